@@ -1,0 +1,188 @@
+/*
+ * stcdb.h — C ABI of the MI355X co-occurrence pipeline (libstcdb_hip.so).
+ *
+ * This is the drop-in boundary for the reference's GPU path. The reference
+ * selects its GPU path at compile time (USE_GPU, src/Makefile:28-32) and
+ * binds three C++ functions:
+ *
+ *   vector<gpu_info*> get_gpus()                                src/cuda/cuda_common.cu:48
+ *   workbench *cuda_create_device_bench(workbench*, gpu_info*)  src/cuda/geometry.cu:1395
+ *   void process_with_gpu(workbench*, workbench*, gpu_info*)    src/cuda/geometry.cu:1573
+ *     (declared at src/tracing/process.cpp:373-376, called at :423 and :462)
+ *
+ * They pass whole workbench structs by pointer and exit() on any CUDA error
+ * (CUDA_SAFE_CALL, src/cuda/cuda_util.cuh:19-27). Here the same operations
+ * take plain structs, pointers and sizes, return status codes, and keep all
+ * device state behind an opaque context. All functions are thread-compatible:
+ * one context per host thread.
+ *
+ * Status codes: STC_OK (0) or a negative STC_E_* value; stc_last_error()
+ * gives the message of the last failure on a context.
+ */
+#ifndef STCDB_H_
+#define STCDB_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define STC_OK 0
+#define STC_E_ARG (-1)       /* invalid argument or configuration */
+#define STC_E_HIP (-2)       /* HIP runtime error (no device, launch failure, ...) */
+#define STC_E_CAPACITY (-3)  /* a device buffer would overflow and could not grow */
+#define STC_E_INPUT (-4)     /* a point lies outside the map box */
+#define STC_E_STATE (-5)     /* call out of order (e.g. inspection before any step) */
+
+/* compat bits */
+#define STC_COMPAT_HALF_REDUCTION 1u /* reproduce geometry.cu:865-888 and 955-974 */
+
+/* Configuration: the subset of the reference's `configuration`
+ * (src/util/config.h:15-86) that the GPU path reads. */
+typedef struct stc_config {
+    uint32_t num_objects;         /* config->num_objects */
+    uint32_t grid_capacity;       /* config->grid_capacity: a QTree block splits above it */
+    double grid_amplify;          /* config->grid_amplify: points stored per leaf = amplify * capacity */
+    uint32_t min_meet_time;       /* config->min_meet_time (seconds) */
+    uint32_t max_meet_time;       /* config->max_meet_time (seconds); longer meetings go to the oversize buffer */
+    uint32_t start_time;          /* config->start_time */
+    uint32_t kv_restriction;      /* flush once the memtable holds more records (config->kv_restriction) */
+    uint32_t ctf_count;           /* config->CTF_count, a square */
+    uint32_t compat;              /* STC_COMPAT_* bits */
+    uint64_t num_meeting_buckets; /* initial meeting-table size; 0 = 4 * num_objects. Grows on demand. */
+    double reach_distance;        /* config->reach_distance (metres) */
+    double area_threshold;        /* workbench::dProbFalse: object MBR area (deg^2) above which it is oversize */
+    uint32_t keep_pairs;          /* 1: keep each step's pair list for stc_get_pairs */
+    uint32_t reserved;
+} stc_config;
+
+typedef struct stc_box {
+    double low[2];
+    double high[2];
+} stc_box;
+
+/* Per-step counters, the fields process_with_gpu copies back to the host
+ * workbench (geometry.cu:1655-1745) plus device-side stage timings. */
+typedef struct stc_step_out {
+    uint64_t nodes;            /* QTree nodes built this step */
+    uint64_t leaves;           /* QTree leaves (grids) */
+    uint64_t border_points;    /* filter_list_index */
+    uint64_t cross_units;      /* point-leaf checks added by the border lookup */
+    uint64_t candidates;       /* point pairs whose distance was evaluated */
+    uint64_t pairs;            /* unique pairs within reach_distance */
+    uint64_t active_meetings;  /* num_active_meetings */
+    uint64_t expired;          /* meetings that ended this step (emitted or not) */
+    uint64_t new_records;      /* records added to the memtable this step */
+    uint64_t kv_count;         /* records in the memtable after the step (before a flush) */
+    uint64_t overflow_points;  /* points beyond the leaf buffer (not stored as targets) */
+    uint64_t ambiguous_pairs;  /* pair decisions re-evaluated on the host (distance within 16 ulp of r) */
+    uint64_t ambiguous_nodes;  /* box-distance decisions within 16 ulp of r (decided on the device) */
+    uint64_t table_buckets;    /* meeting-table size */
+    uint64_t table_occupied;   /* buckets holding a key (live or retired) */
+    uint32_t flushed;          /* 1 if this step flushed a run */
+    uint32_t rebuilt;          /* 1 if the meeting table was rehashed this step */
+    float ms_partition;        /* QTree build + partition */
+    float ms_filter;           /* border lookup */
+    float ms_refine;           /* pair refinement */
+    float ms_meetings;         /* table insert + identify */
+    float ms_flush;            /* flush (0 when no flush) */
+    float ms_total;
+} stc_step_out;
+
+/* CTF metadata of a flushed run: the fields of the reference's CTF
+ * (src/tracing/step_merge.h:64-109) without its host pointers. */
+typedef struct stc_ctf_meta {
+    float mbr[4];              /* ctf_mbr: low0, low1, high0, high1 */
+    uint32_t kv_capacity;
+    uint32_t start_time_min, start_time_max;
+    uint32_t end_time_min, end_time_max;
+    uint32_t bitmap_size;      /* bytes of the bitmap written with the CTF meta */
+    uint16_t key_bit, id_bit, duration_bit, end_bit;
+    uint16_t low_x_bit, low_y_bit, edge_bit, mbr_bit;
+    uint16_t x_grid, y_grid;
+    uint32_t flags;            /* bit0: the reference would fail an assert on this CTF */
+} stc_ctf_meta;
+
+/* A flushed run (the reference's CTB, src/tracing/step_merge.h:111-131, and
+ * its oversize buffer, :44-62). */
+typedef struct stc_run_info {
+    uint32_t kv_count;         /* records stored in the CTFs */
+    uint32_t oversize_count;   /* records in the oversize buffer */
+    uint32_t ctf_count;
+    uint32_t num_objects;
+    uint32_t o_start_min, o_start_max, o_end_min, o_end_max; /* oversize buffer time bounds */
+    uint32_t end_time_min;     /* time base of the end offsets in this run's keys */
+    uint32_t cur_time;         /* timestep that triggered the flush */
+    uint32_t sid_count, oversize_oid_count;
+    uint64_t key_bytes;        /* packed CTF key bytes over all CTFs */
+} stc_run_info;
+
+/* Device description (the reference's gpu_info, src/cuda/mygpu.h:18-50). */
+typedef struct stc_device_info {
+    int32_t device_id;
+    char name[256];
+    uint64_t mem_size;
+    int32_t num_cus;
+    int32_t clock_khz;
+} stc_device_info;
+
+/* ---- device discovery: replaces get_gpus() (cuda_common.cu:48-72) ---- */
+int stc_device_count(int32_t *count);
+int stc_device_info_get(int32_t device, stc_device_info *info);
+
+/* ---- lifecycle: replaces cuda_create_device_bench (geometry.cu:1395-1565) ----
+ * Allocates all device state for `cfg` on the current HIP device. map_mbr is
+ * the map box (tracer::mbr). */
+int stc_create(const stc_config *cfg, const stc_box *map_mbr, void **ctx);
+int stc_destroy(void *ctx);
+const char *stc_last_error(void *ctx);
+
+/* ---- one timestep: replaces process_with_gpu (geometry.cu:1573-2218) ----
+ * xy: num_objects AoS (x, y) doubles in degrees (the reference's Point array,
+ * geometry.h:23-56). stc_step copies host memory to the device first;
+ * stc_step_device takes a device pointer already resident in HBM. t is
+ * bench->cur_time. A flush runs inside the step when the memtable exceeds
+ * kv_restriction, as in process_with_gpu; its run is then available through
+ * stc_num_runs/stc_get_run. */
+int stc_step(void *ctx, const double *xy_host, uint32_t t, stc_step_out *out);
+int stc_step_device(void *ctx, const double *xy_dev, uint32_t t, stc_step_out *out);
+
+/* ---- inspection of the current state (copies to host memory) ----
+ * Each getter writes at most `cap` items and stores the total count in *n. */
+int stc_get_pairs(void *ctx, uint64_t *keys, uint64_t cap, uint64_t *n);   /* last step, sorted, oid1<<32|oid2 */
+int stc_get_active(void *ctx, uint64_t *keys, uint32_t *start, uint32_t *end, float *mbr, uint64_t cap, uint64_t *n);
+int stc_get_records(void *ctx, uint64_t *keys, float *boxes, uint64_t cap, uint64_t *n); /* memtable, keys as lo,hi */
+int stc_get_leaves(void *ctx, double *boxes, uint32_t *counts, uint64_t cap, uint64_t *n); /* last step's leaves */
+
+/* ---- flushed runs ---- */
+int stc_num_runs(void *ctx, uint32_t *n);
+int stc_get_run_info(void *ctx, uint32_t run, stc_run_info *info);
+/* any output pointer may be NULL; sizes follow stc_run_info:
+ * ctfs[ctf_count], key_bytes[key_bytes], bitmaps[ctf_count * 8192],
+ * sids[num_objects], o_keys[2 * oversize_count], o_boxes[4 * oversize_count],
+ * o_bitmap[8192], sorted_keys[2 * kv_count], sorted_boxes[4 * kv_count] */
+int stc_get_run(void *ctx, uint32_t run, stc_ctf_meta *ctfs, uint8_t *key_bytes, uint8_t *bitmaps,
+                uint16_t *sids, uint64_t *o_keys, float *o_boxes, uint8_t *o_bitmap,
+                uint64_t *sorted_keys, float *sorted_boxes);
+int stc_drop_runs(void *ctx); /* release host copies of flushed runs */
+
+/* ---- seeded synthetic traces (the reference's generator is unseeded) ----
+ * Parameter and state layouts are those of stcdb_amd/csrc/stc_gen.h. */
+typedef struct stc_gen_params stc_gen_params;
+typedef struct stc_walker stc_walker;
+int stc_gen_init_host(const stc_gen_params *p, stc_walker *w, uint32_t first, uint32_t count);
+/* advance host walkers from step t-1 to t (t >= 1) */
+int stc_gen_step_host(const stc_gen_params *p, stc_walker *w, uint32_t count, uint32_t t);
+int stc_gen_pos_host(const stc_gen_params *p, const stc_walker *w, uint32_t count, double *xy);
+/* device-resident trace: walkers live in HBM; stc_trace_next advances to t
+ * (t == 0 returns the initial positions) and returns a device pointer to
+ * num_objects AoS points valid until the next call. */
+int stc_trace_create(const stc_gen_params *p, void **trace);
+int stc_trace_next(void *trace, uint32_t t, const double **xy_dev);
+int stc_trace_destroy(void *trace);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* STCDB_H_ */

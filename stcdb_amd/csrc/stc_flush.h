@@ -1,0 +1,66 @@
+/*
+ * stc_flush.h — sort-based flush of the memtable into a run of CTFs.
+ */
+#ifndef STC_FLUSH_H_
+#define STC_FLUSH_H_
+
+#include "stc_internal.h"
+#include "../../include/stcdb.h"
+#include <vector>
+
+struct FlushWork {
+    uint32_t n = 0, ctf_count = 0;
+    /* device scratch, grown on demand */
+    uint64_t *mid = nullptr, *mid_alt = nullptr;   /* per-object STR keys */
+    uint32_t *oids = nullptr, *oids_alt = nullptr;
+    uint32_t *capacity = nullptr;                  /* per-CTF record counts */
+    uint32_t *ctf_base = nullptr;                  /* per-CTF first record */
+    uint64_t *ctf_byte_base = nullptr;             /* per-CTF first packed byte */
+    stc_ctf_meta *ctfs = nullptr;
+    uint8_t *bitmaps = nullptr;                    /* ctf_count * 8192 */
+    uint32_t *obj_base = nullptr, *obj_fill = nullptr; /* per-object record slots */
+    uint32_t *scan_temp = nullptr;
+    size_t scan_temp_words = 0;
+    uint32_t *flags = nullptr;                     /* device counters of the flush */
+    /* record-sized scratch */
+    uint64_t rec_cap = 0;
+    uint64_t *keys_out = nullptr;                  /* sorted keys (lo, hi) */
+    float4 *boxes_out = nullptr;
+    uint32_t *rec_perm = nullptr;
+    uint8_t *key_bytes = nullptr;
+    uint64_t key_bytes_cap = 0;
+};
+
+struct FlushArgs {
+    uint32_t n, ctf_count;
+    uint64_t kv_count;
+    uint32_t kv_restriction;
+    uint32_t end_time_min, cur_time, max_meet_time;
+    uint32_t sid_count, oversize_oid_count;
+    uint32_t compat;
+    double area_threshold;
+    double map[4];
+    uint64_t *rec_keys;      /* (lo, hi) per record */
+    float4 *rec_boxes;
+    uint32_t *same_pid_count;
+    uint4 *o_boxs;
+    uint16_t *sids;
+};
+
+struct FlushOut {
+    stc_run_info info;
+    std::vector<stc_ctf_meta> ctfs;
+    std::vector<uint8_t> key_bytes, bitmaps, o_bitmap;
+    std::vector<uint16_t> sids;
+    std::vector<uint64_t> o_keys, sorted_keys;
+    std::vector<float> o_boxes, sorted_boxes;
+};
+
+int stc_flush_init(FlushWork *w, uint32_t n, uint32_t ctf_count);
+void stc_flush_free(FlushWork *w);
+int stc_flush_run(FlushWork *w, const FlushArgs *a, FlushOut *out, hipStream_t s);
+
+/* CTF::get_ctf_bits (src/tracing/step_merge.cpp:140-186), host arithmetic */
+void stc_ctf_bits(stc_ctf_meta *c, const double *map, uint32_t num_objects, uint32_t max_meet_time);
+
+#endif

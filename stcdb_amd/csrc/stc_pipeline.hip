@@ -1,0 +1,1282 @@
+/*
+ * stc_pipeline.hip — per-timestep co-occurrence pipeline on gfx950 and the
+ * C ABI of include/stcdb.h.
+ *
+ * One timestep (the work of process_with_gpu, src/cuda/geometry.cu:1573-2218):
+ *
+ *  1. prep      per point: QTree cell (16384 x 16384 grid over the map, as
+ *               cuda_build_qtree geometry.cu:1270-1279), its Morton code, and
+ *               the point's longitude scale dpm_lon(y) (stc_geo.h).
+ *  2. sort      stable radix sort of the Morton codes.
+ *  3. tree      top-down QTree build over the sorted codes: a block splits when
+ *               it holds more than grid_capacity points; blocks at the finest
+ *               level split once more into four halves. This is the QTree that
+ *               the reference's bottom-up merge builds (geometry.cu:1295-1393),
+ *               with the same node box expressions. Counts come from binary
+ *               searches on the sorted codes, not a 2 GiB dense histogram.
+ *  4. partition per point: descend by mid_x/mid_y to its leaf and record the
+ *               deepest ancestor whose top/right edge is at least the border
+ *               buffer away (cuda_partition, geometry.cu:221-271).
+ *  5. leaves    stable sort of points by leaf: each leaf's members in id order,
+ *               stored in leaf order as SoA x, y, dpm_lon, id. A leaf keeps its
+ *               lowest grid_amplify * grid_capacity ids as targets.
+ *  6. lookup    border points search the subtree under their last valid node
+ *               for leaves within reach on their top or right (cuda_filtering,
+ *               geometry.cu:295-392): a counting pass, a prefix sum, and an
+ *               emitting pass, so no global atomics and a deterministic order.
+ *  7. refine    one wavefront per check unit; lanes stride over the target
+ *               leaf's members with coalesced SoA loads; hits are compacted with
+ *               a ballot and one atomic per wavefront (cuda_refinement,
+ *               geometry.cu:424-497).
+ *  8. meetings  open-addressing table keyed by (oid1 << 32 | oid2) with 64-bit
+ *               CAS: insert or extend, MBR by ordered-integer atomic min/max.
+ *  9. identify  full-table scan: meetings not seen this step expire; those that
+ *               lasted at least min_meet_time and stay inside the map emit two
+ *               records each (cuda_identify_meetings, geometry.cu:595-737).
+ *               Expired entries keep their key (so probe chains stay intact)
+ *               until the table is rehashed.
+ * 10. flush     when the memtable exceeds kv_restriction (stc_flush.hip).
+ */
+#include "stc_internal.h"
+#include "stc_geo.h"
+#include "stc_gen.h"
+#include "stc_flush.h"
+#include "../../include/stcdb.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#define ONE_DIM 16384u
+#define NLEVELS 15
+#define T_BRANCH 1u
+#define T_LEAF 2u
+#define EMPTY_KEY 0xFFFFFFFFFFFFFFFFull
+#define STACK_DEPTH 96
+
+static thread_local std::string g_last_fail;
+
+int stc_hip_fail(hipError_t e, const char *expr, const char *file, int line) {
+    char buf[512];
+    snprintf(buf, sizeof(buf), "%s failed: %s (%s:%d)", expr, hipGetErrorString(e), file, line);
+    g_last_fail = buf;
+    return STC_E_HIP;
+}
+
+/* ------------------------------------------------------------------ */
+/* device structures                                                   */
+/* ------------------------------------------------------------------ */
+
+struct Node {                 /* QTSchema (src/index/QTree.h:49-62) without counters */
+    double lo0, lo1, hi0, hi1;
+    double mid_x, mid_y;
+    uint32_t child[4];
+    uint32_t type, leaf;
+    uint32_t code, level;     /* Morton code and level of the block (branches) */
+};
+
+struct FrontierItem { uint32_t node, code; };
+
+struct __align__(32) Bucket { /* meeting_unit (workbench.h:62-83) with an ordered-integer MBR */
+    unsigned long long key;
+    uint32_t start, end;
+    uint32_t mbr[4];          /* ordered ints of low0, low1, high0, high1 */
+};
+
+struct PairHit {              /* 24 B: key and the float MBR of the two positions */
+    unsigned long long key;
+    float mbr[4];
+};
+
+struct Counters {             /* device-side counters, copied back once per sync point */
+    unsigned long long units;
+    unsigned long long candidates;
+    unsigned long long pairs;
+    unsigned long long ambiguous;
+    unsigned long long ambiguous_nodes;
+    unsigned long long live, expired, retired;
+    unsigned long long kv_count;
+    uint32_t nodes, leaves;
+    uint32_t frontier[NLEVELS + 1];
+    uint32_t bad_points;
+    uint32_t border;
+    uint32_t overflow;
+    uint32_t sid_count, oversize_oid_count;
+    uint32_t table_fail;
+};
+
+__host__ __device__ inline uint32_t f2o(float f) {
+    uint32_t b;
+    memcpy(&b, &f, 4);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__host__ __device__ inline float o2f(uint32_t o) {
+    uint32_t b = (o & 0x80000000u) ? (o & 0x7FFFFFFFu) : ~o;
+    float f;
+    memcpy(&f, &b, 4);
+    return f;
+}
+
+__device__ inline uint32_t part1by1(uint32_t x) {
+    x &= 0x0000ffff;
+    x = (x | (x << 8)) & 0x00FF00FF;
+    x = (x | (x << 4)) & 0x0F0F0F0F;
+    x = (x | (x << 2)) & 0x33333333;
+    x = (x | (x << 1)) & 0x55555555;
+    return x;
+}
+__device__ inline uint32_t compact1by1(uint32_t x) {
+    x &= 0x55555555;
+    x = (x | (x >> 1)) & 0x33333333;
+    x = (x | (x >> 2)) & 0x0F0F0F0F;
+    x = (x | (x >> 4)) & 0x00FF00FF;
+    x = (x | (x >> 8)) & 0x0000FFFF;
+    return x;
+}
+
+__device__ inline uint64_t hash_slot(unsigned long long key, int log2b) {
+    return (uint64_t)((key * 0x9E3779B97F4A7C15ull) >> (64 - log2b));
+}
+
+/* ------------------------------------------------------------------ */
+/* kernels: prep, tree, partition                                      */
+/* ------------------------------------------------------------------ */
+
+struct MapBox { double lo0, lo1, hi0, hi1; };
+
+__global__ void k_prep(const double2 *__restrict__ xy, uint32_t n, MapBox m, uint32_t *__restrict__ codes,
+                       uint32_t *__restrict__ idx, double *__restrict__ dlon, Counters *cnt) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double2 p = xy[i];
+    double fx = (p.x - m.lo0) / (m.hi0 - m.lo0) * ONE_DIM;
+    double fy = (p.y - m.lo1) / (m.hi1 - m.lo1) * ONE_DIM;
+    uint32_t code = 0;
+    if (fx > -1.0 && fx < (double)ONE_DIM && fy > -1.0 && fy < (double)ONE_DIM) {
+        code = part1by1((uint32_t)fx) | (part1by1((uint32_t)fy) << 1);
+    } else {
+        atomicAdd(&cnt->bad_points, 1u);
+    }
+    codes[i] = code;
+    idx[i] = i;
+    dlon[i] = stc_dpm_lon(p.y);
+}
+
+/* box of block (x, y) with xdim blocks per side (geometry.cu:1306-1311) */
+__device__ inline void block_box(Node &nd, MapBox m, uint32_t x, uint32_t y, uint32_t xdim) {
+    nd.lo0 = m.lo0 + (double)x * (m.hi0 - m.lo0) / (double)xdim;
+    nd.lo1 = m.lo1 + (double)y * (m.hi1 - m.lo1) / (double)xdim;
+    nd.hi0 = m.lo0 + (double)(x + 1) * (m.hi0 - m.lo0) / (double)xdim;
+    nd.hi1 = m.lo1 + (double)(y + 1) * (m.hi1 - m.lo1) / (double)xdim;
+    nd.mid_x = (nd.lo0 + nd.hi0) / 2;
+    nd.mid_y = (nd.lo1 + nd.hi1) / 2;
+}
+
+/* leaf child i of parent, by halving (geometry.cu:1318-1334) */
+__device__ inline void halving_leaf(Node &nd, const Node &p, int i) {
+    double xhalf = p.mid_x - p.lo0;
+    double yhalf = p.mid_y - p.lo1;
+    nd.lo0 = p.lo0 + (double)(i % 2 == 1) * xhalf;
+    nd.lo1 = p.lo1 + (double)(i / 2 == 1) * yhalf;
+    nd.hi0 = p.mid_x + (double)(i % 2 == 1) * xhalf;
+    nd.hi1 = p.mid_y + (double)(i / 2 == 1) * yhalf;
+    nd.mid_x = (nd.lo0 + nd.hi0) / 2;
+    nd.mid_y = (nd.lo1 + nd.hi1) / 2;
+}
+
+__device__ inline uint32_t lower_bound_u32(const uint32_t *a, uint32_t n, uint32_t v) {
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (a[mid] < v) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+__global__ void k_tree_root(Node *nodes, MapBox m, FrontierItem *frontier, Counters *cnt, uint32_t n, uint32_t cap) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    Node nd;
+    block_box(nd, m, 0, 0, 1);
+    nd.type = T_BRANCH;
+    nd.leaf = 0;
+    nd.code = 0;
+    nd.level = NLEVELS - 1;
+    for (int c = 0; c < 4; c++) nd.child[c] = 0;
+    nodes[0] = nd;
+    cnt->nodes = 1;
+    cnt->leaves = 0;
+    for (int l = 0; l <= NLEVELS; l++) cnt->frontier[l] = 0;
+    if (n > cap) {
+        frontier[0].node = 0;
+        frontier[0].code = 0;
+        cnt->frontier[NLEVELS - 1] = 1;
+    }
+}
+
+/* expand the branch blocks of level L (L >= 1): each of the four sub-blocks
+ * becomes a branch when it holds more than cap points, else a halving leaf */
+__global__ void k_tree_level(Node *nodes, MapBox m, const uint32_t *sorted_codes, uint32_t n, uint32_t cap, int L,
+                             const FrontierItem *fin, FrontierItem *fout, Counters *cnt, uint32_t node_cap,
+                             uint32_t front_cap) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cnt->frontier[L]) return;
+    FrontierItem f = fin[i];
+    Node parent = nodes[f.node];
+    int Lc = L - 1;
+    uint32_t xdim = ONE_DIM >> Lc;
+    uint32_t child_ids[4];
+    uint32_t nb = 0;
+    bool is_branch[4];
+    for (int c = 0; c < 4; c++) {
+        uint32_t sub = (f.code << 2) | (uint32_t)c;
+        uint32_t lo = sub << (2 * Lc);
+        uint32_t a = lower_bound_u32(sorted_codes, n, lo);
+        uint32_t b = lower_bound_u32(sorted_codes, n, lo + (1u << (2 * Lc)));
+        is_branch[c] = (b - a) > cap;
+        nb += is_branch[c];
+    }
+    uint32_t base = atomicAdd(&cnt->nodes, 4u);
+    if (base + 4 > node_cap) { atomicAdd(&cnt->table_fail, 1u); return; }
+    uint32_t nleaf = 4 - nb;
+    uint32_t lbase = nleaf ? atomicAdd(&cnt->leaves, nleaf) : 0;
+    uint32_t fbase = nb ? atomicAdd(&cnt->frontier[Lc], nb) : 0;
+    if (fbase + nb > front_cap) { atomicAdd(&cnt->table_fail, 1u); return; }
+    for (int c = 0; c < 4; c++) {
+        uint32_t id = base + c;
+        child_ids[c] = id;
+        Node nd;
+        uint32_t sub = (f.code << 2) | (uint32_t)c;
+        for (int k = 0; k < 4; k++) nd.child[k] = 0;
+        if (is_branch[c]) {
+            block_box(nd, m, compact1by1(sub), compact1by1(sub >> 1), xdim);
+            nd.type = T_BRANCH;
+            nd.leaf = 0;
+            nd.code = sub;
+            nd.level = Lc;
+            fout[fbase].node = id;
+            fout[fbase].code = sub;
+            fbase++;
+        } else {
+            halving_leaf(nd, parent, c);
+            nd.type = T_LEAF;
+            nd.leaf = lbase++;
+            nd.code = sub;
+            nd.level = Lc;
+        }
+        nodes[id] = nd;
+    }
+    for (int c = 0; c < 4; c++) nodes[f.node].child[c] = child_ids[c];
+}
+
+/* finest level: each overfull cell becomes a branch of four halving leaves */
+__global__ void k_tree_finest(Node *nodes, const FrontierItem *fin, Counters *cnt, uint32_t node_cap) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cnt->frontier[0]) return;
+    FrontierItem f = fin[i];
+    Node parent = nodes[f.node];
+    uint32_t base = atomicAdd(&cnt->nodes, 4u);
+    if (base + 4 > node_cap) { atomicAdd(&cnt->table_fail, 1u); return; }
+    uint32_t lbase = atomicAdd(&cnt->leaves, 4u);
+    for (int c = 0; c < 4; c++) {
+        Node nd;
+        halving_leaf(nd, parent, c);
+        for (int k = 0; k < 4; k++) nd.child[k] = 0;
+        nd.type = T_LEAF;
+        nd.leaf = lbase + c;
+        nd.code = f.code;
+        nd.level = 0;
+        nodes[base + c] = nd;
+        nodes[f.node].child[c] = base + c;
+    }
+}
+
+__global__ void k_leaf_nodes(const Node *nodes, uint32_t n_nodes, uint32_t *leaf_node) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_nodes) return;
+    if (nodes[i].type == T_LEAF) leaf_node[nodes[i].leaf] = i;
+}
+
+__global__ void k_partition(const double2 *__restrict__ xy, uint32_t n, const Node *__restrict__ nodes, double xbuf,
+                            double ybuf, uint32_t *__restrict__ pt_leaf, uint32_t *__restrict__ pt_lv,
+                            uint32_t *__restrict__ leaf_count, Counters *cnt) {
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    bool border = false;
+    if (p < n) {
+        double2 q = xy[p];
+        uint32_t cur = 0, lv = 0;
+        for (int depth = 0; depth < 64; depth++) {
+            const Node &nd = nodes[cur];
+            int loc = (q.y > nd.mid_y) * 2 + (q.x > nd.mid_x);
+            cur = nd.child[loc];
+            const Node &c = nodes[cur];
+            if (q.x + xbuf < c.hi0 && q.y + ybuf < c.hi1) lv = cur;
+            if (c.type == T_LEAF) break;
+        }
+        uint32_t leaf = nodes[cur].leaf;
+        pt_leaf[p] = leaf;
+        border = (lv != cur);
+        pt_lv[p] = border ? lv : 0xFFFFFFFFu;
+        atomicAdd(&leaf_count[leaf], 1u);
+    }
+    uint64_t b = __ballot(border);
+    if ((threadIdx.x & (STC_WAVE - 1)) == 0 && b) atomicAdd(&cnt->border, (uint32_t)__popcll(b));
+}
+
+__global__ void k_iota(uint32_t *a, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = i;
+}
+
+/* leaf-ordered SoA and each point's rank within its leaf */
+__global__ void k_gather(const double2 *__restrict__ xy, const double *__restrict__ dlon,
+                         const uint32_t *__restrict__ lp, const uint32_t *__restrict__ pt_leaf,
+                         const uint32_t *__restrict__ leaf_start, uint32_t n, double *__restrict__ lx,
+                         double *__restrict__ ly, double *__restrict__ ld, uint32_t *__restrict__ pt_rank) {
+    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    uint32_t p = lp[j];
+    double2 q = xy[p];
+    lx[j] = q.x;
+    ly[j] = q.y;
+    ld[j] = dlon[p];
+    pt_rank[p] = j - leaf_start[pt_leaf[p]];
+}
+
+__global__ void k_overflow(const uint32_t *leaf_count, uint32_t nl, uint32_t buf, Counters *cnt) {
+    uint32_t l = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t over = 0;
+    if (l < nl && leaf_count[l] > buf) over = leaf_count[l] - buf;
+    if (over) atomicAdd(&cnt->overflow, over);
+}
+
+/* ------------------------------------------------------------------ */
+/* border lookup (two passes)                                          */
+/* ------------------------------------------------------------------ */
+
+template <bool EMIT>
+__global__ void k_lookup(const double2 *__restrict__ xy, const double *__restrict__ dlon, uint32_t n,
+                         const Node *__restrict__ nodes, const uint32_t *__restrict__ pt_lv, double r, double band,
+                         uint32_t *__restrict__ count, const uint32_t *__restrict__ offset,
+                         uint2 *__restrict__ units, Counters *cnt) {
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    uint32_t lv = pt_lv[p];
+    if (lv == 0xFFFFFFFFu) {
+        if (!EMIT) count[p] = 0;
+        return;
+    }
+    double2 q = xy[p];
+    double dl = dlon[p];
+    uint32_t stk[STACK_DEPTH];
+    int sp = 0;
+    stk[sp++] = lv;
+    uint32_t found = 0, amb = 0;
+    uint32_t out = EMIT ? offset[p] : 0;
+    while (sp > 0) {
+        uint32_t cur = stk[--sp];
+        const Node &nd = nodes[cur];
+        for (int i = 0; i < 4; i++) {
+            uint32_t c = nd.child[i];
+            const Node &ch = nodes[c];
+            double d = stc_box_dist(ch.lo0, ch.lo1, ch.hi0, ch.hi1, q.x, q.y, dl);
+            if (fabs(d - r) <= band) amb++;
+            if (d <= r) {
+                if (ch.type == T_LEAF) {
+                    if (q.y < ch.lo1 || (q.y < ch.hi1 && q.x < ch.lo0)) {
+                        if (EMIT) units[out + found] = make_uint2(p, ch.leaf);
+                        found++;
+                    }
+                } else if (sp < STACK_DEPTH) {
+                    stk[sp++] = c;
+                }
+            }
+        }
+    }
+    if (!EMIT) {
+        count[p] = found;
+        if (amb) atomicAdd(&cnt->ambiguous_nodes, (unsigned long long)amb);
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* refinement                                                          */
+/* ------------------------------------------------------------------ */
+
+__global__ __launch_bounds__(256) void k_refine(
+    const double2 *__restrict__ xy, const double *__restrict__ dlon, uint32_t n, const uint32_t *__restrict__ pt_leaf,
+    const uint32_t *__restrict__ pt_rank, const uint32_t *__restrict__ leaf_start,
+    const uint32_t *__restrict__ leaf_count, const uint32_t *__restrict__ lp, const double *__restrict__ lx,
+    const double *__restrict__ ly, const uint2 *__restrict__ units, uint64_t n_units_total, uint32_t buf, double r,
+    double band, PairHit *__restrict__ pairs, uint64_t pair_cap, uint2 *__restrict__ amb, uint32_t amb_cap,
+    Counters *cnt) {
+    const int lane = threadIdx.x & (STC_WAVE - 1);
+    const uint64_t wid = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / STC_WAVE;
+    const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) / STC_WAVE;
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (STC_WAVE - lane));
+    unsigned long long cand = 0;
+    for (uint64_t u = wid; u < n_units_total; u += nw) {
+        uint32_t p, leaf, begin;
+        if (u < n) {
+            p = (uint32_t)u;
+            leaf = pt_leaf[p];
+            begin = pt_rank[p] + 1;
+        } else {
+            uint2 un = units[u - n];
+            p = un.x;
+            leaf = un.y;
+            begin = 0;
+        }
+        uint32_t s = leaf_count[leaf];
+        if (s > buf) s = buf;
+        if (begin >= s) continue;
+        uint32_t st = leaf_start[leaf];
+        double2 pp = xy[p];
+        double pd = dlon[p];
+        float pfx = (float)pp.x, pfy = (float)pp.y;
+        for (uint32_t m0 = begin; m0 < s; m0 += STC_WAVE) {
+            uint32_t m = m0 + lane;
+            bool hit = false, ambg = false;
+            uint32_t q = 0;
+            double qx = 0, qy = 0;
+            if (m < s) {
+                q = lp[st + m];
+                qx = lx[st + m];
+                qy = ly[st + m];
+                double d = stc_point_dist(pp.x, pp.y, qx, qy, pd);
+                if (fabs(d - r) <= band) ambg = true;
+                else hit = d <= r;
+            }
+            cand += (m < s) ? 1 : 0;
+            uint64_t hm = __ballot(hit);
+            if (hm) {
+                unsigned long long basei = 0;
+                if (lane == 0) basei = atomicAdd(&cnt->pairs, (unsigned long long)__popcll(hm));
+                basei = __shfl(basei, 0, STC_WAVE);
+                if (hit) {
+                    uint64_t k = basei + (uint64_t)__popcll(hm & lt);
+                    if (k < pair_cap) {
+                        uint32_t a = p < q ? p : q, b = p < q ? q : p;
+                        PairHit h;
+                        h.key = ((unsigned long long)a << 32) | b;
+                        float qfx = (float)qx, qfy = (float)qy;
+                        h.mbr[0] = fminf(pfx, qfx);
+                        h.mbr[1] = fminf(pfy, qfy);
+                        h.mbr[2] = fmaxf(pfx, qfx);
+                        h.mbr[3] = fmaxf(pfy, qfy);
+                        pairs[k] = h;
+                    }
+                }
+            }
+            if (ambg) {
+                unsigned long long k = atomicAdd(&cnt->ambiguous, 1ull);
+                if (k < amb_cap) amb[k] = make_uint2(p, q);
+            }
+        }
+    }
+    /* reduce candidate counts over the wavefront */
+    for (int o = STC_WAVE / 2; o > 0; o >>= 1) cand += __shfl_down(cand, o, STC_WAVE);
+    if (lane == 0 && cand) atomicAdd(&cnt->candidates, cand);
+}
+
+/* ------------------------------------------------------------------ */
+/* meeting table                                                       */
+/* ------------------------------------------------------------------ */
+
+__global__ void k_table_clear(Bucket *t, uint64_t nb) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nb) return;
+    Bucket b;
+    b.key = EMPTY_KEY;
+    b.start = 0;
+    b.end = 0xFFFFFFFFu;
+    b.mbr[0] = b.mbr[1] = 0xFFFFFFFFu;
+    b.mbr[2] = b.mbr[3] = 0u;
+    t[i] = b;
+}
+
+__device__ inline Bucket *table_find_or_claim(Bucket *t, int log2b, unsigned long long key, bool *claimed) {
+    uint64_t mask = (1ull << log2b) - 1;
+    uint64_t h = hash_slot(key, log2b);
+    for (uint64_t probe = 0; probe <= mask; probe++) {
+        Bucket *b = &t[h];
+        unsigned long long k = __atomic_load_n(&b->key, __ATOMIC_RELAXED);
+        if (k == key) { *claimed = false; return b; }
+        if (k == EMPTY_KEY) {
+            unsigned long long prev = atomicCAS(&b->key, EMPTY_KEY, key);
+            if (prev == EMPTY_KEY) { *claimed = true; return b; }
+            if (prev == key) { *claimed = false; return b; }
+        }
+        h = (h + 1) & mask;
+    }
+    return nullptr;
+}
+
+/* insert or extend (geometry.cu:469-494, without the 5-probe limit) */
+__global__ void k_table_insert(Bucket *t, int log2b, const PairHit *__restrict__ pairs, uint64_t np, uint32_t now,
+                               Counters *cnt) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    PairHit h = pairs[i];
+    bool claimed;
+    Bucket *b = table_find_or_claim(t, log2b, h.key, &claimed);
+    if (!b) { atomicAdd(&cnt->table_fail, 1u); return; }
+    uint32_t old_end = atomicExch(&b->end, now);
+    if (claimed || (old_end != now && old_end != now - 1)) b->start = now; /* new or restarted meeting */
+    atomicMin(&b->mbr[0], f2o(h.mbr[0]));
+    atomicMin(&b->mbr[1], f2o(h.mbr[1]));
+    atomicMax(&b->mbr[2], f2o(h.mbr[2]));
+    atomicMax(&b->mbr[3], f2o(h.mbr[3]));
+}
+
+/* float_to_uint (cuda_util.cuh:115-119), saturating as the reference GPU converts */
+__device__ inline uint32_t f2q(float xy) {
+    xy += 180;
+    float v = xy * 100000;
+    if (!(v > 0.0f)) return 0;
+    if (v >= 4294967296.0f) return 0xFFFFFFFFu;
+    return (uint32_t)v;
+}
+
+struct MemtableDev {
+    uint64_t *keys;        /* the reference's 128-bit key as (lo, hi) words:
+                              hi = pid << 26 | target, lo = duration << 32 | end offset */
+    float4 *boxes;
+    uint64_t cap;
+    uint32_t *same_pid_count;
+    uint4 *o_boxs;         /* quantized per-object MBR: low0, low1, high0, high1 */
+    uint16_t *sids;
+};
+
+/* identify completed meetings (geometry.cu:595-737) */
+__global__ void k_identify(Bucket *t, uint64_t nb, uint32_t now, uint32_t min_meet, uint32_t max_meet,
+                           uint32_t end_time_min, MapBox m, MemtableDev mt, Counters *cnt) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool live = false, expired = false, retired = false;
+    if (i < nb) {
+        Bucket b = t[i];
+        if (b.key != EMPTY_KEY) {
+            if (b.end == now) {
+                live = true;
+            } else if (b.end == now - 1) {
+                expired = true;
+                float lo0 = o2f(b.mbr[0]), lo1 = o2f(b.mbr[1]), hi0 = o2f(b.mbr[2]), hi1 = o2f(b.mbr[3]);
+                if (now - b.start >= min_meet + 1 && (double)lo0 >= m.lo0 && (double)hi0 <= m.hi0 &&
+                    (double)lo1 >= m.lo1 && (double)hi1 <= m.hi1) {
+                    uint32_t duration = b.end - b.start;
+                    uint32_t pid = (uint32_t)(b.key >> 32), target = (uint32_t)b.key;
+                    if (duration > max_meet) {
+                        atomicAdd(&cnt->oversize_oid_count, 1u);
+                        mt.sids[pid] = 1;
+                        mt.sids[target] = 1;
+                    }
+                    unsigned long long k = atomicAdd(&cnt->kv_count, 2ull);
+                    uint32_t q0 = f2q(lo0), q1 = f2q(lo1), q2 = f2q(hi0), q3 = f2q(hi1);
+                    for (int s = 0; s < 2; s++) {
+                        uint32_t a = s ? target : pid, c = s ? pid : target;
+                        if (k + s < mt.cap) {
+                            mt.keys[2 * (k + s)] = ((uint64_t)duration << 32) | (uint32_t)(b.end - end_time_min);
+                            mt.keys[2 * (k + s) + 1] = ((uint64_t)a << 26) | (uint64_t)c;
+                            mt.boxes[k + s] = make_float4(lo0, lo1, hi0, hi1);
+                        }
+                        if (atomicAdd(&mt.same_pid_count[a], 1u) == 0) atomicAdd(&cnt->sid_count, 1u);
+                        atomicMin(&mt.o_boxs[a].x, q0);
+                        atomicMax(&mt.o_boxs[a].z, q2);
+                        atomicMin(&mt.o_boxs[a].y, q1);
+                        atomicMax(&mt.o_boxs[a].w, q3);
+                    }
+                }
+                /* retire: key stays for the probe chains, MBR resets for a restart */
+                t[i].mbr[0] = 0xFFFFFFFFu;
+                t[i].mbr[1] = 0xFFFFFFFFu;
+                t[i].mbr[2] = 0u;
+                t[i].mbr[3] = 0u;
+            } else {
+                retired = true;
+            }
+        }
+    }
+    uint64_t bl = __ballot(live), be = __ballot(expired), br = __ballot(retired);
+    if ((threadIdx.x & (STC_WAVE - 1)) == 0) {
+        if (bl) atomicAdd(&cnt->live, (unsigned long long)__popcll(bl));
+        if (be) atomicAdd(&cnt->expired, (unsigned long long)__popcll(be));
+        if (br) atomicAdd(&cnt->retired, (unsigned long long)__popcll(br));
+    }
+}
+
+/* move live entries into a fresh table */
+__global__ void k_table_rehash(const Bucket *src, uint64_t nsrc, Bucket *dst, int log2b, uint32_t now,
+                               Counters *cnt) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nsrc) return;
+    Bucket b = src[i];
+    if (b.key == EMPTY_KEY || b.end != now) return;
+    bool claimed;
+    Bucket *d = table_find_or_claim(dst, log2b, b.key, &claimed);
+    if (!d) { atomicAdd(&cnt->table_fail, 1u); return; }
+    d->start = b.start;
+    d->end = b.end;
+    d->mbr[0] = b.mbr[0];
+    d->mbr[1] = b.mbr[1];
+    d->mbr[2] = b.mbr[2];
+    d->mbr[3] = b.mbr[3];
+}
+
+__global__ void k_fill_u32(uint32_t *a, uint64_t n, uint32_t v) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+__global__ void k_fill_obox(uint4 *a, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = make_uint4(100000000u, 100000000u, 0u, 0u); /* cuda_init_o_boxs */
+}
+
+/* ------------------------------------------------------------------ */
+/* host context                                                        */
+/* ------------------------------------------------------------------ */
+
+template <typename T>
+struct DBuf {
+    T *p = nullptr;
+    size_t cap = 0;
+    int ensure(size_t n) {
+        if (n <= cap && p) return 0;
+        size_t nc = std::max<size_t>(n + n / 2, 1024);
+        if (p) { hipFree(p); p = nullptr; cap = 0; }
+        if (hipMalloc((void **)&p, nc * sizeof(T)) != hipSuccess) { p = nullptr; return -1; }
+        cap = nc;
+        return 0;
+    }
+    void release() { if (p) hipFree(p); p = nullptr; cap = 0; }
+};
+
+struct RunHost {
+    stc_run_info info;
+    std::vector<stc_ctf_meta> ctfs;
+    std::vector<uint8_t> key_bytes, bitmaps, o_bitmap;
+    std::vector<uint16_t> sids;
+    std::vector<uint64_t> o_keys, sorted_keys;
+    std::vector<float> o_boxes, sorted_boxes;
+};
+
+struct StcCtx {
+    stc_config cfg;
+    MapBox map;
+    std::string err;
+    hipStream_t s = nullptr;
+    uint32_t n = 0, cap = 0, buf = 0;
+    double x_buffer = 0, y_buffer = 0, band = 0;
+    int device = 0;
+
+    DBuf<double2> xy_in;
+    DBuf<uint32_t> codes, idx, codes_alt, idx_alt, temp;
+    DBuf<double> dlon, lx, ly, ld;
+    DBuf<Node> nodes;
+    DBuf<FrontierItem> frontier_a, frontier_b;
+    DBuf<uint32_t> leaf_count, leaf_start, leaf_node;
+    DBuf<uint32_t> pt_leaf, pt_lv, pt_rank, lp, lkeys;
+    DBuf<uint32_t> ucount, uoffset;
+    DBuf<uint2> units;
+    DBuf<PairHit> pairs;
+    DBuf<uint2> amb;
+    Bucket *table = nullptr;
+    uint64_t table_buckets = 0;
+    int log2b = 0;
+    /* memtable */
+    uint64_t *rec_keys = nullptr;
+    float4 *rec_boxes = nullptr;
+    uint64_t rec_cap = 0;
+    uint32_t *same_pid_count = nullptr;
+    uint4 *o_boxs = nullptr;
+    uint16_t *sids = nullptr;
+    Counters *dcnt = nullptr;
+    Counters *hcnt = nullptr;  /* pinned */
+    Counters *hinit = nullptr; /* pinned: initial counter values of a step */
+    uint32_t sid_count = 0, oversize_oid_count = 0;
+    uint32_t end_time_min = 0;
+    uint64_t kv_count = 0;
+    uint64_t last_live = 0;
+    uint32_t last_t = 0;
+    bool stepped = false;
+    uint32_t last_nodes = 0, last_leaves = 0;
+    uint64_t last_pairs = 0;
+    std::vector<PairHit> host_pairs_extra;
+    std::vector<RunHost> runs;
+    FlushWork flushw;
+    hipEvent_t ev[8];
+};
+
+static int fail(StcCtx *c, int code, const std::string &msg) {
+    c->err = msg;
+    return code;
+}
+#define CK(expr)                                                                   \
+    do {                                                                           \
+        hipError_t e_ = (expr);                                                    \
+        if (e_ != hipSuccess) {                                                    \
+            stc_hip_fail(e_, #expr, __FILE__, __LINE__);                           \
+            return fail(c, STC_E_HIP, g_last_fail);                                \
+        }                                                                          \
+    } while (0)
+#define CKRC(expr)                                                                 \
+    do {                                                                           \
+        int rc_ = (expr);                                                          \
+        if (rc_) return fail(c, rc_ < 0 ? rc_ : STC_E_HIP, g_last_fail.empty() ? #expr : g_last_fail); \
+    } while (0)
+
+static inline uint32_t blocks_for(uint64_t n, uint32_t t) { return (uint32_t)((n + t - 1) / t); }
+
+static int sync_counters(StcCtx *c) {
+    CK(hipMemcpyAsync(c->hcnt, c->dcnt, sizeof(Counters), hipMemcpyDeviceToHost, c->s));
+    CK(hipStreamSynchronize(c->s));
+    return 0;
+}
+
+static int alloc_table(StcCtx *c, uint64_t nb, Bucket **out) {
+    Bucket *t = nullptr;
+    CK(hipMalloc((void **)&t, nb * sizeof(Bucket)));
+    hipLaunchKernelGGL(k_table_clear, dim3(blocks_for(nb, 256)), dim3(256), 0, c->s, t, nb);
+    CK(hipGetLastError());
+    *out = t;
+    return 0;
+}
+
+static int ensure_records(StcCtx *c, uint64_t need) {
+    if (need <= c->rec_cap) return 0;
+    uint64_t nc = std::max<uint64_t>(need + need / 2, 1 << 16);
+    uint64_t *k = nullptr;
+    float4 *b = nullptr;
+    CK(hipMalloc((void **)&k, nc * 16));
+    CK(hipMalloc((void **)&b, nc * 16));
+    if (c->kv_count) {
+        CK(hipMemcpyAsync(k, c->rec_keys, c->kv_count * 16, hipMemcpyDeviceToDevice, c->s));
+        CK(hipMemcpyAsync(b, c->rec_boxes, c->kv_count * 16, hipMemcpyDeviceToDevice, c->s));
+    }
+    CK(hipStreamSynchronize(c->s));
+    if (c->rec_keys) hipFree(c->rec_keys);
+    if (c->rec_boxes) hipFree(c->rec_boxes);
+    c->rec_keys = k;
+    c->rec_boxes = b;
+    c->rec_cap = nc;
+    return 0;
+}
+
+static int ilog2_ceil(uint64_t v) {
+    int l = 0;
+    while ((1ull << l) < v) l++;
+    return l;
+}
+
+extern "C" {
+
+int stc_device_count(int32_t *count) {
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) { *count = 0; return STC_E_HIP; }
+    *count = n;
+    return STC_OK;
+}
+
+int stc_device_info_get(int32_t device, stc_device_info *info) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return STC_E_HIP;
+    memset(info, 0, sizeof(*info));
+    info->device_id = device;
+    snprintf(info->name, sizeof(info->name), "%s", prop.gcnArchName);
+    info->mem_size = prop.totalGlobalMem;
+    info->num_cus = prop.multiProcessorCount;
+    info->clock_khz = prop.clockRate;
+    return STC_OK;
+}
+
+int stc_create(const stc_config *cfg, const stc_box *mbr, void **out) {
+    if (!cfg || !mbr || !out) return STC_E_ARG;
+    *out = nullptr;
+    StcCtx *c = new StcCtx();
+    c->cfg = *cfg;
+    c->map = MapBox{mbr->low[0], mbr->low[1], mbr->high[0], mbr->high[1]};
+    c->n = cfg->num_objects;
+    c->cap = cfg->grid_capacity;
+    c->buf = (uint32_t)(cfg->grid_amplify * cfg->grid_capacity);
+    uint32_t C = cfg->ctf_count, split = (uint32_t)std::sqrt((double)C);
+    if (c->n == 0 || c->cap == 0 || c->buf == 0 || !(cfg->reach_distance > 0) || C == 0 || split * split != C ||
+        !(mbr->high[0] > mbr->low[0]) || !(mbr->high[1] > mbr->low[1]) || c->n >= (1u << 26)) {
+        delete c;
+        return STC_E_ARG;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        delete c;
+        return STC_E_HIP;
+    }
+    hipGetDevice(&c->device);
+    /* border buffers from the map's lowest latitude, host formula (filter.cpp:30-31) */
+    double absla = std::fabs(mbr->low[1]);
+    if (absla == 90) absla = 89;
+    c->x_buffer = cfg->reach_distance * (360.0 / (std::sin((90 - absla) * STC_PI_REF / 180) * 40076.0 * 1000.0));
+    c->y_buffer = cfg->reach_distance * STC_DPM_LAT;
+    c->band = 16.0 * (std::nextafter(cfg->reach_distance, INFINITY) - cfg->reach_distance);
+    c->end_time_min = cfg->start_time + cfg->min_meet_time; /* process.cpp:404 */
+
+    auto bad = [&](const char *what) {
+        c->err = what;
+        stc_destroy(c);
+        return STC_E_HIP;
+    };
+    if (hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking) != hipSuccess) return bad("stream");
+    for (int i = 0; i < 8; i++) hipEventCreate(&c->ev[i]);
+    uint32_t n = c->n;
+    size_t node_cap = 1 + 4ull * NLEVELS * (n / (c->cap + 1) + 1);
+    size_t front_cap = n / (c->cap + 1) + 2;
+    if (c->xy_in.ensure(n) || c->codes.ensure(n) || c->idx.ensure(n) || c->codes_alt.ensure(n) ||
+        c->idx_alt.ensure(n) || c->dlon.ensure(n) || c->lx.ensure(n) || c->ly.ensure(n) || c->ld.ensure(n) ||
+        c->nodes.ensure(node_cap) || c->frontier_a.ensure(front_cap) || c->frontier_b.ensure(front_cap) ||
+        c->pt_leaf.ensure(n) || c->pt_lv.ensure(n) || c->pt_rank.ensure(n) || c->lp.ensure(n) ||
+        c->lkeys.ensure(n) || c->ucount.ensure(n) || c->uoffset.ensure(n) || c->units.ensure(4ull * n) ||
+        c->pairs.ensure(std::max<size_t>(4ull * n, 1 << 16)) || c->amb.ensure(1 << 16) ||
+        c->leaf_count.ensure(node_cap) || c->leaf_start.ensure(node_cap) || c->leaf_node.ensure(node_cap) ||
+        c->temp.ensure(stc_radix_temp_words((uint32_t)std::max<size_t>(n, node_cap)) + 64))
+        return bad("device allocation failed");
+    if (hipMalloc((void **)&c->dcnt, sizeof(Counters)) != hipSuccess) return bad("counters");
+    if (hipHostMalloc((void **)&c->hcnt, sizeof(Counters), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&c->hinit, sizeof(Counters), hipHostMallocDefault) != hipSuccess)
+        return bad("pinned");
+    uint64_t nb = cfg->num_meeting_buckets ? cfg->num_meeting_buckets : 4ull * n;
+    c->log2b = std::max(ilog2_ceil(nb), 10);
+    c->table_buckets = 1ull << c->log2b;
+    if (alloc_table(c, c->table_buckets, &c->table)) return bad("table");
+    if (hipMalloc((void **)&c->same_pid_count, 4ull * n) != hipSuccess ||
+        hipMalloc((void **)&c->o_boxs, 16ull * n) != hipSuccess || hipMalloc((void **)&c->sids, 2ull * n) != hipSuccess)
+        return bad("per-object state");
+    hipMemsetAsync(c->same_pid_count, 0, 4ull * n, c->s);
+    hipMemsetAsync(c->sids, 0, 2ull * n, c->s);
+    hipLaunchKernelGGL(k_fill_obox, dim3(blocks_for(n, 256)), dim3(256), 0, c->s, c->o_boxs, n);
+    if (ensure_records(c, std::max<uint64_t>(1 << 16, (uint64_t)cfg->kv_restriction + 2))) return bad("records");
+    if (stc_flush_init(&c->flushw, n, C)) return bad("flush buffers");
+    if (hipStreamSynchronize(c->s) != hipSuccess) return bad("init sync");
+    *out = c;
+    return STC_OK;
+}
+
+int stc_destroy(void *vc) {
+    StcCtx *c = (StcCtx *)vc;
+    if (!c) return STC_OK;
+    if (c->s) hipStreamSynchronize(c->s);
+    c->xy_in.release(); c->codes.release(); c->idx.release(); c->codes_alt.release(); c->idx_alt.release();
+    c->temp.release(); c->dlon.release(); c->lx.release(); c->ly.release(); c->ld.release(); c->nodes.release();
+    c->frontier_a.release(); c->frontier_b.release(); c->leaf_count.release(); c->leaf_start.release();
+    c->leaf_node.release(); c->pt_leaf.release(); c->pt_lv.release(); c->pt_rank.release(); c->lp.release();
+    c->lkeys.release(); c->ucount.release(); c->uoffset.release(); c->units.release(); c->pairs.release();
+    c->amb.release();
+    if (c->table) hipFree(c->table);
+    if (c->rec_keys) hipFree(c->rec_keys);
+    if (c->rec_boxes) hipFree(c->rec_boxes);
+    if (c->same_pid_count) hipFree(c->same_pid_count);
+    if (c->o_boxs) hipFree(c->o_boxs);
+    if (c->sids) hipFree(c->sids);
+    if (c->dcnt) hipFree(c->dcnt);
+    if (c->hcnt) hipHostFree(c->hcnt);
+    if (c->hinit) hipHostFree(c->hinit);
+    stc_flush_free(&c->flushw);
+    for (int i = 0; i < 8; i++) if (c->ev[i]) hipEventDestroy(c->ev[i]);
+    if (c->s) hipStreamDestroy(c->s);
+    delete c;
+    return STC_OK;
+}
+
+const char *stc_last_error(void *vc) {
+    StcCtx *c = (StcCtx *)vc;
+    if (!c) return g_last_fail.c_str();
+    return c->err.c_str();
+}
+
+/* host re-evaluation of ambiguous pair decisions with the C library's sin,
+ * the reference CPU formula (geometry.h:40-48, util.h:59-66) */
+static double host_dpm_lon(double lat) {
+    return 360.0 / (std::sin((90 - std::fabs(lat)) * STC_PI_REF / 180) * 40076.0 * 1000.0);
+}
+
+static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out) {
+    const uint32_t n = c->n;
+    hipStream_t s = c->s;
+    Counters *cnt = c->dcnt;
+    memset(c->hinit, 0, sizeof(Counters));
+    c->hinit->kv_count = c->kv_count;
+    c->hinit->sid_count = c->sid_count;
+    c->hinit->oversize_oid_count = c->oversize_oid_count;
+    CK(hipMemcpyAsync(cnt, c->hinit, sizeof(Counters), hipMemcpyHostToDevice, s));
+    CK(hipEventRecord(c->ev[0], s));
+
+    /* 1-2: cells, Morton codes, dpm_lon; sort codes */
+    hipLaunchKernelGGL(k_prep, dim3(blocks_for(n, 256)), dim3(256), 0, s, xy, n, c->map, c->codes.p, c->idx.p,
+                       c->dlon.p, cnt);
+    CK(hipGetLastError());
+    CKRC(stc_radix_sort_u32(c->codes.p, c->idx.p, c->codes_alt.p, c->idx_alt.p, n, 28, c->temp.p, s));
+
+    /* 3: tree */
+    uint32_t node_cap = (uint32_t)c->nodes.cap;
+    hipLaunchKernelGGL(k_tree_root, dim3(1), dim3(64), 0, s, c->nodes.p, c->map, c->frontier_a.p, cnt, n, c->cap);
+    FrontierItem *fin = c->frontier_a.p, *fout = c->frontier_b.p;
+    for (int L = NLEVELS - 1; L >= 1; L--) {
+        uint64_t bound = std::min<uint64_t>(1ull << (2 * (NLEVELS - 1 - L)), c->frontier_a.cap);
+        hipLaunchKernelGGL(k_tree_level, dim3(blocks_for(bound, 128)), dim3(128), 0, s, c->nodes.p, c->map,
+                           c->codes.p, n, c->cap, L, fin, fout, cnt, node_cap, (uint32_t)c->frontier_a.cap);
+        std::swap(fin, fout);
+    }
+    {
+        uint64_t bound = c->frontier_a.cap;
+        hipLaunchKernelGGL(k_tree_finest, dim3(blocks_for(bound, 128)), dim3(128), 0, s, c->nodes.p, fin, cnt,
+                           node_cap);
+    }
+    CK(hipGetLastError());
+    CKRC(sync_counters(c));
+    Counters h = *c->hcnt;
+    if (h.bad_points) return fail(c, STC_E_INPUT, "point outside the map box");
+    if (h.table_fail) return fail(c, STC_E_CAPACITY, "QTree node capacity exceeded");
+    if (n <= c->cap) return fail(c, STC_E_ARG, "root does not split: too few objects for grid_capacity");
+    uint32_t n_nodes = h.nodes, n_leaves = h.leaves;
+    c->last_nodes = n_nodes;
+    c->last_leaves = n_leaves;
+
+    /* 4: partition */
+    CK(hipMemsetAsync(c->leaf_count.p, 0, 4ull * n_leaves, s));
+    hipLaunchKernelGGL(k_leaf_nodes, dim3(blocks_for(n_nodes, 256)), dim3(256), 0, s, c->nodes.p, n_nodes,
+                       c->leaf_node.p);
+    hipLaunchKernelGGL(k_partition, dim3(blocks_for(n, 256)), dim3(256), 0, s, xy, n, c->nodes.p, c->x_buffer,
+                       c->y_buffer, c->pt_leaf.p, c->pt_lv.p, c->leaf_count.p, cnt);
+    CK(hipGetLastError());
+    /* 5: leaves in id order */
+    CKRC(stc_exclusive_scan_u32(c->leaf_count.p, c->leaf_start.p, n_leaves, nullptr, c->temp.p, s));
+    CK(hipMemcpyAsync(c->lkeys.p, c->pt_leaf.p, 4ull * n, hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL(k_iota, dim3(blocks_for(n, 256)), dim3(256), 0, s, c->lp.p, n);
+    CKRC(stc_radix_sort_u32(c->lkeys.p, c->lp.p, c->codes_alt.p, c->idx_alt.p, n,
+                            std::max(1, ilog2_ceil(n_leaves)), c->temp.p, s));
+    hipLaunchKernelGGL(k_gather, dim3(blocks_for(n, 256)), dim3(256), 0, s, xy, c->dlon.p, c->lp.p, c->pt_leaf.p,
+                       c->leaf_start.p, n, c->lx.p, c->ly.p, c->ld.p, c->pt_rank.p);
+    hipLaunchKernelGGL(k_overflow, dim3(blocks_for(n_leaves, 256)), dim3(256), 0, s, c->leaf_count.p, n_leaves,
+                       c->buf, cnt);
+    CK(hipGetLastError());
+    CK(hipEventRecord(c->ev[1], s));
+
+    /* 6: border lookup (count, scan, emit) */
+    double r = c->cfg.reach_distance;
+    hipLaunchKernelGGL(k_lookup<false>, dim3(blocks_for(n, 128)), dim3(128), 0, s, xy, c->dlon.p, n, c->nodes.p,
+                       c->pt_lv.p, r, c->band, c->ucount.p, (const uint32_t *)nullptr, (uint2 *)nullptr, cnt);
+    CK(hipGetLastError());
+    uint32_t *units_total = (uint32_t *)((char *)cnt + offsetof(Counters, units));
+    CKRC(stc_exclusive_scan_u32(c->ucount.p, c->uoffset.p, n, units_total, c->temp.p, s));
+    CKRC(sync_counters(c));
+    h = *c->hcnt;
+    uint64_t n_units = (uint32_t)h.units;
+    if (c->units.ensure(n_units + 1)) return fail(c, STC_E_CAPACITY, "unit buffer allocation failed");
+    hipLaunchKernelGGL(k_lookup<true>, dim3(blocks_for(n, 128)), dim3(128), 0, s, xy, c->dlon.p, n, c->nodes.p,
+                       c->pt_lv.p, r, c->band, c->ucount.p, (const uint32_t *)c->uoffset.p, c->units.p, cnt);
+    CK(hipGetLastError());
+    CK(hipEventRecord(c->ev[2], s));
+
+    /* 7: refinement; rerun with a larger pair buffer if it overflowed */
+    uint64_t total_units = (uint64_t)n + n_units;
+    uint32_t grid = (uint32_t)std::min<uint64_t>(blocks_for(total_units * STC_WAVE, 256), 256u * 64u);
+    for (int attempt = 0; attempt < 4; attempt++) {
+        CK(hipMemsetAsync(&cnt->candidates, 0, 3 * sizeof(unsigned long long), s)); /* candidates, pairs, ambiguous */
+        hipLaunchKernelGGL(k_refine, dim3(grid), dim3(256), 0, s, xy, c->dlon.p, n, c->pt_leaf.p, c->pt_rank.p,
+                           c->leaf_start.p, c->leaf_count.p, c->lp.p, c->lx.p, c->ly.p, c->units.p, total_units,
+                           c->buf, r, c->band, c->pairs.p, (uint64_t)c->pairs.cap, c->amb.p, (uint32_t)c->amb.cap,
+                           cnt);
+        CK(hipGetLastError());
+        CKRC(sync_counters(c));
+        h = *c->hcnt;
+        if (h.pairs <= c->pairs.cap && h.ambiguous <= c->amb.cap) break;
+        if (c->pairs.ensure(h.pairs + 1024) || c->amb.ensure(h.ambiguous + 1024))
+            return fail(c, STC_E_CAPACITY, "pair buffer allocation failed");
+    }
+    if (h.pairs > c->pairs.cap || h.ambiguous > c->amb.cap) return fail(c, STC_E_CAPACITY, "pair buffer overflow");
+    uint64_t n_pairs = h.pairs;
+    uint64_t n_amb = h.ambiguous;
+    if (n_amb) {
+        /* decide the band with the C library's sin and append accepted pairs */
+        std::vector<uint2> ab(n_amb);
+        CK(hipMemcpy(ab.data(), c->amb.p, n_amb * sizeof(uint2), hipMemcpyDeviceToHost));
+        std::vector<PairHit> extra;
+        for (const uint2 &e : ab) {
+            double2 pp, qq;
+            CK(hipMemcpy(&pp, xy + e.x, sizeof(double2), hipMemcpyDeviceToHost));
+            CK(hipMemcpy(&qq, xy + e.y, sizeof(double2), hipMemcpyDeviceToHost));
+            double dx = (pp.x - qq.x) / host_dpm_lon(pp.y);
+            double dy = (pp.y - qq.y) / STC_DPM_LAT;
+            if (std::sqrt(dx * dx + dy * dy) <= r) {
+                PairHit ph;
+                uint32_t a = std::min(e.x, e.y), b = std::max(e.x, e.y);
+                ph.key = ((unsigned long long)a << 32) | b;
+                float pfx = (float)pp.x, pfy = (float)pp.y, qfx = (float)qq.x, qfy = (float)qq.y;
+                ph.mbr[0] = std::min(pfx, qfx); ph.mbr[1] = std::min(pfy, qfy);
+                ph.mbr[2] = std::max(pfx, qfx); ph.mbr[3] = std::max(pfy, qfy);
+                extra.push_back(ph);
+            }
+        }
+        if (!extra.empty()) {
+            if (c->pairs.cap < n_pairs + extra.size()) {
+                /* grow preserving contents */
+                PairHit *np_ = nullptr;
+                CK(hipMalloc((void **)&np_, (n_pairs + extra.size() + 1024) * sizeof(PairHit)));
+                CK(hipMemcpy(np_, c->pairs.p, n_pairs * sizeof(PairHit), hipMemcpyDeviceToDevice));
+                hipFree(c->pairs.p);
+                c->pairs.p = np_;
+                c->pairs.cap = n_pairs + extra.size() + 1024;
+            }
+            CK(hipMemcpy(c->pairs.p + n_pairs, extra.data(), extra.size() * sizeof(PairHit), hipMemcpyHostToDevice));
+            n_pairs += extra.size();
+        }
+    }
+    c->last_pairs = n_pairs;
+    CK(hipEventRecord(c->ev[3], s));
+
+    /* 8-9: meeting table insert and identify */
+    uint64_t need_rec = c->kv_count + 2 * c->last_live + 2;
+    if (ensure_records(c, need_rec)) return fail(c, STC_E_CAPACITY, "record buffer allocation failed");
+    if (n_pairs)
+        hipLaunchKernelGGL(k_table_insert, dim3(blocks_for(n_pairs, 256)), dim3(256), 0, s, c->table, c->log2b,
+                           c->pairs.p, n_pairs, t, cnt);
+    MemtableDev mt{c->rec_keys, c->rec_boxes, c->rec_cap, c->same_pid_count, c->o_boxs, c->sids};
+    hipLaunchKernelGGL(k_identify, dim3(blocks_for(c->table_buckets, 256)), dim3(256), 0, s, c->table,
+                       c->table_buckets, t, c->cfg.min_meet_time, c->cfg.max_meet_time, c->end_time_min, c->map, mt,
+                       cnt);
+    CK(hipGetLastError());
+    CKRC(sync_counters(c));
+    h = *c->hcnt;
+    if (h.table_fail) return fail(c, STC_E_CAPACITY, "meeting table full");
+    c->kv_count = h.kv_count;
+    c->sid_count = h.sid_count;
+    c->oversize_oid_count = h.oversize_oid_count;
+    c->last_live = h.live;
+    uint64_t occupied = h.live + h.expired + h.retired;
+    uint32_t rebuilt = 0;
+    /* rehash when keys (live or retired) fill half the table; grow to keep live <= 1/4 */
+    if (occupied * 2 > c->table_buckets) {
+        uint64_t nb = c->table_buckets;
+        while (h.live * 4 > nb) nb *= 2;
+        Bucket *nt = nullptr;
+        CKRC(alloc_table(c, nb, &nt));
+        int nl2 = ilog2_ceil(nb);
+        hipLaunchKernelGGL(k_table_rehash, dim3(blocks_for(c->table_buckets, 256)), dim3(256), 0, s, c->table,
+                           c->table_buckets, nt, nl2, t, cnt);
+        CK(hipGetLastError());
+        CK(hipStreamSynchronize(s));
+        hipFree(c->table);
+        c->table = nt;
+        c->table_buckets = nb;
+        c->log2b = nl2;
+        rebuilt = 1;
+        occupied = h.live;
+    }
+    CK(hipEventRecord(c->ev[4], s));
+
+    /* 10: flush */
+    uint32_t flushed = 0;
+    if (c->kv_count > c->cfg.kv_restriction) {
+        RunHost run;
+        FlushArgs fa;
+        fa.n = n;
+        fa.ctf_count = c->cfg.ctf_count;
+        fa.kv_count = c->kv_count;
+        fa.kv_restriction = c->cfg.kv_restriction;
+        fa.end_time_min = c->end_time_min;
+        fa.cur_time = t;
+        fa.max_meet_time = c->cfg.max_meet_time;
+        fa.area_threshold = c->cfg.area_threshold;
+        fa.compat = c->cfg.compat;
+        fa.map[0] = c->map.lo0; fa.map[1] = c->map.lo1; fa.map[2] = c->map.hi0; fa.map[3] = c->map.hi1;
+        fa.sid_count = c->sid_count;
+        fa.oversize_oid_count = c->oversize_oid_count;
+        fa.rec_keys = c->rec_keys;
+        fa.rec_boxes = c->rec_boxes;
+        fa.same_pid_count = c->same_pid_count;
+        fa.o_boxs = c->o_boxs;
+        fa.sids = c->sids;
+        FlushOut fo;
+        int rc = stc_flush_run(&c->flushw, &fa, &fo, s);
+        if (rc) return fail(c, rc, std::string("flush: ") + g_last_fail);
+        run.info = fo.info;
+        run.ctfs.swap(fo.ctfs);
+        run.key_bytes.swap(fo.key_bytes);
+        run.bitmaps.swap(fo.bitmaps);
+        run.o_bitmap.swap(fo.o_bitmap);
+        run.sids.swap(fo.sids);
+        run.o_keys.swap(fo.o_keys);
+        run.sorted_keys.swap(fo.sorted_keys);
+        run.o_boxes.swap(fo.o_boxes);
+        run.sorted_boxes.swap(fo.sorted_boxes);
+        c->runs.push_back(std::move(run));
+        /* reset the memtable (geometry.cu:2111-2131) */
+        CK(hipMemsetAsync(c->same_pid_count, 0, 4ull * n, s));
+        CK(hipMemsetAsync(c->sids, 0, 2ull * n, s));
+        hipLaunchKernelGGL(k_fill_obox, dim3(blocks_for(n, 256)), dim3(256), 0, s, c->o_boxs, n);
+        CK(hipGetLastError());
+        c->kv_count = 0;
+        c->sid_count = 0;
+        c->oversize_oid_count = 0;
+        c->end_time_min = t; /* process.cpp:271 with end_time_max = cur_time */
+        flushed = 1;
+    }
+    CK(hipEventRecord(c->ev[5], s));
+    CK(hipEventSynchronize(c->ev[5]));
+
+    c->stepped = true;
+    c->last_t = t;
+    if (out) {
+        memset(out, 0, sizeof(*out));
+        out->nodes = n_nodes;
+        out->leaves = n_leaves;
+        out->border_points = h.border;
+        out->cross_units = n_units;
+        out->candidates = h.candidates;
+        out->pairs = n_pairs;
+        out->active_meetings = h.live;
+        out->expired = h.expired;
+        out->kv_count = h.kv_count;
+        out->overflow_points = h.overflow;
+        out->ambiguous_pairs = n_amb;
+        out->ambiguous_nodes = h.ambiguous_nodes;
+        out->table_buckets = c->table_buckets;
+        out->table_occupied = occupied;
+        out->flushed = flushed;
+        out->rebuilt = rebuilt;
+        hipEventElapsedTime(&out->ms_partition, c->ev[0], c->ev[1]);
+        hipEventElapsedTime(&out->ms_filter, c->ev[1], c->ev[2]);
+        hipEventElapsedTime(&out->ms_refine, c->ev[2], c->ev[3]);
+        hipEventElapsedTime(&out->ms_meetings, c->ev[3], c->ev[4]);
+        hipEventElapsedTime(&out->ms_flush, c->ev[4], c->ev[5]);
+        hipEventElapsedTime(&out->ms_total, c->ev[0], c->ev[5]);
+    }
+    return STC_OK;
+}
+
+int stc_step_device(void *vc, const double *xy_dev, uint32_t t, stc_step_out *out) {
+    StcCtx *c = (StcCtx *)vc;
+    if (!c || !xy_dev) return STC_E_ARG;
+    uint64_t kv_before = c->kv_count;
+    int rc = step_impl(c, (const double2 *)xy_dev, t, out);
+    if (rc == STC_OK && out) out->new_records = out->kv_count - kv_before;
+    return rc;
+}
+
+int stc_step(void *vc, const double *xy_host, uint32_t t, stc_step_out *out) {
+    StcCtx *c = (StcCtx *)vc;
+    if (!c || !xy_host) return STC_E_ARG;
+    CK(hipMemcpyAsync(c->xy_in.p, xy_host, 16ull * c->n, hipMemcpyHostToDevice, c->s));
+    return stc_step_device(c, (const double *)c->xy_in.p, t, out);
+}
+
+int stc_get_pairs(void *vc, uint64_t *keys, uint64_t cap, uint64_t *n) {
+    StcCtx *c = (StcCtx *)vc;
+    if (!c || !n) return STC_E_ARG;
+    if (!c->stepped) return fail(c, STC_E_STATE, "no step processed");
+    std::vector<PairHit> ph(c->last_pairs);
+    if (c->last_pairs) CK(hipMemcpy(ph.data(), c->pairs.p, c->last_pairs * sizeof(PairHit), hipMemcpyDeviceToHost));
+    std::vector<uint64_t> k(ph.size());
+    for (size_t i = 0; i < ph.size(); i++) k[i] = ph[i].key;
+    std::sort(k.begin(), k.end());
+    k.erase(std::unique(k.begin(), k.end()), k.end());
+    *n = k.size();
+    if (keys) memcpy(keys, k.data(), std::min<uint64_t>(cap, k.size()) * 8);
+    return STC_OK;
+}
+
+int stc_get_active(void *vc, uint64_t *keys, uint32_t *start, uint32_t *end, float *mbr, uint64_t cap, uint64_t *n) {
+    StcCtx *c = (StcCtx *)vc;
+    if (!c || !n) return STC_E_ARG;
+    if (!c->stepped) return fail(c, STC_E_STATE, "no step processed");
+    std::vector<Bucket> tb(c->table_buckets);
+    CK(hipMemcpy(tb.data(), c->table, c->table_buckets * sizeof(Bucket), hipMemcpyDeviceToHost));
+    std::vector<Bucket> live;
+    for (const Bucket &b : tb)
+        if (b.key != EMPTY_KEY && b.end == c->last_t) live.push_back(b);
+    std::sort(live.begin(), live.end(), [](const Bucket &a, const Bucket &b) { return a.key < b.key; });
+    *n = live.size();
+    for (uint64_t i = 0; i < std::min<uint64_t>(cap, live.size()); i++) {
+        if (keys) keys[i] = live[i].key;
+        if (start) start[i] = live[i].start;
+        if (end) end[i] = live[i].end;
+        if (mbr) for (int q = 0; q < 4; q++) mbr[4 * i + q] = o2f(live[i].mbr[q]);
+    }
+    return STC_OK;
+}
+
+int stc_get_records(void *vc, uint64_t *keys, float *boxes, uint64_t cap, uint64_t *n) {
+    StcCtx *c = (StcCtx *)vc;
+    if (!c || !n) return STC_E_ARG;
+    *n = c->kv_count;
+    uint64_t m = std::min<uint64_t>(cap, c->kv_count);
+    if (m && keys) CK(hipMemcpy(keys, c->rec_keys, m * 16, hipMemcpyDeviceToHost));
+    if (m && boxes) CK(hipMemcpy(boxes, c->rec_boxes, m * 16, hipMemcpyDeviceToHost));
+    return STC_OK;
+}
+
+int stc_get_leaves(void *vc, double *boxes, uint32_t *counts, uint64_t cap, uint64_t *n) {
+    StcCtx *c = (StcCtx *)vc;
+    if (!c || !n) return STC_E_ARG;
+    if (!c->stepped) return fail(c, STC_E_STATE, "no step processed");
+    std::vector<Node> nd(c->last_nodes);
+    std::vector<uint32_t> lc(c->last_leaves);
+    CK(hipMemcpy(nd.data(), c->nodes.p, nd.size() * sizeof(Node), hipMemcpyDeviceToHost));
+    if (!lc.empty()) CK(hipMemcpy(lc.data(), c->leaf_count.p, lc.size() * 4, hipMemcpyDeviceToHost));
+    struct L { double lo1, lo0, hi0, hi1; uint32_t cnt; };
+    std::vector<L> ls;
+    for (const Node &x : nd)
+        if (x.type == T_LEAF) ls.push_back(L{x.lo1, x.lo0, x.hi0, x.hi1, lc[x.leaf]});
+    std::sort(ls.begin(), ls.end(), [](const L &a, const L &b) {
+        if (a.lo1 != b.lo1) return a.lo1 < b.lo1;
+        return a.lo0 < b.lo0;
+    });
+    *n = ls.size();
+    for (uint64_t i = 0; i < std::min<uint64_t>(cap, ls.size()); i++) {
+        if (boxes) {
+            boxes[4 * i] = ls[i].lo0; boxes[4 * i + 1] = ls[i].lo1;
+            boxes[4 * i + 2] = ls[i].hi0; boxes[4 * i + 3] = ls[i].hi1;
+        }
+        if (counts) counts[i] = ls[i].cnt;
+    }
+    return STC_OK;
+}
+
+int stc_num_runs(void *vc, uint32_t *n) {
+    StcCtx *c = (StcCtx *)vc;
+    if (!c || !n) return STC_E_ARG;
+    *n = (uint32_t)c->runs.size();
+    return STC_OK;
+}
+
+int stc_get_run_info(void *vc, uint32_t run, stc_run_info *info) {
+    StcCtx *c = (StcCtx *)vc;
+    if (!c || !info || run >= c->runs.size()) return STC_E_ARG;
+    *info = c->runs[run].info;
+    return STC_OK;
+}
+
+int stc_get_run(void *vc, uint32_t run, stc_ctf_meta *ctfs, uint8_t *key_bytes, uint8_t *bitmaps, uint16_t *sids,
+                uint64_t *o_keys, float *o_boxes, uint8_t *o_bitmap, uint64_t *sorted_keys, float *sorted_boxes) {
+    StcCtx *c = (StcCtx *)vc;
+    if (!c || run >= c->runs.size()) return STC_E_ARG;
+    const RunHost &R = c->runs[run];
+    if (ctfs) memcpy(ctfs, R.ctfs.data(), R.ctfs.size() * sizeof(stc_ctf_meta));
+    if (key_bytes) memcpy(key_bytes, R.key_bytes.data(), R.key_bytes.size());
+    if (bitmaps) memcpy(bitmaps, R.bitmaps.data(), R.bitmaps.size());
+    if (sids) memcpy(sids, R.sids.data(), R.sids.size() * 2);
+    if (o_keys) memcpy(o_keys, R.o_keys.data(), R.o_keys.size() * 8);
+    if (o_boxes) memcpy(o_boxes, R.o_boxes.data(), R.o_boxes.size() * 4);
+    if (o_bitmap) memcpy(o_bitmap, R.o_bitmap.data(), R.o_bitmap.size());
+    if (sorted_keys) memcpy(sorted_keys, R.sorted_keys.data(), R.sorted_keys.size() * 8);
+    if (sorted_boxes) memcpy(sorted_boxes, R.sorted_boxes.data(), R.sorted_boxes.size() * 4);
+    return STC_OK;
+}
+
+int stc_drop_runs(void *vc) {
+    StcCtx *c = (StcCtx *)vc;
+    if (!c) return STC_E_ARG;
+    c->runs.clear();
+    return STC_OK;
+}
+
+} /* extern "C" */

@@ -1,0 +1,207 @@
+"""Host-side engine over the C ABI.
+
+Pipeline mirrors the reference's tracer/workbench pair on the GPU path
+(src/tracing/trace.h:24-54, src/tracing/process.cpp:378-634): it owns the
+device context created by stc_create (cuda_create_device_bench) and runs one
+stc_step (process_with_gpu) per timestep. Traces come from the seeded
+generator, on the host (HostTrace) or resident in HBM (DeviceTrace).
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+
+
+def _check(rc, ctx=None):
+    if rc != L.STC_OK:
+        lib = L.load()
+        msg = lib.stc_last_error(ctx) if ctx else b""
+        raise L.StcError(rc, (msg or b"").decode(errors="replace"))
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+def _gen_struct(gp):
+    g = L.StcGenParams()
+    for k, v in gp.items():
+        setattr(g, k, v)
+    return g
+
+
+class HostTrace:
+    """Seeded walk computed on the host; yields (N, 2) float64 arrays."""
+
+    def __init__(self, gen):
+        self.lib = L.load()
+        self.gp = _gen_struct(gen)
+        self.n = gen["num_objects"]
+        self.w = (L.StcWalker * self.n)()
+        _check(self.lib.stc_gen_init_host(C.byref(self.gp), self.w, 0, self.n))
+        self.t = 0
+
+    def positions(self):
+        xy = np.empty((self.n, 2), dtype=np.float64)
+        _check(self.lib.stc_gen_pos_host(C.byref(self.gp), self.w, self.n, _ptr(xy)))
+        return xy
+
+    def advance(self):
+        self.t += 1
+        _check(self.lib.stc_gen_step_host(C.byref(self.gp), self.w, self.n, self.t))
+
+    def steps(self, count):
+        for t in range(count):
+            if t > 0:
+                self.advance()
+            yield t, self.positions()
+
+
+class DeviceTrace:
+    """Seeded walk resident in HBM; next(t) returns a device pointer."""
+
+    def __init__(self, gen):
+        self.lib = L.load()
+        self.gp = _gen_struct(gen)
+        self.h = C.c_void_p()
+        _check(self.lib.stc_trace_create(C.byref(self.gp), C.byref(self.h)))
+
+    def next(self, t):
+        p = C.c_void_p()
+        _check(self.lib.stc_trace_next(self.h, t, C.byref(p)))
+        return p
+
+    def close(self):
+        if self.h:
+            self.lib.stc_trace_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Pipeline:
+    """One device context: the per-timestep co-occurrence pipeline."""
+
+    def __init__(self, cfg, box):
+        self.lib = L.load()
+        self.cfg = dict(cfg)
+        c = L.StcConfig()
+        for k, v in cfg.items():
+            setattr(c, k, v)
+        b = L.StcBox()
+        b.low[0], b.low[1] = box[0]
+        b.high[0], b.high[1] = box[1]
+        self.ctx = C.c_void_p()
+        rc = self.lib.stc_create(C.byref(c), C.byref(b), C.byref(self.ctx))
+        if rc != L.STC_OK:
+            raise L.StcError(rc, "stc_create failed: " + (self.lib.stc_last_error(None) or b"").decode())
+        self.n = cfg["num_objects"]
+
+    def step(self, xy, t):
+        """one timestep from host points (N, 2) float64"""
+        xy = np.ascontiguousarray(xy, dtype=np.float64)
+        assert xy.shape == (self.n, 2)
+        out = L.StcStepOut()
+        _check(self.lib.stc_step(self.ctx, _ptr(xy), t, C.byref(out)), self.ctx)
+        return out.as_dict()
+
+    def step_device(self, xy_dev, t):
+        out = L.StcStepOut()
+        _check(self.lib.stc_step_device(self.ctx, xy_dev, t, C.byref(out)), self.ctx)
+        return out.as_dict()
+
+    def _count(self, fn, *args):
+        n = C.c_uint64()
+        _check(fn(self.ctx, *args, 0, C.byref(n)), self.ctx)
+        return n.value
+
+    def pairs(self):
+        n = self._count(self.lib.stc_get_pairs, None)
+        keys = np.empty(n, dtype=np.uint64)
+        m = C.c_uint64()
+        _check(self.lib.stc_get_pairs(self.ctx, _ptr(keys), n, C.byref(m)), self.ctx)
+        return keys
+
+    def active(self):
+        n = self._count(self.lib.stc_get_active, None, None, None, None)
+        keys = np.empty(n, dtype=np.uint64)
+        start = np.empty(n, dtype=np.uint32)
+        end = np.empty(n, dtype=np.uint32)
+        mbr = np.empty((n, 4), dtype=np.float32)
+        m = C.c_uint64()
+        _check(self.lib.stc_get_active(self.ctx, _ptr(keys), _ptr(start), _ptr(end), _ptr(mbr), n, C.byref(m)),
+               self.ctx)
+        return keys, start, end, mbr
+
+    def records(self):
+        n = self._count(self.lib.stc_get_records, None, None)
+        keys = np.empty((n, 2), dtype=np.uint64)
+        boxes = np.empty((n, 4), dtype=np.float32)
+        m = C.c_uint64()
+        _check(self.lib.stc_get_records(self.ctx, _ptr(keys), _ptr(boxes), n, C.byref(m)), self.ctx)
+        return keys, boxes
+
+    def leaves(self):
+        n = self._count(self.lib.stc_get_leaves, None, None)
+        boxes = np.empty((n, 4), dtype=np.float64)
+        counts = np.empty(n, dtype=np.uint32)
+        m = C.c_uint64()
+        _check(self.lib.stc_get_leaves(self.ctx, _ptr(boxes), _ptr(counts), n, C.byref(m)), self.ctx)
+        return boxes, counts
+
+    def num_runs(self):
+        n = C.c_uint32()
+        _check(self.lib.stc_num_runs(self.ctx, C.byref(n)), self.ctx)
+        return n.value
+
+    def run(self, i):
+        info = L.StcRunInfo()
+        _check(self.lib.stc_get_run_info(self.ctx, i, C.byref(info)), self.ctx)
+        nc, no, nk, nf = info.ctf_count, info.oversize_count, info.kv_count, info.num_objects
+        ctfs = (L.StcCtfMeta * nc)()
+        key_bytes = np.empty(info.key_bytes, dtype=np.uint8)
+        bitmaps = np.empty(nc * 8192, dtype=np.uint8)
+        sids = np.empty(nf, dtype=np.uint16)
+        o_keys = np.empty((no, 2), dtype=np.uint64)
+        o_boxes = np.empty((no, 4), dtype=np.float32)
+        o_bitmap = np.empty(8192, dtype=np.uint8)
+        sk = np.empty((nk, 2), dtype=np.uint64)
+        sb = np.empty((nk, 4), dtype=np.float32)
+        _check(self.lib.stc_get_run(self.ctx, i, ctfs, _ptr(key_bytes), _ptr(bitmaps), _ptr(sids), _ptr(o_keys),
+                                    _ptr(o_boxes), _ptr(o_bitmap), _ptr(sk), _ptr(sb)), self.ctx)
+        return dict(info={k: getattr(info, k) for k, _ in info._fields_}, ctfs=ctfs_to_numpy(ctfs),
+                    key_bytes=key_bytes, bitmaps=bitmaps, sids=sids, o_keys=o_keys, o_boxes=o_boxes,
+                    o_bitmap=o_bitmap, sorted_keys=sk, sorted_boxes=sb)
+
+    def drop_runs(self):
+        _check(self.lib.stc_drop_runs(self.ctx), self.ctx)
+
+    def close(self):
+        if self.ctx:
+            self.lib.stc_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+CTF_DTYPE = np.dtype([
+    ("mbr", np.float32, 4), ("kv_capacity", np.uint32), ("start_time_min", np.uint32),
+    ("start_time_max", np.uint32), ("end_time_min", np.uint32), ("end_time_max", np.uint32),
+    ("bitmap_size", np.uint32), ("key_bit", np.uint16), ("id_bit", np.uint16), ("duration_bit", np.uint16),
+    ("end_bit", np.uint16), ("low_x_bit", np.uint16), ("low_y_bit", np.uint16), ("edge_bit", np.uint16),
+    ("mbr_bit", np.uint16), ("x_grid", np.uint16), ("y_grid", np.uint16), ("flags", np.uint32),
+])
+
+
+def ctfs_to_numpy(arr):
+    raw = bytes(memoryview(arr))
+    return np.frombuffer(raw, dtype=CTF_DTYPE).copy()
