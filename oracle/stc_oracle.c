@@ -875,8 +875,11 @@ static int flush(orc *o, uint32_t t) {
     }
 
     /* cuda_write_bitmap (geometry.cu:986-1021) and write_key_and_mbr (1073-1107) */
-    uint64_t total_bytes = 0;
+    uint64_t total_bytes = 0, total_cap = 0;
+    int raster_capped = 0;
     for (uint32_t c = 0; c < C; c++) total_bytes += (uint64_t)capacity[c] * R->ctfs[c].key_bit / 8;
+    for (uint32_t c = 0; c < C; c++) total_cap += capacity[c];
+    if (total_cap != nf) bad_capacity = 1;
     R->key_bytes = (uint8_t *)calloc(total_bytes ? total_bytes : 1, 1);
     if (!R->key_bytes) goto oom;
     key_index = 0;
@@ -898,9 +901,12 @@ static int flush(orc *o, uint32_t t) {
             uint32_t high1 = sat_u32((b[3] - m[1]) / (m[3] - m[1]) * (float)ctf->y_grid);
             if (ctf->x_grid) {
                 uint64_t room = total_bits - base_bit; /* bits left in the array from this slot */
-                for (uint64_t j = low1; j <= high1 && j * ctf->x_grid < room; j++)
-                    for (uint64_t i = low0; i <= high0 && i + j * ctf->x_grid < room; i++)
+                uint32_t done = 0;   /* at most 2^20 cells per record (only reachable in compat mode) */
+                for (uint64_t j = low1; j <= high1 && j * ctf->x_grid < room && done < (1u << 20); j++)
+                    for (uint64_t i = low0; i <= high0 && i + j * ctf->x_grid < room && done < (1u << 20);
+                         i++, done++)
                         set_bit(R->bitmaps, total_bits, base_bit + (uint32_t)(i + j * ctf->x_grid));
+                if (done >= (1u << 20)) raster_capped = 1;
             }
             /* serialize_mbr, f_box form (geometry.cu:161-171) */
             uint64_t v0 = sat_u64((b[0] - m[0]) / (m[2] - m[0]) * (float)((1ull << ctf->low_x_bit) - 1));
@@ -926,6 +932,7 @@ static int flush(orc *o, uint32_t t) {
         bytes_index += (uint64_t)capacity[c] * ctf->key_bit / 8;
     }
     if (bad_capacity) for (uint32_t c = 0; c < C; c++) R->ctfs[c].flags |= 2;
+    if (raster_capped) for (uint32_t c = 0; c < C; c++) R->ctfs[c].flags |= 8;
     if (!(nt < o->cfg.kv_restriction / 100)) for (uint32_t c = 0; c < C; c++) R->ctfs[c].flags |= 4;
 
     /* outputs */
@@ -1078,10 +1085,15 @@ void orc_get_records(void *vo, uint64_t *keys, float *boxes) {
 }
 
 uint32_t orc_num_runs(void *o) { return ((orc *)o)->n_runs; }
-void orc_get_run_info(void *vo, uint32_t r, orc_run_info *info) { *info = ((orc *)vo)->runs[r].info; }
+void orc_get_run_info(void *vo, uint32_t r, orc_run_info *info) {
+    orc *o = (orc *)vo;
+    if (r >= o->n_runs) { memset(info, 0, sizeof(*info)); return; }
+    *info = o->runs[r].info;
+}
 void orc_get_run(void *vo, uint32_t r, orc_ctf *ctfs, uint8_t *key_bytes, uint8_t *bitmaps,
                  uint16_t *sids, uint64_t *o_keys, float *o_boxes, uint8_t *o_bitmap,
                  uint64_t *sorted_keys, float *sorted_boxes) {
+    if (r >= ((orc *)vo)->n_runs) return;
     const run *R = &((orc *)vo)->runs[r];
     uint32_t C = R->info.ctf_count;
     if (ctfs) memcpy(ctfs, R->ctfs, sizeof(orc_ctf) * C);

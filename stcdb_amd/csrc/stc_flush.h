@@ -19,14 +19,17 @@ struct FlushWork {
     stc_ctf_meta *ctfs = nullptr;
     uint8_t *bitmaps = nullptr;                    /* ctf_count * 8192 */
     uint32_t *obj_base = nullptr, *obj_fill = nullptr; /* per-object record slots */
+    uint8_t *part_tail = nullptr;                  /* per object: records go to the oversize buffer */
+    uint32_t *obj_key = nullptr, *obj_id = nullptr, *obj_cnt = nullptr; /* object placement order */
     uint32_t *scan_temp = nullptr;
     size_t scan_temp_words = 0;
     uint32_t *flags = nullptr;                     /* device counters of the flush */
     /* record-sized scratch */
     uint64_t rec_cap = 0;
-    uint64_t *keys_out = nullptr;                  /* sorted keys (lo, hi) */
+    uint64_t *keys_out = nullptr;                  /* sorted keys (lo, hi): CTF records, then the oversize tail */
     float4 *boxes_out = nullptr;
-    uint32_t *rec_perm = nullptr;
+    uint64_t *tkeys = nullptr;                     /* records grouped by object, before ranking */
+    float4 *tboxes = nullptr;
     uint8_t *key_bytes = nullptr;
     uint64_t key_bytes_cap = 0;
 };
@@ -38,6 +41,7 @@ struct FlushArgs {
     uint32_t end_time_min, cur_time, max_meet_time;
     uint32_t sid_count, oversize_oid_count;
     uint32_t compat;
+    uint32_t keep_sorted;    /* copy the sorted CTF records back to the host */
     double area_threshold;
     double map[4];
     uint64_t *rec_keys;      /* (lo, hi) per record */

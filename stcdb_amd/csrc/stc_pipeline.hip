@@ -606,13 +606,13 @@ __global__ void k_identify(Bucket *t, uint64_t nb, uint32_t now, uint32_t min_me
     }
 }
 
-/* move live entries into a fresh table */
-__global__ void k_table_rehash(const Bucket *src, uint64_t nsrc, Bucket *dst, int log2b, uint32_t now,
+/* move the entries last seen at `keep_end` into a fresh table */
+__global__ void k_table_rehash(const Bucket *src, uint64_t nsrc, Bucket *dst, int log2b, uint32_t keep_end,
                                Counters *cnt) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nsrc) return;
     Bucket b = src[i];
-    if (b.key == EMPTY_KEY || b.end != now) return;
+    if (b.key == EMPTY_KEY || b.end != keep_end) return;
     bool claimed;
     Bucket *d = table_find_or_claim(dst, log2b, b.key, &claimed);
     if (!d) { atomicAdd(&cnt->table_fail, 1u); return; }
@@ -698,6 +698,7 @@ struct StcCtx {
     uint32_t end_time_min = 0;
     uint64_t kv_count = 0;
     uint64_t last_live = 0;
+    uint64_t occupied = 0;
     uint32_t last_t = 0;
     bool stepped = false;
     uint32_t last_nodes = 0, last_leaves = 0;
@@ -767,6 +768,26 @@ static int ilog2_ceil(uint64_t v) {
     int l = 0;
     while ((1ull << l) < v) l++;
     return l;
+}
+
+/* rehash the entries last seen at keep_end into a table of at least
+ * 4 * expected_keys buckets (and no smaller than the current one) */
+static int table_rehash(StcCtx *c, uint32_t keep_end, uint64_t expected_keys) {
+    uint64_t nb = c->table_buckets;
+    while (nb < 4 * expected_keys) nb *= 2;
+    Bucket *nt = nullptr;
+    int rc = alloc_table(c, nb, &nt);
+    if (rc) return rc;
+    int nl2 = ilog2_ceil(nb);
+    hipLaunchKernelGGL(k_table_rehash, dim3(blocks_for(c->table_buckets, 256)), dim3(256), 0, c->s, c->table,
+                       c->table_buckets, nt, nl2, keep_end, c->dcnt);
+    CK(hipGetLastError());
+    CK(hipStreamSynchronize(c->s));
+    CK(hipFree(c->table));
+    c->table = nt;
+    c->table_buckets = nb;
+    c->log2b = nl2;
+    return 0;
 }
 
 extern "C" {
@@ -1033,7 +1054,14 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
     c->last_pairs = n_pairs;
     CK(hipEventRecord(c->ev[3], s));
 
-    /* 8-9: meeting table insert and identify */
+    /* 8-9: meeting table insert and identify; first make room for every pair
+     * being a new key (the table keeps at most half of its buckets occupied) */
+    uint32_t rebuilt = 0;
+    if ((c->occupied + n_pairs) * 2 > c->table_buckets) {
+        CKRC(table_rehash(c, t - 1, c->last_live + n_pairs));
+        c->occupied = c->last_live;
+        rebuilt = 1;
+    }
     uint64_t need_rec = c->kv_count + 2 * c->last_live + 2;
     if (ensure_records(c, need_rec)) return fail(c, STC_E_CAPACITY, "record buffer allocation failed");
     if (n_pairs)
@@ -1052,25 +1080,13 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
     c->oversize_oid_count = h.oversize_oid_count;
     c->last_live = h.live;
     uint64_t occupied = h.live + h.expired + h.retired;
-    uint32_t rebuilt = 0;
-    /* rehash when keys (live or retired) fill half the table; grow to keep live <= 1/4 */
+    /* drop retired keys once live and retired keys fill half the table */
     if (occupied * 2 > c->table_buckets) {
-        uint64_t nb = c->table_buckets;
-        while (h.live * 4 > nb) nb *= 2;
-        Bucket *nt = nullptr;
-        CKRC(alloc_table(c, nb, &nt));
-        int nl2 = ilog2_ceil(nb);
-        hipLaunchKernelGGL(k_table_rehash, dim3(blocks_for(c->table_buckets, 256)), dim3(256), 0, s, c->table,
-                           c->table_buckets, nt, nl2, t, cnt);
-        CK(hipGetLastError());
-        CK(hipStreamSynchronize(s));
-        hipFree(c->table);
-        c->table = nt;
-        c->table_buckets = nb;
-        c->log2b = nl2;
+        CKRC(table_rehash(c, t, h.live));
         rebuilt = 1;
         occupied = h.live;
     }
+    c->occupied = occupied;
     CK(hipEventRecord(c->ev[4], s));
 
     /* 10: flush */
@@ -1087,6 +1103,7 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
         fa.max_meet_time = c->cfg.max_meet_time;
         fa.area_threshold = c->cfg.area_threshold;
         fa.compat = c->cfg.compat;
+        fa.keep_sorted = (c->cfg.keep_pairs & 2) ? 1 : 0;
         fa.map[0] = c->map.lo0; fa.map[1] = c->map.lo1; fa.map[2] = c->map.hi0; fa.map[3] = c->map.hi1;
         fa.sid_count = c->sid_count;
         fa.oversize_oid_count = c->oversize_oid_count;

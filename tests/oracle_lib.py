@@ -221,6 +221,8 @@ class Oracle:
         return self.lib.orc_num_runs(self.h)
 
     def run(self, i):
+        if i >= self.num_runs():
+            raise IndexError(f"run {i} of {self.num_runs()}")
         info = OrcRunInfo()
         self.lib.orc_get_run_info(self.h, i, C.byref(info))
         nc, no, nk, nf = info.ctf_count, info.oversize_count, info.kv_count, info.num_objects
