@@ -1,0 +1,197 @@
+#!/usr/bin/env python3
+"""Ingest throughput of the per-timestep co-occurrence pipeline on one MI355X.
+
+A step is one timestep of the workload: every object's position goes through
+the QTree build and partition, the border lookup, pair refinement fused with
+the meeting-table update, meeting identification and, whenever the memtable
+exceeds its restriction, the sort-based flush into a run of CTFs
+(process_with_gpu in the reference, src/cuda/geometry.cu:1573-2218).
+
+The default workload is the 10,000,000-object street-grid walk with 20
+Gaussian hotspots (r = 20 m, minimum meeting 10 s, runs of 8 Mi records in
+256 CTFs) named by the metric "point-updates/s ingested at 10M objects". The
+trace is generated on the device before timing, so the timed region holds
+only pipeline steps on points already resident in HBM.
+
+Output: one JSON line (metric, value, roofline of the dominant kernel from
+HIP events on the library's stream, CPU baseline from the oracle on a bounded
+sample of the same workload).
+"""
+import argparse
+import ctypes
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def hip():
+    lib = ctypes.CDLL("libamdhip64.so")
+    lib.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    lib.hipMemcpy.restype = ctypes.c_int
+    return lib
+
+
+def mix64(z):
+    z = z + np.uint64(0x9E3779B97F4A7C15)
+    z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return z ^ (z >> np.uint64(31))
+
+
+def gen_rand(seed, oid, step, salt):
+    """stc_rand of stcdb_amd/csrc/stc_gen.h, vectorised over oid"""
+    with np.errstate(over="ignore"):
+        h = mix64(np.uint64(seed) ^ (np.uint64(salt) * np.uint64(0xD1B54A32D192ED03)))
+        h = mix64(h ^ oid)
+        return mix64(h ^ (np.uint64(step) * np.uint64(0x8CB92BA72F3D8DD7)))
+
+
+def cpu_sample_ids(gen):
+    """A bounded sample of the workload that keeps its density: every object
+    of hotspot 0 and every 20th of the uniformly started objects."""
+    n = gen["num_objects"]
+    oid = np.arange(n, dtype=np.uint64)
+    if gen["n_hotspots"] == 0:
+        return np.nonzero(oid % np.uint64(20) == 0)[0]
+    u = (gen_rand(gen["seed"], oid, 0, 3) >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+    hot = u < gen["hotspot_frac"]
+    hid = gen_rand(gen["seed"], oid, 0, 4) % np.uint64(gen["n_hotspots"])
+    return np.nonzero((hot & (hid == 0)) | (~hot & (oid % np.uint64(20) == 0)))[0]
+
+
+def cpu_baseline(w, frames_host, steps):
+    """the CPU oracle (single thread) on the sample over `steps` timesteps"""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    ids = cpu_sample_ids(w["gen"])
+    cfg = dict(w["cfg"])
+    cfg["num_objects"] = int(len(ids))
+    orc = O.Oracle(cfg, w["box"])
+    t0 = time.perf_counter()
+    pairs = 0
+    for t in range(steps):
+        c = orc.step(np.ascontiguousarray(frames_host[t][ids]), t)
+        pairs += c["pairs"]
+    dt = time.perf_counter() - t0
+    orc.close()
+    return dict(value=len(ids) * steps / dt, unit="point-updates/s", cores=1, kind="port",
+                sample=f"CPU oracle, 1 thread, {steps} timesteps of {len(ids)} objects (all of hotspot 0 and every "
+                       f"20th uniformly started object of the same workload, density preserved); "
+                       f"{pairs} pairs found; {dt:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="c3")
+    ap.add_argument("--objects", type=int, default=0, help="override the object count")
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one MI355X: --gpus {args.gpus} is not supported (the pipeline is single-GPU; "
+              f"see the sharding design for several GPUs)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from stcdb_amd import DeviceTrace, Pipeline, workload
+
+    over = {}
+    if args.objects:
+        over["num_objects"] = args.objects
+    w = workload(args.workload, **over)
+    cfg, gen = w["cfg"], w["gen"]
+    n = cfg["num_objects"]
+    total = args.warmup + args.steps
+
+    # inputs for every step, generated on the device before timing
+    frames = torch.empty((total, n, 2), dtype=torch.float64, device="cuda")
+    trace = DeviceTrace(gen)
+    h = hip()
+    for t in range(total):
+        p = trace.next(t)
+        assert h.hipMemcpy(frames[t].data_ptr(), p.value, n * 16, 3) == 0
+    trace.close()
+    torch.cuda.synchronize()
+    print(f"[bench] workload {args.workload}: {n} objects, {total} frames resident", file=sys.stderr, flush=True)
+
+    pipe = Pipeline(cfg, w["box"])
+    for t in range(args.warmup):
+        r = pipe.step_device(ctypes.c_void_p(frames[t].data_ptr()), t)
+        print(f"[bench] warmup step {t}: {r['ms_total']:.1f} ms, {r['pairs']} pairs", file=sys.stderr, flush=True)
+
+    outs = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for t in range(args.warmup, total):
+        outs.append(pipe.step_device(ctypes.c_void_p(frames[t].data_ptr()), t))
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    for i, r in enumerate(outs):
+        print(f"[bench] step {args.warmup + i}: {r['ms_total']:.1f} ms (refine {r['ms_refine_kernel']:.1f}, identify "
+              f"{r['ms_identify_kernel']:.1f}, flush {r['ms_flush']:.1f}) pairs {r['pairs']} records {r['new_records']}"
+              f" flushed {r['flushed']}", file=sys.stderr, flush=True)
+
+    value = n * args.steps / elapsed
+    # roofline of the dominant kernel (algorithmic bytes per launch / its HIP-event time)
+    refine_ms = float(np.mean([r["ms_refine_kernel"] for r in outs]))
+    ident_ms = float(np.mean([r["ms_identify_kernel"] for r in outs]))
+    pairs = float(np.mean([r["pairs"] for r in outs]))
+    if refine_ms >= ident_ms:
+        # pair/refine 16N + 4N + 8P, fused with the table insert/extend 64P
+        alg = 20.0 * n + 72.0 * pairs
+        name, ms = "k_refine (pair refinement fused with meeting-table insert)", refine_ms
+    else:
+        buckets = float(np.mean([r["table_buckets"] for r in outs]))
+        meets = float(np.mean([r["new_records"] for r in outs])) / 2
+        alg = 32.0 * buckets + 64.0 * meets
+        name, ms = "k_identify (meeting expiry scan)", ident_ms
+    achieved = alg / (ms * 1e-3) / 1e9
+    roof = dict(bound="hbm", kernel=name, achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, alg_bytes_per_launch=alg,
+                avg_ms=round(ms, 3))
+
+    cpu = None
+    if not args.no_cpu:
+        frames_host = [frames[t].cpu().numpy() for t in range(args.cpu_steps)]
+        cpu = cpu_baseline(w, frames_host, args.cpu_steps)
+        cpu["cpu_model"] = platform.processor() or platform.machine()
+
+    line = {
+        "metric": "point-updates/s ingested at 10M objects",
+        "value": round(value, 1),
+        "unit": "point-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed * 1000 / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded street-grid walk generated on the device)",
+        "config": {"workload": args.workload, "objects": n, "reach_m": cfg["reach_distance"],
+                   "min_meet_s": cfg["min_meet_time"], "leaf_capacity": cfg["grid_capacity"],
+                   "kv_restriction": cfg["kv_restriction"], "ctf_count": cfg["ctf_count"],
+                   "pairs_per_step": int(pairs),
+                   "flushes": int(sum(r["flushed"] for r in outs))},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    pipe.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -88,6 +88,10 @@ typedef struct stc_step_out {
     float ms_meetings;         /* table insert + identify */
     float ms_flush;            /* flush (0 when no flush) */
     float ms_total;
+    float ms_refine_kernel;    /* the fused refinement + table-insert kernel (last launch) */
+    float ms_identify_kernel;  /* the identify kernel */
+    uint64_t continuing;       /* pairs that continued a meeting seen at the previous step */
+    uint64_t new_keys;         /* pairs that opened a new table entry */
 } stc_step_out;
 
 /* CTF metadata of a flushed run: the fields of the reference's CTF

@@ -67,6 +67,10 @@ class StcStepOut(C.Structure):
         ("ms_meetings", C.c_float),
         ("ms_flush", C.c_float),
         ("ms_total", C.c_float),
+        ("ms_refine_kernel", C.c_float),
+        ("ms_identify_kernel", C.c_float),
+        ("continuing", C.c_uint64),
+        ("new_keys", C.c_uint64),
     ]
 
     def as_dict(self):

@@ -96,6 +96,9 @@ struct Counters {             /* device-side counters, copied back once per sync
     unsigned long long candidates;
     unsigned long long pairs;
     unsigned long long ambiguous;
+    unsigned long long pairs_listed;
+    unsigned long long new_keys;
+    unsigned long long continuing;
     unsigned long long ambiguous_nodes;
     unsigned long long live, expired, retired;
     unsigned long long kv_count;
@@ -137,8 +140,14 @@ __device__ inline uint32_t compact1by1(uint32_t x) {
     return x;
 }
 
-__device__ inline uint64_t hash_slot(unsigned long long key, int log2b) {
-    return (uint64_t)((key * 0x9E3779B97F4A7C15ull) >> (64 - log2b));
+/* home slot of a key in a table of nb buckets: a 64-bit mix of the key
+ * mapped onto [0, nb) by a multiply-shift (nb need not be a power of two) */
+__device__ inline uint64_t hash_slot(unsigned long long key, uint64_t nb) {
+    uint64_t z = key + 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (uint64_t)(((unsigned __int128)z * nb) >> 64);
 }
 
 /* ------------------------------------------------------------------ */
@@ -402,7 +411,77 @@ __global__ void k_lookup(const double2 *__restrict__ xy, const double *__restric
 }
 
 /* ------------------------------------------------------------------ */
-/* refinement                                                          */
+/* meeting table                                                       */
+/* ------------------------------------------------------------------ */
+
+#define MAX_PROBES 1024u
+
+__device__ inline unsigned long long wave_sum_u64(unsigned long long v) {
+    for (int o = STC_WAVE / 2; o > 0; o >>= 1) v += __shfl_down(v, o, STC_WAVE);
+    return v;
+}
+
+__global__ void k_table_clear(Bucket *t, uint64_t nb) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nb) return;
+    Bucket b;
+    b.key = EMPTY_KEY;
+    b.start = 0;
+    b.end = 0xFFFFFFFFu;
+    b.mbr[0] = b.mbr[1] = 0xFFFFFFFFu;
+    b.mbr[2] = b.mbr[3] = 0u;
+    t[i] = b;
+}
+
+/* linear probing from the key's hash slot; claims an empty bucket with a
+ * 64-bit CAS. Returns null after MAX_PROBES probes (table too full). */
+__device__ inline Bucket *table_find_or_claim(Bucket *t, uint64_t nb, unsigned long long key, bool *claimed) {
+    uint64_t h = hash_slot(key, nb);
+    for (uint32_t probe = 0; probe < MAX_PROBES; probe++) {
+        Bucket *b = &t[h];
+        unsigned long long k = __atomic_load_n(&b->key, __ATOMIC_RELAXED);
+        if (k == key) { *claimed = false; return b; }
+        if (k == EMPTY_KEY) {
+            unsigned long long prev = atomicCAS(&b->key, EMPTY_KEY, key);
+            if (prev == EMPTY_KEY) { *claimed = true; return b; }
+            if (prev == key) { *claimed = false; return b; }
+        }
+        h = (h + 1 == nb) ? 0 : h + 1;
+    }
+    return nullptr;
+}
+
+/* insert or extend one pair seen at `now` (geometry.cu:469-494, without the
+ * 5-probe drop). Idempotent within a step: inserting the same pair twice at
+ * the same `now` leaves the same state. Returns -1 when the table is full,
+ * 1 for a new key, 2 when it continues a meeting seen at now - 1, else 0. */
+__device__ inline int table_upsert(Bucket *t, uint64_t nb, unsigned long long key, const float *mbr, uint32_t now) {
+    bool claimed;
+    Bucket *b = table_find_or_claim(t, nb, key, &claimed);
+    if (!b) return -1;
+    uint32_t old_end = atomicExch(&b->end, now);
+    if (claimed || (old_end != now && old_end != now - 1)) b->start = now; /* new or restarted meeting */
+    atomicMin(&b->mbr[0], f2o(mbr[0]));
+    atomicMin(&b->mbr[1], f2o(mbr[1]));
+    atomicMax(&b->mbr[2], f2o(mbr[2]));
+    atomicMax(&b->mbr[3], f2o(mbr[3]));
+    if (claimed) return 1;
+    return (old_end == now - 1) ? 2 : 0;
+}
+
+/* insert a list of pairs (host-resolved ambiguous pairs) */
+__global__ void k_table_insert(Bucket *t, uint64_t nb, const PairHit *__restrict__ pairs, uint64_t np, uint32_t now,
+                               Counters *cnt) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    PairHit h = pairs[i];
+    int rc = table_upsert(t, nb, h.key, h.mbr, now);
+    if (rc < 0) atomicAdd(&cnt->table_fail, 1u);
+    if (rc == 2) atomicAdd(&cnt->continuing, 1ull);
+}
+
+/* ------------------------------------------------------------------ */
+/* refinement fused with the table insert                               */
 /* ------------------------------------------------------------------ */
 
 __global__ __launch_bounds__(256) void k_refine(
@@ -410,13 +489,13 @@ __global__ __launch_bounds__(256) void k_refine(
     const uint32_t *__restrict__ pt_rank, const uint32_t *__restrict__ leaf_start,
     const uint32_t *__restrict__ leaf_count, const uint32_t *__restrict__ lp, const double *__restrict__ lx,
     const double *__restrict__ ly, const uint2 *__restrict__ units, uint64_t n_units_total, uint32_t buf, double r,
-    double band, PairHit *__restrict__ pairs, uint64_t pair_cap, uint2 *__restrict__ amb, uint32_t amb_cap,
-    Counters *cnt) {
+    double band, Bucket *table, uint64_t nb, uint32_t now, PairHit *__restrict__ pairs, uint64_t pair_cap,
+    uint2 *__restrict__ amb, uint32_t amb_cap, Counters *cnt) {
     const int lane = threadIdx.x & (STC_WAVE - 1);
     const uint64_t wid = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / STC_WAVE;
     const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) / STC_WAVE;
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (STC_WAVE - lane));
-    unsigned long long cand = 0;
+    unsigned long long cand = 0, hits = 0, fresh = 0, fails = 0, cont = 0;
     for (uint64_t u = wid; u < n_units_total; u += nw) {
         uint32_t p, leaf, begin;
         if (u < n) {
@@ -448,24 +527,27 @@ __global__ __launch_bounds__(256) void k_refine(
                 double d = stc_point_dist(pp.x, pp.y, qx, qy, pd);
                 if (fabs(d - r) <= band) ambg = true;
                 else hit = d <= r;
+                cand++;
             }
-            cand += (m < s) ? 1 : 0;
-            uint64_t hm = __ballot(hit);
-            if (hm) {
-                unsigned long long basei = 0;
-                if (lane == 0) basei = atomicAdd(&cnt->pairs, (unsigned long long)__popcll(hm));
-                basei = __shfl(basei, 0, STC_WAVE);
-                if (hit) {
-                    uint64_t k = basei + (uint64_t)__popcll(hm & lt);
+            if (hit) {
+                uint32_t a = p < q ? p : q, b = p < q ? q : p;
+                unsigned long long key = ((unsigned long long)a << 32) | b;
+                float qfx = (float)qx, qfy = (float)qy;
+                float mbr[4] = {fminf(pfx, qfx), fminf(pfy, qfy), fmaxf(pfx, qfx), fmaxf(pfy, qfy)};
+                /* once the table has overflowed this pass only counts; the host
+                 * grows the table and repeats the pass */
+                int rc = __atomic_load_n(&cnt->table_fail, __ATOMIC_RELAXED) ? -1
+                                                                             : table_upsert(table, nb, key, mbr, now);
+                hits++;
+                if (rc == 1) fresh++;
+                if (rc == 2) cont++;
+                if (rc < 0) fails++;
+                if (pairs) { /* inspection copy of the step's pairs */
+                    unsigned long long k = atomicAdd(&cnt->pairs_listed, 1ull);
                     if (k < pair_cap) {
-                        uint32_t a = p < q ? p : q, b = p < q ? q : p;
                         PairHit h;
-                        h.key = ((unsigned long long)a << 32) | b;
-                        float qfx = (float)qx, qfy = (float)qy;
-                        h.mbr[0] = fminf(pfx, qfx);
-                        h.mbr[1] = fminf(pfy, qfy);
-                        h.mbr[2] = fmaxf(pfx, qfx);
-                        h.mbr[3] = fmaxf(pfy, qfy);
+                        h.key = key;
+                        h.mbr[0] = mbr[0]; h.mbr[1] = mbr[1]; h.mbr[2] = mbr[2]; h.mbr[3] = mbr[3];
                         pairs[k] = h;
                     }
                 }
@@ -476,59 +558,19 @@ __global__ __launch_bounds__(256) void k_refine(
             }
         }
     }
-    /* reduce candidate counts over the wavefront */
-    for (int o = STC_WAVE / 2; o > 0; o >>= 1) cand += __shfl_down(cand, o, STC_WAVE);
-    if (lane == 0 && cand) atomicAdd(&cnt->candidates, cand);
-}
-
-/* ------------------------------------------------------------------ */
-/* meeting table                                                       */
-/* ------------------------------------------------------------------ */
-
-__global__ void k_table_clear(Bucket *t, uint64_t nb) {
-    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nb) return;
-    Bucket b;
-    b.key = EMPTY_KEY;
-    b.start = 0;
-    b.end = 0xFFFFFFFFu;
-    b.mbr[0] = b.mbr[1] = 0xFFFFFFFFu;
-    b.mbr[2] = b.mbr[3] = 0u;
-    t[i] = b;
-}
-
-__device__ inline Bucket *table_find_or_claim(Bucket *t, int log2b, unsigned long long key, bool *claimed) {
-    uint64_t mask = (1ull << log2b) - 1;
-    uint64_t h = hash_slot(key, log2b);
-    for (uint64_t probe = 0; probe <= mask; probe++) {
-        Bucket *b = &t[h];
-        unsigned long long k = __atomic_load_n(&b->key, __ATOMIC_RELAXED);
-        if (k == key) { *claimed = false; return b; }
-        if (k == EMPTY_KEY) {
-            unsigned long long prev = atomicCAS(&b->key, EMPTY_KEY, key);
-            if (prev == EMPTY_KEY) { *claimed = true; return b; }
-            if (prev == key) { *claimed = false; return b; }
-        }
-        h = (h + 1) & mask;
+    (void)lt;
+    cand = wave_sum_u64(cand);
+    hits = wave_sum_u64(hits);
+    fresh = wave_sum_u64(fresh);
+    fails = wave_sum_u64(fails);
+    cont = wave_sum_u64(cont);
+    if (lane == 0) {
+        if (cont) atomicAdd(&cnt->continuing, cont);
+        if (cand) atomicAdd(&cnt->candidates, cand);
+        if (hits) atomicAdd(&cnt->pairs, hits);
+        if (fresh) atomicAdd(&cnt->new_keys, fresh);
+        if (fails) atomicAdd(&cnt->table_fail, (uint32_t)fails);
     }
-    return nullptr;
-}
-
-/* insert or extend (geometry.cu:469-494, without the 5-probe limit) */
-__global__ void k_table_insert(Bucket *t, int log2b, const PairHit *__restrict__ pairs, uint64_t np, uint32_t now,
-                               Counters *cnt) {
-    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= np) return;
-    PairHit h = pairs[i];
-    bool claimed;
-    Bucket *b = table_find_or_claim(t, log2b, h.key, &claimed);
-    if (!b) { atomicAdd(&cnt->table_fail, 1u); return; }
-    uint32_t old_end = atomicExch(&b->end, now);
-    if (claimed || (old_end != now && old_end != now - 1)) b->start = now; /* new or restarted meeting */
-    atomicMin(&b->mbr[0], f2o(h.mbr[0]));
-    atomicMin(&b->mbr[1], f2o(h.mbr[1]));
-    atomicMax(&b->mbr[2], f2o(h.mbr[2]));
-    atomicMax(&b->mbr[3], f2o(h.mbr[3]));
 }
 
 /* float_to_uint (cuda_util.cuh:115-119), saturating as the reference GPU converts */
@@ -550,71 +592,111 @@ struct MemtableDev {
     uint16_t *sids;
 };
 
-/* identify completed meetings (geometry.cu:595-737) */
-__global__ void k_identify(Bucket *t, uint64_t nb, uint32_t now, uint32_t min_meet, uint32_t max_meet,
-                           uint32_t end_time_min, MapBox m, MemtableDev mt, Counters *cnt) {
-    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    bool live = false, expired = false, retired = false;
-    if (i < nb) {
-        Bucket b = t[i];
-        if (b.key != EMPTY_KEY) {
-            if (b.end == now) {
-                live = true;
-            } else if (b.end == now - 1) {
-                expired = true;
-                float lo0 = o2f(b.mbr[0]), lo1 = o2f(b.mbr[1]), hi0 = o2f(b.mbr[2]), hi1 = o2f(b.mbr[3]);
-                if (now - b.start >= min_meet + 1 && (double)lo0 >= m.lo0 && (double)hi0 <= m.hi0 &&
-                    (double)lo1 >= m.lo1 && (double)hi1 <= m.hi1) {
-                    uint32_t duration = b.end - b.start;
-                    uint32_t pid = (uint32_t)(b.key >> 32), target = (uint32_t)b.key;
-                    if (duration > max_meet) {
-                        atomicAdd(&cnt->oversize_oid_count, 1u);
-                        mt.sids[pid] = 1;
-                        mt.sids[target] = 1;
-                    }
-                    unsigned long long k = atomicAdd(&cnt->kv_count, 2ull);
-                    uint32_t q0 = f2q(lo0), q1 = f2q(lo1), q2 = f2q(hi0), q3 = f2q(hi1);
-                    for (int s = 0; s < 2; s++) {
-                        uint32_t a = s ? target : pid, c = s ? pid : target;
-                        if (k + s < mt.cap) {
-                            mt.keys[2 * (k + s)] = ((uint64_t)duration << 32) | (uint32_t)(b.end - end_time_min);
-                            mt.keys[2 * (k + s) + 1] = ((uint64_t)a << 26) | (uint64_t)c;
-                            mt.boxes[k + s] = make_float4(lo0, lo1, hi0, hi1);
-                        }
-                        if (atomicAdd(&mt.same_pid_count[a], 1u) == 0) atomicAdd(&cnt->sid_count, 1u);
-                        atomicMin(&mt.o_boxs[a].x, q0);
-                        atomicMax(&mt.o_boxs[a].z, q2);
-                        atomicMin(&mt.o_boxs[a].y, q1);
-                        atomicMax(&mt.o_boxs[a].w, q3);
-                    }
+#define ID_STAGE 128 /* per-wavefront staging of expired buckets that emit records */
+
+/* identify completed meetings (geometry.cu:595-737). Grid-stride over the
+ * table; emitting buckets are staged per wavefront in LDS and written in
+ * batches, so the memtable counter sees one atomic per batch. */
+__global__ __launch_bounds__(256) void k_identify(Bucket *t, uint64_t nb, uint32_t now, uint32_t min_meet,
+                                                  uint32_t max_meet, uint32_t end_time_min, MapBox m,
+                                                  MemtableDev mt, Counters *cnt) {
+    __shared__ uint64_t stage[256 / STC_WAVE][ID_STAGE];
+    const int lane = threadIdx.x & (STC_WAVE - 1), w = threadIdx.x / STC_WAVE;
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (STC_WAVE - lane));
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    unsigned long long nlive = 0, nexp = 0, nret = 0;
+    uint32_t staged = 0; /* wave-uniform */
+    const uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x + (uint64_t)w * STC_WAVE;
+    for (uint64_t base = i0; base < nb; base += stride) {
+        uint64_t i = base + lane;
+        bool emit = false, retire_now = false;
+        if (i < nb) {
+            Bucket b = t[i];
+            if (b.key != EMPTY_KEY) {
+                if (b.end == now) {
+                    nlive++;
+                } else if (b.end == now - 1) {
+                    nexp++;
+                    float lo0 = o2f(b.mbr[0]), lo1 = o2f(b.mbr[1]), hi0 = o2f(b.mbr[2]), hi1 = o2f(b.mbr[3]);
+                    emit = (now - b.start >= min_meet + 1 && (double)lo0 >= m.lo0 && (double)hi0 <= m.hi0 &&
+                            (double)lo1 >= m.lo1 && (double)hi1 <= m.hi1);
+                    retire_now = !emit;
+                } else {
+                    nret++;
                 }
-                /* retire: key stays for the probe chains, MBR resets for a restart */
-                t[i].mbr[0] = 0xFFFFFFFFu;
-                t[i].mbr[1] = 0xFFFFFFFFu;
-                t[i].mbr[2] = 0u;
-                t[i].mbr[3] = 0u;
-            } else {
-                retired = true;
             }
         }
+        /* retire expired buckets: the key stays for the probe chains, the MBR
+         * resets for a later restart. Emitting buckets retire after their
+         * records are written. */
+        if (retire_now) {
+            t[i].mbr[0] = 0xFFFFFFFFu;
+            t[i].mbr[1] = 0xFFFFFFFFu;
+            t[i].mbr[2] = 0u;
+            t[i].mbr[3] = 0u;
+        }
+        uint64_t em = __ballot(emit);
+        if (emit) stage[w][staged + __popcll(em & lt)] = i;
+        staged += (uint32_t)__popcll(em);
+        bool flush = staged > ID_STAGE - STC_WAVE || (base + stride >= nb);
+        __builtin_amdgcn_wave_barrier();
+        if (flush && staged) {
+            unsigned long long kbase = 0;
+            if (lane == 0) kbase = atomicAdd(&cnt->kv_count, 2ull * staged);
+            kbase = __shfl(kbase, 0, STC_WAVE);
+            for (uint32_t e = lane; e < staged; e += STC_WAVE) {
+                uint64_t bi = stage[w][e];
+                Bucket b = t[bi];
+                float lo0 = o2f(b.mbr[0]), lo1 = o2f(b.mbr[1]), hi0 = o2f(b.mbr[2]), hi1 = o2f(b.mbr[3]);
+                uint32_t duration = b.end - b.start;
+                uint32_t pid = (uint32_t)(b.key >> 32), target = (uint32_t)b.key;
+                if (duration > max_meet) {
+                    atomicAdd(&cnt->oversize_oid_count, 1u);
+                    mt.sids[pid] = 1;
+                    mt.sids[target] = 1;
+                }
+                unsigned long long k = kbase + 2ull * e;
+                uint32_t q0 = f2q(lo0), q1 = f2q(lo1), q2 = f2q(hi0), q3 = f2q(hi1);
+                for (int sdir = 0; sdir < 2; sdir++) {
+                    uint32_t a = sdir ? target : pid, c = sdir ? pid : target;
+                    if (k + sdir < mt.cap) {
+                        mt.keys[2 * (k + sdir)] = ((uint64_t)duration << 32) | (uint32_t)(b.end - end_time_min);
+                        mt.keys[2 * (k + sdir) + 1] = ((uint64_t)a << 26) | (uint64_t)c;
+                        mt.boxes[k + sdir] = make_float4(lo0, lo1, hi0, hi1);
+                    }
+                    if (atomicAdd(&mt.same_pid_count[a], 1u) == 0) atomicAdd(&cnt->sid_count, 1u);
+                    atomicMin(&mt.o_boxs[a].x, q0);
+                    atomicMax(&mt.o_boxs[a].z, q2);
+                    atomicMin(&mt.o_boxs[a].y, q1);
+                    atomicMax(&mt.o_boxs[a].w, q3);
+                }
+                t[bi].mbr[0] = 0xFFFFFFFFu;
+                t[bi].mbr[1] = 0xFFFFFFFFu;
+                t[bi].mbr[2] = 0u;
+                t[bi].mbr[3] = 0u;
+            }
+            staged = 0;
+        }
     }
-    uint64_t bl = __ballot(live), be = __ballot(expired), br = __ballot(retired);
-    if ((threadIdx.x & (STC_WAVE - 1)) == 0) {
-        if (bl) atomicAdd(&cnt->live, (unsigned long long)__popcll(bl));
-        if (be) atomicAdd(&cnt->expired, (unsigned long long)__popcll(be));
-        if (br) atomicAdd(&cnt->retired, (unsigned long long)__popcll(br));
+    nlive = wave_sum_u64(nlive);
+    nexp = wave_sum_u64(nexp);
+    nret = wave_sum_u64(nret);
+    if (lane == 0) {
+        if (nlive) atomicAdd(&cnt->live, nlive);
+        if (nexp) atomicAdd(&cnt->expired, nexp);
+        if (nret) atomicAdd(&cnt->retired, nret);
     }
 }
 
-/* move the entries last seen at `keep_end` into a fresh table */
-__global__ void k_table_rehash(const Bucket *src, uint64_t nsrc, Bucket *dst, int log2b, uint32_t keep_end,
+/* move the entries last seen at or after `keep_from` into a fresh table */
+__global__ void k_table_rehash(const Bucket *src, uint64_t nsrc, Bucket *dst, uint64_t ndst, uint32_t keep_from,
                                Counters *cnt) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nsrc) return;
     Bucket b = src[i];
-    if (b.key == EMPTY_KEY || b.end != keep_end) return;
+    if (b.key == EMPTY_KEY || b.end == 0xFFFFFFFFu || b.end < keep_from) return;
     bool claimed;
-    Bucket *d = table_find_or_claim(dst, log2b, b.key, &claimed);
+    Bucket *d = table_find_or_claim(dst, ndst, b.key, &claimed);
     if (!d) { atomicAdd(&cnt->table_fail, 1u); return; }
     d->start = b.start;
     d->end = b.end;
@@ -699,14 +781,17 @@ struct StcCtx {
     uint64_t kv_count = 0;
     uint64_t last_live = 0;
     uint64_t occupied = 0;
+    uint64_t last_new = 0;
+    uint64_t last_listed = 0;
     uint32_t last_t = 0;
     bool stepped = false;
     uint32_t last_nodes = 0, last_leaves = 0;
     uint64_t last_pairs = 0;
     std::vector<PairHit> host_pairs_extra;
+    DBuf<PairHit> amb_pairs;
     std::vector<RunHost> runs;
     FlushWork flushw;
-    hipEvent_t ev[8];
+    hipEvent_t ev[10];
 };
 
 static int fail(StcCtx *c, int code, const std::string &msg) {
@@ -770,23 +855,20 @@ static int ilog2_ceil(uint64_t v) {
     return l;
 }
 
-/* rehash the entries last seen at keep_end into a table of at least
- * 4 * expected_keys buckets (and no smaller than the current one) */
-static int table_rehash(StcCtx *c, uint32_t keep_end, uint64_t expected_keys) {
-    uint64_t nb = c->table_buckets;
-    while (nb < 4 * expected_keys) nb *= 2;
+/* move the entries last seen at or after keep_from into a table sized for
+ * expected_keys at a load of at most 0.4 (and no smaller than the current one) */
+static int table_rehash(StcCtx *c, uint32_t keep_from, uint64_t expected_keys) {
+    uint64_t nb = std::max<uint64_t>(c->table_buckets, expected_keys * 5 / 2 + 1024);
     Bucket *nt = nullptr;
     int rc = alloc_table(c, nb, &nt);
     if (rc) return rc;
-    int nl2 = ilog2_ceil(nb);
     hipLaunchKernelGGL(k_table_rehash, dim3(blocks_for(c->table_buckets, 256)), dim3(256), 0, c->s, c->table,
-                       c->table_buckets, nt, nl2, keep_end, c->dcnt);
+                       c->table_buckets, nt, nb, keep_from, c->dcnt);
     CK(hipGetLastError());
     CK(hipStreamSynchronize(c->s));
     CK(hipFree(c->table));
     c->table = nt;
     c->table_buckets = nb;
-    c->log2b = nl2;
     return 0;
 }
 
@@ -847,7 +929,7 @@ int stc_create(const stc_config *cfg, const stc_box *mbr, void **out) {
         return STC_E_HIP;
     };
     if (hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking) != hipSuccess) return bad("stream");
-    for (int i = 0; i < 8; i++) hipEventCreate(&c->ev[i]);
+    for (int i = 0; i < 10; i++) hipEventCreate(&c->ev[i]);
     uint32_t n = c->n;
     size_t node_cap = 1 + 4ull * NLEVELS * (n / (c->cap + 1) + 1);
     size_t front_cap = n / (c->cap + 1) + 2;
@@ -865,8 +947,7 @@ int stc_create(const stc_config *cfg, const stc_box *mbr, void **out) {
         hipHostMalloc((void **)&c->hinit, sizeof(Counters), hipHostMallocDefault) != hipSuccess)
         return bad("pinned");
     uint64_t nb = cfg->num_meeting_buckets ? cfg->num_meeting_buckets : 4ull * n;
-    c->log2b = std::max(ilog2_ceil(nb), 10);
-    c->table_buckets = 1ull << c->log2b;
+    c->table_buckets = std::max<uint64_t>(nb, 1024);
     if (alloc_table(c, c->table_buckets, &c->table)) return bad("table");
     if (hipMalloc((void **)&c->same_pid_count, 4ull * n) != hipSuccess ||
         hipMalloc((void **)&c->o_boxs, 16ull * n) != hipSuccess || hipMalloc((void **)&c->sids, 2ull * n) != hipSuccess)
@@ -891,6 +972,7 @@ int stc_destroy(void *vc) {
     c->leaf_node.release(); c->pt_leaf.release(); c->pt_lv.release(); c->pt_rank.release(); c->lp.release();
     c->lkeys.release(); c->ucount.release(); c->uoffset.release(); c->units.release(); c->pairs.release();
     c->amb.release();
+    c->amb_pairs.release();
     if (c->table) hipFree(c->table);
     if (c->rec_keys) hipFree(c->rec_keys);
     if (c->rec_boxes) hipFree(c->rec_boxes);
@@ -901,7 +983,7 @@ int stc_destroy(void *vc) {
     if (c->hcnt) hipHostFree(c->hcnt);
     if (c->hinit) hipHostFree(c->hinit);
     stc_flush_free(&c->flushw);
-    for (int i = 0; i < 8; i++) if (c->ev[i]) hipEventDestroy(c->ev[i]);
+    for (int i = 0; i < 10; i++) if (c->ev[i]) hipEventDestroy(c->ev[i]);
     if (c->s) hipStreamDestroy(c->s);
     delete c;
     return STC_OK;
@@ -997,27 +1079,56 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
     CK(hipGetLastError());
     CK(hipEventRecord(c->ev[2], s));
 
-    /* 7: refinement; rerun with a larger pair buffer if it overflowed */
+    /* 7-8: refinement fused with the meeting-table insert. First make room:
+     * after this step's new keys at most half of the buckets hold keys. */
+    uint32_t rebuilt = 0;
+    uint64_t est_new = c->stepped ? 2 * c->last_new + 1024 : (uint64_t)n;
+    if ((c->occupied + est_new) * 2 > c->table_buckets) {
+        CKRC(table_rehash(c, t - 1, c->last_live + est_new));
+        c->occupied = c->last_live;
+        rebuilt = 1;
+    }
+    const bool keep = (c->cfg.keep_pairs & 1) != 0;
     uint64_t total_units = (uint64_t)n + n_units;
     uint32_t grid = (uint32_t)std::min<uint64_t>(blocks_for(total_units * STC_WAVE, 256), 256u * 64u);
-    for (int attempt = 0; attempt < 4; attempt++) {
-        CK(hipMemsetAsync(&cnt->candidates, 0, 3 * sizeof(unsigned long long), s)); /* candidates, pairs, ambiguous */
+    for (int attempt = 0;; attempt++) {
+        CK(hipMemsetAsync(&cnt->candidates, 0, 6 * sizeof(unsigned long long), s)); /* candidates .. continuing */
+        CK(hipMemsetAsync(&cnt->table_fail, 0, sizeof(uint32_t), s));
+        CK(hipEventRecord(c->ev[6], s));
         hipLaunchKernelGGL(k_refine, dim3(grid), dim3(256), 0, s, xy, c->dlon.p, n, c->pt_leaf.p, c->pt_rank.p,
                            c->leaf_start.p, c->leaf_count.p, c->lp.p, c->lx.p, c->ly.p, c->units.p, total_units,
-                           c->buf, r, c->band, c->pairs.p, (uint64_t)c->pairs.cap, c->amb.p, (uint32_t)c->amb.cap,
-                           cnt);
+                           c->buf, r, c->band, c->table, c->table_buckets, t, keep ? c->pairs.p : (PairHit *)nullptr,
+                           (uint64_t)c->pairs.cap, c->amb.p, (uint32_t)c->amb.cap, cnt);
         CK(hipGetLastError());
+        CK(hipEventRecord(c->ev[7], s));
         CKRC(sync_counters(c));
         h = *c->hcnt;
-        if (h.pairs <= c->pairs.cap && h.ambiguous <= c->amb.cap) break;
-        if (c->pairs.ensure(h.pairs + 1024) || c->amb.ensure(h.ambiguous + 1024))
-            return fail(c, STC_E_CAPACITY, "pair buffer allocation failed");
+        bool redo = false;
+        if (h.table_fail) {
+            /* grow, keeping the keys seen last step and this step; reinserting is idempotent */
+            CKRC(table_rehash(c, t - 1, c->last_live + h.pairs + 1024));
+            rebuilt = 1;
+            redo = true;
+        }
+        if (keep && h.pairs_listed > c->pairs.cap) {
+            if (c->pairs.ensure(h.pairs_listed + 1024)) return fail(c, STC_E_CAPACITY, "pair list allocation failed");
+            redo = true;
+        }
+        if (h.ambiguous > c->amb.cap) {
+            if (c->amb.ensure(h.ambiguous + 1024)) return fail(c, STC_E_CAPACITY, "ambiguous list allocation failed");
+            redo = true;
+        }
+        if (!redo) break;
+        if (attempt >= 8) return fail(c, STC_E_CAPACITY, "meeting table cannot hold the step's pairs");
     }
-    if (h.pairs > c->pairs.cap || h.ambiguous > c->amb.cap) return fail(c, STC_E_CAPACITY, "pair buffer overflow");
     uint64_t n_pairs = h.pairs;
     uint64_t n_amb = h.ambiguous;
+    c->last_new = h.new_keys;
+    c->occupied += h.new_keys;
+    c->last_listed = keep ? h.pairs_listed : 0;
+    c->host_pairs_extra.clear();
     if (n_amb) {
-        /* decide the band with the C library's sin and append accepted pairs */
+        /* decide the band with the C library's sin and insert the accepted pairs */
         std::vector<uint2> ab(n_amb);
         CK(hipMemcpy(ab.data(), c->amb.p, n_amb * sizeof(uint2), hipMemcpyDeviceToHost));
         std::vector<PairHit> extra;
@@ -1038,39 +1149,31 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
             }
         }
         if (!extra.empty()) {
-            if (c->pairs.cap < n_pairs + extra.size()) {
-                /* grow preserving contents */
-                PairHit *np_ = nullptr;
-                CK(hipMalloc((void **)&np_, (n_pairs + extra.size() + 1024) * sizeof(PairHit)));
-                CK(hipMemcpy(np_, c->pairs.p, n_pairs * sizeof(PairHit), hipMemcpyDeviceToDevice));
-                hipFree(c->pairs.p);
-                c->pairs.p = np_;
-                c->pairs.cap = n_pairs + extra.size() + 1024;
-            }
-            CK(hipMemcpy(c->pairs.p + n_pairs, extra.data(), extra.size() * sizeof(PairHit), hipMemcpyHostToDevice));
+            if (c->amb_pairs.ensure(extra.size())) return fail(c, STC_E_CAPACITY, "ambiguous pair buffer");
+            CK(hipMemcpy(c->amb_pairs.p, extra.data(), extra.size() * sizeof(PairHit), hipMemcpyHostToDevice));
+            hipLaunchKernelGGL(k_table_insert, dim3(blocks_for(extra.size(), 256)), dim3(256), 0, s, c->table,
+                               c->table_buckets, c->amb_pairs.p, (uint64_t)extra.size(), t, cnt);
+            CK(hipGetLastError());
             n_pairs += extra.size();
+            c->occupied += extra.size();
+            c->host_pairs_extra = extra;
         }
     }
     c->last_pairs = n_pairs;
     CK(hipEventRecord(c->ev[3], s));
 
-    /* 8-9: meeting table insert and identify; first make room for every pair
-     * being a new key (the table keeps at most half of its buckets occupied) */
-    uint32_t rebuilt = 0;
-    if ((c->occupied + n_pairs) * 2 > c->table_buckets) {
-        CKRC(table_rehash(c, t - 1, c->last_live + n_pairs));
-        c->occupied = c->last_live;
-        rebuilt = 1;
-    }
-    uint64_t need_rec = c->kv_count + 2 * c->last_live + 2;
+    /* 9: identify: meetings live at t - 1 and not continued now expire */
+    CKRC(sync_counters(c));
+    uint64_t h_cont = c->hcnt->continuing;
+    uint64_t expiring = c->last_live > h_cont ? c->last_live - h_cont : 0;
+    uint64_t need_rec = c->kv_count + 2 * expiring + 2;
     if (ensure_records(c, need_rec)) return fail(c, STC_E_CAPACITY, "record buffer allocation failed");
-    if (n_pairs)
-        hipLaunchKernelGGL(k_table_insert, dim3(blocks_for(n_pairs, 256)), dim3(256), 0, s, c->table, c->log2b,
-                           c->pairs.p, n_pairs, t, cnt);
     MemtableDev mt{c->rec_keys, c->rec_boxes, c->rec_cap, c->same_pid_count, c->o_boxs, c->sids};
-    hipLaunchKernelGGL(k_identify, dim3(blocks_for(c->table_buckets, 256)), dim3(256), 0, s, c->table,
-                       c->table_buckets, t, c->cfg.min_meet_time, c->cfg.max_meet_time, c->end_time_min, c->map, mt,
-                       cnt);
+    uint32_t id_grid = (uint32_t)std::min<uint64_t>(blocks_for(c->table_buckets, 256), 4096);
+    CK(hipEventRecord(c->ev[8], s));
+    hipLaunchKernelGGL(k_identify, dim3(id_grid), dim3(256), 0, s, c->table, c->table_buckets, t,
+                       c->cfg.min_meet_time, c->cfg.max_meet_time, c->end_time_min, c->map, mt, cnt);
+    CK(hipEventRecord(c->ev[9], s));
     CK(hipGetLastError());
     CKRC(sync_counters(c));
     h = *c->hcnt;
@@ -1166,6 +1269,10 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
         hipEventElapsedTime(&out->ms_meetings, c->ev[3], c->ev[4]);
         hipEventElapsedTime(&out->ms_flush, c->ev[4], c->ev[5]);
         hipEventElapsedTime(&out->ms_total, c->ev[0], c->ev[5]);
+        hipEventElapsedTime(&out->ms_refine_kernel, c->ev[6], c->ev[7]);
+        hipEventElapsedTime(&out->ms_identify_kernel, c->ev[8], c->ev[9]);
+        out->continuing = h_cont;
+        out->new_keys = c->last_new;
     }
     return STC_OK;
 }
@@ -1190,10 +1297,12 @@ int stc_get_pairs(void *vc, uint64_t *keys, uint64_t cap, uint64_t *n) {
     StcCtx *c = (StcCtx *)vc;
     if (!c || !n) return STC_E_ARG;
     if (!c->stepped) return fail(c, STC_E_STATE, "no step processed");
-    std::vector<PairHit> ph(c->last_pairs);
-    if (c->last_pairs) CK(hipMemcpy(ph.data(), c->pairs.p, c->last_pairs * sizeof(PairHit), hipMemcpyDeviceToHost));
+    if (!(c->cfg.keep_pairs & 1)) return fail(c, STC_E_STATE, "pairs are kept only with keep_pairs bit 0 set");
+    std::vector<PairHit> ph(c->last_listed);
+    if (c->last_listed) CK(hipMemcpy(ph.data(), c->pairs.p, c->last_listed * sizeof(PairHit), hipMemcpyDeviceToHost));
     std::vector<uint64_t> k(ph.size());
     for (size_t i = 0; i < ph.size(); i++) k[i] = ph[i].key;
+    for (const PairHit &e : c->host_pairs_extra) k.push_back(e.key);
     std::sort(k.begin(), k.end());
     k.erase(std::unique(k.begin(), k.end()), k.end());
     *n = k.size();

@@ -22,7 +22,8 @@ def sorted_records(keys, boxes):
 
 
 def run_parity(w, steps, check_every=1, tag=""):
-    cfg, box = w["cfg"], w["box"]
+    cfg, box = dict(w["cfg"]), w["box"]
+    cfg["keep_pairs"] = 1
     print(f"[parity {tag}] n={cfg['num_objects']} cap={cfg['grid_capacity']} r={cfg['reach_distance']} "
           f"steps={steps}", flush=True)
     gpu = Pipeline(cfg, box)
@@ -104,7 +105,7 @@ def test_device_trace_matches_host_trace():
 
 
 def test_step_device_equals_step_host():
-    w = workload("c1", num_objects=4000)
+    w = workload("c1", num_objects=4000, keep_pairs=1)
     a = Pipeline(w["cfg"], w["box"])
     b = Pipeline(w["cfg"], w["box"])
     dt = DeviceTrace(w["gen"])
