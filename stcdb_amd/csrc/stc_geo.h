@@ -7,12 +7,15 @@
  * where dpm_lon(lat) = 360 / (sin((90 - |lat|) * PI / 180) * 40076 * 1000)
  * and PI = 3.14159265 (src/cuda/cuda_util.cuh:41-53, src/util/util.h:38, 59-66).
  * Every operation is a correctly rounded IEEE double operation once FMA
- * contraction is off, except sin. The CPU path uses the C library's sin, which
- * returns the correctly rounded result for these arguments; the device does
- * the same with stc_sin_dd below: sin evaluated in double-double arithmetic
- * (about 104 significant bits) and rounded once. Pair decisions are therefore
- * bit-identical between the GPU and the CPU oracle. dpm_lon depends only on
- * the probe point, so it is computed once per point per step.
+ * contraction is off, except sin. The device evaluates sin in double-double
+ * arithmetic (about 104 significant bits) and rounds once (stc_sin_dd below),
+ * which gives the correctly rounded sine. The C library's sin, which the CPU
+ * path uses, differs from the correctly rounded value by one ulp for about
+ * 0.1% of arguments; that moves a distance by at most a few ulp. The device
+ * therefore decides every pair whose distance lies more than 16 ulp away from
+ * r, and hands the pairs inside that band to the host, which re-evaluates them
+ * with the C library's sin: pair decisions equal the CPU formula's. dpm_lon
+ * depends only on the probe point, so it is computed once per point per step.
  *
  * Everything here must be compiled with -ffp-contract=off.
  */
@@ -30,6 +33,7 @@
 
 #define STC_PI_REF 3.14159265
 #define STC_DPM_LAT (360.0 / (40076.0 * 1000.0))
+#define STC_INV_DPM_LAT (40076.0 * 1000.0 / 360.0)
 
 /* (-1)^k / (2k+1)! as unevaluated double-double sums, k = 0..17 */
 #define STC_SIN_TERMS 18

@@ -157,7 +157,8 @@ __device__ inline uint64_t hash_slot(unsigned long long key, uint64_t nb) {
 struct MapBox { double lo0, lo1, hi0, hi1; };
 
 __global__ void k_prep(const double2 *__restrict__ xy, uint32_t n, MapBox m, uint32_t *__restrict__ codes,
-                       uint32_t *__restrict__ idx, double *__restrict__ dlon, Counters *cnt) {
+                       uint32_t *__restrict__ idx, double *__restrict__ dlon, double *__restrict__ idlon,
+                       Counters *cnt) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double2 p = xy[i];
@@ -171,7 +172,9 @@ __global__ void k_prep(const double2 *__restrict__ xy, uint32_t n, MapBox m, uin
     }
     codes[i] = code;
     idx[i] = i;
-    dlon[i] = stc_dpm_lon(p.y);
+    double d = stc_dpm_lon(p.y);
+    dlon[i] = d;
+    idlon[i] = 1.0 / d;
 }
 
 /* box of block (x, y) with xdim blocks per side (geometry.cu:1306-1311) */
@@ -451,22 +454,52 @@ __device__ inline Bucket *table_find_or_claim(Bucket *t, uint64_t nb, unsigned l
     return nullptr;
 }
 
-/* insert or extend one pair seen at `now` (geometry.cu:469-494, without the
- * 5-probe drop). Idempotent within a step: inserting the same pair twice at
- * the same `now` leaves the same state. Returns -1 when the table is full,
+/* Insert or extend one pair seen at `now` (geometry.cu:469-494, without the
+ * 5-probe drop). Within a step a key is updated only by the threads that
+ * found this very pair, and all of them carry the same two positions, so the
+ * bucket is read once and rewritten with plain stores: concurrent writers
+ * store identical values and the result is the same in any order. Only
+ * claiming an empty bucket needs a CAS. Returns -1 when the table is full,
  * 1 for a new key, 2 when it continues a meeting seen at now - 1, else 0. */
 __device__ inline int table_upsert(Bucket *t, uint64_t nb, unsigned long long key, const float *mbr, uint32_t now) {
-    bool claimed;
-    Bucket *b = table_find_or_claim(t, nb, key, &claimed);
-    if (!b) return -1;
-    uint32_t old_end = atomicExch(&b->end, now);
-    if (claimed || (old_end != now && old_end != now - 1)) b->start = now; /* new or restarted meeting */
-    atomicMin(&b->mbr[0], f2o(mbr[0]));
-    atomicMin(&b->mbr[1], f2o(mbr[1]));
-    atomicMax(&b->mbr[2], f2o(mbr[2]));
-    atomicMax(&b->mbr[3], f2o(mbr[3]));
-    if (claimed) return 1;
-    return (old_end == now - 1) ? 2 : 0;
+    uint64_t h = hash_slot(key, nb);
+    for (uint32_t probe = 0; probe < MAX_PROBES; probe++) {
+        Bucket *b = &t[h];
+        Bucket cur = *b;
+        bool claimed = false;
+        if (cur.key == EMPTY_KEY) {
+            unsigned long long prev = atomicCAS(&b->key, EMPTY_KEY, key);
+            if (prev == EMPTY_KEY) {
+                claimed = true;
+            } else if (prev == key) {
+                cur = *b;
+            } else {
+                h = (h + 1 == nb) ? 0 : h + 1;
+                continue;
+            }
+        } else if (cur.key != key) {
+            h = (h + 1 == nb) ? 0 : h + 1;
+            continue;
+        }
+        if (claimed) {
+            b->start = now;
+            b->end = now;
+            b->mbr[0] = f2o(mbr[0]);
+            b->mbr[1] = f2o(mbr[1]);
+            b->mbr[2] = f2o(mbr[2]);
+            b->mbr[3] = f2o(mbr[3]);
+            return 1;
+        }
+        uint32_t old_end = cur.end;
+        if (old_end != now && old_end != now - 1) b->start = now; /* restarted (retired key) or racing a claim */
+        b->end = now;
+        b->mbr[0] = min(cur.mbr[0], f2o(mbr[0]));
+        b->mbr[1] = min(cur.mbr[1], f2o(mbr[1]));
+        b->mbr[2] = max(cur.mbr[2], f2o(mbr[2]));
+        b->mbr[3] = max(cur.mbr[3], f2o(mbr[3]));
+        return (old_end == now - 1) ? 2 : 0;
+    }
+    return -1;
 }
 
 /* insert a list of pairs (host-resolved ambiguous pairs) */
@@ -485,11 +518,12 @@ __global__ void k_table_insert(Bucket *t, uint64_t nb, const PairHit *__restrict
 /* ------------------------------------------------------------------ */
 
 __global__ __launch_bounds__(256) void k_refine(
-    const double2 *__restrict__ xy, const double *__restrict__ dlon, uint32_t n, const uint32_t *__restrict__ pt_leaf,
-    const uint32_t *__restrict__ pt_rank, const uint32_t *__restrict__ leaf_start,
+    const double2 *__restrict__ xy, const double *__restrict__ dlon, const double *__restrict__ idlon, uint32_t n,
+    const uint32_t *__restrict__ pt_leaf, const uint32_t *__restrict__ pt_rank, const uint32_t *__restrict__ leaf_start,
     const uint32_t *__restrict__ leaf_count, const uint32_t *__restrict__ lp, const double *__restrict__ lx,
     const double *__restrict__ ly, const uint2 *__restrict__ units, uint64_t n_units_total, uint32_t buf, double r,
-    double band, Bucket *table, uint64_t nb, uint32_t now, PairHit *__restrict__ pairs, uint64_t pair_cap,
+    double band, double r2_lo, double r2_hi, Bucket *table, uint64_t nb, uint32_t now,
+    PairHit *__restrict__ pairs, uint64_t pair_cap,
     uint2 *__restrict__ amb, uint32_t amb_cap, Counters *cnt) {
     const int lane = threadIdx.x & (STC_WAVE - 1);
     const uint64_t wid = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / STC_WAVE;
@@ -514,6 +548,7 @@ __global__ __launch_bounds__(256) void k_refine(
         uint32_t st = leaf_start[leaf];
         double2 pp = xy[p];
         double pd = dlon[p];
+        double pinv = idlon[p];
         float pfx = (float)pp.x, pfy = (float)pp.y;
         for (uint32_t m0 = begin; m0 < s; m0 += STC_WAVE) {
             uint32_t m = m0 + lane;
@@ -521,12 +556,22 @@ __global__ __launch_bounds__(256) void k_refine(
             uint32_t q = 0;
             double qx = 0, qy = 0;
             if (m < s) {
-                q = lp[st + m];
                 qx = lx[st + m];
                 qy = ly[st + m];
-                double d = stc_point_dist(pp.x, pp.y, qx, qy, pd);
-                if (fabs(d - r) <= band) ambg = true;
-                else hit = d <= r;
+                /* squared distance with reciprocal multiplies; it is within
+                 * 1.2e-15 (relative) of the formula's, so only a band of 1e-12
+                 * around r^2 needs the exact divisions and square root */
+                double ax = (pp.x - qx) * pinv;
+                double ay = (pp.y - qy) * STC_INV_DPM_LAT;
+                double s2 = ax * ax + ay * ay;
+                if (s2 < r2_lo) {
+                    hit = true;
+                } else if (s2 <= r2_hi) {
+                    double d = stc_point_dist(pp.x, pp.y, qx, qy, pd);
+                    if (fabs(d - r) <= band) ambg = true;
+                    else hit = d <= r;
+                }
+                if (hit || ambg) q = lp[st + m];
                 cand++;
             }
             if (hit) {
@@ -688,13 +733,62 @@ __global__ __launch_bounds__(256) void k_identify(Bucket *t, uint64_t nb, uint32
     }
 }
 
-/* move the entries last seen at or after `keep_from` into a fresh table */
-__global__ void k_table_rehash(const Bucket *src, uint64_t nsrc, Bucket *dst, uint64_t ndst, uint32_t keep_from,
-                               Counters *cnt) {
+/* Rehash in three passes over the old table, chunked by workgroup:
+ * count the kept buckets per chunk, prefix-sum the counts, and compact the
+ * kept buckets in order into a staging array; the old table is then freed
+ * and the staged buckets are inserted into the new one. */
+#define RH_CHUNK 8192u
+
+__device__ inline bool keep_bucket(const Bucket &b, uint32_t keep_from) {
+    return b.key != EMPTY_KEY && b.end != 0xFFFFFFFFu && b.end >= keep_from;
+}
+
+__global__ __launch_bounds__(256) void k_stage_count(const Bucket *src, uint64_t nsrc, uint32_t keep_from,
+                                                     uint32_t *chunk_count) {
+    __shared__ uint32_t wsum[4];
+    uint64_t lo = (uint64_t)blockIdx.x * RH_CHUNK, hi = min(lo + RH_CHUNK, nsrc);
+    uint32_t c = 0;
+    for (uint64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) c += keep_bucket(src[i], keep_from);
+    for (int o = STC_WAVE / 2; o > 0; o >>= 1) c += __shfl_down(c, o, STC_WAVE);
+    if ((threadIdx.x & (STC_WAVE - 1)) == 0) wsum[threadIdx.x / STC_WAVE] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) chunk_count[blockIdx.x] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+}
+
+__global__ __launch_bounds__(256) void k_stage_write(const Bucket *src, uint64_t nsrc, uint32_t keep_from,
+                                                     const uint32_t *chunk_offset, Bucket *stage) {
+    __shared__ uint32_t wbase[5];
+    const int lane = threadIdx.x & (STC_WAVE - 1), w = threadIdx.x / STC_WAVE;
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (STC_WAVE - lane));
+    uint64_t lo = (uint64_t)blockIdx.x * RH_CHUNK, hi = min(lo + RH_CHUNK, nsrc);
+    uint64_t out = chunk_offset[blockIdx.x];
+    for (uint64_t base = lo; base < hi; base += blockDim.x) {
+        uint64_t i = base + threadIdx.x;
+        Bucket b;
+        bool k = false;
+        if (i < hi) {
+            b = src[i];
+            k = keep_bucket(b, keep_from);
+        }
+        uint64_t m = __ballot(k);
+        if (lane == 0) wbase[w] = (uint32_t)__popcll(m);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t acc = 0;
+            for (int q = 0; q < 4; q++) { uint32_t v = wbase[q]; wbase[q] = acc; acc += v; }
+            wbase[4] = acc;
+        }
+        __syncthreads();
+        if (k) stage[out + wbase[w] + __popcll(m & lt)] = b;
+        out += wbase[4];
+        __syncthreads();
+    }
+}
+
+__global__ void k_table_unstage(const Bucket *stage, uint64_t n, Bucket *dst, uint64_t ndst, Counters *cnt) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nsrc) return;
-    Bucket b = src[i];
-    if (b.key == EMPTY_KEY || b.end == 0xFFFFFFFFu || b.end < keep_from) return;
+    if (i >= n) return;
+    Bucket b = stage[i];
     bool claimed;
     Bucket *d = table_find_or_claim(dst, ndst, b.key, &claimed);
     if (!d) { atomicAdd(&cnt->table_fail, 1u); return; }
@@ -754,7 +848,7 @@ struct StcCtx {
 
     DBuf<double2> xy_in;
     DBuf<uint32_t> codes, idx, codes_alt, idx_alt, temp;
-    DBuf<double> dlon, lx, ly, ld;
+    DBuf<double> dlon, idlon, lx, ly, ld;
     DBuf<Node> nodes;
     DBuf<FrontierItem> frontier_a, frontier_b;
     DBuf<uint32_t> leaf_count, leaf_start, leaf_node;
@@ -789,6 +883,7 @@ struct StcCtx {
     uint64_t last_pairs = 0;
     std::vector<PairHit> host_pairs_extra;
     DBuf<PairHit> amb_pairs;
+    DBuf<uint32_t> chunk_count, chunk_offset;
     std::vector<RunHost> runs;
     FlushWork flushw;
     hipEvent_t ev[10];
@@ -856,19 +951,47 @@ static int ilog2_ceil(uint64_t v) {
 }
 
 /* move the entries last seen at or after keep_from into a table sized for
- * expected_keys at a load of at most 0.4 (and no smaller than the current one) */
+ * expected_keys at a load of at most 0.4 (see the kernels above) */
 static int table_rehash(StcCtx *c, uint32_t keep_from, uint64_t expected_keys) {
-    uint64_t nb = std::max<uint64_t>(c->table_buckets, expected_keys * 5 / 2 + 1024);
+    hipStream_t s = c->s;
+    uint64_t nold = c->table_buckets;
+    uint32_t chunks = (uint32_t)((nold + RH_CHUNK - 1) / RH_CHUNK);
+    if (c->chunk_count.ensure(chunks + 1) || c->chunk_offset.ensure(chunks + 1))
+        return fail(c, STC_E_CAPACITY, "rehash chunk counts");
+    hipLaunchKernelGGL(k_stage_count, dim3(chunks), dim3(256), 0, s, c->table, nold, keep_from, c->chunk_count.p);
+    CK(hipGetLastError());
+    CKRC(stc_exclusive_scan_u32(c->chunk_count.p, c->chunk_offset.p, chunks, c->chunk_offset.p + chunks, c->temp.p,
+                                s));
+    uint32_t kept = 0;
+    CK(hipMemcpyAsync(&kept, c->chunk_offset.p + chunks, 4, hipMemcpyDeviceToHost, s));
+    CK(hipStreamSynchronize(s));
+    Bucket *stage = nullptr;
+    if (kept) {
+        CK(hipMalloc((void **)&stage, (uint64_t)kept * sizeof(Bucket)));
+        hipLaunchKernelGGL(k_stage_write, dim3(chunks), dim3(256), 0, s, c->table, nold, keep_from,
+                           c->chunk_offset.p, stage);
+        CK(hipGetLastError());
+        CK(hipStreamSynchronize(s));
+    }
+    CK(hipFree(c->table));
+    c->table = nullptr;
+    uint64_t nb = std::max<uint64_t>(std::max<uint64_t>(expected_keys, kept) * 5 / 2 + 1024, 1024);
     Bucket *nt = nullptr;
     int rc = alloc_table(c, nb, &nt);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_table_rehash, dim3(blocks_for(c->table_buckets, 256)), dim3(256), 0, c->s, c->table,
-                       c->table_buckets, nt, nb, keep_from, c->dcnt);
-    CK(hipGetLastError());
-    CK(hipStreamSynchronize(c->s));
-    CK(hipFree(c->table));
+    if (rc) {
+        if (stage) (void)hipFree(stage);
+        return rc;
+    }
     c->table = nt;
     c->table_buckets = nb;
+    if (kept) {
+        hipLaunchKernelGGL(k_table_unstage, dim3(blocks_for(kept, 256)), dim3(256), 0, s, stage, (uint64_t)kept, nt,
+                           nb, c->dcnt);
+        CK(hipGetLastError());
+        CK(hipStreamSynchronize(s));
+        CK(hipFree(stage));
+    }
+    c->occupied = kept;
     return 0;
 }
 
@@ -934,7 +1057,7 @@ int stc_create(const stc_config *cfg, const stc_box *mbr, void **out) {
     size_t node_cap = 1 + 4ull * NLEVELS * (n / (c->cap + 1) + 1);
     size_t front_cap = n / (c->cap + 1) + 2;
     if (c->xy_in.ensure(n) || c->codes.ensure(n) || c->idx.ensure(n) || c->codes_alt.ensure(n) ||
-        c->idx_alt.ensure(n) || c->dlon.ensure(n) || c->lx.ensure(n) || c->ly.ensure(n) || c->ld.ensure(n) ||
+        c->idx_alt.ensure(n) || c->dlon.ensure(n) || c->idlon.ensure(n) || c->lx.ensure(n) || c->ly.ensure(n) || c->ld.ensure(n) ||
         c->nodes.ensure(node_cap) || c->frontier_a.ensure(front_cap) || c->frontier_b.ensure(front_cap) ||
         c->pt_leaf.ensure(n) || c->pt_lv.ensure(n) || c->pt_rank.ensure(n) || c->lp.ensure(n) ||
         c->lkeys.ensure(n) || c->ucount.ensure(n) || c->uoffset.ensure(n) || c->units.ensure(4ull * n) ||
@@ -967,12 +1090,14 @@ int stc_destroy(void *vc) {
     if (!c) return STC_OK;
     if (c->s) hipStreamSynchronize(c->s);
     c->xy_in.release(); c->codes.release(); c->idx.release(); c->codes_alt.release(); c->idx_alt.release();
-    c->temp.release(); c->dlon.release(); c->lx.release(); c->ly.release(); c->ld.release(); c->nodes.release();
+    c->temp.release(); c->dlon.release(); c->idlon.release(); c->lx.release(); c->ly.release(); c->ld.release(); c->nodes.release();
     c->frontier_a.release(); c->frontier_b.release(); c->leaf_count.release(); c->leaf_start.release();
     c->leaf_node.release(); c->pt_leaf.release(); c->pt_lv.release(); c->pt_rank.release(); c->lp.release();
     c->lkeys.release(); c->ucount.release(); c->uoffset.release(); c->units.release(); c->pairs.release();
     c->amb.release();
     c->amb_pairs.release();
+    c->chunk_count.release();
+    c->chunk_offset.release();
     if (c->table) hipFree(c->table);
     if (c->rec_keys) hipFree(c->rec_keys);
     if (c->rec_boxes) hipFree(c->rec_boxes);
@@ -1014,7 +1139,7 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
 
     /* 1-2: cells, Morton codes, dpm_lon; sort codes */
     hipLaunchKernelGGL(k_prep, dim3(blocks_for(n, 256)), dim3(256), 0, s, xy, n, c->map, c->codes.p, c->idx.p,
-                       c->dlon.p, cnt);
+                       c->dlon.p, c->idlon.p, cnt);
     CK(hipGetLastError());
     CKRC(stc_radix_sort_u32(c->codes.p, c->idx.p, c->codes_alt.p, c->idx_alt.p, n, 28, c->temp.p, s));
 
@@ -1082,10 +1207,9 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
     /* 7-8: refinement fused with the meeting-table insert. First make room:
      * after this step's new keys at most half of the buckets hold keys. */
     uint32_t rebuilt = 0;
-    uint64_t est_new = c->stepped ? 2 * c->last_new + 1024 : (uint64_t)n;
-    if ((c->occupied + est_new) * 2 > c->table_buckets) {
+    uint64_t est_new = c->stepped ? c->last_new / 2 + 1024 : (uint64_t)n;
+    if ((c->occupied + est_new) * 10 > c->table_buckets * 7) {
         CKRC(table_rehash(c, t - 1, c->last_live + est_new));
-        c->occupied = c->last_live;
         rebuilt = 1;
     }
     const bool keep = (c->cfg.keep_pairs & 1) != 0;
@@ -1095,9 +1219,10 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
         CK(hipMemsetAsync(&cnt->candidates, 0, 6 * sizeof(unsigned long long), s)); /* candidates .. continuing */
         CK(hipMemsetAsync(&cnt->table_fail, 0, sizeof(uint32_t), s));
         CK(hipEventRecord(c->ev[6], s));
-        hipLaunchKernelGGL(k_refine, dim3(grid), dim3(256), 0, s, xy, c->dlon.p, n, c->pt_leaf.p, c->pt_rank.p,
-                           c->leaf_start.p, c->leaf_count.p, c->lp.p, c->lx.p, c->ly.p, c->units.p, total_units,
-                           c->buf, r, c->band, c->table, c->table_buckets, t, keep ? c->pairs.p : (PairHit *)nullptr,
+        hipLaunchKernelGGL(k_refine, dim3(grid), dim3(256), 0, s, xy, c->dlon.p, c->idlon.p, n, c->pt_leaf.p,
+                           c->pt_rank.p, c->leaf_start.p, c->leaf_count.p, c->lp.p, c->lx.p, c->ly.p, c->units.p,
+                           total_units, c->buf, r, c->band, r * r * (1.0 - 1e-12), r * r * (1.0 + 1e-12), c->table,
+                           c->table_buckets, t, keep ? c->pairs.p : (PairHit *)nullptr,
                            (uint64_t)c->pairs.cap, c->amb.p, (uint32_t)c->amb.cap, cnt);
         CK(hipGetLastError());
         CK(hipEventRecord(c->ev[7], s));
@@ -1109,6 +1234,7 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
             CKRC(table_rehash(c, t - 1, c->last_live + h.pairs + 1024));
             rebuilt = 1;
             redo = true;
+            CK(hipMemsetAsync(&cnt->table_fail, 0, sizeof(uint32_t), s));
         }
         if (keep && h.pairs_listed > c->pairs.cap) {
             if (c->pairs.ensure(h.pairs_listed + 1024)) return fail(c, STC_E_CAPACITY, "pair list allocation failed");
@@ -1183,13 +1309,13 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
     c->oversize_oid_count = h.oversize_oid_count;
     c->last_live = h.live;
     uint64_t occupied = h.live + h.expired + h.retired;
+    c->occupied = occupied;
     /* drop retired keys once live and retired keys fill half the table */
     if (occupied * 2 > c->table_buckets) {
         CKRC(table_rehash(c, t, h.live));
         rebuilt = 1;
-        occupied = h.live;
+        occupied = c->occupied;
     }
-    c->occupied = occupied;
     CK(hipEventRecord(c->ev[4], s));
 
     /* 10: flush */
