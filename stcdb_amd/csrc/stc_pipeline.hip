@@ -54,7 +54,15 @@
 #define NLEVELS 15
 #define T_BRANCH 1u
 #define T_LEAF 2u
+/* Bucket keys are pid << 32 | target with pid, target < 2^26. Keys whose high
+ * word is all ones mark free buckets (the low word tags the identify pass that
+ * freed it; a cleared table holds EMPTY_KEY); high word 0xFFFFFFFE marks a
+ * tombstone, an occupied bucket that matches no key. */
 #define EMPTY_KEY 0xFFFFFFFFFFFFFFFFull
+#define FREE_KEY(tag) (0xFFFFFFFF00000000ull | (unsigned long long)(uint32_t)(tag))
+#define TOMB_KEY(tag) (0xFFFFFFFE00000000ull | (unsigned long long)(uint32_t)(tag))
+__host__ __device__ inline bool key_free(unsigned long long k) { return (k >> 32) == 0xFFFFFFFFull; }
+__host__ __device__ inline bool key_tomb(unsigned long long k) { return (k >> 32) == 0xFFFFFFFEull; }
 #define STACK_DEPTH 96
 
 static thread_local std::string g_last_fail;
@@ -100,7 +108,7 @@ struct Counters {             /* device-side counters, copied back once per sync
     unsigned long long new_keys;
     unsigned long long continuing;
     unsigned long long ambiguous_nodes;
-    unsigned long long live, expired, retired;
+    unsigned long long live, expired, retired, freed, tombs;
     unsigned long long kv_count;
     uint32_t nodes, leaves;
     uint32_t frontier[NLEVELS + 1];
@@ -444,9 +452,9 @@ __device__ inline Bucket *table_find_or_claim(Bucket *t, uint64_t nb, unsigned l
         Bucket *b = &t[h];
         unsigned long long k = __atomic_load_n(&b->key, __ATOMIC_RELAXED);
         if (k == key) { *claimed = false; return b; }
-        if (k == EMPTY_KEY) {
-            unsigned long long prev = atomicCAS(&b->key, EMPTY_KEY, key);
-            if (prev == EMPTY_KEY) { *claimed = true; return b; }
+        if (key_free(k)) {
+            unsigned long long prev = atomicCAS(&b->key, k, key);
+            if (prev == k) { *claimed = true; return b; }
             if (prev == key) { *claimed = false; return b; }
         }
         h = (h + 1 == nb) ? 0 : h + 1;
@@ -467,9 +475,9 @@ __device__ inline int table_upsert(Bucket *t, uint64_t nb, unsigned long long ke
         Bucket *b = &t[h];
         Bucket cur = *b;
         bool claimed = false;
-        if (cur.key == EMPTY_KEY) {
-            unsigned long long prev = atomicCAS(&b->key, EMPTY_KEY, key);
-            if (prev == EMPTY_KEY) {
+        if (key_free(cur.key)) {
+            unsigned long long prev = atomicCAS(&b->key, cur.key, key);
+            if (prev == cur.key) {
                 claimed = true;
             } else if (prev == key) {
                 cur = *b;
@@ -637,99 +645,182 @@ struct MemtableDev {
     uint16_t *sids;
 };
 
-#define ID_STAGE 128 /* per-wavefront staging of expired buckets that emit records */
+#define ID_STAGE 128 /* per-wavefront staging of records to emit */
 
-/* identify completed meetings (geometry.cu:595-737). Grid-stride over the
- * table; emitting buckets are staged per wavefront in LDS and written in
- * batches, so the memtable counter sees one atomic per batch. */
+/* write the two memtable records of each staged meeting (geometry.cu:641-705) */
+__device__ inline void emit_staged(const Bucket *st, uint32_t staged, int lane, uint32_t max_meet,
+                                   uint32_t end_time_min, MemtableDev mt, Counters *cnt) {
+    unsigned long long kbase = 0;
+    if (lane == 0) kbase = atomicAdd(&cnt->kv_count, 2ull * staged);
+    kbase = __shfl(kbase, 0, STC_WAVE);
+    for (uint32_t e = lane; e < staged; e += STC_WAVE) {
+        Bucket b = st[e];
+        float lo0 = o2f(b.mbr[0]), lo1 = o2f(b.mbr[1]), hi0 = o2f(b.mbr[2]), hi1 = o2f(b.mbr[3]);
+        uint32_t duration = b.end - b.start;
+        uint32_t pid = (uint32_t)(b.key >> 32), target = (uint32_t)b.key;
+        if (duration > max_meet) {
+            atomicAdd(&cnt->oversize_oid_count, 1u);
+            mt.sids[pid] = 1;
+            mt.sids[target] = 1;
+        }
+        unsigned long long k = kbase + 2ull * e;
+        uint32_t q0 = f2q(lo0), q1 = f2q(lo1), q2 = f2q(hi0), q3 = f2q(hi1);
+        for (int sdir = 0; sdir < 2; sdir++) {
+            uint32_t a = sdir ? target : pid, c = sdir ? pid : target;
+            if (k + sdir < mt.cap) {
+                mt.keys[2 * (k + sdir)] = ((uint64_t)duration << 32) | (uint32_t)(b.end - end_time_min);
+                mt.keys[2 * (k + sdir) + 1] = ((uint64_t)a << 26) | (uint64_t)c;
+                mt.boxes[k + sdir] = make_float4(lo0, lo1, hi0, hi1);
+            }
+            if (atomicAdd(&mt.same_pid_count[a], 1u) == 0) atomicAdd(&cnt->sid_count, 1u);
+            atomicMin(&mt.o_boxs[a].x, q0);
+            atomicMax(&mt.o_boxs[a].z, q2);
+            atomicMin(&mt.o_boxs[a].y, q1);
+            atomicMax(&mt.o_boxs[a].w, q3);
+        }
+    }
+}
+
+__device__ inline uint64_t slot_next(uint64_t j, uint64_t nb) { return (j + 1 == nb) ? 0 : j + 1; }
+
+__device__ inline void write_marker(Bucket *b, unsigned long long key) {
+    Bucket x;
+    x.key = key;
+    x.start = 0;
+    x.end = 0xFFFFFFFFu;
+    x.mbr[0] = x.mbr[1] = 0xFFFFFFFFu;
+    x.mbr[2] = x.mbr[3] = 0u;
+    *b = x;
+}
+
+/* identify completed meetings (geometry.cu:595-737) and delete every bucket
+ * that is not live at `now` from the table.
+ *
+ * Deletion is linear probing's backward shift, run per probe cluster (a
+ * maximal run of occupied buckets). Each cluster is walked by the thread
+ * whose bucket is the free slot just before it: only buckets that were free
+ * when the pass began start a walk, and those are never written during the
+ * pass (buckets freed by the pass carry the tag `now`, so they start no walk).
+ * The walker visits the cluster in probe order, turns each dead bucket into a
+ * hole, and moves each live bucket into the first hole at or after its home
+ * slot, so every remaining key keeps an unbroken probe chain. Holes are kept
+ * in a 64-slot window behind the walker; a hole that falls out of the window
+ * becomes a tombstone (occupied, never matches a key, deleted by a later
+ * pass). At the cluster's end the remaining holes become free buckets.
+ *
+ * Walkers read their own cluster only, and the buckets of a wavefront's
+ * 64-slot chunk come from one coalesced load staged in LDS. Expiring buckets
+ * that emit records are staged per wavefront and written in batches, so the
+ * memtable counter sees one atomic per batch. */
 __global__ __launch_bounds__(256) void k_identify(Bucket *t, uint64_t nb, uint32_t now, uint32_t min_meet,
                                                   uint32_t max_meet, uint32_t end_time_min, MapBox m,
                                                   MemtableDev mt, Counters *cnt) {
-    __shared__ uint64_t stage[256 / STC_WAVE][ID_STAGE];
+    __shared__ Bucket chunk[256 / STC_WAVE][STC_WAVE];
+    __shared__ Bucket stage[256 / STC_WAVE][ID_STAGE];
     const int lane = threadIdx.x & (STC_WAVE - 1), w = threadIdx.x / STC_WAVE;
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (STC_WAVE - lane));
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    unsigned long long nlive = 0, nexp = 0, nret = 0;
+    const unsigned long long free_now = FREE_KEY(now);
+    const unsigned long long tomb = TOMB_KEY(now);
+    unsigned long long nlive = 0, nexp = 0, nret = 0, nfreed = 0, ntomb = 0;
     uint32_t staged = 0; /* wave-uniform */
-    const uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x + (uint64_t)w * STC_WAVE;
-    for (uint64_t base = i0; base < nb; base += stride) {
-        uint64_t i = base + lane;
-        bool emit = false, retire_now = false;
+    for (uint64_t c0 = (uint64_t)blockIdx.x * blockDim.x + (uint64_t)w * STC_WAVE; c0 < nb; c0 += stride) {
+        const uint64_t i = c0 + lane;
+        bool walking = false;
         if (i < nb) {
             Bucket b = t[i];
-            if (b.key != EMPTY_KEY) {
-                if (b.end == now) {
-                    nlive++;
-                } else if (b.end == now - 1) {
-                    nexp++;
-                    float lo0 = o2f(b.mbr[0]), lo1 = o2f(b.mbr[1]), hi0 = o2f(b.mbr[2]), hi1 = o2f(b.mbr[3]);
-                    emit = (now - b.start >= min_meet + 1 && (double)lo0 >= m.lo0 && (double)hi0 <= m.hi0 &&
-                            (double)lo1 >= m.lo1 && (double)hi1 <= m.hi1);
-                    retire_now = !emit;
-                } else {
-                    nret++;
-                }
-            }
+            chunk[w][lane] = b;
+            walking = key_free(b.key) && b.key != free_now;
         }
-        /* retire expired buckets: the key stays for the probe chains, the MBR
-         * resets for a later restart. Emitting buckets retire after their
-         * records are written. */
-        if (retire_now) {
-            t[i].mbr[0] = 0xFFFFFFFFu;
-            t[i].mbr[1] = 0xFFFFFFFFu;
-            t[i].mbr[2] = 0u;
-            t[i].mbr[3] = 0u;
-        }
-        uint64_t em = __ballot(emit);
-        if (emit) stage[w][staged + __popcll(em & lt)] = i;
-        staged += (uint32_t)__popcll(em);
-        bool flush = staged > ID_STAGE - STC_WAVE || (base + stride >= nb);
         __builtin_amdgcn_wave_barrier();
-        if (flush && staged) {
-            unsigned long long kbase = 0;
-            if (lane == 0) kbase = atomicAdd(&cnt->kv_count, 2ull * staged);
-            kbase = __shfl(kbase, 0, STC_WAVE);
-            for (uint32_t e = lane; e < staged; e += STC_WAVE) {
-                uint64_t bi = stage[w][e];
-                Bucket b = t[bi];
-                float lo0 = o2f(b.mbr[0]), lo1 = o2f(b.mbr[1]), hi0 = o2f(b.mbr[2]), hi1 = o2f(b.mbr[3]);
-                uint32_t duration = b.end - b.start;
-                uint32_t pid = (uint32_t)(b.key >> 32), target = (uint32_t)b.key;
-                if (duration > max_meet) {
-                    atomicAdd(&cnt->oversize_oid_count, 1u);
-                    mt.sids[pid] = 1;
-                    mt.sids[target] = 1;
-                }
-                unsigned long long k = kbase + 2ull * e;
-                uint32_t q0 = f2q(lo0), q1 = f2q(lo1), q2 = f2q(hi0), q3 = f2q(hi1);
-                for (int sdir = 0; sdir < 2; sdir++) {
-                    uint32_t a = sdir ? target : pid, c = sdir ? pid : target;
-                    if (k + sdir < mt.cap) {
-                        mt.keys[2 * (k + sdir)] = ((uint64_t)duration << 32) | (uint32_t)(b.end - end_time_min);
-                        mt.keys[2 * (k + sdir) + 1] = ((uint64_t)a << 26) | (uint64_t)c;
-                        mt.boxes[k + sdir] = make_float4(lo0, lo1, hi0, hi1);
+        uint64_t j = slot_next(i, nb), wb = j, holes = 0;
+        uint32_t rel = 0; /* j - wb */
+        while (__ballot(walking)) {
+            bool emit = false;
+            Bucket b;
+            if (walking) {
+                uint64_t d = j - c0; /* j < c0 wraps to a large value and is read from memory */
+                b = (d < STC_WAVE) ? chunk[w][d] : t[j];
+                if (key_free(b.key)) {
+                    /* end of the cluster: the remaining holes become free buckets */
+                    while (holes) {
+                        int q = __builtin_ctzll(holes);
+                        holes &= holes - 1;
+                        uint64_t dst = wb + q;
+                        if (dst >= nb) dst -= nb;
+                        write_marker(&t[dst], free_now);
+                        nfreed++;
                     }
-                    if (atomicAdd(&mt.same_pid_count[a], 1u) == 0) atomicAdd(&cnt->sid_count, 1u);
-                    atomicMin(&mt.o_boxs[a].x, q0);
-                    atomicMax(&mt.o_boxs[a].z, q2);
-                    atomicMin(&mt.o_boxs[a].y, q1);
-                    atomicMax(&mt.o_boxs[a].w, q3);
+                    walking = false;
+                } else {
+                    if (rel == STC_WAVE) { /* slide the hole window by one slot */
+                        if (holes & 1ull) {
+                            write_marker(&t[wb], tomb);
+                            ntomb++;
+                        }
+                        holes >>= 1;
+                        wb = slot_next(wb, nb);
+                        rel--;
+                    }
+                    if (b.end == now && !key_tomb(b.key)) {
+                        nlive++;
+                        if (holes) {
+                            uint64_t h = hash_slot(b.key, nb);
+                            uint64_t back = (j >= h) ? j - h : j + nb - h; /* probe distance of this bucket */
+                            uint64_t mask = holes;
+                            if (back < rel) mask &= ~0ull << (rel - back);
+                            if (mask) {
+                                int q = __builtin_ctzll(mask);
+                                uint64_t dst = wb + q;
+                                if (dst >= nb) dst -= nb;
+                                t[dst] = b;
+                                holes ^= 1ull << q;
+                                holes |= 1ull << rel;
+                            }
+                        }
+                    } else {
+                        if (!key_tomb(b.key) && b.end == now - 1) {
+                            nexp++;
+                            float lo0 = o2f(b.mbr[0]), lo1 = o2f(b.mbr[1]), hi0 = o2f(b.mbr[2]),
+                                  hi1 = o2f(b.mbr[3]);
+                            emit = (now - b.start >= min_meet + 1 && (double)lo0 >= m.lo0 && (double)hi0 <= m.hi0 &&
+                                    (double)lo1 >= m.lo1 && (double)hi1 <= m.hi1);
+                        } else {
+                            nret++;
+                        }
+                        holes |= 1ull << rel;
+                    }
+                    j = slot_next(j, nb);
+                    rel++;
                 }
-                t[bi].mbr[0] = 0xFFFFFFFFu;
-                t[bi].mbr[1] = 0xFFFFFFFFu;
-                t[bi].mbr[2] = 0u;
-                t[bi].mbr[3] = 0u;
             }
-            staged = 0;
+            uint64_t em = __ballot(emit);
+            if (emit) stage[w][staged + __popcll(em & lt)] = b;
+            staged += (uint32_t)__popcll(em);
+            if (staged > ID_STAGE - STC_WAVE) {
+                __builtin_amdgcn_wave_barrier();
+                emit_staged(stage[w], staged, lane, max_meet, end_time_min, mt, cnt);
+                staged = 0;
+                __builtin_amdgcn_wave_barrier();
+            }
         }
+        __builtin_amdgcn_wave_barrier(); /* the chunk buffer is reloaded next iteration */
+    }
+    if (staged) {
+        __builtin_amdgcn_wave_barrier();
+        emit_staged(stage[w], staged, lane, max_meet, end_time_min, mt, cnt);
     }
     nlive = wave_sum_u64(nlive);
     nexp = wave_sum_u64(nexp);
     nret = wave_sum_u64(nret);
+    nfreed = wave_sum_u64(nfreed);
+    ntomb = wave_sum_u64(ntomb);
     if (lane == 0) {
         if (nlive) atomicAdd(&cnt->live, nlive);
         if (nexp) atomicAdd(&cnt->expired, nexp);
         if (nret) atomicAdd(&cnt->retired, nret);
+        if (nfreed) atomicAdd(&cnt->freed, nfreed);
+        if (ntomb) atomicAdd(&cnt->tombs, ntomb);
     }
 }
 
@@ -740,7 +831,7 @@ __global__ __launch_bounds__(256) void k_identify(Bucket *t, uint64_t nb, uint32
 #define RH_CHUNK 8192u
 
 __device__ inline bool keep_bucket(const Bucket &b, uint32_t keep_from) {
-    return b.key != EMPTY_KEY && b.end != 0xFFFFFFFFu && b.end >= keep_from;
+    return !key_free(b.key) && !key_tomb(b.key) && b.end != 0xFFFFFFFFu && b.end >= keep_from;
 }
 
 __global__ __launch_bounds__(256) void k_stage_count(const Bucket *src, uint64_t nsrc, uint32_t keep_from,
@@ -884,6 +975,7 @@ struct StcCtx {
     std::vector<PairHit> host_pairs_extra;
     DBuf<PairHit> amb_pairs;
     DBuf<uint32_t> chunk_count, chunk_offset;
+    DBuf<Bucket> stage;
     std::vector<RunHost> runs;
     FlushWork flushw;
     hipEvent_t ev[10];
@@ -950,8 +1042,10 @@ static int ilog2_ceil(uint64_t v) {
     return l;
 }
 
-/* move the entries last seen at or after keep_from into a table sized for
- * expected_keys at a load of at most 0.4 (see the kernels above) */
+/* Move the entries last seen at or after keep_from into a table sized for
+ * expected_keys at a load of at most 0.4 (see the kernels above). The kept
+ * entries go to a staging buffer that stays allocated between rehashes; the
+ * table is refilled in place unless it must grow. */
 static int table_rehash(StcCtx *c, uint32_t keep_from, uint64_t expected_keys) {
     hipStream_t s = c->s;
     uint64_t nold = c->table_buckets;
@@ -965,32 +1059,33 @@ static int table_rehash(StcCtx *c, uint32_t keep_from, uint64_t expected_keys) {
     uint32_t kept = 0;
     CK(hipMemcpyAsync(&kept, c->chunk_offset.p + chunks, 4, hipMemcpyDeviceToHost, s));
     CK(hipStreamSynchronize(s));
-    Bucket *stage = nullptr;
     if (kept) {
-        CK(hipMalloc((void **)&stage, (uint64_t)kept * sizeof(Bucket)));
+        if (c->stage.ensure(kept)) return fail(c, STC_E_CAPACITY, "rehash staging buffer");
         hipLaunchKernelGGL(k_stage_write, dim3(chunks), dim3(256), 0, s, c->table, nold, keep_from,
-                           c->chunk_offset.p, stage);
+                           c->chunk_offset.p, c->stage.p);
         CK(hipGetLastError());
+    }
+    uint64_t want = std::max<uint64_t>(std::max<uint64_t>(expected_keys, kept) * 5 / 2 + 1024, 1024);
+    uint64_t nb = nold;
+    if (want > nold) {
         CK(hipStreamSynchronize(s));
+        CK(hipFree(c->table));
+        c->table = nullptr;
+        Bucket *nt = nullptr;
+        CKRC(alloc_table(c, want, &nt));
+        c->table = nt;
+        nb = want;
+    } else {
+        hipLaunchKernelGGL(k_table_clear, dim3(blocks_for(nb, 256)), dim3(256), 0, s, c->table, nb);
+        CK(hipGetLastError());
     }
-    CK(hipFree(c->table));
-    c->table = nullptr;
-    uint64_t nb = std::max<uint64_t>(std::max<uint64_t>(expected_keys, kept) * 5 / 2 + 1024, 1024);
-    Bucket *nt = nullptr;
-    int rc = alloc_table(c, nb, &nt);
-    if (rc) {
-        if (stage) (void)hipFree(stage);
-        return rc;
-    }
-    c->table = nt;
     c->table_buckets = nb;
     if (kept) {
-        hipLaunchKernelGGL(k_table_unstage, dim3(blocks_for(kept, 256)), dim3(256), 0, s, stage, (uint64_t)kept, nt,
-                           nb, c->dcnt);
+        hipLaunchKernelGGL(k_table_unstage, dim3(blocks_for(kept, 256)), dim3(256), 0, s, c->stage.p,
+                           (uint64_t)kept, c->table, nb, c->dcnt);
         CK(hipGetLastError());
-        CK(hipStreamSynchronize(s));
-        CK(hipFree(stage));
     }
+    CK(hipStreamSynchronize(s));
     c->occupied = kept;
     return 0;
 }
@@ -1098,6 +1193,7 @@ int stc_destroy(void *vc) {
     c->amb_pairs.release();
     c->chunk_count.release();
     c->chunk_offset.release();
+    c->stage.release();
     if (c->table) hipFree(c->table);
     if (c->rec_keys) hipFree(c->rec_keys);
     if (c->rec_boxes) hipFree(c->rec_boxes);
@@ -1208,7 +1304,7 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
      * after this step's new keys at most half of the buckets hold keys. */
     uint32_t rebuilt = 0;
     uint64_t est_new = c->stepped ? c->last_new / 2 + 1024 : (uint64_t)n;
-    if ((c->occupied + est_new) * 10 > c->table_buckets * 7) {
+    if ((c->occupied + est_new) * 10 > c->table_buckets * 6) {
         CKRC(table_rehash(c, t - 1, c->last_live + est_new));
         rebuilt = 1;
     }
@@ -1308,10 +1404,10 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
     c->sid_count = h.sid_count;
     c->oversize_oid_count = h.oversize_oid_count;
     c->last_live = h.live;
-    uint64_t occupied = h.live + h.expired + h.retired;
+    uint64_t occupied = h.live + h.tombs;
     c->occupied = occupied;
-    /* drop retired keys once live and retired keys fill half the table */
-    if (occupied * 2 > c->table_buckets) {
+    /* rebuild once live keys and tombstones fill 60% of the table */
+    if (occupied * 10 > c->table_buckets * 6) {
         CKRC(table_rehash(c, t, h.live));
         rebuilt = 1;
         occupied = c->occupied;
@@ -1406,6 +1502,8 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
 int stc_step_device(void *vc, const double *xy_dev, uint32_t t, stc_step_out *out) {
     StcCtx *c = (StcCtx *)vc;
     if (!c || !xy_dev) return STC_E_ARG;
+    if (t >= 0xFFFFFFFEu) return fail(c, STC_E_ARG, "timestep out of range");
+    if (c->stepped && t <= c->last_t) return fail(c, STC_E_STATE, "timesteps must increase");
     uint64_t kv_before = c->kv_count;
     int rc = step_impl(c, (const double2 *)xy_dev, t, out);
     if (rc == STC_OK && out) out->new_records = out->kv_count - kv_before;
@@ -1444,7 +1542,7 @@ int stc_get_active(void *vc, uint64_t *keys, uint32_t *start, uint32_t *end, flo
     CK(hipMemcpy(tb.data(), c->table, c->table_buckets * sizeof(Bucket), hipMemcpyDeviceToHost));
     std::vector<Bucket> live;
     for (const Bucket &b : tb)
-        if (b.key != EMPTY_KEY && b.end == c->last_t) live.push_back(b);
+        if (!key_free(b.key) && !key_tomb(b.key) && b.end == c->last_t) live.push_back(b);
     std::sort(live.begin(), live.end(), [](const Bucket &a, const Bucket &b) { return a.key < b.key; });
     *n = live.size();
     for (uint64_t i = 0; i < std::min<uint64_t>(cap, live.size()); i++) {
