@@ -93,7 +93,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=12)
     ap.add_argument("--workload", default="c3")
     ap.add_argument("--objects", type=int, default=0, help="override the object count")
     ap.add_argument("--cpu-steps", type=int, default=2)
@@ -129,6 +129,7 @@ def main():
     pipe = Pipeline(cfg, w["box"])
     for t in range(args.warmup):
         r = pipe.step_device(ctypes.c_void_p(frames[t].data_ptr()), t)
+        pipe.drop_runs()
         print(f"[bench] warmup step {t}: {r['ms_total']:.1f} ms, {r['pairs']} pairs", file=sys.stderr, flush=True)
 
     outs = []
@@ -136,6 +137,9 @@ def main():
     t0 = time.perf_counter()
     for t in range(args.warmup, total):
         outs.append(pipe.step_device(ctypes.c_void_p(frames[t].data_ptr()), t))
+        # flushed runs go to the writer: their pinned copies finish in the background
+        pipe.drop_runs()
+    pipe.sync_runs()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     for i, r in enumerate(outs):

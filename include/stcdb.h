@@ -170,6 +170,10 @@ int stc_get_run(void *ctx, uint32_t run, stc_ctf_meta *ctfs, uint8_t *key_bytes,
                 uint16_t *sids, uint64_t *o_keys, float *o_boxes, uint8_t *o_bitmap,
                 uint64_t *sorted_keys, float *sorted_boxes);
 int stc_drop_runs(void *ctx); /* release host copies of flushed runs */
+/* Wait until all work of the context is done, including the background copies
+ * of flushed runs into host memory (a flush hands its run to a copy stream so
+ * that the copy overlaps the following steps; stc_get_run waits for its own run). */
+int stc_sync(void *ctx);
 
 /* ---- seeded synthetic traces (the reference's generator is unseeded) ----
  * Parameter and state layouts are those of stcdb_amd/csrc/stc_gen.h. */

@@ -178,6 +178,7 @@ SIGNATURES = {
     "stc_get_run_info": (C.c_int, [P, C.c_uint32, C.POINTER(StcRunInfo)]),
     "stc_get_run": (C.c_int, [P, C.c_uint32, P, P, P, P, P, P, P, P, P]),
     "stc_drop_runs": (C.c_int, [P]),
+    "stc_sync": (C.c_int, [P]),
     "stc_gen_init_host": (C.c_int, [C.POINTER(StcGenParams), P, C.c_uint32, C.c_uint32]),
     "stc_gen_step_host": (C.c_int, [C.POINTER(StcGenParams), P, C.c_uint32, C.c_uint32]),
     "stc_gen_pos_host": (C.c_int, [C.POINTER(StcGenParams), P, C.c_uint32, P]),

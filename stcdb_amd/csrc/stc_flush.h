@@ -32,6 +32,14 @@ struct FlushWork {
     float4 *tboxes = nullptr;
     uint8_t *key_bytes = nullptr;
     uint64_t key_bytes_cap = 0;
+    /* run hand-off: the payload is copied to pinned host memory on `cs`,
+     * overlapping the following steps; the next flush waits for `copied`
+     * before it rewrites the buffers the copy reads */
+    uint16_t *sids_out = nullptr;                  /* sids as of the flush */
+    uint8_t *o_bitmap_dev = nullptr;               /* oversize bitmap as of the flush */
+    hipStream_t cs = nullptr;
+    hipEvent_t ready = nullptr, copied = nullptr;
+    bool pending = false;
 };
 
 struct FlushArgs {
@@ -51,14 +59,32 @@ struct FlushArgs {
     uint16_t *sids;
 };
 
+/* Offsets of a run's payload inside one pinned host block. */
+struct RunLayout {
+    size_t key_bytes, bitmaps, o_bitmap, sids, o_keys, o_boxes, sorted_keys, sorted_boxes, counters, total;
+};
+
+/* A pinned host block and the event that completes the copies into it. */
+struct PinnedBlock {
+    uint8_t *p = nullptr;
+    size_t cap = 0;
+    hipEvent_t ev = nullptr;
+};
+
 struct FlushOut {
     stc_run_info info;
-    std::vector<stc_ctf_meta> ctfs;
-    std::vector<uint8_t> key_bytes, bitmaps, o_bitmap;
-    std::vector<uint16_t> sids;
-    std::vector<uint64_t> o_keys, sorted_keys;
-    std::vector<float> o_boxes, sorted_boxes;
+    std::vector<stc_ctf_meta> ctfs;   /* final except flag 8, which is in the copied counters */
+    RunLayout lay;
+    bool has_sorted = false;
+    /* called once the payload size is known; returns a block of at least
+     * `bytes` whose event the flush records after its copies */
+    PinnedBlock (*acquire)(void *ctx, size_t bytes) = nullptr;
+    void *acquire_ctx = nullptr;
+    PinnedBlock blk;
 };
+
+/* FlushCounters::raster_capped of the copied counters (flag 8 of every CTF) */
+uint32_t stc_flush_raster_capped(const void *counters);
 
 int stc_flush_init(FlushWork *w, uint32_t n, uint32_t ctf_count);
 void stc_flush_free(FlushWork *w);

@@ -189,12 +189,32 @@ __global__ void k_make_sid(const uint32_t *oids, uint32_t n, uint16_t *sids, con
 __global__ void k_ctf_capacity(const uint32_t *oids, uint32_t n, const uint16_t *sids,
                                const uint32_t *same_pid_count, uint32_t *capacity, uint32_t ctf_count,
                                FlushCounters *fc) {
+    /* objects id >= z are in CTF order, so a wavefront spans one or two CTFs:
+     * lanes of one CTF add their counts first and one lane updates it */
     uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= n || id < fc->z) return;
-    uint32_t oid = oids[id];
-    uint32_t cid = (uint32_t)sids[oid] - 2;
-    if (cid < ctf_count) atomicAdd(&capacity[cid], same_pid_count[oid]);
-    else atomicAdd(&fc->bad, 1u);
+    const int lane = threadIdx.x & (STC_WAVE - 1);
+    bool on = id < n && id >= fc->z;
+    uint32_t cid = 0, v = 0;
+    if (on) {
+        uint32_t oid = oids[id];
+        cid = (uint32_t)sids[oid] - 2;
+        v = same_pid_count[oid];
+        if (cid >= ctf_count) {
+            atomicAdd(&fc->bad, 1u);
+            on = false;
+        }
+    }
+    uint64_t act = __ballot(on);
+    while (act) {
+        int leader = __ffsll((unsigned long long)act) - 1;
+        uint32_t lc = __shfl(cid, leader, STC_WAVE);
+        bool mine = on && cid == lc;
+        uint32_t sum = mine ? v : 0u;
+        for (int o = STC_WAVE / 2; o > 0; o >>= 1) sum += __shfl_xor(sum, o, STC_WAVE);
+        if (lane == leader) atomicAdd(&capacity[lc], sum);
+        act &= ~__ballot(mine);
+        if (mine) on = false;
+    }
 }
 
 /* object order for placement: objects by (sid, oid) — the CTF records, sid
@@ -513,7 +533,12 @@ int stc_flush_init(FlushWork *w, uint32_t n, uint32_t ctf_count) {
         hipMalloc((void **)&w->ctfs, ctf_count * sizeof(stc_ctf_meta)) ||
         hipMalloc((void **)&w->bitmaps, (size_t)ctf_count * 8192) || hipMalloc((void **)&w->flags, 256) ||
         hipMalloc((void **)&w->part_tail, N) || hipMalloc((void **)&w->obj_key, N * 4) ||
-        hipMalloc((void **)&w->obj_id, N * 4) || hipMalloc((void **)&w->obj_cnt, N * 4))
+        hipMalloc((void **)&w->obj_id, N * 4) || hipMalloc((void **)&w->obj_cnt, N * 4) ||
+        hipMalloc((void **)&w->sids_out, N * 2) || hipMalloc((void **)&w->o_bitmap_dev, 8192))
+        return STC_E_HIP;
+    if (hipStreamCreateWithFlags(&w->cs, hipStreamNonBlocking) ||
+        hipEventCreateWithFlags(&w->ready, hipEventDisableTiming) ||
+        hipEventCreateWithFlags(&w->copied, hipEventDisableTiming))
         return STC_E_HIP;
     w->scan_temp_words = stc_radix_temp_words(n) + 64;
     if (hipMalloc((void **)&w->scan_temp, w->scan_temp_words * 4)) return STC_E_HIP;
@@ -521,7 +546,11 @@ int stc_flush_init(FlushWork *w, uint32_t n, uint32_t ctf_count) {
 }
 
 void stc_flush_free(FlushWork *w) {
-    void *ptrs[] = {w->mid, w->mid_alt, w->oids, w->oids_alt, w->obj_base, w->obj_fill, w->capacity, w->ctf_base,
+    if (w->cs) (void)hipStreamSynchronize(w->cs);
+    if (w->cs) (void)hipStreamDestroy(w->cs);
+    if (w->ready) (void)hipEventDestroy(w->ready);
+    if (w->copied) (void)hipEventDestroy(w->copied);
+    void *ptrs[] = {w->sids_out, w->o_bitmap_dev, w->mid, w->mid_alt, w->oids, w->oids_alt, w->obj_base, w->obj_fill, w->capacity, w->ctf_base,
                     w->ctf_byte_base, w->ctfs, w->bitmaps, w->flags, w->part_tail, w->obj_key, w->obj_id, w->obj_cnt,
                     w->scan_temp, w->keys_out, w->boxes_out, w->tkeys, w->tboxes, w->key_bytes};
     for (void *p : ptrs)
@@ -531,6 +560,7 @@ void stc_flush_free(FlushWork *w) {
 
 static int ensure_rec(FlushWork *w, uint64_t K) {
     if (K <= w->rec_cap) return 0;
+    if (w->pending && hipEventSynchronize(w->copied)) return STC_E_HIP; /* a copy still reads them */
     uint64_t nc = std::max<uint64_t>(K + K / 4, 1 << 16);
     void *ptrs[] = {w->keys_out, w->boxes_out, w->tkeys, w->tboxes, w->key_bytes};
     for (void *p : ptrs)
@@ -549,6 +579,8 @@ int stc_flush_run(FlushWork *w, const FlushArgs *a, FlushOut *out, hipStream_t s
     const uint32_t split = (uint32_t)std::sqrt((double)C);
     const uint64_t K = a->kv_count;
     FRC(ensure_rec(w, K + 1));
+    /* the previous run's copies read keys_out, key_bytes, bitmaps, sids_out and the counters */
+    if (w->pending) FCK(hipStreamWaitEvent(s, w->copied, 0));
     FlushCounters *fc = (FlushCounters *)w->flags;
     FlushCounters init;
     memset(&init, 0, sizeof(init));
@@ -624,38 +656,50 @@ int stc_flush_run(FlushWork *w, const FlushArgs *a, FlushOut *out, hipStream_t s
     if (n_tail)
         hipLaunchKernelGGL(k_o_bitmap, dim3(blocks_for(n_tail, B)), dim3(B), 0, s, w->boxes_out, nf, fc, words,
                            total_bits, a->map[0], a->map[1], a->map[2], a->map[3]);
-    out->o_bitmap.resize(8192);
-    FCK(hipMemcpyAsync(out->o_bitmap.data(), w->bitmaps, 8192, hipMemcpyDeviceToHost, s));
+    FCK(hipMemcpyAsync(w->o_bitmap_dev, w->bitmaps, 8192, hipMemcpyDeviceToDevice, s));
     FCK(hipMemsetAsync(w->bitmaps, 0, 8192, s));
     hipLaunchKernelGGL(k_bitmap_keys, dim3(C), dim3(256), 0, s, w->keys_out, w->boxes_out, w->capacity, w->ctf_base,
                        w->ctf_byte_base, w->ctfs, C, fc, nf, words, w->key_bytes, fc);
     FCK(hipGetLastError());
+    FCK(hipMemcpyAsync(w->sids_out, a->sids, (size_t)n * 2, hipMemcpyDeviceToDevice, s));
 
-    /* outputs */
+    /* outputs: one pinned block, filled on the copy stream */
     out->ctfs = ctfs;
-    out->key_bytes.resize(byte_base[C]);
-    out->bitmaps.resize((size_t)C * 8192);
-    out->sids.resize(n);
-    if (byte_base[C]) FCK(hipMemcpyAsync(out->key_bytes.data(), w->key_bytes, byte_base[C], hipMemcpyDeviceToHost, s));
-    FCK(hipMemcpyAsync(out->bitmaps.data(), w->bitmaps, (size_t)C * 8192, hipMemcpyDeviceToHost, s));
-    FCK(hipMemcpyAsync(out->sids.data(), a->sids, (size_t)n * 2, hipMemcpyDeviceToHost, s));
-    out->o_keys.resize(2 * n_tail);
-    out->o_boxes.resize(4 * n_tail);
+    out->has_sorted = a->keep_sorted && nf;
+    RunLayout &L = out->lay;
+    size_t off = 0;
+    auto place = [&](size_t bytes) { size_t o = off; off += (bytes + 255) & ~(size_t)255; return o; };
+    L.key_bytes = place(byte_base[C]);
+    L.bitmaps = place((size_t)C * 8192);
+    L.o_bitmap = place(8192);
+    L.sids = place((size_t)n * 2);
+    L.o_keys = place(n_tail * 16);
+    L.o_boxes = place(n_tail * 16);
+    L.sorted_keys = place(out->has_sorted ? nf * 16 : 0);
+    L.sorted_boxes = place(out->has_sorted ? nf * 16 : 0);
+    L.counters = place(sizeof(FlushCounters));
+    L.total = off;
+    out->blk = out->acquire(out->acquire_ctx, L.total);
+    if (!out->blk.p) return stc_hip_fail(hipErrorOutOfMemory, "pinned run buffer", __FILE__, __LINE__);
+    uint8_t *dst = out->blk.p;
+    FCK(hipEventRecord(w->ready, s));
+    FCK(hipStreamWaitEvent(w->cs, w->ready, 0));
+    if (byte_base[C]) FCK(hipMemcpyAsync(dst + L.key_bytes, w->key_bytes, byte_base[C], hipMemcpyDeviceToHost, w->cs));
+    FCK(hipMemcpyAsync(dst + L.bitmaps, w->bitmaps, (size_t)C * 8192, hipMemcpyDeviceToHost, w->cs));
+    FCK(hipMemcpyAsync(dst + L.o_bitmap, w->o_bitmap_dev, 8192, hipMemcpyDeviceToHost, w->cs));
+    FCK(hipMemcpyAsync(dst + L.sids, w->sids_out, (size_t)n * 2, hipMemcpyDeviceToHost, w->cs));
     if (n_tail) {
-        FCK(hipMemcpyAsync(out->o_keys.data(), w->keys_out + 2 * nf, n_tail * 16, hipMemcpyDeviceToHost, s));
-        FCK(hipMemcpyAsync(out->o_boxes.data(), w->boxes_out + nf, n_tail * 16, hipMemcpyDeviceToHost, s));
+        FCK(hipMemcpyAsync(dst + L.o_keys, w->keys_out + 2 * nf, n_tail * 16, hipMemcpyDeviceToHost, w->cs));
+        FCK(hipMemcpyAsync(dst + L.o_boxes, w->boxes_out + nf, n_tail * 16, hipMemcpyDeviceToHost, w->cs));
     }
-    if (a->keep_sorted && nf) {
-        out->sorted_keys.resize(2 * nf);
-        out->sorted_boxes.resize(4 * nf);
-        FCK(hipMemcpyAsync(out->sorted_keys.data(), w->keys_out, nf * 16, hipMemcpyDeviceToHost, s));
-        FCK(hipMemcpyAsync(out->sorted_boxes.data(), w->boxes_out, nf * 16, hipMemcpyDeviceToHost, s));
+    if (out->has_sorted) {
+        FCK(hipMemcpyAsync(dst + L.sorted_keys, w->keys_out, nf * 16, hipMemcpyDeviceToHost, w->cs));
+        FCK(hipMemcpyAsync(dst + L.sorted_boxes, w->boxes_out, nf * 16, hipMemcpyDeviceToHost, w->cs));
     }
-    FlushCounters end_fc;
-    FCK(hipMemcpyAsync(&end_fc, fc, sizeof(end_fc), hipMemcpyDeviceToHost, s));
-    FCK(hipStreamSynchronize(s));
-    if (end_fc.raster_capped)
-        for (auto &m : out->ctfs) m.flags |= 8;
+    FCK(hipMemcpyAsync(dst + L.counters, fc, sizeof(FlushCounters), hipMemcpyDeviceToHost, w->cs));
+    FCK(hipEventRecord(w->copied, w->cs));
+    FCK(hipEventRecord(out->blk.ev, w->cs));
+    w->pending = true;
     stc_run_info &info = out->info;
     memset(&info, 0, sizeof(info));
     info.kv_count = (uint32_t)nf;
@@ -672,4 +716,8 @@ int stc_flush_run(FlushWork *w, const FlushArgs *a, FlushOut *out, hipStream_t s
     info.oversize_oid_count = hfc.oversize_oid_count;
     info.key_bytes = byte_base[C];
     return 0;
+}
+
+uint32_t stc_flush_raster_capped(const void *counters) {
+    return ((const FlushCounters *)counters)->raster_capped;
 }

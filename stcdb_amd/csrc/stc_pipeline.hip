@@ -919,14 +919,73 @@ struct DBuf {
     void release() { if (p) hipFree(p); p = nullptr; cap = 0; }
 };
 
+/* A flushed run: metadata on the host, payload in a pinned block that the
+ * copy stream fills while later steps run. */
 struct RunHost {
     stc_run_info info;
     std::vector<stc_ctf_meta> ctfs;
-    std::vector<uint8_t> key_bytes, bitmaps, o_bitmap;
-    std::vector<uint16_t> sids;
-    std::vector<uint64_t> o_keys, sorted_keys;
-    std::vector<float> o_boxes, sorted_boxes;
+    RunLayout lay;
+    bool has_sorted = false;
+    bool finalized = false;
+    PinnedBlock blk;
 };
+
+/* Pinned host blocks for run payloads, recycled once their copies are done. */
+struct PinnedPool {
+    std::vector<PinnedBlock> idle, retiring;
+    void reclaim() {
+        for (size_t i = 0; i < retiring.size();) {
+            if (hipEventQuery(retiring[i].ev) == hipSuccess) {
+                idle.push_back(retiring[i]);
+                retiring[i] = retiring.back();
+                retiring.pop_back();
+            } else {
+                i++;
+            }
+        }
+    }
+    PinnedBlock acquire(size_t bytes) {
+        reclaim();
+        int best = -1;
+        for (size_t i = 0; i < idle.size(); i++)
+            if (idle[i].cap >= bytes && (best < 0 || idle[i].cap < idle[best].cap)) best = (int)i;
+        if (best >= 0) {
+            PinnedBlock b = idle[best];
+            idle.erase(idle.begin() + best);
+            return b;
+        }
+        if (!idle.empty()) { /* too small: replace the largest idle block */
+            PinnedBlock b = idle.back();
+            idle.pop_back();
+            (void)hipHostFree(b.p);
+            (void)hipEventDestroy(b.ev);
+        }
+        PinnedBlock b;
+        size_t cap = bytes + bytes / 4 + (1u << 20);
+        if (hipHostMalloc((void **)&b.p, cap, hipHostMallocDefault) != hipSuccess) return PinnedBlock();
+        if (hipEventCreateWithFlags(&b.ev, hipEventDisableTiming) != hipSuccess) {
+            (void)hipHostFree(b.p);
+            return PinnedBlock();
+        }
+        b.cap = cap;
+        return b;
+    }
+    void release(PinnedBlock b) {
+        if (b.p) retiring.push_back(b);
+    }
+    void clear() {
+        for (auto *v : {&idle, &retiring})
+            for (auto &b : *v) {
+                (void)hipEventSynchronize(b.ev);
+                (void)hipHostFree(b.p);
+                (void)hipEventDestroy(b.ev);
+            }
+        idle.clear();
+        retiring.clear();
+    }
+};
+
+static PinnedBlock pool_acquire(void *pool, size_t bytes) { return ((PinnedPool *)pool)->acquire(bytes); }
 
 struct StcCtx {
     stc_config cfg;
@@ -977,6 +1036,7 @@ struct StcCtx {
     DBuf<uint32_t> chunk_count, chunk_offset;
     DBuf<Bucket> stage;
     std::vector<RunHost> runs;
+    PinnedPool pinned;
     FlushWork flushw;
     hipEvent_t ev[10];
 };
@@ -1204,6 +1264,9 @@ int stc_destroy(void *vc) {
     if (c->hcnt) hipHostFree(c->hcnt);
     if (c->hinit) hipHostFree(c->hinit);
     stc_flush_free(&c->flushw);
+    for (auto &r : c->runs) c->pinned.release(r.blk);
+    c->runs.clear();
+    c->pinned.clear();
     for (int i = 0; i < 10; i++) if (c->ev[i]) hipEventDestroy(c->ev[i]);
     if (c->s) hipStreamDestroy(c->s);
     delete c;
@@ -1438,18 +1501,18 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
         fa.o_boxs = c->o_boxs;
         fa.sids = c->sids;
         FlushOut fo;
+        fo.acquire = pool_acquire;
+        fo.acquire_ctx = &c->pinned;
         int rc = stc_flush_run(&c->flushw, &fa, &fo, s);
-        if (rc) return fail(c, rc, std::string("flush: ") + g_last_fail);
+        if (rc) {
+            c->pinned.release(fo.blk);
+            return fail(c, rc, std::string("flush: ") + g_last_fail);
+        }
         run.info = fo.info;
         run.ctfs.swap(fo.ctfs);
-        run.key_bytes.swap(fo.key_bytes);
-        run.bitmaps.swap(fo.bitmaps);
-        run.o_bitmap.swap(fo.o_bitmap);
-        run.sids.swap(fo.sids);
-        run.o_keys.swap(fo.o_keys);
-        run.sorted_keys.swap(fo.sorted_keys);
-        run.o_boxes.swap(fo.o_boxes);
-        run.sorted_boxes.swap(fo.sorted_boxes);
+        run.lay = fo.lay;
+        run.has_sorted = fo.has_sorted;
+        run.blk = fo.blk;
         c->runs.push_back(std::move(run));
         /* reset the memtable (geometry.cu:2111-2131) */
         CK(hipMemsetAsync(c->same_pid_count, 0, 4ull * n, s));
@@ -1609,22 +1672,42 @@ int stc_get_run(void *vc, uint32_t run, stc_ctf_meta *ctfs, uint8_t *key_bytes, 
                 uint64_t *o_keys, float *o_boxes, uint8_t *o_bitmap, uint64_t *sorted_keys, float *sorted_boxes) {
     StcCtx *c = (StcCtx *)vc;
     if (!c || run >= c->runs.size()) return STC_E_ARG;
-    const RunHost &R = c->runs[run];
+    RunHost &R = c->runs[run];
+    CK(hipEventSynchronize(R.blk.ev));
+    const uint8_t *src = R.blk.p;
+    const RunLayout &L = R.lay;
+    if (!R.finalized) {
+        if (stc_flush_raster_capped(src + L.counters))
+            for (auto &m : R.ctfs) m.flags |= 8;
+        R.finalized = true;
+    }
+    const stc_run_info &I = R.info;
     if (ctfs) memcpy(ctfs, R.ctfs.data(), R.ctfs.size() * sizeof(stc_ctf_meta));
-    if (key_bytes) memcpy(key_bytes, R.key_bytes.data(), R.key_bytes.size());
-    if (bitmaps) memcpy(bitmaps, R.bitmaps.data(), R.bitmaps.size());
-    if (sids) memcpy(sids, R.sids.data(), R.sids.size() * 2);
-    if (o_keys) memcpy(o_keys, R.o_keys.data(), R.o_keys.size() * 8);
-    if (o_boxes) memcpy(o_boxes, R.o_boxes.data(), R.o_boxes.size() * 4);
-    if (o_bitmap) memcpy(o_bitmap, R.o_bitmap.data(), R.o_bitmap.size());
-    if (sorted_keys) memcpy(sorted_keys, R.sorted_keys.data(), R.sorted_keys.size() * 8);
-    if (sorted_boxes) memcpy(sorted_boxes, R.sorted_boxes.data(), R.sorted_boxes.size() * 4);
+    if (key_bytes) memcpy(key_bytes, src + L.key_bytes, I.key_bytes);
+    if (bitmaps) memcpy(bitmaps, src + L.bitmaps, (size_t)I.ctf_count * 8192);
+    if (sids) memcpy(sids, src + L.sids, (size_t)I.num_objects * 2);
+    if (o_keys) memcpy(o_keys, src + L.o_keys, (size_t)I.oversize_count * 16);
+    if (o_boxes) memcpy(o_boxes, src + L.o_boxes, (size_t)I.oversize_count * 16);
+    if (o_bitmap) memcpy(o_bitmap, src + L.o_bitmap, 8192);
+    if (R.has_sorted && sorted_keys) memcpy(sorted_keys, src + L.sorted_keys, (size_t)I.kv_count * 16);
+    if (R.has_sorted && sorted_boxes) memcpy(sorted_boxes, src + L.sorted_boxes, (size_t)I.kv_count * 16);
     return STC_OK;
 }
 
+int stc_sync(void *vc) {
+    StcCtx *c = (StcCtx *)vc;
+    if (!c) return STC_E_ARG;
+    CK(hipStreamSynchronize(c->s));
+    if (c->flushw.cs) CK(hipStreamSynchronize(c->flushw.cs));
+    return STC_OK;
+}
+
+/* Releases the host copies of all flushed runs. Copies still in flight finish
+ * before their memory is reused. */
 int stc_drop_runs(void *vc) {
     StcCtx *c = (StcCtx *)vc;
     if (!c) return STC_E_ARG;
+    for (auto &r : c->runs) c->pinned.release(r.blk);
     c->runs.clear();
     return STC_OK;
 }

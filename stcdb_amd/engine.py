@@ -181,6 +181,10 @@ class Pipeline:
     def drop_runs(self):
         _check(self.lib.stc_drop_runs(self.ctx), self.ctx)
 
+    def sync_runs(self):
+        """wait for every background copy of a flushed run"""
+        _check(self.lib.stc_sync(self.ctx), self.ctx)
+
     def close(self):
         if self.ctx:
             self.lib.stc_destroy(self.ctx)
