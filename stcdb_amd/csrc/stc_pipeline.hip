@@ -693,134 +693,270 @@ __device__ inline void write_marker(Bucket *b, unsigned long long key) {
     *b = x;
 }
 
+/* Stage the buckets of the lanes with `emit` set, writing the staged records
+ * out first when the stage could overflow. Wave-uniform. */
+__device__ inline void stage_push(Bucket *st, uint32_t &staged, bool emit, const Bucket &b, int lane, uint64_t lt,
+                                  uint32_t max_meet, uint32_t end_time_min, MemtableDev mt, Counters *cnt) {
+    uint64_t em = __ballot(emit);
+    if (!em) return;
+    if (staged + (uint32_t)__popcll(em) > ID_STAGE) {
+        __builtin_amdgcn_wave_barrier();
+        emit_staged(st, staged, lane, max_meet, end_time_min, mt, cnt);
+        staged = 0;
+        __builtin_amdgcn_wave_barrier();
+    }
+    if (emit) st[staged + __popcll(em & lt)] = b;
+    staged += (uint32_t)__popcll(em);
+}
+
+struct IdentifyArgs {
+    uint32_t now, min_meet, max_meet, end_time_min;
+    MapBox m;
+};
+
+__device__ inline bool meeting_emits(const Bucket &b, const IdentifyArgs &a) {
+    float lo0 = o2f(b.mbr[0]), lo1 = o2f(b.mbr[1]), hi0 = o2f(b.mbr[2]), hi1 = o2f(b.mbr[3]);
+    return a.now - b.start >= a.min_meet + 1 && (double)lo0 >= a.m.lo0 && (double)hi0 <= a.m.hi0 &&
+           (double)lo1 >= a.m.lo1 && (double)hi1 <= a.m.hi1;
+}
+
+struct IdentifyTally {
+    unsigned long long live = 0, exp = 0, ret = 0, freed = 0, tomb = 0;
+};
+
+/* Backward-shift deletion over one probe cluster that starts after the free
+ * bucket `owner`, walked bucket by bucket from global memory by the lanes with
+ * `walking` set (wave-uniform call). Holes are kept in a 64-slot window behind
+ * the walker; a hole that leaves the window becomes a tombstone, an occupied
+ * bucket that matches no key. Used for clusters longer than the identify
+ * window, which are rare. */
+__device__ void walk_cluster(Bucket *t, uint64_t nb, uint64_t owner, bool walking, const IdentifyArgs &a,
+                             Bucket *st, uint32_t &staged, int lane, uint64_t lt, MemtableDev mt, Counters *cnt,
+                             IdentifyTally &tl) {
+    const unsigned long long free_now = FREE_KEY(a.now), tomb = TOMB_KEY(a.now);
+    uint64_t j = slot_next(owner, nb), wb = j, holes = 0;
+    uint32_t rel = 0;
+    while (__ballot(walking)) {
+        bool emit = false;
+        Bucket b;
+        if (walking) {
+            b = t[j];
+            if (key_free(b.key)) {
+                while (holes) {
+                    int q = __builtin_ctzll(holes);
+                    holes &= holes - 1;
+                    uint64_t dst = wb + q;
+                    if (dst >= nb) dst -= nb;
+                    write_marker(&t[dst], free_now);
+                    tl.freed++;
+                }
+                walking = false;
+            } else {
+                if (rel == STC_WAVE) {
+                    if (holes & 1ull) {
+                        write_marker(&t[wb], tomb);
+                        tl.tomb++;
+                    }
+                    holes >>= 1;
+                    wb = slot_next(wb, nb);
+                    rel--;
+                }
+                if (b.end == a.now && !key_tomb(b.key)) {
+                    tl.live++;
+                    if (holes) {
+                        uint64_t h = hash_slot(b.key, nb);
+                        uint64_t back = (j >= h) ? j - h : j + nb - h;
+                        uint64_t mask = holes;
+                        if (back < rel) mask &= ~0ull << (rel - back);
+                        if (mask) {
+                            int q = __builtin_ctzll(mask);
+                            uint64_t dst = wb + q;
+                            if (dst >= nb) dst -= nb;
+                            t[dst] = b;
+                            holes ^= 1ull << q;
+                            holes |= 1ull << rel;
+                        }
+                    }
+                } else {
+                    if (!key_tomb(b.key) && b.end == a.now - 1) {
+                        tl.exp++;
+                        emit = meeting_emits(b, a);
+                    } else {
+                        tl.ret++;
+                    }
+                    holes |= 1ull << rel;
+                }
+                j = slot_next(j, nb);
+                rel++;
+            }
+        }
+        stage_push(st, staged, emit, b, lane, lt, a.max_meet, a.end_time_min, mt, cnt);
+    }
+}
+
 /* identify completed meetings (geometry.cu:595-737) and delete every bucket
  * that is not live at `now` from the table.
  *
- * Deletion is linear probing's backward shift, run per probe cluster (a
- * maximal run of occupied buckets). Each cluster is walked by the thread
- * whose bucket is the free slot just before it: only buckets that were free
- * when the pass began start a walk, and those are never written during the
- * pass (buckets freed by the pass carry the tag `now`, so they start no walk).
- * The walker visits the cluster in probe order, turns each dead bucket into a
- * hole, and moves each live bucket into the first hole at or after its home
- * slot, so every remaining key keeps an unbroken probe chain. Holes are kept
- * in a 64-slot window behind the walker; a hole that falls out of the window
- * becomes a tombstone (occupied, never matches a key, deleted by a later
- * pass). At the cluster's end the remaining holes become free buckets.
+ * Deletion is linear probing's backward shift, applied per probe cluster (a
+ * maximal run of occupied buckets): each live bucket moves to the first hole
+ * at or after its home slot, holes being dead buckets and the slots that moved
+ * buckets vacated, so every remaining key keeps an unbroken probe chain; the
+ * holes left at the end become free buckets.
  *
- * Walkers read their own cluster only, and the buckets of a wavefront's
- * 64-slot chunk come from one coalesced load staged in LDS. Expiring buckets
- * that emit records are staged per wavefront and written in batches, so the
- * memtable counter sees one atomic per batch. */
-__global__ __launch_bounds__(256) void k_identify(Bucket *t, uint64_t nb, uint32_t now, uint32_t min_meet,
-                                                  uint32_t max_meet, uint32_t end_time_min, MapBox m,
-                                                  MemtableDev mt, Counters *cnt) {
-    __shared__ Bucket chunk[256 / STC_WAVE][STC_WAVE];
+ * Each wavefront owns a contiguous range of 64-bucket chunks and the clusters
+ * that start in them (after a bucket that was free when the pass began; slots
+ * freed by the pass carry the tag `now` and start nothing). A chunk is handled
+ * with the next chunk as lookahead, one bucket per lane and half: ballots give
+ * the free, live and dead masks, each lane finds its cluster from them, and
+ * the hole-filling runs as a scalar loop over 128-bit masks that visits only
+ * live buckets with a hole before them in their cluster. The rare cluster
+ * that leaves the 128-bucket window is walked from memory (walk_cluster).
+ * Clusters are disjoint, so no two lanes ever write the same bucket, and a
+ * cluster's buckets are read before its owner writes any of them.
+ *
+ * Expiring buckets that emit records are staged per wavefront and written in
+ * batches, so the memtable counter sees one atomic per batch. */
+__global__ __launch_bounds__(256) void k_identify(Bucket *t, uint64_t nb, IdentifyArgs a, MemtableDev mt,
+                                                  Counters *cnt) {
     __shared__ Bucket stage[256 / STC_WAVE][ID_STAGE];
     const int lane = threadIdx.x & (STC_WAVE - 1), w = threadIdx.x / STC_WAVE;
-    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (STC_WAVE - lane));
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    const unsigned long long free_now = FREE_KEY(now);
-    const unsigned long long tomb = TOMB_KEY(now);
-    unsigned long long nlive = 0, nexp = 0, nret = 0, nfreed = 0, ntomb = 0;
-    uint32_t staged = 0; /* wave-uniform */
-    for (uint64_t c0 = (uint64_t)blockIdx.x * blockDim.x + (uint64_t)w * STC_WAVE; c0 < nb; c0 += stride) {
-        const uint64_t i = c0 + lane;
-        bool walking = false;
-        if (i < nb) {
-            Bucket b = t[i];
-            chunk[w][lane] = b;
-            walking = key_free(b.key) && b.key != free_now;
-        }
-        __builtin_amdgcn_wave_barrier();
-        uint64_t j = slot_next(i, nb), wb = j, holes = 0;
-        uint32_t rel = 0; /* j - wb */
-        while (__ballot(walking)) {
-            bool emit = false;
-            Bucket b;
-            if (walking) {
-                uint64_t d = j - c0; /* j < c0 wraps to a large value and is read from memory */
-                b = (d < STC_WAVE) ? chunk[w][d] : t[j];
-                if (key_free(b.key)) {
-                    /* end of the cluster: the remaining holes become free buckets */
-                    while (holes) {
-                        int q = __builtin_ctzll(holes);
-                        holes &= holes - 1;
-                        uint64_t dst = wb + q;
-                        if (dst >= nb) dst -= nb;
-                        write_marker(&t[dst], free_now);
-                        nfreed++;
-                    }
-                    walking = false;
-                } else {
-                    if (rel == STC_WAVE) { /* slide the hole window by one slot */
-                        if (holes & 1ull) {
-                            write_marker(&t[wb], tomb);
-                            ntomb++;
-                        }
-                        holes >>= 1;
-                        wb = slot_next(wb, nb);
-                        rel--;
-                    }
-                    if (b.end == now && !key_tomb(b.key)) {
-                        nlive++;
-                        if (holes) {
-                            uint64_t h = hash_slot(b.key, nb);
-                            uint64_t back = (j >= h) ? j - h : j + nb - h; /* probe distance of this bucket */
-                            uint64_t mask = holes;
-                            if (back < rel) mask &= ~0ull << (rel - back);
-                            if (mask) {
-                                int q = __builtin_ctzll(mask);
-                                uint64_t dst = wb + q;
-                                if (dst >= nb) dst -= nb;
-                                t[dst] = b;
-                                holes ^= 1ull << q;
-                                holes |= 1ull << rel;
-                            }
-                        }
-                    } else {
-                        if (!key_tomb(b.key) && b.end == now - 1) {
-                            nexp++;
-                            float lo0 = o2f(b.mbr[0]), lo1 = o2f(b.mbr[1]), hi0 = o2f(b.mbr[2]),
-                                  hi1 = o2f(b.mbr[3]);
-                            emit = (now - b.start >= min_meet + 1 && (double)lo0 >= m.lo0 && (double)hi0 <= m.hi0 &&
-                                    (double)lo1 >= m.lo1 && (double)hi1 <= m.hi1);
-                        } else {
-                            nret++;
-                        }
-                        holes |= 1ull << rel;
-                    }
-                    j = slot_next(j, nb);
-                    rel++;
+    const uint64_t lt = (1ull << lane) - 1;           /* lanes below */
+    const uint64_t gt = ~((2ull << lane) - 1);        /* lanes above */
+    const unsigned long long free_now = FREE_KEY(a.now);
+    const uint64_t nchunks = nb / STC_WAVE;
+    const uint64_t nwaves = (uint64_t)gridDim.x * (256 / STC_WAVE);
+    const uint64_t wid = (uint64_t)blockIdx.x * (256 / STC_WAVE) + w;
+    const uint64_t c_begin = wid * nchunks / nwaves, c_end = (wid + 1) * nchunks / nwaves;
+    Bucket *st = stage[w];
+    IdentifyTally tl;
+    uint32_t staged = 0;
+    if (c_begin < c_end) {
+        auto at = [&](uint64_t chunk) -> uint64_t {
+            uint64_t p = chunk * STC_WAVE + lane;
+            return p >= nb ? p - nb : p;
+        };
+        Bucket A = t[at(c_begin)], B = t[at(c_begin + 1)];
+        for (uint64_t c = c_begin; c < c_end; c++) {
+            Bucket N;
+            if (c + 1 < c_end) N = t[at(c + 2)]; /* prefetch */
+            const uint64_t pa = at(c), pb = at(c + 1);
+            const bool fA = key_free(A.key), fB = key_free(B.key);
+            const bool lA = !fA && !key_tomb(A.key) && A.end == a.now;
+            const bool lB = !fB && !key_tomb(B.key) && B.end == a.now;
+            const uint64_t FA = __ballot(fA), FB = __ballot(fB);
+            const uint64_t OA = __ballot(fA && A.key != free_now);
+            /* A-half lane: owned when the nearest free slot below it is an original free slot of this chunk */
+            bool ownA = false, ownB = false, longA = false, longB = false, pastHoleA = false, pastHoleB = false;
+            uint64_t below = FA & lt;
+            int pf = 63 - __builtin_clzll(below | 1ull); /* valid when below != 0 */
+            if (!fA && below && ((OA >> pf) & 1ull)) {
+                ownA = true;
+                longA = (FA & gt) == 0 && FB == 0;
+            }
+            /* B-half lane: owned when no free slot lies between the last free slot of the chunk and it */
+            int pl = 63 - __builtin_clzll(FA | 1ull);
+            if (!fB && (FB & lt) == 0 && FA && ((OA >> pl) & 1ull)) {
+                ownB = true;
+                longB = (FB & gt) == 0;
+            }
+            const bool partA = ownA && !longA, partB = ownB && !longB;
+            const bool dA = partA && !lA, dB = partB && !lB;
+            uint64_t HA = __ballot(dA), HB = __ballot(dB); /* holes: the owned dead buckets */
+            if (partA && lA) pastHoleA = (HA & lt & ~((2ull << pf) - 1)) != 0;
+            if (partB && lB) pastHoleB = (HB & lt) != 0 || (HA & ~((2ull << pl) - 1)) != 0;
+            /* tallies and emission */
+            bool emitA = false, emitB = false;
+            if (partA) {
+                if (lA) tl.live++;
+                else if (!key_tomb(A.key) && A.end == a.now - 1) { tl.exp++; emitA = meeting_emits(A, a); }
+                else tl.ret++;
+            }
+            if (partB) {
+                if (lB) tl.live++;
+                else if (!key_tomb(B.key) && B.end == a.now - 1) { tl.exp++; emitB = meeting_emits(B, a); }
+                else tl.ret++;
+            }
+            stage_push(st, staged, emitA, A, lane, lt, a.max_meet, a.end_time_min, mt, cnt);
+            stage_push(st, staged, emitB, B, lane, lt, a.max_meet, a.end_time_min, mt, cnt);
+            /* home slots, relative to the window, of the live buckets that may move */
+            int hA = 0, hB = 0;
+            if (pastHoleA) {
+                uint64_t h = hash_slot(A.key, nb);
+                uint64_t back = (pa >= h) ? pa - h : pa + nb - h;
+                hA = back > (uint64_t)lane ? 0 : lane - (int)back;
+            }
+            if (pastHoleB) {
+                uint64_t h = hash_slot(B.key, nb);
+                uint64_t back = (pb >= h) ? pb - h : pb + nb - h;
+                hB = back > (uint64_t)(64 + lane) ? 0 : 64 + lane - (int)back;
+            }
+            uint64_t MA = __ballot(pastHoleA), MB = __ballot(pastHoleB);
+            int dstA = -1, dstB = -1;
+            while (MA) { /* scalar: first hole at or after the home slot */
+                int x = __builtin_ctzll(MA);
+                MA &= MA - 1;
+                int hx = __builtin_amdgcn_readlane(hA, x);
+                uint64_t e = HA & (~0ull << hx) & ((1ull << x) - 1);
+                if (e) {
+                    int q = __builtin_ctzll(e);
+                    HA ^= 1ull << q;
+                    HA |= 1ull << x;
+                    if (lane == x) dstA = q;
                 }
             }
-            uint64_t em = __ballot(emit);
-            if (emit) stage[w][staged + __popcll(em & lt)] = b;
-            staged += (uint32_t)__popcll(em);
-            if (staged > ID_STAGE - STC_WAVE) {
-                __builtin_amdgcn_wave_barrier();
-                emit_staged(stage[w], staged, lane, max_meet, end_time_min, mt, cnt);
-                staged = 0;
-                __builtin_amdgcn_wave_barrier();
+            while (MB) {
+                int x = __builtin_ctzll(MB);
+                MB &= MB - 1;
+                int hx = __builtin_amdgcn_readlane(hB, x);
+                uint64_t eA = hx < 64 ? (HA & (~0ull << hx)) : 0ull;
+                uint64_t eB = HB & (~0ull << (hx < 64 ? 0 : hx - 64)) & ((1ull << x) - 1);
+                if (eA) {
+                    int q = __builtin_ctzll(eA);
+                    HA ^= 1ull << q;
+                    HB |= 1ull << x;
+                    if (lane == x) dstB = q;
+                } else if (eB) {
+                    int q = __builtin_ctzll(eB);
+                    HB ^= 1ull << q;
+                    HB |= 1ull << x;
+                    if (lane == x) dstB = 64 + q;
+                }
             }
+            const uint64_t c0 = c * STC_WAVE;
+            auto slot = [&](int r) -> uint64_t {
+                uint64_t p = c0 + (uint64_t)r;
+                return p >= nb ? p - nb : p;
+            };
+            if (dstA >= 0) t[slot(dstA)] = A;
+            if (dstB >= 0) t[slot(dstB)] = B;
+            if ((HA >> lane) & 1ull) {
+                write_marker(&t[pa], free_now);
+                tl.freed++;
+            }
+            if ((HB >> lane) & 1ull) {
+                write_marker(&t[pb], free_now);
+                tl.freed++;
+            }
+            /* the cluster after the chunk's last free slot, when it leaves the window */
+            const uint64_t LG = __ballot(longA || longB);
+            const bool long_owner = LG != 0 && lane == pl;
+            walk_cluster(t, nb, pa, long_owner, a, st, staged, lane, lt, mt, cnt, tl);
+            A = B;
+            B = N;
         }
-        __builtin_amdgcn_wave_barrier(); /* the chunk buffer is reloaded next iteration */
     }
     if (staged) {
         __builtin_amdgcn_wave_barrier();
-        emit_staged(stage[w], staged, lane, max_meet, end_time_min, mt, cnt);
+        emit_staged(st, staged, lane, a.max_meet, a.end_time_min, mt, cnt);
     }
-    nlive = wave_sum_u64(nlive);
-    nexp = wave_sum_u64(nexp);
-    nret = wave_sum_u64(nret);
-    nfreed = wave_sum_u64(nfreed);
-    ntomb = wave_sum_u64(ntomb);
+    unsigned long long v0 = wave_sum_u64(tl.live), v1 = wave_sum_u64(tl.exp), v2 = wave_sum_u64(tl.ret),
+                       v3 = wave_sum_u64(tl.freed), v4 = wave_sum_u64(tl.tomb);
     if (lane == 0) {
-        if (nlive) atomicAdd(&cnt->live, nlive);
-        if (nexp) atomicAdd(&cnt->expired, nexp);
-        if (nret) atomicAdd(&cnt->retired, nret);
-        if (nfreed) atomicAdd(&cnt->freed, nfreed);
-        if (ntomb) atomicAdd(&cnt->tombs, ntomb);
+        if (v0) atomicAdd(&cnt->live, v0);
+        if (v1) atomicAdd(&cnt->expired, v1);
+        if (v2) atomicAdd(&cnt->retired, v2);
+        if (v3) atomicAdd(&cnt->freed, v3);
+        if (v4) atomicAdd(&cnt->tombs, v4);
     }
 }
 
@@ -1067,6 +1203,9 @@ static int sync_counters(StcCtx *c) {
     return 0;
 }
 
+/* the identify pass works on 64-bucket chunks: table sizes are multiples of 64 */
+static uint64_t round_buckets(uint64_t nb) { return (nb + 63) & ~63ull; }
+
 static int alloc_table(StcCtx *c, uint64_t nb, Bucket **out) {
     Bucket *t = nullptr;
     CK(hipMalloc((void **)&t, nb * sizeof(Bucket)));
@@ -1125,7 +1264,7 @@ static int table_rehash(StcCtx *c, uint32_t keep_from, uint64_t expected_keys) {
                            c->chunk_offset.p, c->stage.p);
         CK(hipGetLastError());
     }
-    uint64_t want = std::max<uint64_t>(std::max<uint64_t>(expected_keys, kept) * 5 / 2 + 1024, 1024);
+    uint64_t want = round_buckets(std::max<uint64_t>(expected_keys, kept) * 5 / 2 + 1024);
     uint64_t nb = nold;
     if (want > nold) {
         CK(hipStreamSynchronize(s));
@@ -1225,7 +1364,7 @@ int stc_create(const stc_config *cfg, const stc_box *mbr, void **out) {
         hipHostMalloc((void **)&c->hinit, sizeof(Counters), hipHostMallocDefault) != hipSuccess)
         return bad("pinned");
     uint64_t nb = cfg->num_meeting_buckets ? cfg->num_meeting_buckets : 4ull * n;
-    c->table_buckets = std::max<uint64_t>(nb, 1024);
+    c->table_buckets = round_buckets(std::max<uint64_t>(nb, 1024));
     if (alloc_table(c, c->table_buckets, &c->table)) return bad("table");
     if (hipMalloc((void **)&c->same_pid_count, 4ull * n) != hipSuccess ||
         hipMalloc((void **)&c->o_boxs, 16ull * n) != hipSuccess || hipMalloc((void **)&c->sids, 2ull * n) != hipSuccess)
@@ -1454,10 +1593,11 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
     uint64_t need_rec = c->kv_count + 2 * expiring + 2;
     if (ensure_records(c, need_rec)) return fail(c, STC_E_CAPACITY, "record buffer allocation failed");
     MemtableDev mt{c->rec_keys, c->rec_boxes, c->rec_cap, c->same_pid_count, c->o_boxs, c->sids};
-    uint32_t id_grid = (uint32_t)std::min<uint64_t>(blocks_for(c->table_buckets, 256), 4096);
+    /* one wavefront per range of at least 16 chunks */
+    uint32_t id_grid = (uint32_t)std::max<uint64_t>(std::min<uint64_t>(c->table_buckets / (64 * 16 * 4), 4096), 1);
+    IdentifyArgs ia{t, c->cfg.min_meet_time, c->cfg.max_meet_time, c->end_time_min, c->map};
     CK(hipEventRecord(c->ev[8], s));
-    hipLaunchKernelGGL(k_identify, dim3(id_grid), dim3(256), 0, s, c->table, c->table_buckets, t,
-                       c->cfg.min_meet_time, c->cfg.max_meet_time, c->end_time_min, c->map, mt, cnt);
+    hipLaunchKernelGGL(k_identify, dim3(id_grid), dim3(256), 0, s, c->table, c->table_buckets, ia, mt, cnt);
     CK(hipEventRecord(c->ev[9], s));
     CK(hipGetLastError());
     CKRC(sync_counters(c));
