@@ -31,6 +31,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+PROFILES = os.path.join(ROOT, "profiles")
 
 
 def hip():
@@ -89,6 +90,26 @@ def cpu_baseline(w, frames_host, steps):
                        f"{pairs} pairs found; {dt:.1f} s")
 
 
+def measured_traffic(workload, n):
+    """HBM bytes per launch of the dominant kernel, from the committed rocprofv3
+    counter summary for this workload (FETCH_SIZE and WRITE_SIZE, calibrated
+    with a copy of known size); None when no summary covers it."""
+    path = os.path.join(PROFILES, f"k_refine_{workload}_traffic.md")
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    for line in open(path):
+        if ":" in line and line.startswith("- `"):
+            k, v = line[3:].split("`", 1)[0], line.split(":", 1)[1].strip()
+            vals[k] = v
+    try:
+        if int(vals["objects"]) != n:
+            return None
+        return int(vals["traffic_bytes_per_launch"])
+    except (KeyError, ValueError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -97,7 +118,11 @@ def main():
     ap.add_argument("--workload", default="c3")
     ap.add_argument("--objects", type=int, default=0, help="override the object count")
     ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--host-steps", type=int, default=3,
+                    help="extra steps fed from pinned host memory (PCIe-inclusive rate, reported beside value)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="copy a buffer of this size once before the steps (calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one MI355X: --gpus {args.gpus} is not supported (the pipeline is single-GPU; "
@@ -114,17 +139,28 @@ def main():
     cfg, gen = w["cfg"], w["gen"]
     n = cfg["num_objects"]
     total = args.warmup + args.steps
+    nframes = total + args.host_steps
 
     # inputs for every step, generated on the device before timing
-    frames = torch.empty((total, n, 2), dtype=torch.float64, device="cuda")
+    frames = torch.empty((nframes, n, 2), dtype=torch.float64, device="cuda")
     trace = DeviceTrace(gen)
     h = hip()
-    for t in range(total):
+    for t in range(nframes):
         p = trace.next(t)
         assert h.hipMemcpy(frames[t].data_ptr(), p.value, n * 16, 3) == 0
     trace.close()
     torch.cuda.synchronize()
-    print(f"[bench] workload {args.workload}: {n} objects, {total} frames resident", file=sys.stderr, flush=True)
+    print(f"[bench] workload {args.workload}: {n} objects, {nframes} frames resident", file=sys.stderr, flush=True)
+
+    if args.copy_probe_mib:
+        src = torch.ones(args.copy_probe_mib << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"[bench] copy probe: {args.copy_probe_mib} MiB read, {args.copy_probe_mib} MiB written", file=sys.stderr,
+              flush=True)
+        del src, dst
 
     pipe = Pipeline(cfg, w["box"])
     for t in range(args.warmup):
@@ -148,6 +184,21 @@ def main():
               f" flushed {r['flushed']}", file=sys.stderr, flush=True)
 
     value = n * args.steps / elapsed
+
+    # the same pipeline fed from pinned host memory: stc_step includes the H2D copy of the points
+    host_value = None
+    if args.host_steps:
+        pinned = torch.empty((args.host_steps, n, 2), dtype=torch.float64, pin_memory=True)
+        pinned.copy_(frames[total:total + args.host_steps])
+        del frames
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(args.host_steps):
+            pipe.step_host_ptr(ctypes.c_void_p(pinned[i].data_ptr()), total + i)
+            pipe.drop_runs()
+        pipe.sync_runs()
+        torch.cuda.synchronize()
+        host_value = n * args.host_steps / (time.perf_counter() - t0)
     # roofline of the dominant kernel (algorithmic bytes per launch / its HIP-event time)
     refine_ms = float(np.mean([r["ms_refine_kernel"] for r in outs]))
     ident_ms = float(np.mean([r["ms_identify_kernel"] for r in outs]))
@@ -162,13 +213,21 @@ def main():
         alg = 32.0 * buckets + 64.0 * meets
         name, ms = "k_identify (meeting expiry scan)", ident_ms
     achieved = alg / (ms * 1e-3) / 1e9
+    traffic = measured_traffic(args.workload, n) if refine_ms >= ident_ms else None
     roof = dict(bound="hbm", kernel=name, achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, alg_bytes_per_launch=alg,
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic, alg_bytes_per_launch=alg,
                 avg_ms=round(ms, 3))
 
     cpu = None
     if not args.no_cpu:
-        frames_host = [frames[t].cpu().numpy() for t in range(args.cpu_steps)]
+        trace = DeviceTrace(gen)
+        frames_host = []
+        for t in range(args.cpu_steps):
+            p = trace.next(t)
+            buf = np.empty((n, 2), dtype=np.float64)
+            assert h.hipMemcpy(buf.ctypes.data, p.value, n * 16, 2) == 0
+            frames_host.append(buf)
+        trace.close()
         cpu = cpu_baseline(w, frames_host, args.cpu_steps)
         cpu["cpu_model"] = platform.processor() or platform.machine()
 
@@ -190,6 +249,7 @@ def main():
                    "kv_restriction": cfg["kv_restriction"], "ctf_count": cfg["ctf_count"],
                    "pairs_per_step": int(pairs),
                    "flushes": int(sum(r["flushed"] for r in outs))},
+        "host_input_value": round(host_value, 1) if host_value else None,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

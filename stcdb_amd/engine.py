@@ -110,6 +110,12 @@ class Pipeline:
         _check(self.lib.stc_step(self.ctx, _ptr(xy), t, C.byref(out)), self.ctx)
         return out.as_dict()
 
+    def step_host_ptr(self, xy_host, t):
+        """one timestep from a raw host pointer to N AoS float64 points (e.g. pinned memory)"""
+        out = L.StcStepOut()
+        _check(self.lib.stc_step(self.ctx, xy_host, t, C.byref(out)), self.ctx)
+        return out.as_dict()
+
     def step_device(self, xy_dev, t):
         out = L.StcStepOut()
         _check(self.lib.stc_step_device(self.ctx, xy_dev, t, C.byref(out)), self.ctx)
