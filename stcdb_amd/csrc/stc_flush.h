@@ -36,6 +36,8 @@ struct FlushWork {
      * overlapping the following steps; the next flush waits for `copied`
      * before it rewrites the buffers the copy reads */
     uint16_t *sids_out = nullptr;                  /* sids as of the flush */
+    uint4 *chunks = nullptr;                       /* (ctf, first, count) work items of the key/bitmap pass */
+    size_t chunks_cap = 0;
     uint8_t *o_bitmap_dev = nullptr;               /* oversize bitmap as of the flush */
     hipStream_t cs = nullptr;
     hipEvent_t ready = nullptr, copied = nullptr;

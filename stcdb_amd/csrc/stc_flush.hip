@@ -427,16 +427,25 @@ __global__ void k_o_bitmap(const float4 *__restrict__ sboxes, uint64_t tail_firs
 }
 
 /* F7: bitmap and key packing, one workgroup per CTF */
+#define BK_CHUNK 2048u /* records per block of k_bitmap_keys */
+
+/* cuda_write_bitmap + write_key_and_mbr (geometry.cu:986-1021, 1073-1107) for
+ * one chunk of one CTF's records: the chunk's bitmap bits are gathered in an
+ * LDS copy of the CTF bitmap and OR-ed into it once, and its packed keys are
+ * assembled in LDS and written as one contiguous byte range. Chunk = (ctf,
+ * first record in the CTF, record count). */
 __global__ __launch_bounds__(256) void k_bitmap_keys(const uint64_t *__restrict__ skeys,
                                                      const float4 *__restrict__ sboxes,
-                                                     const uint32_t *__restrict__ capacity,
+                                                     const uint4 *__restrict__ chunks,
                                                      const uint32_t *__restrict__ ctf_base,
                                                      const uint64_t *__restrict__ ctf_byte_base,
                                                      const stc_ctf_meta *__restrict__ ctfs, uint32_t ctf_count,
-                                                     const FlushCounters *fc, uint64_t nf, uint32_t *__restrict__ words,
+                                                     uint64_t nf, uint32_t *__restrict__ words,
                                                      uint8_t *__restrict__ key_bytes, FlushCounters *fcw) {
     __shared__ uint32_t bm[BITMAP_WORDS];
-    uint32_t c = blockIdx.x;
+    __shared__ uint8_t kst[BK_CHUNK * 16];
+    const uint4 ch = chunks[blockIdx.x];
+    const uint32_t c = ch.x, k0 = ch.y, kn = ch.z;
     for (uint32_t i = threadIdx.x; i < BITMAP_WORDS; i += blockDim.x) bm[i] = 0;
     __syncthreads();
     stc_ctf_meta ctf = ctfs[c];
@@ -450,11 +459,9 @@ __global__ __launch_bounds__(256) void k_bitmap_keys(const uint64_t *__restrict_
     const uint32_t s_mbr = ctf.mbr_bit, s_end = ctf.end_bit, s_dur = ctf.duration_bit, s_id = ctf.id_bit;
     const float longer = fmaxf(m2 - m0, m3 - m1);
     const uint64_t first = ctf_base[c];
-    const uint32_t cap = capacity[c];
-    uint8_t *dst0 = key_bytes + ctf_byte_base[c];
-    for (uint32_t k = threadIdx.x; k < cap; k += blockDim.x) {
-        uint64_t rel = (uint64_t)ctf_base[c] + k;
-        if (rel >= nf) break;
+    /* records at or past nf are not in the CTF region (only when the capacities disagree with nf) */
+    const uint32_t kend = (first + k0 >= nf) ? k0 : (uint32_t)min<uint64_t>(k0 + kn, nf - first);
+    for (uint32_t k = k0 + threadIdx.x; k < kend; k += blockDim.x) {
         uint64_t kid = first + k;
         float4 b = sboxes[kid];
         /* cuda_write_bitmap */
@@ -492,10 +499,15 @@ __global__ __launch_bounds__(256) void k_bitmap_keys(const uint64_t *__restrict_
         unsigned __int128 tk = (sh1 < 128 ? pid << sh1 : 0) + (sh2 < 128 ? target << sh2 : 0) +
                                (sh3 < 128 ? dur << sh3 : 0) + (s_mbr < 128 ? end << s_mbr : 0) +
                                (unsigned __int128)vm;
-        uint8_t *dst = dst0 + (uint64_t)k * kb;
+        uint8_t *dst = kst + (k - k0) * kb;
         for (uint32_t q = 0; q < kb && q < 16; q++) dst[q] = (uint8_t)(tk >> (8 * q));
     }
     __syncthreads();
+    /* packed keys: one contiguous range, written byte-coalesced */
+    const uint32_t nbytes = (kend - k0) * min(kb, 16u);
+    uint8_t *gdst = key_bytes + ctf_byte_base[c] + (uint64_t)k0 * kb;
+    if (kb <= 16)
+        for (uint32_t i = threadIdx.x; i < nbytes; i += blockDim.x) gdst[i] = kst[i];
     uint32_t *gw = words + (uint64_t)c * BITMAP_WORDS;
     for (uint32_t i = threadIdx.x; i < BITMAP_WORDS; i += blockDim.x)
         if (bm[i]) atomicOr(&gw[i], bm[i]);
@@ -550,7 +562,7 @@ void stc_flush_free(FlushWork *w) {
     if (w->cs) (void)hipStreamDestroy(w->cs);
     if (w->ready) (void)hipEventDestroy(w->ready);
     if (w->copied) (void)hipEventDestroy(w->copied);
-    void *ptrs[] = {w->sids_out, w->o_bitmap_dev, w->mid, w->mid_alt, w->oids, w->oids_alt, w->obj_base, w->obj_fill, w->capacity, w->ctf_base,
+    void *ptrs[] = {w->chunks, w->sids_out, w->o_bitmap_dev, w->mid, w->mid_alt, w->oids, w->oids_alt, w->obj_base, w->obj_fill, w->capacity, w->ctf_base,
                     w->ctf_byte_base, w->ctfs, w->bitmaps, w->flags, w->part_tail, w->obj_key, w->obj_id, w->obj_cnt,
                     w->scan_temp, w->keys_out, w->boxes_out, w->tkeys, w->tboxes, w->key_bytes};
     for (void *p : ptrs)
@@ -658,9 +670,24 @@ int stc_flush_run(FlushWork *w, const FlushArgs *a, FlushOut *out, hipStream_t s
                            total_bits, a->map[0], a->map[1], a->map[2], a->map[3]);
     FCK(hipMemcpyAsync(w->o_bitmap_dev, w->bitmaps, 8192, hipMemcpyDeviceToDevice, s));
     FCK(hipMemsetAsync(w->bitmaps, 0, 8192, s));
-    hipLaunchKernelGGL(k_bitmap_keys, dim3(C), dim3(256), 0, s, w->keys_out, w->boxes_out, w->capacity, w->ctf_base,
-                       w->ctf_byte_base, w->ctfs, C, fc, nf, words, w->key_bytes, fc);
-    FCK(hipGetLastError());
+    /* chunks of at most BK_CHUNK records, CTF by CTF */
+    std::vector<uint4> chunks;
+    for (uint32_t c = 0; c < C; c++)
+        for (uint32_t k = 0; k < cap[c]; k += BK_CHUNK) chunks.push_back(make_uint4(c, k, std::min(BK_CHUNK, cap[c] - k), 0));
+    if (!chunks.empty()) {
+        if (chunks.size() > w->chunks_cap) {
+            if (w->chunks) FCK(hipFree(w->chunks));
+            w->chunks = nullptr;
+            w->chunks_cap = 0;
+            size_t nc = chunks.size() + chunks.size() / 2 + 64;
+            FCK(hipMalloc((void **)&w->chunks, nc * sizeof(uint4)));
+            w->chunks_cap = nc;
+        }
+        FCK(hipMemcpyAsync(w->chunks, chunks.data(), chunks.size() * sizeof(uint4), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_bitmap_keys, dim3((uint32_t)chunks.size()), dim3(256), 0, s, w->keys_out, w->boxes_out,
+                           w->chunks, w->ctf_base, w->ctf_byte_base, w->ctfs, C, nf, words, w->key_bytes, fc);
+        FCK(hipGetLastError());
+    }
     FCK(hipMemcpyAsync(w->sids_out, a->sids, (size_t)n * 2, hipMemcpyDeviceToDevice, s));
 
     /* outputs: one pinned block, filled on the copy stream */
