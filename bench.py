@@ -90,11 +90,11 @@ def cpu_baseline(w, frames_host, steps):
                        f"{pairs} pairs found; {dt:.1f} s")
 
 
-def measured_traffic(workload, n):
-    """HBM bytes per launch of the dominant kernel, from the committed rocprofv3
-    counter summary for this workload (FETCH_SIZE and WRITE_SIZE, calibrated
-    with a copy of known size); None when no summary covers it."""
-    path = os.path.join(PROFILES, f"k_refine_{workload}_traffic.md")
+def measured_traffic(kernel, workload, n):
+    """HBM bytes per launch of a kernel, from the committed rocprofv3 counter
+    summary for this workload (FETCH_SIZE and WRITE_SIZE, calibrated with a copy
+    of known size); None when no summary covers it."""
+    path = os.path.join(PROFILES, f"{kernel}_{workload}_traffic.md")
     if not os.path.exists(path):
         return None
     vals = {}
@@ -179,9 +179,9 @@ def main():
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     for i, r in enumerate(outs):
-        print(f"[bench] step {args.warmup + i}: {r['ms_total']:.1f} ms (refine {r['ms_refine_kernel']:.1f}, identify "
-              f"{r['ms_identify_kernel']:.1f}, flush {r['ms_flush']:.1f}) pairs {r['pairs']} records {r['new_records']}"
-              f" flushed {r['flushed']}", file=sys.stderr, flush=True)
+        print(f"[bench] step {args.warmup + i}: {r['ms_total']:.1f} ms (refine {r['ms_refine_kernel']:.1f}, "
+              f"identify {r['ms_identify_kernel']:.1f}, flush {r['ms_flush']:.1f}) "
+              f"pairs {r['pairs']} records {r['new_records']} flushed {r['flushed']}", file=sys.stderr, flush=True)
 
     value = n * args.steps / elapsed
 
@@ -200,23 +200,26 @@ def main():
         torch.cuda.synchronize()
         host_value = n * args.host_steps / (time.perf_counter() - t0)
     # roofline of the dominant kernel (algorithmic bytes per launch / its HIP-event time)
-    refine_ms = float(np.mean([r["ms_refine_kernel"] for r in outs]))
-    ident_ms = float(np.mean([r["ms_identify_kernel"] for r in outs]))
-    pairs = float(np.mean([r["pairs"] for r in outs]))
-    if refine_ms >= ident_ms:
-        # pair/refine 16N + 4N + 8P, fused with the table insert/extend 64P
-        alg = 20.0 * n + 72.0 * pairs
-        name, ms = "k_refine (pair refinement fused with meeting-table insert)", refine_ms
-    else:
-        buckets = float(np.mean([r["table_buckets"] for r in outs]))
-        meets = float(np.mean([r["new_records"] for r in outs])) / 2
-        alg = 32.0 * buckets + 64.0 * meets
-        name, ms = "k_identify (meeting expiry scan)", ident_ms
+    def mean(k):
+        return float(np.mean([r[k] for r in outs]))
+
+    pairs = mean("pairs")
+    kernels = {
+        # pair/refine (coords 16N + pids 4N in, pair keys 8P out) fused with the
+        # hash insert/extend (one 32 B bucket read and one 32 B write per pair)
+        "k_refine": ("pair refinement fused with the meeting-table insert/extend", mean("ms_refine_kernel"),
+                     20.0 * n + 72.0 * pairs),
+        # expire/identify: 32 B per bucket scanned, 2 records x 32 B per completed meeting
+        "k_identify": ("meeting expiry scan and emission", mean("ms_identify_kernel"),
+                       32.0 * mean("table_buckets") + 32.0 * mean("new_records")),
+    }
+    kname = max(kernels, key=lambda k: kernels[k][1])
+    desc, ms, alg = kernels[kname]
     achieved = alg / (ms * 1e-3) / 1e9
-    traffic = measured_traffic(args.workload, n) if refine_ms >= ident_ms else None
-    roof = dict(bound="hbm", kernel=name, achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic, alg_bytes_per_launch=alg,
-                avg_ms=round(ms, 3))
+    roof = dict(bound="hbm", kernel=f"{kname} ({desc})", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=measured_traffic(kname, args.workload, n),
+                alg_bytes_per_launch=alg, avg_ms=round(ms, 3),
+                kernels_ms={k: round(v[1], 3) for k, v in kernels.items()})
 
     cpu = None
     if not args.no_cpu:

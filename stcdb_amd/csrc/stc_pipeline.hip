@@ -1242,9 +1242,11 @@ static int ilog2_ceil(uint64_t v) {
 }
 
 /* Move the entries last seen at or after keep_from into a table sized for
- * expected_keys at a load of at most 0.4 (see the kernels above). The kept
- * entries go to a staging buffer that stays allocated between rehashes; the
- * table is refilled in place unless it must grow. */
+ * expected_keys at a load of at most 1/3 (see the kernels above): fewer
+ * probes per insert make the refine pass faster than the larger table makes
+ * the identify scan slower. The kept entries go to a staging buffer that
+ * stays allocated between rehashes; the table is refilled in place unless it
+ * must grow. */
 static int table_rehash(StcCtx *c, uint32_t keep_from, uint64_t expected_keys) {
     hipStream_t s = c->s;
     uint64_t nold = c->table_buckets;
@@ -1264,7 +1266,7 @@ static int table_rehash(StcCtx *c, uint32_t keep_from, uint64_t expected_keys) {
                            c->chunk_offset.p, c->stage.p);
         CK(hipGetLastError());
     }
-    uint64_t want = round_buckets(std::max<uint64_t>(expected_keys, kept) * 5 / 2 + 1024);
+    uint64_t want = round_buckets(std::max<uint64_t>(expected_keys, kept) * 3 + 1024);
     uint64_t nb = nold;
     if (want > nold) {
         CK(hipStreamSynchronize(s));
