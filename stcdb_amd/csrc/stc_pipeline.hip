@@ -878,17 +878,20 @@ __global__ __launch_bounds__(256) void k_identify(Bucket *t, uint64_t nb, Identi
             }
             stage_push(st, staged, emitA, A, lane, lt, a.max_meet, a.end_time_min, mt, cnt);
             stage_push(st, staged, emitB, B, lane, lt, a.max_meet, a.end_time_min, mt, cnt);
-            /* home slots, relative to the window, of the live buckets that may move */
+            /* home slots, relative to the window, of the live buckets that may move;
+             * a bucket in its home slot never moves */
             int hA = 0, hB = 0;
             if (pastHoleA) {
                 uint64_t h = hash_slot(A.key, nb);
                 uint64_t back = (pa >= h) ? pa - h : pa + nb - h;
                 hA = back > (uint64_t)lane ? 0 : lane - (int)back;
+                pastHoleA = back != 0;
             }
             if (pastHoleB) {
                 uint64_t h = hash_slot(B.key, nb);
                 uint64_t back = (pb >= h) ? pb - h : pb + nb - h;
                 hB = back > (uint64_t)(64 + lane) ? 0 : 64 + lane - (int)back;
+                pastHoleB = back != 0;
             }
             uint64_t MA = __ballot(pastHoleA), MB = __ballot(pastHoleB);
             int dstA = -1, dstB = -1;
