@@ -53,8 +53,9 @@ typedef struct stc_config {
     uint64_t num_meeting_buckets; /* initial meeting-table size; 0 = 4 * num_objects. Grows on demand. */
     double reach_distance;        /* config->reach_distance (metres) */
     double area_threshold;        /* workbench::dProbFalse: object MBR area (deg^2) above which it is oversize */
-    uint32_t keep_pairs;          /* 1: keep each step's pair list for stc_get_pairs */
-    uint32_t reserved;
+    uint32_t keep_pairs;          /* bit 0: keep each step's pair list for stc_get_pairs;
+                                     bit 1: keep each run's sorted records for stc_get_run */
+    uint32_t device_runs;         /* nonzero: keep every flushed run in HBM for stc_query until stc_drop_runs */
 } stc_config;
 
 typedef struct stc_box {
@@ -174,6 +175,47 @@ int stc_drop_runs(void *ctx); /* release host copies of flushed runs */
  * of flushed runs into host memory (a flush hands its run to a copy stream so
  * that the copy overlaps the following steps; stc_get_run waits for its own run). */
 int stc_sync(void *ctx);
+
+/* ---- queries over the resident state ----
+ * The reference answers object-history, range and time queries from its
+ * flushed runs (workbench::id_search_in_CTB, mbr_search_in_disk,
+ * search_time_in_disk: src/tracing/workbench.cpp:350-572; CTF::search_SSTable,
+ * time_search: src/tracing/step_merge.cpp:528-584). stc_query_batch answers a batch
+ * over everything the context holds after the last step, with exact set
+ * semantics:
+ *   STC_Q_OID   meetings with pid == oid;
+ *   STC_Q_RANGE meetings whose MBR intersects box (closed bounds, compared in
+ *               double), each meeting once (the record with pid < target);
+ *   STC_Q_PAIR  meetings with pid == oid and target == oid2;
+ * each restricted to meetings whose [start, end] overlaps [t0, t1]. Sources:
+ * live meetings (start, end = last step, f32 MBR; both directions), memtable
+ * records (end = offset + time base, start = end - duration) and the runs kept
+ * in HBM (device_runs), whose keys are decoded as CTF::ctf_get_key_oid,
+ * parse_key and new_parse_mbr decode them (step_merge.cpp:325-379). counts[q]
+ * is the number of hits of query q; at most `cap` hits are stored, sorted by
+ * (query, source, pid, target, start, end); *n_hits is the total. */
+#define STC_Q_OID 1u
+#define STC_Q_RANGE 2u
+#define STC_Q_PAIR 3u
+
+typedef struct stc_query {
+    uint32_t kind;
+    uint32_t oid, oid2;
+    uint32_t t0, t1;           /* inclusive time window */
+    uint32_t pad;
+    double box[4];             /* STC_Q_RANGE: low0, low1, high0, high1 (degrees) */
+} stc_query;
+
+typedef struct stc_hit {
+    uint32_t query;            /* index of the query in the batch */
+    uint32_t pid, target;      /* the meeting as seen from pid */
+    uint32_t start, end;
+    uint32_t source;           /* 0 live, 1 memtable, 2 + r run r (stc_get_run numbering) */
+    double mbr[4];             /* f32 MBR (live, memtable) or the decoded CTF MBR (runs) */
+} stc_hit;
+
+int stc_query_batch(void *ctx, const stc_query *q, uint32_t nq, stc_hit *hits, uint64_t cap, uint64_t *n_hits,
+              uint64_t *counts);
 
 /* ---- seeded synthetic traces (the reference's generator is unseeded) ----
  * Parameter and state layouts are those of stcdb_amd/csrc/stc_gen.h. */

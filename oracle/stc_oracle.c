@@ -1114,6 +1114,128 @@ void orc_get_run(void *vo, uint32_t r, orc_ctf *ctfs, uint8_t *key_bytes, uint8_
 }
 
 /* ------------------------------------------------------------------ */
+/* queries over the resident state (see stc_oracle.h)                  */
+/* ------------------------------------------------------------------ */
+
+typedef struct qhit_ctx {
+    orc_hit *hits;
+    uint64_t cap, n;
+    uint64_t *counts;
+} qhit_ctx;
+
+static void q_emit(qhit_ctx *h, uint32_t qi, uint32_t pid, uint32_t target, uint32_t start, uint32_t end,
+                   uint32_t source, const double *mbr) {
+    h->counts[qi]++;
+    if (h->n < h->cap) {
+        orc_hit *x = &h->hits[h->n];
+        x->query = qi; x->pid = pid; x->target = target; x->start = start; x->end = end; x->source = source;
+        memcpy(x->mbr, mbr, sizeof(x->mbr));
+    }
+    h->n++;
+}
+
+static int q_time(const orc_query *q, uint32_t start, uint32_t end) { return start <= q->t1 && end >= q->t0; }
+static int q_box(const orc_query *q, const double *m) {
+    return m[0] <= q->box[2] && m[2] >= q->box[0] && m[1] <= q->box[3] && m[3] >= q->box[1];
+}
+
+/* one meeting as seen from its owner `pid` */
+static void q_match(const orc_query *qs, uint32_t nq, qhit_ctx *h, uint32_t pid, uint32_t target, uint32_t start,
+                    uint32_t end, uint32_t source, const double *mbr) {
+    for (uint32_t qi = 0; qi < nq; qi++) {
+        const orc_query *q = &qs[qi];
+        if (!q_time(q, start, end)) continue;
+        if ((q->kind == 1 && pid == q->oid) || (q->kind == 3 && pid == q->oid && target == q->oid2) ||
+            (q->kind == 2 && pid < target && q_box(q, mbr)))
+            q_emit(h, qi, pid, target, start, end, source, mbr);
+    }
+}
+
+/* CTF::ctf_get_key_oid, parse_key and new_parse_mbr (step_merge.cpp:325-379)
+ * on the little-endian packed key at p */
+static void decode_packed(const orc_ctf *c, const uint8_t *p, uint32_t *pid, uint32_t *target, uint32_t *start,
+                          uint32_t *end, double *box) {
+    u128 k = 0;
+    memcpy(&k, p, c->key_bit / 8);
+    uint64_t vm = (uint64_t)(k & (((u128)1 << c->mbr_bit) - 1));
+    u128 r = k >> c->mbr_bit;
+    uint32_t e = (uint32_t)(r & ((1ull << c->end_bit) - 1));
+    r >>= c->end_bit;
+    uint32_t d = (uint32_t)(r & ((1ull << c->duration_bit) - 1));
+    r >>= c->duration_bit;
+    *target = (uint32_t)(r & ((1ull << c->id_bit) - 1));
+    r >>= c->id_bit;
+    *pid = (uint32_t)(r & ((1ull << c->id_bit) - 1));
+    *end = e + c->end_time_min;
+    *start = *end - d;
+    uint64_t mask_edge = (1ull << c->edge_bit) - 1, mask_low_y = (1ull << c->low_y_bit) - 1,
+             mask_low_x = (1ull << c->low_x_bit) - 1;
+    uint64_t y = vm & mask_edge;
+    vm >>= c->edge_bit;
+    uint64_t x = vm & mask_edge;
+    vm >>= c->edge_bit;
+    uint64_t low1 = vm & mask_low_y;
+    vm >>= c->low_y_bit;
+    uint64_t low0 = vm & mask_low_x;
+    float w0 = c->mbr[2] - c->mbr[0], w1 = c->mbr[3] - c->mbr[1];
+    float longer = w0 > w1 ? w0 : w1;
+    box[0] = (double)c->mbr[0] + (double)low0 / (double)mask_low_x * (double)w0;
+    box[1] = (double)c->mbr[1] + (double)low1 / (double)mask_low_y * (double)w1;
+    box[2] = box[0] + (double)x / (double)mask_edge * (double)longer;
+    box[3] = box[1] + (double)y / (double)mask_edge * (double)longer;
+}
+
+uint32_t orc_end_time_min(void *o) { return ((orc *)o)->end_time_min; }
+
+uint64_t orc_query_batch(void *vo, const orc_query *qs, uint32_t nq, orc_hit *hits, uint64_t cap, uint64_t *counts) {
+    orc *o = (orc *)vo;
+    qhit_ctx h = {hits, cap, 0, counts};
+    memset(counts, 0, sizeof(uint64_t) * nq);
+    double m[4];
+    /* live meetings: both directions */
+    for (uint64_t i = 0; i < o->n_active; i++) {
+        const meeting *mt = &o->active[i];
+        for (int k = 0; k < 4; k++) m[k] = (double)mt->mbr[k];
+        uint32_t a = (uint32_t)(mt->key >> 32), b = (uint32_t)mt->key;
+        q_match(qs, nq, &h, a, b, mt->start, mt->end, 0, m);
+        q_match(qs, nq, &h, b, a, mt->start, mt->end, 0, m);
+    }
+    /* memtable records */
+    for (uint64_t i = 0; i < o->n_recs; i++) {
+        const record *rc = &o->recs[i];
+        uint32_t pid = (uint32_t)((rc->key >> 90) & ((1ull << OID_BIT) - 1));
+        uint32_t target = (uint32_t)((rc->key >> 64) & ((1ull << OID_BIT) - 1));
+        uint32_t end = key_end(rc->key) + o->end_time_min, start = end - key_dur(rc->key);
+        for (int k = 0; k < 4; k++) m[k] = (double)rc->box[k];
+        q_match(qs, nq, &h, pid, target, start, end, 1, m);
+    }
+    /* runs: every CTF record and every oversize record */
+    for (uint32_t r = 0; r < o->n_runs; r++) {
+        const run *R = &o->runs[r];
+        uint64_t off = 0;
+        for (uint32_t c = 0; c < R->info.ctf_count; c++) {
+            const orc_ctf *ctf = &R->ctfs[c];
+            uint32_t kb = ctf->key_bit / 8;
+            for (uint32_t i = 0; i < ctf->kv_capacity; i++) {
+                uint32_t pid, target, start, end;
+                decode_packed(ctf, R->key_bytes + off + (uint64_t)i * kb, &pid, &target, &start, &end, m);
+                q_match(qs, nq, &h, pid, target, start, end, 2 + r, m);
+            }
+            off += (uint64_t)ctf->kv_capacity * kb;
+        }
+        for (uint32_t i = 0; i < R->info.oversize_count; i++) {
+            u128 k = R->o_keys[i];
+            uint32_t pid = (uint32_t)((k >> 90) & ((1ull << OID_BIT) - 1));
+            uint32_t target = (uint32_t)((k >> 64) & ((1ull << OID_BIT) - 1));
+            uint32_t end = key_end(k) + R->info.end_time_min, start = end - key_dur(k);
+            for (int j = 0; j < 4; j++) m[j] = (double)R->o_boxes[4 * i + j];
+            q_match(qs, nq, &h, pid, target, start, end, 2 + r, m);
+        }
+    }
+    return h.n;
+}
+
+/* ------------------------------------------------------------------ */
 /* brute force pairs (independent of the tree)                         */
 /* ------------------------------------------------------------------ */
 

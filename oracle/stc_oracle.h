@@ -128,6 +128,28 @@ void orc_ctf_bits(orc_ctf *ctf, const double *map4, uint32_t num_objects, uint32
 /* brute force: all unordered pairs (i < j) with distance measured from probe i
  * (probe_low = 1) or probe j (probe_low = 0) within r; returns count, writes
  * up to cap keys (unsorted). Uses a uniform grid to prune. */
+/* Queries over the state after the last step (exact set semantics):
+ *   kind 1, object history: meetings of `oid` (as record owner) whose
+ *           [start, end] overlaps [t0, t1];
+ *   kind 2, square range: meetings whose MBR intersects `box` (closed, in
+ *           double) and whose time overlaps, each meeting once (pid < target);
+ *   kind 3, pair lookup: meetings with pid == oid and target == oid2.
+ * Sources: live meetings (start, end = last step, f32 MBR), memtable records
+ * (end = offset + end_time_min, start = end - duration) and every run (keys
+ * decoded as CTF::ctf_get_key_oid / parse_key / new_parse_mbr,
+ * src/tracing/step_merge.cpp:325-379; oversize records unpacked). source: 0
+ * live, 1 memtable, 2 + r run r. Hits beyond `cap` are counted, not stored. */
+typedef struct orc_query {
+    uint32_t kind, oid, oid2, t0, t1, pad;
+    double box[4];
+} orc_query;
+typedef struct orc_hit {
+    uint32_t query, pid, target, start, end, source;
+    double mbr[4];
+} orc_hit;
+uint64_t orc_query_batch(void *o, const orc_query *q, uint32_t nq, orc_hit *hits, uint64_t cap, uint64_t *counts);
+uint32_t orc_end_time_min(void *o); /* time base of the memtable's end offsets */
+
 uint64_t orc_brute_pairs(const double *xy, uint32_t n, double r, int probe_low,
                          uint64_t *keys, uint64_t cap);
 

@@ -34,7 +34,7 @@ class StcConfig(C.Structure):
         ("reach_distance", C.c_double),
         ("area_threshold", C.c_double),
         ("keep_pairs", C.c_uint32),
-        ("reserved", C.c_uint32),
+        ("device_runs", C.c_uint32),
     ]
 
 
@@ -179,6 +179,7 @@ SIGNATURES = {
     "stc_get_run": (C.c_int, [P, C.c_uint32, P, P, P, P, P, P, P, P, P]),
     "stc_drop_runs": (C.c_int, [P]),
     "stc_sync": (C.c_int, [P]),
+    "stc_query_batch": (C.c_int, [P, P, C.c_uint32, P, C.c_uint64, U64P, P]),
     "stc_gen_init_host": (C.c_int, [C.POINTER(StcGenParams), P, C.c_uint32, C.c_uint32]),
     "stc_gen_step_host": (C.c_int, [C.POINTER(StcGenParams), P, C.c_uint32, C.c_uint32]),
     "stc_gen_pos_host": (C.c_int, [C.POINTER(StcGenParams), P, C.c_uint32, P]),

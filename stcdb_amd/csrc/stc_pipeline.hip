@@ -41,6 +41,8 @@
 #include "stc_geo.h"
 #include "stc_gen.h"
 #include "stc_flush.h"
+#include "stc_table.h"
+#include "stc_query.h"
 #include "../../include/stcdb.h"
 
 #include <algorithm>
@@ -54,15 +56,6 @@
 #define NLEVELS 15
 #define T_BRANCH 1u
 #define T_LEAF 2u
-/* Bucket keys are pid << 32 | target with pid, target < 2^26. Keys whose high
- * word is all ones mark free buckets (the low word tags the identify pass that
- * freed it; a cleared table holds EMPTY_KEY); high word 0xFFFFFFFE marks a
- * tombstone, an occupied bucket that matches no key. */
-#define EMPTY_KEY 0xFFFFFFFFFFFFFFFFull
-#define FREE_KEY(tag) (0xFFFFFFFF00000000ull | (unsigned long long)(uint32_t)(tag))
-#define TOMB_KEY(tag) (0xFFFFFFFE00000000ull | (unsigned long long)(uint32_t)(tag))
-__host__ __device__ inline bool key_free(unsigned long long k) { return (k >> 32) == 0xFFFFFFFFull; }
-__host__ __device__ inline bool key_tomb(unsigned long long k) { return (k >> 32) == 0xFFFFFFFEull; }
 #define STACK_DEPTH 96
 
 static thread_local std::string g_last_fail;
@@ -87,12 +80,6 @@ struct Node {                 /* QTSchema (src/index/QTree.h:49-62) without coun
 };
 
 struct FrontierItem { uint32_t node, code; };
-
-struct __align__(32) Bucket { /* meeting_unit (workbench.h:62-83) with an ordered-integer MBR */
-    unsigned long long key;
-    uint32_t start, end;
-    uint32_t mbr[4];          /* ordered ints of low0, low1, high0, high1 */
-};
 
 struct PairHit {              /* 24 B: key and the float MBR of the two positions */
     unsigned long long key;
@@ -119,17 +106,6 @@ struct Counters {             /* device-side counters, copied back once per sync
     uint32_t table_fail;
 };
 
-__host__ __device__ inline uint32_t f2o(float f) {
-    uint32_t b;
-    memcpy(&b, &f, 4);
-    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
-}
-__host__ __device__ inline float o2f(uint32_t o) {
-    uint32_t b = (o & 0x80000000u) ? (o & 0x7FFFFFFFu) : ~o;
-    float f;
-    memcpy(&f, &b, 4);
-    return f;
-}
 
 __device__ inline uint32_t part1by1(uint32_t x) {
     x &= 0x0000ffff;
@@ -1067,6 +1043,8 @@ struct RunHost {
     bool has_sorted = false;
     bool finalized = false;
     PinnedBlock blk;
+    uint8_t *dev = nullptr;           /* HBM copy for queries (device_runs) */
+    QueryRunDev qdev;
 };
 
 /* Pinned host blocks for run payloads, recycled once their copies are done. */
@@ -1176,6 +1154,7 @@ struct StcCtx {
     DBuf<Bucket> stage;
     std::vector<RunHost> runs;
     PinnedPool pinned;
+    QueryBufs *qb = nullptr;
     FlushWork flushw;
     hipEvent_t ev[10];
 };
@@ -1250,6 +1229,46 @@ static int ilog2_ceil(uint64_t v) {
  * the identify scan slower. The kept entries go to a staging buffer that
  * stays allocated between rehashes; the table is refilled in place unless it
  * must grow. */
+/* Copy a just-flushed run out of the flush buffers into its own HBM block
+ * (packed keys, CTF metadata and bases, sids, oversize buffer) for queries. */
+static int keep_run_on_device(StcCtx *c, RunHost &run) {
+    const FlushWork &w = c->flushw;
+    const uint32_t C = run.info.ctf_count, n = run.info.num_objects, nt = run.info.oversize_count;
+    const uint64_t nf = run.info.kv_count;
+    size_t off = 0;
+    auto place = [&](size_t bytes) { size_t o = off; off += (bytes + 255) & ~(size_t)255; return o; };
+    const size_t o_keys = place(run.info.key_bytes), o_ctfs = place(C * sizeof(stc_ctf_meta)),
+                 o_bb = place((C + 1) * 8ull), o_rb = place((C + 1) * 4ull), o_sids = place(n * 2ull),
+                 o_ok = place(nt * 16ull), o_ob = place(nt * 16ull);
+    CK(hipMalloc((void **)&run.dev, std::max<size_t>(off, 256)));
+    hipStream_t s = c->s;
+    uint8_t *d = run.dev;
+    if (run.info.key_bytes) CK(hipMemcpyAsync(d + o_keys, w.key_bytes, run.info.key_bytes, hipMemcpyDeviceToDevice, s));
+    /* the host metadata is final but for flag 8, which queries do not read */
+    CK(hipMemcpyAsync(d + o_ctfs, run.ctfs.data(), C * sizeof(stc_ctf_meta), hipMemcpyHostToDevice, s));
+    CK(hipMemcpyAsync(d + o_bb, w.ctf_byte_base, (C + 1) * 8ull, hipMemcpyDeviceToDevice, s));
+    CK(hipMemcpyAsync(d + o_rb, w.ctf_base, (C + 1) * 4ull, hipMemcpyDeviceToDevice, s));
+    CK(hipMemcpyAsync(d + o_sids, w.sids_out, n * 2ull, hipMemcpyDeviceToDevice, s));
+    if (nt) {
+        CK(hipMemcpyAsync(d + o_ok, w.keys_out + 2 * nf, nt * 16ull, hipMemcpyDeviceToDevice, s));
+        CK(hipMemcpyAsync(d + o_ob, w.boxes_out + nf, nt * 16ull, hipMemcpyDeviceToDevice, s));
+    }
+    QueryRunDev &q = run.qdev;
+    q.key_bytes = d + o_keys;
+    q.ctfs = (const stc_ctf_meta *)(d + o_ctfs);
+    q.byte_base = (const uint64_t *)(d + o_bb);
+    q.rec_base = (const uint32_t *)(d + o_rb);
+    q.sids = (const uint16_t *)(d + o_sids);
+    q.o_keys = (const uint64_t *)(d + o_ok);
+    q.o_boxes = (const float4 *)(d + o_ob);
+    q.ctf_count = C;
+    q.n_tail = nt;
+    q.end_time_min = run.info.end_time_min;
+    q.source = 0; /* set per query from the run's position */
+    CK(hipStreamSynchronize(s)); /* the host metadata vector is read by the copy above */
+    return 0;
+}
+
 static int table_rehash(StcCtx *c, uint32_t keep_from, uint64_t expected_keys) {
     hipStream_t s = c->s;
     uint64_t nold = c->table_buckets;
@@ -1408,9 +1427,14 @@ int stc_destroy(void *vc) {
     if (c->hcnt) hipHostFree(c->hcnt);
     if (c->hinit) hipHostFree(c->hinit);
     stc_flush_free(&c->flushw);
-    for (auto &r : c->runs) c->pinned.release(r.blk);
+    for (auto &r : c->runs) {
+        c->pinned.release(r.blk);
+        if (r.dev) (void)hipFree(r.dev);
+    }
     c->runs.clear();
     c->pinned.clear();
+    stc_query_bufs_free(c->qb);
+    c->qb = nullptr;
     for (int i = 0; i < 10; i++) if (c->ev[i]) hipEventDestroy(c->ev[i]);
     if (c->s) hipStreamDestroy(c->s);
     delete c;
@@ -1658,6 +1682,7 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
         run.lay = fo.lay;
         run.has_sorted = fo.has_sorted;
         run.blk = fo.blk;
+        if (c->cfg.device_runs) CKRC(keep_run_on_device(c, run));
         c->runs.push_back(std::move(run));
         /* reset the memtable (geometry.cu:2111-2131) */
         CK(hipMemsetAsync(c->same_pid_count, 0, 4ull * n, s));
@@ -1852,8 +1877,49 @@ int stc_sync(void *vc) {
 int stc_drop_runs(void *vc) {
     StcCtx *c = (StcCtx *)vc;
     if (!c) return STC_E_ARG;
-    for (auto &r : c->runs) c->pinned.release(r.blk);
+    for (auto &r : c->runs) {
+        c->pinned.release(r.blk);
+        if (r.dev) {
+            (void)hipStreamSynchronize(c->s);
+            (void)hipFree(r.dev);
+            r.dev = nullptr;
+        }
+    }
     c->runs.clear();
+    return STC_OK;
+}
+
+int stc_query_batch(void *vc, const stc_query *q, uint32_t nq, stc_hit *hits, uint64_t cap, uint64_t *n_hits,
+              uint64_t *counts) {
+    StcCtx *c = (StcCtx *)vc;
+    if (!c || (!q && nq) || !n_hits) return STC_E_ARG;
+    if (!c->stepped) return fail(c, STC_E_STATE, "no step processed");
+    for (uint32_t i = 0; i < nq; i++)
+        if (q[i].kind < STC_Q_OID || q[i].kind > STC_Q_PAIR) return fail(c, STC_E_ARG, "unknown query kind");
+    if (!c->qb) c->qb = stc_query_bufs_new();
+    QueryState st;
+    st.table = c->table;
+    st.nb = c->table_buckets;
+    st.last_t = c->last_t;
+    st.stepped = 1;
+    st.rec_keys = c->rec_keys;
+    st.rec_boxes = c->rec_boxes;
+    st.n_rec = c->kv_count;
+    st.end_time_min = c->end_time_min;
+    st.num_objects = c->n;
+    st.map[0] = c->map.lo0;
+    st.map[1] = c->map.lo1;
+    st.map[2] = c->map.hi0;
+    st.map[3] = c->map.hi1;
+    for (uint32_t r = 0; r < c->runs.size(); r++) {
+        if (!c->runs[r].dev) continue; /* kept on the host only */
+        QueryRunDev d = c->runs[r].qdev;
+        d.source = 2 + r;
+        st.runs.push_back(d);
+        st.run_ctfs.push_back(c->runs[r].ctfs.data());
+    }
+    int rc = stc_query_run(c->qb, st, q, nq, hits, cap, n_hits, counts, c->s);
+    if (rc) return fail(c, rc, std::string("query: ") + g_last_fail);
     return STC_OK;
 }
 

@@ -1,0 +1,50 @@
+/*
+ * stc_query.h — queries over the resident state (internal interface).
+ *
+ * The state after a step: live meetings in the table (end == last step),
+ * memtable records, and the flushed runs kept in HBM (packed CTF keys, CTF
+ * metadata, per-object sids, the oversize buffer). stc_query_run answers a
+ * batch of object-history, square-range and pair queries with the set
+ * semantics documented at stc_query_batch in include/stcdb.h.
+ */
+#ifndef STC_QUERY_H_
+#define STC_QUERY_H_
+
+#include "stc_internal.h"
+#include "stc_table.h"
+#include "../../include/stcdb.h"
+#include <vector>
+
+struct QueryRunDev {                 /* one run kept in HBM */
+    const uint8_t *key_bytes;        /* packed CTF keys, CTF after CTF */
+    const stc_ctf_meta *ctfs;        /* [ctf_count] */
+    const uint64_t *byte_base;       /* [ctf_count + 1] first byte of each CTF */
+    const uint32_t *rec_base;        /* [ctf_count + 1] first record of each CTF */
+    const uint16_t *sids;            /* [num_objects] 0: no record, 1: oversize buffer, 2 + c: CTF c */
+    const uint64_t *o_keys;          /* [2 * n_tail] (lo, hi) record keys of the oversize buffer */
+    const float4 *o_boxes;           /* [n_tail] */
+    uint32_t ctf_count, n_tail, end_time_min, source;
+};
+
+struct QueryState {
+    const Bucket *table;
+    uint64_t nb;
+    uint32_t last_t;
+    uint32_t stepped;
+    const uint64_t *rec_keys;        /* memtable records (lo, hi) */
+    const float4 *rec_boxes;
+    uint64_t n_rec;
+    uint32_t end_time_min;           /* time base of the memtable's end offsets */
+    uint32_t num_objects;
+    double map[4];
+    std::vector<QueryRunDev> runs;
+    std::vector<const stc_ctf_meta *> run_ctfs;  /* host copies of each run's CTF metadata */
+};
+
+struct QueryBufs;                    /* device scratch of the query path, grown on demand */
+QueryBufs *stc_query_bufs_new();
+void stc_query_bufs_free(QueryBufs *b);
+int stc_query_run(QueryBufs *b, const QueryState &st, const stc_query *q, uint32_t nq, stc_hit *hits, uint64_t cap,
+                  uint64_t *n_hits, uint64_t *counts, hipStream_t s);
+
+#endif /* STC_QUERY_H_ */

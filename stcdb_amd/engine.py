@@ -31,6 +31,14 @@ def _gen_struct(gp):
     return g
 
 
+# stc_query / stc_hit of include/stcdb.h
+QUERY_DTYPE = np.dtype([("kind", "<u4"), ("oid", "<u4"), ("oid2", "<u4"), ("t0", "<u4"), ("t1", "<u4"),
+                        ("pad", "<u4"), ("box", "<f8", (4,))])
+HIT_DTYPE = np.dtype([("query", "<u4"), ("pid", "<u4"), ("target", "<u4"), ("start", "<u4"), ("end", "<u4"),
+                      ("source", "<u4"), ("mbr", "<f8", (4,))])
+Q_OID, Q_RANGE, Q_PAIR = 1, 2, 3
+
+
 class HostTrace:
     """Seeded walk computed on the host; yields (N, 2) float64 arrays."""
 
@@ -183,6 +191,20 @@ class Pipeline:
         return dict(info={k: getattr(info, k) for k, _ in info._fields_}, ctfs=ctfs_to_numpy(ctfs),
                     key_bytes=key_bytes, bitmaps=bitmaps, sids=sids, o_keys=o_keys, o_boxes=o_boxes,
                     o_bitmap=o_bitmap, sorted_keys=sk, sorted_boxes=sb)
+
+    def query(self, q, cap=None):
+        """answer a batch of queries (QUERY_DTYPE) over the state after the last
+        step; returns (hits as HIT_DTYPE sorted by query, source, pid, target,
+        start, end; counts per query)"""
+        q = np.ascontiguousarray(q, dtype=QUERY_DTYPE)
+        counts = np.zeros(len(q), dtype=np.uint64)
+        n = C.c_uint64()
+        if cap is None:
+            _check(self.lib.stc_query_batch(self.ctx, _ptr(q), len(q), None, 0, C.byref(n), _ptr(counts)), self.ctx)
+            cap = n.value
+        hits = np.zeros(cap, dtype=HIT_DTYPE)
+        _check(self.lib.stc_query_batch(self.ctx, _ptr(q), len(q), _ptr(hits), cap, C.byref(n), _ptr(counts)), self.ctx)
+        return hits[:min(n.value, cap)], counts
 
     def drop_runs(self):
         _check(self.lib.stc_drop_runs(self.ctx), self.ctx)

@@ -32,6 +32,17 @@ class OrcRunInfo(C.Structure):
                [("key_bytes", C.c_uint64)]
 
 
+QUERY_DTYPE = np.dtype([("kind", "<u4"), ("oid", "<u4"), ("oid2", "<u4"), ("t0", "<u4"), ("t1", "<u4"),
+                        ("pad", "<u4"), ("box", "<f8", (4,))])
+HIT_DTYPE = np.dtype([("query", "<u4"), ("pid", "<u4"), ("target", "<u4"), ("start", "<u4"), ("end", "<u4"),
+                      ("source", "<u4"), ("mbr", "<f8", (4,))])
+
+
+def sort_hits(h):
+    """canonical order of query results: (query, source, pid, target, start, end)"""
+    return h[np.lexsort((h["end"], h["start"], h["target"], h["pid"], h["source"], h["query"]))]
+
+
 CTF_DTYPE = np.dtype([
     ("mbr", np.float32, 4), ("kv_capacity", np.uint32), ("start_time_min", np.uint32),
     ("start_time_max", np.uint32), ("end_time_min", np.uint32), ("end_time_max", np.uint32),
@@ -91,6 +102,10 @@ def load():
         lib.orc_conv_d2i32.restype = C.c_int32
         lib.orc_conv_d2i32.argtypes = [C.c_double]
         lib.orc_ctf_bits.argtypes = [V, V, C.c_uint32, C.c_uint32]
+        lib.orc_query_batch.restype = C.c_uint64
+        lib.orc_query_batch.argtypes = [V, V, C.c_uint32, V, C.c_uint64, V]
+        lib.orc_end_time_min.restype = C.c_uint32
+        lib.orc_end_time_min.argtypes = [V]
         lib.orc_brute_pairs.restype = C.c_uint64
         lib.orc_brute_pairs.argtypes = [V, C.c_uint32, C.c_double, C.c_int, V, C.c_uint64]
         _lib = lib
@@ -240,6 +255,18 @@ class Oracle:
         return dict(info={k: getattr(info, k) for k, _ in info._fields_}, ctfs=ctfs, key_bytes=key_bytes,
                     bitmaps=bitmaps, sids=sids, o_keys=o_keys, o_boxes=o_boxes, o_bitmap=o_bitmap,
                     sorted_keys=sk, sorted_boxes=sb)
+
+    def end_time_min(self):
+        return self.lib.orc_end_time_min(self.h)
+
+    def query(self, q):
+        """answer a batch of queries (QUERY_DTYPE); returns (hits sorted, counts)"""
+        q = np.ascontiguousarray(q, dtype=QUERY_DTYPE)
+        counts = np.zeros(len(q), dtype=np.uint64)
+        n = self.lib.orc_query_batch(self.h, _p(q), len(q), None, 0, _p(counts))
+        hits = np.zeros(n, dtype=HIT_DTYPE)
+        self.lib.orc_query_batch(self.h, _p(q), len(q), _p(hits), n, _p(counts))
+        return sort_hits(hits), counts
 
     def close(self):
         if self.h:

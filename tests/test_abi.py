@@ -52,3 +52,25 @@ def test_struct_layouts_match_the_c_compiler(tmp_path):
         assert int(got[cname]) == ctypes.sizeof(py), cname
         for f, _ in py._fields_:
             assert int(got[f"{cname}.{f}"]) == getattr(py, f).offset, (cname, f)
+
+
+def test_query_record_layouts_match_the_c_compiler(tmp_path):
+    """the numpy records used for stc_query / stc_hit have gcc's layout"""
+    from stcdb_amd.engine import HIT_DTYPE, QUERY_DTYPE
+    dtypes = {"stc_query": QUERY_DTYPE, "stc_hit": HIT_DTYPE}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "include/stcdb.h"', "int main(void){"]
+    for cname, dt in dtypes.items():
+        lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
+        for f in dt.names:
+            lines.append(f'printf("{cname}.{f} %zu\\n", offsetof({cname}, {f}));')
+    lines.append("return 0;}")
+    src = tmp_path / "qlayout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "qlayout"
+    subprocess.run(["gcc", "-I", ROOT, "-o", str(exe), str(src)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = dict(line.rsplit(" ", 1) for line in out if line)
+    for cname, dt in dtypes.items():
+        assert int(got[cname]) == dt.itemsize, cname
+        for f in dt.names:
+            assert int(got[f"{cname}.{f}"]) == dt.fields[f][1], (cname, f)
