@@ -1045,6 +1045,7 @@ struct RunHost {
     PinnedBlock blk;
     uint8_t *dev = nullptr;           /* HBM copy for queries (device_runs) */
     QueryRunDev qdev;
+    std::vector<CtfStats> stats;      /* per CTF, for planning range queries */
 };
 
 /* Pinned host blocks for run payloads, recycled once their copies are done. */
@@ -1266,7 +1267,7 @@ static int keep_run_on_device(StcCtx *c, RunHost &run) {
     q.end_time_min = run.info.end_time_min;
     q.source = 0; /* set per query from the run's position */
     CK(hipStreamSynchronize(s)); /* the host metadata vector is read by the copy above */
-    return 0;
+    return stc_query_ctf_stats(q, run.stats, s) ? fail(c, STC_E_HIP, std::string("run statistics: ") + g_last_fail) : 0;
 }
 
 static int table_rehash(StcCtx *c, uint32_t keep_from, uint64_t expected_keys) {
@@ -1917,6 +1918,7 @@ int stc_query_batch(void *vc, const stc_query *q, uint32_t nq, stc_hit *hits, ui
         d.source = 2 + r;
         st.runs.push_back(d);
         st.run_ctfs.push_back(c->runs[r].ctfs.data());
+        st.run_stats.push_back(c->runs[r].stats.data());
     }
     int rc = stc_query_run(c->qb, st, q, nq, hits, cap, n_hits, counts, c->s);
     if (rc) return fail(c, rc, std::string("query: ") + g_last_fail);

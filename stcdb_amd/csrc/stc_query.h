@@ -26,6 +26,14 @@ struct QueryRunDev {                 /* one run kept in HBM */
     uint32_t ctf_count, n_tail, end_time_min, source;
 };
 
+/* exact summary of a CTF's decoded records, for planning range queries */
+struct CtfStats {
+    double box[4];                   /* union of the decoded record boxes */
+    uint32_t start_min, end_max;     /* over all records */
+    uint32_t n_fwd;                  /* records with pid < target */
+    uint32_t regular;                /* 1: every box has low <= high, every start <= end, no NaN */
+};
+
 struct QueryState {
     const Bucket *table;
     uint64_t nb;
@@ -39,6 +47,7 @@ struct QueryState {
     double map[4];
     std::vector<QueryRunDev> runs;
     std::vector<const stc_ctf_meta *> run_ctfs;  /* host copies of each run's CTF metadata */
+    std::vector<const CtfStats *> run_stats;     /* per run: [ctf_count] */
 };
 
 struct QueryBufs;                    /* device scratch of the query path, grown on demand */
@@ -46,5 +55,7 @@ QueryBufs *stc_query_bufs_new();
 void stc_query_bufs_free(QueryBufs *b);
 int stc_query_run(QueryBufs *b, const QueryState &st, const stc_query *q, uint32_t nq, stc_hit *hits, uint64_t cap,
                   uint64_t *n_hits, uint64_t *counts, hipStream_t s);
+/* decode every record of a run kept in HBM once and summarise each CTF */
+int stc_query_ctf_stats(const QueryRunDev &run, std::vector<CtfStats> &out, hipStream_t s);
 
 #endif /* STC_QUERY_H_ */

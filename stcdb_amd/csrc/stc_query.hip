@@ -23,6 +23,7 @@
 #include <cstring>
 
 #define QGRID 64
+#define QLDS 4096 /* batches up to this many queries count in LDS, block by block */
 #define MASK26 ((1u << 26) - 1u)
 
 struct QDev {
@@ -36,9 +37,11 @@ struct QDev {
     const uint32_t *g_list;
     double x0, y0, dx, dy;
     stc_hit *hits;
-    uint64_t cap;
+    uint64_t cap;                    /* 0: counts only */
     unsigned long long *n_hits;
     unsigned long long *counts;
+    uint32_t nq;
+    uint32_t lds;                    /* nq <= QLDS: per-block counters in LDS */
 };
 
 struct QueryBufs {
@@ -109,9 +112,25 @@ __host__ __device__ inline uint32_t oid_slot(uint32_t oid, uint32_t bits) {
     return (uint32_t)(((uint64_t)oid * 0x9E3779B97F4A7C15ull) >> (64 - bits));
 }
 
-__device__ inline void q_emit(const QDev &Q, uint32_t qi, uint32_t pid, uint32_t target, uint32_t start, uint32_t end,
-                              uint32_t source, const double *m) {
-    atomicAdd(&Q.counts[qi], 1ull);
+/* per-block query counters: cleared on entry, added to the global counts on
+ * exit (one atomic per query and block instead of one per hit) */
+__device__ inline void lc_clear(const QDev &Q, unsigned int *lc) {
+    if (Q.lds)
+        for (uint32_t i = threadIdx.x; i < Q.nq; i += blockDim.x) lc[i] = 0;
+    __syncthreads();
+}
+__device__ inline void lc_flush(const QDev &Q, unsigned int *lc) {
+    __syncthreads();
+    if (Q.lds)
+        for (uint32_t i = threadIdx.x; i < Q.nq; i += blockDim.x)
+            if (lc[i]) atomicAdd(&Q.counts[i], (unsigned long long)lc[i]);
+}
+
+__device__ inline void q_emit(const QDev &Q, unsigned int *lc, uint32_t qi, uint32_t pid, uint32_t target,
+                              uint32_t start, uint32_t end, uint32_t source, const double *m) {
+    if (Q.lds) atomicAdd(&lc[qi], 1u);
+    else atomicAdd(&Q.counts[qi], 1ull);
+    if (!Q.cap) return;
     unsigned long long k = atomicAdd(Q.n_hits, 1ull);
     if (k < Q.cap) {
         stc_hit h;
@@ -135,8 +154,8 @@ __device__ inline bool q_box(const stc_query &q, const double *m) {
 }
 
 /* object and pair queries of one meeting seen from pid */
-__device__ inline void match_oid(const QDev &Q, uint32_t pid, uint32_t target, uint32_t s, uint32_t e, uint32_t src,
-                                 const double *m) {
+__device__ inline void match_oid(const QDev &Q, unsigned int *lc, uint32_t pid, uint32_t target, uint32_t s,
+                                 uint32_t e, uint32_t src, const double *m) {
     if (!Q.oh_bits) return;
     const uint32_t mask = (1u << Q.oh_bits) - 1u;
     uint32_t h = oid_slot(pid, Q.oh_bits);
@@ -150,13 +169,13 @@ __device__ inline void match_oid(const QDev &Q, uint32_t pid, uint32_t target, u
         const stc_query &q = Q.q[qi];
         if (!q_time(q, s, e)) continue;
         if (q.kind == STC_Q_OID || (q.kind == STC_Q_PAIR && q.oid2 == target))
-            q_emit(Q, (uint32_t)qi, pid, target, s, e, src, m);
+            q_emit(Q, lc, (uint32_t)qi, pid, target, s, e, src, m);
     }
 }
 
 /* range queries of one meeting (called once per meeting) */
-__device__ inline void match_range(const QDev &Q, uint32_t pid, uint32_t target, uint32_t s, uint32_t e, uint32_t src,
-                                   const double *m) {
+__device__ inline void match_range(const QDev &Q, unsigned int *lc, uint32_t pid, uint32_t target, uint32_t s,
+                                   uint32_t e, uint32_t src, const double *m) {
     if (!Q.n_range) return;
     const int cx0 = grid_cell(m[0], Q.x0, Q.dx), cx1 = grid_cell(m[2], Q.x0, Q.dx);
     const int cy0 = grid_cell(m[1], Q.y0, Q.dy), cy1 = grid_cell(m[3], Q.y0, Q.dy);
@@ -169,7 +188,7 @@ __device__ inline void match_range(const QDev &Q, uint32_t pid, uint32_t target,
                 if (!q_time(q, s, e) || !q_box(q, m)) continue;
                 if (grid_cell(fmax(m[0], q.box[0]), Q.x0, Q.dx) != cx || grid_cell(fmax(m[1], q.box[1]), Q.y0, Q.dy) != cy)
                     continue;
-                q_emit(Q, qi, pid, target, s, e, src, m);
+                q_emit(Q, lc, qi, pid, target, s, e, src, m);
             }
         }
 }
@@ -231,40 +250,41 @@ __device__ inline void decode_tail(const uint64_t *o_keys, const float4 *o_boxes
 }
 
 __global__ __launch_bounds__(256) void k_q_live(const Bucket *__restrict__ t, uint64_t nb, uint32_t last_t, QDev Q) {
+    __shared__ unsigned int lc[QLDS];
+    lc_clear(Q, lc);
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nb; i += stride) {
         Bucket b = t[i];
         if (key_free(b.key) || key_tomb(b.key) || b.end != last_t) continue;
         const uint32_t a = (uint32_t)(b.key >> 32), c = (uint32_t)b.key;
         double m[4] = {o2f(b.mbr[0]), o2f(b.mbr[1]), o2f(b.mbr[2]), o2f(b.mbr[3])};
-        match_oid(Q, a, c, b.start, b.end, 0, m);
-        match_oid(Q, c, a, b.start, b.end, 0, m);
-        match_range(Q, a, c, b.start, b.end, 0, m);
+        match_oid(Q, lc, a, c, b.start, b.end, 0, m);
+        match_oid(Q, lc, c, a, b.start, b.end, 0, m);
+        match_range(Q, lc, a, c, b.start, b.end, 0, m);
     }
+    lc_flush(Q, lc);
 }
 
 __global__ __launch_bounds__(256) void k_q_mem(const uint64_t *__restrict__ keys, const float4 *__restrict__ boxes,
                                                uint64_t n, uint32_t base, QDev Q) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    uint32_t pid, target, start, end;
-    double m[4];
-    decode_tail(keys, boxes, (uint32_t)i, base, &pid, &target, &start, &end, m);
-    match_oid(Q, pid, target, start, end, 1, m);
-    if (pid < target) match_range(Q, pid, target, start, end, 1, m);
+    __shared__ unsigned int lc[QLDS];
+    lc_clear(Q, lc);
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        uint32_t pid, target, start, end;
+        double m[4];
+        decode_tail(keys, boxes, (uint32_t)i, base, &pid, &target, &start, &end, m);
+        match_oid(Q, lc, pid, target, start, end, 1, m);
+        if (pid < target) match_range(Q, lc, pid, target, start, end, 1, m);
+    }
+    lc_flush(Q, lc);
 }
 
 /* object and pair queries against one run: the object's CTF by its sid, then a
  * binary search of the CTF's pid-sorted packed keys; oversize objects (sid 1)
  * scan the oversize buffer */
-__global__ __launch_bounds__(256) void k_q_run_oid(const QueryRunDev *__restrict__ runs, uint32_t nruns,
-                                                   const uint32_t *__restrict__ oq, uint32_t noq, uint32_t n_objects,
-                                                   QDev Q) {
-    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= (uint64_t)noq * nruns) return;
-    const uint32_t qi = oq[g % noq], r = (uint32_t)(g / noq);
+__device__ inline void run_oid(const QDev &Q, unsigned int *lc, const QueryRunDev &R, uint32_t qi, uint32_t n_objects) {
     const stc_query q = Q.q[qi];
-    const QueryRunDev R = runs[r];
     if (q.oid >= n_objects) return;
     const uint16_t sid = R.sids[q.oid];
     uint32_t pid, target, start, end;
@@ -274,7 +294,7 @@ __global__ __launch_bounds__(256) void k_q_run_oid(const QueryRunDev *__restrict
         for (uint32_t i = 0; i < R.n_tail; i++) {
             decode_tail(R.o_keys, R.o_boxes, i, R.end_time_min, &pid, &target, &start, &end, m);
             if (pid != q.oid || !q_time(q, start, end)) continue;
-            if (q.kind == STC_Q_OID || target == q.oid2) q_emit(Q, qi, pid, target, start, end, R.source, m);
+            if (q.kind == STC_Q_OID || target == q.oid2) q_emit(Q, lc, qi, pid, target, start, end, R.source, m);
         }
         return;
     }
@@ -294,14 +314,26 @@ __global__ __launch_bounds__(256) void k_q_run_oid(const QueryRunDev *__restrict
         if (packed_pid(ctf, k) != q.oid) break;
         decode_packed(ctf, k, &pid, &target, &start, &end, m);
         if (!q_time(q, start, end)) continue;
-        if (q.kind == STC_Q_OID || target == q.oid2) q_emit(Q, qi, pid, target, start, end, R.source, m);
+        if (q.kind == STC_Q_OID || target == q.oid2) q_emit(Q, lc, qi, pid, target, start, end, R.source, m);
     }
+}
+
+__global__ __launch_bounds__(256) void k_q_run_oid(const QueryRunDev *__restrict__ runs, uint32_t nruns,
+                                                   const uint32_t *__restrict__ oq, uint32_t noq, uint32_t n_objects,
+                                                   QDev Q) {
+    __shared__ unsigned int lc[QLDS];
+    lc_clear(Q, lc);
+    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < (uint64_t)noq * nruns) run_oid(Q, lc, runs[g / noq], oq[g % noq], n_objects);
+    lc_flush(Q, lc);
 }
 
 /* range queries: one block per (query, run, CTF) candidate; CTF 0xFFFFFFFF is
  * the run's oversize buffer */
 __global__ __launch_bounds__(256) void k_q_run_range(const QueryRunDev *__restrict__ runs,
                                                      const uint4 *__restrict__ cand, QDev Q) {
+    __shared__ unsigned int lc[QLDS];
+    lc_clear(Q, lc);
     const uint4 cd = cand[blockIdx.x];
     const uint32_t qi = cd.x;
     const stc_query q = Q.q[qi];
@@ -312,16 +344,77 @@ __global__ __launch_bounds__(256) void k_q_run_range(const QueryRunDev *__restri
         for (uint32_t i = threadIdx.x; i < R.n_tail; i += blockDim.x) {
             decode_tail(R.o_keys, R.o_boxes, i, R.end_time_min, &pid, &target, &start, &end, m);
             if (pid < target && q_time(q, start, end) && q_box(q, m))
-                q_emit(Q, qi, pid, target, start, end, R.source, m);
+                q_emit(Q, lc, qi, pid, target, start, end, R.source, m);
         }
-        return;
+    } else {
+        const stc_ctf_meta ctf = R.ctfs[cd.z];
+        const uint32_t kb = ctf.key_bit / 8;
+        const uint8_t *base = R.key_bytes + R.byte_base[cd.z];
+        for (uint32_t i = threadIdx.x; i < ctf.kv_capacity; i += blockDim.x) {
+            decode_packed(ctf, load_packed(base + (uint64_t)i * kb, kb), &pid, &target, &start, &end, m);
+            if (pid < target && q_time(q, start, end) && q_box(q, m))
+                q_emit(Q, lc, qi, pid, target, start, end, R.source, m);
+        }
     }
-    const stc_ctf_meta ctf = R.ctfs[cd.z];
+    lc_flush(Q, lc);
+}
+
+/* one block per CTF: exact union box, time span and forward-record count */
+__global__ __launch_bounds__(256) void k_ctf_stats(QueryRunDev R, CtfStats *out) {
+    __shared__ double sb[4][256];
+    __shared__ uint32_t su[4][256];
+    const uint32_t c = blockIdx.x, tid = threadIdx.x;
+    const stc_ctf_meta ctf = R.ctfs[c];
     const uint32_t kb = ctf.key_bit / 8;
-    const uint8_t *base = R.key_bytes + R.byte_base[cd.z];
-    for (uint32_t i = threadIdx.x; i < ctf.kv_capacity; i += blockDim.x) {
+    const uint8_t *base = R.key_bytes + R.byte_base[c];
+    double b0 = INFINITY, b1 = INFINITY, b2 = -INFINITY, b3 = -INFINITY;
+    uint32_t smin = 0xFFFFFFFFu, emax = 0, fwd = 0, irregular = 0;
+    for (uint32_t i = tid; i < ctf.kv_capacity; i += blockDim.x) {
+        uint32_t pid, target, start, end;
+        double m[4];
         decode_packed(ctf, load_packed(base + (uint64_t)i * kb, kb), &pid, &target, &start, &end, m);
-        if (pid < target && q_time(q, start, end) && q_box(q, m)) q_emit(Q, qi, pid, target, start, end, R.source, m);
+        if (!(m[0] <= m[2]) || !(m[1] <= m[3]) || start > end) irregular = 1; /* also catches NaN */
+        b0 = fmin(b0, m[0]);
+        b1 = fmin(b1, m[1]);
+        b2 = fmax(b2, m[2]);
+        b3 = fmax(b3, m[3]);
+        smin = min(smin, start);
+        emax = max(emax, end);
+        fwd += pid < target;
+    }
+    sb[0][tid] = b0;
+    sb[1][tid] = b1;
+    sb[2][tid] = b2;
+    sb[3][tid] = b3;
+    su[0][tid] = smin;
+    su[1][tid] = emax;
+    su[2][tid] = fwd;
+    su[3][tid] = irregular;
+    __syncthreads();
+    for (uint32_t o = blockDim.x / 2; o > 0; o >>= 1) {
+        if (tid < o) {
+            sb[0][tid] = fmin(sb[0][tid], sb[0][tid + o]);
+            sb[1][tid] = fmin(sb[1][tid], sb[1][tid + o]);
+            sb[2][tid] = fmax(sb[2][tid], sb[2][tid + o]);
+            sb[3][tid] = fmax(sb[3][tid], sb[3][tid + o]);
+            su[0][tid] = min(su[0][tid], su[0][tid + o]);
+            su[1][tid] = max(su[1][tid], su[1][tid + o]);
+            su[2][tid] += su[2][tid + o];
+            su[3][tid] |= su[3][tid + o];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        CtfStats st;
+        st.box[0] = sb[0][0];
+        st.box[1] = sb[1][0];
+        st.box[2] = sb[2][0];
+        st.box[3] = sb[3][0];
+        st.start_min = su[0][0];
+        st.end_max = su[1][0];
+        st.n_fwd = su[2][0];
+        st.regular = !su[3][0];
+        out[c] = st;
     }
 }
 
@@ -332,6 +425,20 @@ static uint32_t blocks(uint64_t n, uint32_t b) { return (uint32_t)std::max<uint6
         hipError_t e_ = (expr);                                                                     \
         if (e_ != hipSuccess) return stc_hip_fail(e_, #expr, __FILE__, __LINE__);                   \
     } while (0)
+
+int stc_query_ctf_stats(const QueryRunDev &run, std::vector<CtfStats> &out, hipStream_t s) {
+    out.assign(run.ctf_count, CtfStats());
+    if (!run.ctf_count) return STC_OK;
+    CtfStats *d = nullptr;
+    QCK(hipMalloc((void **)&d, run.ctf_count * sizeof(CtfStats)));
+    hipLaunchKernelGGL(k_ctf_stats, dim3(run.ctf_count), dim3(256), 0, s, run, d);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(out.data(), d, run.ctf_count * sizeof(CtfStats), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    (void)hipFree(d);
+    if (e != hipSuccess) return stc_hip_fail(e, "CTF statistics", __FILE__, __LINE__);
+    return STC_OK;
+}
 
 /* the decoded boxes of a CTF lie in [low, low + |extent| + longer edge] per
  * axis (new_parse_mbr): a conservative test, widened by 1e-6 degrees for the
@@ -430,9 +537,13 @@ int stc_query_run(QueryBufs *B, const QueryState &st, const stc_query *q, uint32
     Q.dx = dx;
     Q.dy = dy;
     Q.hits = B->hits;
-    Q.cap = cap;
+    Q.cap = hits ? cap : 0;
     Q.n_hits = B->counters;
     Q.counts = B->counters + 1;
+    Q.nq = nq;
+    Q.lds = nq <= QLDS;
+    /* counts a planner adds without a kernel (CTFs wholly inside a range query) */
+    std::vector<unsigned long long> planned(nq, 0);
 
     /* live meetings and memtable */
     if (st.stepped && st.table && st.nb) {
@@ -441,8 +552,8 @@ int stc_query_run(QueryBufs *B, const QueryState &st, const stc_query *q, uint32
         QCK(hipGetLastError());
     }
     if (st.n_rec) {
-        hipLaunchKernelGGL(k_q_mem, dim3(blocks(st.n_rec, 256)), dim3(256), 0, s, st.rec_keys, st.rec_boxes, st.n_rec,
-                           st.end_time_min, Q);
+        hipLaunchKernelGGL(k_q_mem, dim3(std::min<uint32_t>(blocks(st.n_rec, 256), 8192)), dim3(256), 0, s,
+                           st.rec_keys, st.rec_boxes, st.n_rec, st.end_time_min, Q);
         QCK(hipGetLastError());
     }
     /* runs */
@@ -458,14 +569,33 @@ int stc_query_run(QueryBufs *B, const QueryState &st, const stc_query *q, uint32
         }
         if (!rq.empty()) {
             std::vector<uint4> cand;
-            for (uint32_t qi : rq)
+            for (uint32_t qi : rq) {
+                const stc_query &qq = q[qi];
                 for (uint32_t r = 0; r < nruns; r++) {
                     const stc_ctf_meta *ctfs = st.run_ctfs[r];
-                    for (uint32_t c = 0; c < st.runs[r].ctf_count; c++)
-                        if (ctfs[c].kv_capacity && ctf_may_intersect(ctfs[c], q[qi].box))
-                            cand.push_back(make_uint4(qi, r, c, 0));
+                    const CtfStats *cs = st.run_stats[r];
+                    for (uint32_t c = 0; c < st.runs[r].ctf_count; c++) {
+                        if (!ctfs[c].kv_capacity) continue;
+                        if (cs && cs[c].regular) {
+                            const CtfStats &x = cs[c];
+                            /* no record can match */
+                            if (!(x.box[0] <= qq.box[2] && x.box[2] >= qq.box[0] && x.box[1] <= qq.box[3] &&
+                                  x.box[3] >= qq.box[1]) || x.start_min > qq.t1 || x.end_max < qq.t0)
+                                continue;
+                            /* every forward record matches: count without decoding */
+                            if (!Q.cap && x.box[0] >= qq.box[0] && x.box[2] <= qq.box[2] && x.box[1] >= qq.box[1] &&
+                                x.box[3] <= qq.box[3] && x.start_min >= qq.t0 && x.end_max <= qq.t1) {
+                                planned[qi] += x.n_fwd;
+                                continue;
+                            }
+                        } else if (!ctf_may_intersect(ctfs[c], qq.box)) {
+                            continue;
+                        }
+                        cand.push_back(make_uint4(qi, r, c, 0));
+                    }
                     if (st.runs[r].n_tail) cand.push_back(make_uint4(qi, r, 0xFFFFFFFFu, 0));
                 }
+            }
             if (!cand.empty()) {
                 if (grow(&B->cand, &B->cand_cap, cand.size()))
                     return stc_hip_fail(hipErrorOutOfMemory, "query candidates", __FILE__, __LINE__);
@@ -480,10 +610,14 @@ int stc_query_run(QueryBufs *B, const QueryState &st, const stc_query *q, uint32
     std::vector<unsigned long long> cnt((uint64_t)nq + 1);
     QCK(hipMemcpyAsync(cnt.data(), B->counters, cnt.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     QCK(hipStreamSynchronize(s));
-    *n_hits = cnt[0];
-    if (counts)
-        for (uint32_t i = 0; i < nq; i++) counts[i] = cnt[i + 1];
-    uint64_t stored = std::min<uint64_t>(cnt[0], cap);
+    uint64_t total = 0;
+    for (uint32_t i = 0; i < nq; i++) {
+        const uint64_t ci = cnt[i + 1] + planned[i];
+        total += ci;
+        if (counts) counts[i] = ci;
+    }
+    *n_hits = total;
+    uint64_t stored = Q.cap ? std::min<uint64_t>(cnt[0], cap) : 0;
     if (stored && hits) {
         QCK(hipMemcpy(hits, B->hits, stored * sizeof(stc_hit), hipMemcpyDeviceToHost));
         std::sort(hits, hits + stored, [](const stc_hit &a, const stc_hit &b) {

@@ -11,7 +11,8 @@ lon -87.9401, lat 41.6446): x = lon0 + X * dpm_lon(lat0), y = lat0 + Y * dpm_lat
       hotspots (sigma 500 m), 3,600 steps, r = 20 m, minimum meeting 10 s
   c3  10,000,000 objects, c2's walk and hotspot mix, 600 steps, memtable
       runs of 8 Mi records, 256 CTFs
-  c5  5,000,000 objects with c2's mix, the serving workload
+  c5  5,000,000 objects with c2's mix, runs of 8 Mi records kept in HBM for
+      queries: the serving workload (ingest at 1 Hz plus 2,000 queries/s)
 """
 import math
 
@@ -67,7 +68,8 @@ WORKLOADS = {
                cfg=pipeline_config(10_000_000, 20.0, 10, grid_capacity=200, kv_restriction=8 * 1024 * 1024,
                                    ctf_count=256)),
     "c5": dict(gen=gen_params(5_000_000, 500, seed=5, hotspots=20), steps=3600,
-               cfg=pipeline_config(5_000_000, 20.0, 10, grid_capacity=200)),
+               cfg=dict(pipeline_config(5_000_000, 20.0, 10, grid_capacity=200, kv_restriction=8 * 1024 * 1024,
+                                        ctf_count=256), device_runs=1)),
 }
 
 

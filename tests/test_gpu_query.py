@@ -61,3 +61,30 @@ def test_query_counts_survive_a_small_cap():
     assert len(hits) == 7 and np.array_equal(gc, oc)
     gpu.close()
     orc.close()
+
+
+def test_count_only_queries_match_oracle():
+    # counts without hit lists take the planner's shortcut for CTFs wholly inside a range query
+    w = dense()
+    gpu = Pipeline(w["cfg"], w["box"])
+    orc = O.Oracle(w["cfg"], w["box"])
+    for t, xy in HostTrace(w["gen"]).steps(44):
+        gpu.step(xy, t)
+        orc.step(xy, t)
+    rng = np.random.default_rng(8)
+    q = make_queries(orc, w, rng, 120)
+    (lo0, lo1), (hi0, hi1) = w["box"]
+    big = q[:30].copy()
+    big["kind"] = 2
+    big["t0"], big["t1"] = 0, 1000
+    for i in range(30):  # nested boxes from a quarter of the map to all of it
+        f = 0.25 + 0.75 * i / 29
+        big[i]["box"] = (lo0, lo1, lo0 + f * (hi0 - lo0), lo1 + f * (hi1 - lo1))
+    q = np.concatenate([q, big])
+    _, oc = orc.query(q)
+    hits, gc = gpu.query(q, cap=0)
+    print(f"[query counts] {len(q)} queries, {int(oc.sum())} hits", flush=True)
+    assert len(hits) == 0 and oc[-1] > 100
+    assert np.array_equal(gc, oc)
+    gpu.close()
+    orc.close()
