@@ -176,6 +176,17 @@ int stc_drop_runs(void *ctx); /* release host copies of flushed runs */
  * that the copy overlaps the following steps; stc_get_run waits for its own run). */
 int stc_sync(void *ctx);
 
+/* ---- flushed runs on disk ----
+ * Writes run `run` as the reference's straight_dump lays it out
+ * (src/tracing/process.cpp:254-348): <dir>/<prefix>_SSTable_<ctb>-<c> (packed
+ * keys of CTF c), <dir>/<prefix>_CTB<ctb> (the 128-byte CTB image, sids,
+ * oversize keys, boxes and bitmap; workbench.cpp:597-614) and
+ * <dir>/<prefix>_STcL<ctb>-<c> (the 80-byte CTF image and its bitmap;
+ * step_merge.cpp:644-650). Pointer fields of the images are zero; CTB fields
+ * the reference's GPU path leaves unset carry the run's CTF count, start-time
+ * bounds, end-time base and flush time. */
+int stc_dump_run(void *ctx, uint32_t run, const char *dir, const char *prefix, uint32_t ctb);
+
 /* ---- queries over the resident state ----
  * The reference answers object-history, range and time queries from its
  * flushed runs (workbench::id_search_in_CTB, mbr_search_in_disk,

@@ -449,7 +449,17 @@ __device__ inline int table_upsert(Bucket *t, uint64_t nb, unsigned long long ke
     uint64_t h = hash_slot(key, nb);
     for (uint32_t probe = 0; probe < MAX_PROBES; probe++) {
         Bucket *b = &t[h];
-        Bucket cur = *b;
+        Bucket cur; /* the key half first; the MBR half only for the matching bucket */
+        {
+            const uint4 lo4 = *(const uint4 *)b; /* key, start, end */
+            cur.key = ((unsigned long long)lo4.y << 32) | lo4.x;
+            cur.start = lo4.z;
+            cur.end = lo4.w;
+        }
+        if (cur.key == key) {
+            const uint4 hi4 = *((const uint4 *)b + 1);
+            cur.mbr[0] = hi4.x; cur.mbr[1] = hi4.y; cur.mbr[2] = hi4.z; cur.mbr[3] = hi4.w;
+        }
         bool claimed = false;
         if (key_free(cur.key)) {
             unsigned long long prev = atomicCAS(&b->key, cur.key, key);
@@ -477,10 +487,12 @@ __device__ inline int table_upsert(Bucket *t, uint64_t nb, unsigned long long ke
         uint32_t old_end = cur.end;
         if (old_end != now && old_end != now - 1) b->start = now; /* restarted (retired key) or racing a claim */
         b->end = now;
-        b->mbr[0] = min(cur.mbr[0], f2o(mbr[0]));
-        b->mbr[1] = min(cur.mbr[1], f2o(mbr[1]));
-        b->mbr[2] = max(cur.mbr[2], f2o(mbr[2]));
-        b->mbr[3] = max(cur.mbr[3], f2o(mbr[3]));
+        {
+            const uint32_t n0 = min(cur.mbr[0], f2o(mbr[0])), n1 = min(cur.mbr[1], f2o(mbr[1]));
+            const uint32_t n2 = max(cur.mbr[2], f2o(mbr[2])), n3 = max(cur.mbr[3], f2o(mbr[3]));
+            if (n0 != cur.mbr[0] || n1 != cur.mbr[1] || n2 != cur.mbr[2] || n3 != cur.mbr[3])
+                *((uint4 *)b + 1) = make_uint4(n0, n1, n2, n3); /* only a grown MBR is written */
+        }
         return (old_end == now - 1) ? 2 : 0;
     }
     return -1;
@@ -1870,6 +1882,101 @@ int stc_sync(void *vc) {
     if (!c) return STC_E_ARG;
     CK(hipStreamSynchronize(c->s));
     if (c->flushw.cs) CK(hipStreamSynchronize(c->flushw.cs));
+    return STC_OK;
+}
+
+/* x86-64 images of the reference's CTF and CTB classes (step_merge.h:64-131) */
+struct RefCtfImage {
+    float mbr[4];
+    uint64_t keys, bitmap;
+    uint32_t kv_capacity, start_time_min, start_time_max, end_time_min, end_time_max, bitmap_size;
+    uint16_t key_bit, id_bit, duration_bit, end_bit, low_x_bit, low_y_bit, edge_bit, mbr_bit, x_grid, y_grid;
+    uint32_t pad;
+};
+struct RefCtbImage {
+    uint64_t ctfs;
+    uint32_t ctf_count, start_time_min, start_time_max, end_time_min, end_time_max, pad0;
+    uint64_t first_widpid, sids, bitmaps, bitmap_mbrs, ctf_capacity;
+    uint32_t o_count, pad1;
+    uint64_t o_keys, o_boxes, o_bitmaps;
+    uint32_t o_start_time_min, o_start_time_max, o_end_time_min, o_end_time_max;
+    uint64_t box_rtree;
+};
+static_assert(sizeof(RefCtfImage) == 80 && sizeof(RefCtbImage) == 128, "reference struct images");
+
+static bool write_file(const std::string &path, const void *a, size_t na, const void *b = nullptr, size_t nb = 0,
+                       const void *c = nullptr, size_t nc = 0, const void *d = nullptr, size_t nd = 0,
+                       const void *e = nullptr, size_t ne = 0) {
+    FILE *f = fopen(path.c_str(), "wb");
+    if (!f) return false;
+    bool ok = true;
+    const void *p[] = {a, b, c, d, e};
+    const size_t n[] = {na, nb, nc, nd, ne};
+    for (int i = 0; i < 5 && ok; i++)
+        if (n[i]) ok = fwrite(p[i], 1, n[i], f) == n[i];
+    ok = (fclose(f) == 0) && ok;
+    return ok;
+}
+
+int stc_dump_run(void *vc, uint32_t run, const char *dir, const char *prefix, uint32_t ctb) {
+    StcCtx *c = (StcCtx *)vc;
+    if (!c || !dir || !prefix || run >= c->runs.size()) return STC_E_ARG;
+    RunHost &R = c->runs[run];
+    CK(hipEventSynchronize(R.blk.ev));
+    const uint8_t *src = R.blk.p;
+    const RunLayout &L = R.lay;
+    const stc_run_info &I = R.info;
+    const std::string base = std::string(dir) + "/" + prefix;
+    uint64_t off = 0;
+    uint32_t smin = 0xFFFFFFFFu, smax = 0;
+    for (uint32_t k = 0; k < I.ctf_count; k++) {
+        const stc_ctf_meta &m = R.ctfs[k];
+        const uint64_t nbytes = (uint64_t)m.kv_capacity * (m.key_bit / 8);
+        const std::string tag = std::to_string(ctb) + "-" + std::to_string(k);
+        if (!write_file(base + "_SSTable_" + tag, src + L.key_bytes + off, nbytes))
+            return fail(c, STC_E_ARG, "cannot write " + base + "_SSTable_" + tag);
+        off += nbytes;
+        RefCtfImage img;
+        memset(&img, 0, sizeof(img));
+        memcpy(img.mbr, m.mbr, sizeof(img.mbr));
+        img.kv_capacity = m.kv_capacity;
+        img.start_time_min = m.start_time_min;
+        img.start_time_max = m.start_time_max;
+        img.end_time_min = m.end_time_min;
+        img.end_time_max = m.end_time_max;
+        img.bitmap_size = m.bitmap_size;
+        img.key_bit = m.key_bit;
+        img.id_bit = m.id_bit;
+        img.duration_bit = m.duration_bit;
+        img.end_bit = m.end_bit;
+        img.low_x_bit = m.low_x_bit;
+        img.low_y_bit = m.low_y_bit;
+        img.edge_bit = m.edge_bit;
+        img.mbr_bit = m.mbr_bit;
+        img.x_grid = m.x_grid;
+        img.y_grid = m.y_grid;
+        if (!write_file(base + "_STcL" + tag, &img, sizeof(img), src + L.bitmaps + (uint64_t)k * 8192,
+                        std::min<uint32_t>(m.bitmap_size, 8192)))
+            return fail(c, STC_E_ARG, "cannot write " + base + "_STcL" + tag);
+        smin = std::min(smin, m.start_time_min);
+        smax = std::max(smax, m.start_time_max);
+    }
+    RefCtbImage img;
+    memset(&img, 0, sizeof(img));
+    img.ctf_count = I.ctf_count;
+    img.start_time_min = I.ctf_count ? smin : 0;
+    img.start_time_max = smax;
+    img.end_time_min = I.end_time_min;
+    img.end_time_max = I.cur_time;
+    img.o_count = I.oversize_count;
+    img.o_start_time_min = I.o_start_min;
+    img.o_start_time_max = I.o_start_max;
+    img.o_end_time_min = I.o_end_min;
+    img.o_end_time_max = I.o_end_max;
+    if (!write_file(base + "_CTB" + std::to_string(ctb), &img, sizeof(img), src + L.sids, (size_t)I.num_objects * 2,
+                    src + L.o_keys, (size_t)I.oversize_count * 16, src + L.o_boxes, (size_t)I.oversize_count * 16,
+                    src + L.o_bitmap, 8192))
+        return fail(c, STC_E_ARG, "cannot write " + base + "_CTB" + std::to_string(ctb));
     return STC_OK;
 }
 

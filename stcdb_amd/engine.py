@@ -206,6 +206,10 @@ class Pipeline:
         _check(self.lib.stc_query_batch(self.ctx, _ptr(q), len(q), _ptr(hits), cap, C.byref(n), _ptr(counts)), self.ctx)
         return hits[:min(n.value, cap)], counts
 
+    def dump_run(self, i, directory, prefix, ctb):
+        """write run i in the reference's file layout (stc_dump_run)"""
+        _check(self.lib.stc_dump_run(self.ctx, i, directory.encode(), prefix.encode(), ctb), self.ctx)
+
     def drop_runs(self):
         _check(self.lib.stc_drop_runs(self.ctx), self.ctx)
 
