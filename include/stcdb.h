@@ -228,6 +228,14 @@ typedef struct stc_hit {
 int stc_query_batch(void *ctx, const stc_query *q, uint32_t nq, stc_hit *hits, uint64_t cap, uint64_t *n_hits,
               uint64_t *counts);
 
+/* Live contact search (the reference's search_single / search_multi inside
+ * cuda_identify_meetings, src/cuda/geometry.cu:605-662, driven from
+ * src/tracing/process.cpp:465-495): the meetings live at the last step that
+ * have lasted at least min_meet_time + 1 steps and involve one of `pids`, as
+ * stc_hit records seen from the searched object (source 0). A second call
+ * with the targets found is the reference's second hop. */
+int stc_live_search(void *ctx, const uint32_t *pids, uint32_t n, stc_hit *hits, uint64_t cap, uint64_t *n_hits);
+
 /* ---- seeded synthetic traces (the reference's generator is unseeded) ----
  * Parameter and state layouts are those of stcdb_amd/csrc/stc_gen.h. */
 typedef struct stc_gen_params stc_gen_params;

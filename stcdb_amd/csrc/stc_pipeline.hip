@@ -1885,6 +1885,28 @@ int stc_sync(void *vc) {
     return STC_OK;
 }
 
+int stc_live_search(void *vc, const uint32_t *pids, uint32_t n, stc_hit *hits, uint64_t cap, uint64_t *n_hits) {
+    StcCtx *c = (StcCtx *)vc;
+    if (!c || (!pids && n) || !n_hits) return STC_E_ARG;
+    if (!c->stepped) return fail(c, STC_E_STATE, "no step processed");
+    /* live at t and established (t - start >= min_meet + 1): start <= t - min_meet - 1 and end >= t,
+     * the overlap predicate of an object query with t0 = t, t1 = t - min_meet - 1 */
+    const uint32_t t = c->last_t, m = c->cfg.min_meet_time + 1;
+    if (t < m) {
+        *n_hits = 0;
+        return STC_OK;
+    }
+    std::vector<stc_query> q(n);
+    for (uint32_t i = 0; i < n; i++) {
+        memset(&q[i], 0, sizeof(stc_query));
+        q[i].kind = STC_Q_OID;
+        q[i].oid = pids[i];
+        q[i].t0 = t;
+        q[i].t1 = t - m;
+    }
+    return stc_query_batch(vc, q.data(), n, hits, cap, n_hits, nullptr);
+}
+
 /* x86-64 images of the reference's CTF and CTB classes (step_merge.h:64-131) */
 struct RefCtfImage {
     float mbr[4];

@@ -206,6 +206,17 @@ class Pipeline:
         _check(self.lib.stc_query_batch(self.ctx, _ptr(q), len(q), _ptr(hits), cap, C.byref(n), _ptr(counts)), self.ctx)
         return hits[:min(n.value, cap)], counts
 
+    def live_search(self, pids):
+        """established live contacts of the given objects (stc_live_search);
+        hits as HIT_DTYPE with `query` the index into pids"""
+        pids = np.ascontiguousarray(pids, dtype=np.uint32)
+        n = C.c_uint64()
+        _check(self.lib.stc_live_search(self.ctx, _ptr(pids), len(pids), None, 0, C.byref(n)), self.ctx)
+        hits = np.zeros(n.value, dtype=HIT_DTYPE)
+        _check(self.lib.stc_live_search(self.ctx, _ptr(pids), len(pids), _ptr(hits), len(hits), C.byref(n)),
+               self.ctx)
+        return hits
+
     def dump_run(self, i, directory, prefix, ctb):
         """write run i in the reference's file layout (stc_dump_run)"""
         _check(self.lib.stc_dump_run(self.ctx, i, directory.encode(), prefix.encode(), ctb), self.ctx)
