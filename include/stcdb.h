@@ -91,7 +91,8 @@ typedef struct stc_step_out {
     float ms_total;
     float ms_refine_kernel;    /* the fused refinement + table-insert kernel (last launch) */
     float ms_identify_kernel;  /* the identify kernel */
-    uint64_t continuing;       /* pairs that continued a meeting seen at the previous step */
+    uint64_t continuing;       /* pairs that continued a meeting seen at the previous step (a pair found
+                                  twice in one step, an ulp sliver between leaves, may count twice) */
     uint64_t new_keys;         /* pairs that opened a new table entry */
 } stc_step_out;
 

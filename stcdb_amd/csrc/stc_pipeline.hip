@@ -460,13 +460,14 @@ __device__ inline int table_upsert(Bucket *t, uint64_t nb, unsigned long long ke
             const uint4 hi4 = *((const uint4 *)b + 1);
             cur.mbr[0] = hi4.x; cur.mbr[1] = hi4.y; cur.mbr[2] = hi4.z; cur.mbr[3] = hi4.w;
         }
-        bool claimed = false;
+        bool claimed = false, raced = false;
         if (key_free(cur.key)) {
             unsigned long long prev = atomicCAS(&b->key, cur.key, key);
             if (prev == cur.key) {
                 claimed = true;
-            } else if (prev == key) {
+            } else if (prev == key) { /* another thread claimed this very pair this step */
                 cur = *b;
+                raced = true;
             } else {
                 h = (h + 1 == nb) ? 0 : h + 1;
                 continue;
@@ -493,7 +494,9 @@ __device__ inline int table_upsert(Bucket *t, uint64_t nb, unsigned long long ke
             if (n0 != cur.mbr[0] || n1 != cur.mbr[1] || n2 != cur.mbr[2] || n3 != cur.mbr[3])
                 *((uint4 *)b + 1) = make_uint4(n0, n1, n2, n3); /* only a grown MBR is written */
         }
-        return (old_end == now - 1) ? 2 : 0;
+        /* a free bucket's end field reads 0xFFFFFFFF until its claimer writes it: at
+         * now == 0 that equals now - 1, so nothing continues into step 0 */
+        return (!raced && now != 0 && old_end == now - 1) ? 2 : 0;
     }
     return -1;
 }
@@ -506,6 +509,7 @@ __global__ void k_table_insert(Bucket *t, uint64_t nb, const PairHit *__restrict
     PairHit h = pairs[i];
     int rc = table_upsert(t, nb, h.key, h.mbr, now);
     if (rc < 0) atomicAdd(&cnt->table_fail, 1u);
+    if (rc == 1) atomicAdd(&cnt->new_keys, 1ull);
     if (rc == 2) atomicAdd(&cnt->continuing, 1ull);
 }
 
@@ -1740,7 +1744,7 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
         hipEventElapsedTime(&out->ms_refine_kernel, c->ev[6], c->ev[7]);
         hipEventElapsedTime(&out->ms_identify_kernel, c->ev[8], c->ev[9]);
         out->continuing = h_cont;
-        out->new_keys = c->last_new;
+        out->new_keys = h.new_keys;
     }
     return STC_OK;
 }
