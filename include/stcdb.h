@@ -172,6 +172,16 @@ int stc_get_run(void *ctx, uint32_t run, stc_ctf_meta *ctfs, uint8_t *key_bytes,
                 uint16_t *sids, uint64_t *o_keys, float *o_boxes, uint8_t *o_bitmap,
                 uint64_t *sorted_keys, float *sorted_boxes);
 int stc_drop_runs(void *ctx); /* release host copies of flushed runs */
+/* Compaction: merge runs [first, first + count) into one run that takes their
+ * place (later runs move down by count - 1). The records of the inputs are
+ * decoded as the reference's compaction reads them (CTF::parse_key and
+ * CTF::new_parse_mbr_f_box, tools/compaction.cpp:320-336), their end times
+ * re-based on the earliest input base (compaction.cpp:483-487), and flushed
+ * again: STR partition, CTF encoding and bitmaps as in a step's flush, with
+ * objects oversize in any input kept oversize (compaction.cpp:259-268). The
+ * inputs must be kept in HBM (stc_config.device_runs). Replaces the offline
+ * merge of tools/compaction.cpp:729-1206. */
+int stc_compact_runs(void *ctx, uint32_t first, uint32_t count);
 /* Wait until all work of the context is done, including the background copies
  * of flushed runs into host memory (a flush hands its run to a copy stream so
  * that the copy overlaps the following steps; stc_get_run waits for its own run). */

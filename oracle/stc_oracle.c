@@ -123,6 +123,7 @@ typedef struct orc {
     uint32_t (*o_boxs)[4];      /* quantized per-object MBR: low0, low1, high0, high1 */
     uint16_t *sids;
     uint32_t sid_count, oversize_oid_count;
+    int sticky;                 /* compaction flush: objects entering with sids 1 stay oversize */
 
     run *runs; uint32_t n_runs;
 } orc;
@@ -759,7 +760,7 @@ static int flush(orc *o, uint32_t t) {
             float low0 = q2f(o->o_boxs[id][0]), high0 = q2f(o->o_boxs[id][2]);
             float low1 = q2f(o->o_boxs[id][1]), high1 = q2f(o->o_boxs[id][3]);
             float area = (high1 - low1) * (high0 - low0);
-            if ((double)area > o->cfg.area_threshold) {
+            if ((o->sticky && o->sids[id] == 1) || (double)area > o->cfg.area_threshold) {
                 o->oversize_oid_count++;
                 o->sids[id] = 1;
                 continue;
@@ -1186,6 +1187,141 @@ static void decode_packed(const orc_ctf *c, const uint8_t *p, uint32_t *pid, uin
 }
 
 uint32_t orc_end_time_min(void *o) { return ((orc *)o)->end_time_min; }
+
+/* ------------------------------------------------------------------ */
+/* compaction                                                          */
+/* ------------------------------------------------------------------ */
+
+/* CTF::new_parse_mbr_f_box (step_merge.cpp:381-401): each corner is the double
+ * expression of new_parse_mbr stored to float, and the high corner adds to
+ * the stored (float) low corner */
+static void decode_fbox(const orc_ctf *c, const uint8_t *p, float *b) {
+    u128 k = 0;
+    memcpy(&k, p, c->key_bit / 8);
+    uint64_t vm = (uint64_t)(k & (((u128)1 << c->mbr_bit) - 1));
+    uint64_t mask_edge = (1ull << c->edge_bit) - 1, mask_low_y = (1ull << c->low_y_bit) - 1,
+             mask_low_x = (1ull << c->low_x_bit) - 1;
+    uint64_t y = vm & mask_edge;
+    vm >>= c->edge_bit;
+    uint64_t x = vm & mask_edge;
+    vm >>= c->edge_bit;
+    uint64_t low1 = vm & mask_low_y;
+    vm >>= c->low_y_bit;
+    uint64_t low0 = vm & mask_low_x;
+    float w0 = c->mbr[2] - c->mbr[0], w1 = c->mbr[3] - c->mbr[1];
+    float longer = w0 > w1 ? w0 : w1;
+    b[0] = (float)((double)c->mbr[0] + (double)low0 / (double)mask_low_x * (double)w0);
+    b[1] = (float)((double)c->mbr[1] + (double)low1 / (double)mask_low_y * (double)w1);
+    b[2] = (float)((double)b[0] + (double)x / (double)mask_edge * (double)longer);
+    b[3] = (float)((double)b[1] + (double)y / (double)mask_edge * (double)longer);
+}
+
+static int compact_add(orc *o, record *rc, uint32_t pid, uint32_t target, uint32_t dur, uint32_t end, uint32_t base,
+                       const float *box) {
+    if (pid >= o->cfg.num_objects || target >= o->cfg.num_objects) return -1;
+    rc->key = ((u128)pid << (OID_BIT + DURATION_BIT + END_BIT)) + ((u128)target << (DURATION_BIT + END_BIT)) +
+              ((u128)dur << END_BIT) + (u128)(uint32_t)(end - base);
+    memcpy(rc->box, box, sizeof(rc->box));
+    o->same_pid_count[pid]++;
+    uint32_t *ob = o->o_boxs[pid], q;
+    q = f2q(box[0]); if (q < ob[0]) ob[0] = q;
+    q = f2q(box[2]); if (q > ob[2]) ob[2] = q;
+    q = f2q(box[1]); if (q < ob[1]) ob[1] = q;
+    q = f2q(box[3]); if (q > ob[3]) ob[3] = q;
+    return 0;
+}
+
+/* Merge runs [first, first + count) into one run in their place: every record
+ * of the inputs, decoded as tools/compaction.cpp:320-336 reads them, becomes a
+ * memtable record whose end offset is relative to the earliest input base
+ * (compaction.cpp:483-487); the memtable is flushed at the latest input's
+ * flush time, with the objects oversize in any input kept oversize
+ * (compaction.cpp:259-268). */
+int orc_compact_runs(void *vo, uint32_t first, uint32_t count) {
+    orc *o = (orc *)vo;
+    if (count == 0 || (uint64_t)first + count > o->n_runs) { set_err(o, "run range out of bounds"); return -1; }
+    if (count == 1) return 0;
+    const uint32_t N = o->cfg.num_objects;
+    uint64_t total = 0;
+    uint32_t base = 0xFFFFFFFFu, tmax = 0;
+    for (uint32_t r = first; r < first + count; r++) {
+        const run *R = &o->runs[r];
+        for (uint32_t c = 0; c < R->info.ctf_count; c++) total += R->ctfs[c].kv_capacity;
+        total += R->info.oversize_count;
+        if (R->info.end_time_min < base) base = R->info.end_time_min;
+        if (R->info.cur_time > tmax) tmax = R->info.cur_time;
+    }
+    /* the memtable being filled by steps is set aside */
+    record *s_recs = o->recs; uint64_t s_n = o->n_recs, s_cap = o->cap_recs;
+    uint32_t *s_spc = o->same_pid_count; uint32_t (*s_obox)[4] = o->o_boxs; uint16_t *s_sids = o->sids;
+    uint32_t s_sid = o->sid_count, s_over = o->oversize_oid_count, s_etm = o->end_time_min,
+             s_kvr = o->cfg.kv_restriction;
+    o->recs = (record *)malloc(sizeof(record) * (total ? total : 1));
+    o->cap_recs = total ? total : 1;
+    o->n_recs = 0;
+    o->same_pid_count = (uint32_t *)calloc(N, sizeof(uint32_t));
+    o->o_boxs = (uint32_t (*)[4])malloc(sizeof(uint32_t) * 4 * N);
+    o->sids = (uint16_t *)calloc(N, sizeof(uint16_t));
+    int rc = -1;
+    if (!o->recs || !o->same_pid_count || !o->o_boxs || !o->sids) { set_err(o, "out of memory in compaction"); goto done; }
+    for (uint32_t i = 0; i < N; i++) {
+        o->o_boxs[i][0] = o->o_boxs[i][1] = 100000000u;
+        o->o_boxs[i][2] = o->o_boxs[i][3] = 0;
+    }
+    for (uint32_t r = first; r < first + count; r++) {
+        const run *R = &o->runs[r];
+        uint64_t off = 0;
+        for (uint32_t c = 0; c < R->info.ctf_count; c++) {
+            const orc_ctf *ctf = &R->ctfs[c];
+            uint32_t kb = ctf->key_bit / 8;
+            for (uint32_t i = 0; i < ctf->kv_capacity; i++) {
+                const uint8_t *p = R->key_bytes + off + (uint64_t)i * kb;
+                uint32_t pid, target, start, end;
+                double unused[4];
+                float box[4];
+                decode_packed(ctf, p, &pid, &target, &start, &end, unused);
+                decode_fbox(ctf, p, box);
+                if (compact_add(o, &o->recs[o->n_recs++], pid, target, end - start, end, base, box)) goto bad_id;
+            }
+            off += (uint64_t)ctf->kv_capacity * kb;
+        }
+        for (uint32_t i = 0; i < R->info.oversize_count; i++) {
+            u128 k = R->o_keys[i];
+            uint32_t pid = key_oid(k), target = (uint32_t)((k >> 64) & ((1ull << OID_BIT) - 1));
+            if (compact_add(o, &o->recs[o->n_recs++], pid, target, key_dur(k), key_end(k) + R->info.end_time_min, base,
+                            R->o_boxes + 4 * i))
+                goto bad_id;
+        }
+        for (uint32_t id = 0; id < N; id++)
+            if (R->sids[id] == 1) o->sids[id] = 1;
+    }
+    o->sid_count = 0;
+    for (uint32_t id = 0; id < N; id++) o->sid_count += o->same_pid_count[id] ? 1 : 0;
+    o->oversize_oid_count = 0; /* counted by the flush */
+    o->end_time_min = base;
+    if (count > 0xFFFFFFFFu / (s_kvr ? s_kvr : 1)) o->cfg.kv_restriction = 0xFFFFFFFFu;
+    else o->cfg.kv_restriction = s_kvr * count;
+    o->sticky = 1;
+    rc = flush(o, tmax);
+    o->sticky = 0;
+    if (rc == 0) { /* the new run replaces its inputs */
+        run merged = o->runs[o->n_runs - 1];
+        for (uint32_t r = first; r < first + count; r++) free_run(&o->runs[r]);
+        memmove(o->runs + first + 1, o->runs + first + count, sizeof(run) * (o->n_runs - 1 - (first + count)));
+        o->runs[first] = merged;
+        o->n_runs -= count;
+    }
+    goto done;
+bad_id:
+    set_err(o, "an input run holds a record whose object id is not below num_objects");
+done:
+    free(o->recs); free(o->same_pid_count); free(o->o_boxs); free(o->sids);
+    o->recs = s_recs; o->n_recs = s_n; o->cap_recs = s_cap;
+    o->same_pid_count = s_spc; o->o_boxs = s_obox; o->sids = s_sids;
+    o->sid_count = s_sid; o->oversize_oid_count = s_over; o->end_time_min = s_etm;
+    o->cfg.kv_restriction = s_kvr;
+    return rc;
+}
 
 uint64_t orc_query_batch(void *vo, const orc_query *qs, uint32_t nq, orc_hit *hits, uint64_t cap, uint64_t *counts) {
     orc *o = (orc *)vo;

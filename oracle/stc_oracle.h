@@ -149,6 +149,8 @@ typedef struct orc_hit {
 } orc_hit;
 uint64_t orc_query_batch(void *o, const orc_query *q, uint32_t nq, orc_hit *hits, uint64_t cap, uint64_t *counts);
 uint32_t orc_end_time_min(void *o); /* time base of the memtable's end offsets */
+/* merge runs [first, first + count) into one run in their place (tools/compaction.cpp) */
+int orc_compact_runs(void *o, uint32_t first, uint32_t count);
 
 uint64_t orc_brute_pairs(const double *xy, uint32_t n, double r, int probe_low,
                          uint64_t *keys, uint64_t cap);

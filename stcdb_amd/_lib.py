@@ -178,6 +178,7 @@ SIGNATURES = {
     "stc_get_run_info": (C.c_int, [P, C.c_uint32, C.POINTER(StcRunInfo)]),
     "stc_get_run": (C.c_int, [P, C.c_uint32, P, P, P, P, P, P, P, P, P]),
     "stc_drop_runs": (C.c_int, [P]),
+    "stc_compact_runs": (C.c_int, [P, C.c_uint32, C.c_uint32]),
     "stc_sync": (C.c_int, [P]),
     "stc_live_search": (C.c_int, [P, P, C.c_uint32, P, C.c_uint64, U64P]),
     "stc_dump_run": (C.c_int, [P, C.c_uint32, C.c_char_p, C.c_char_p, C.c_uint32]),

@@ -52,6 +52,7 @@ struct FlushArgs {
     uint32_t sid_count, oversize_oid_count;
     uint32_t compat;
     uint32_t keep_sorted;    /* copy the sorted CTF records back to the host */
+    uint32_t sticky_oversize; /* compaction: objects entering with sids 1 stay oversize (compaction.cpp:259-268) */
     double area_threshold;
     double map[4];
     uint64_t *rec_keys;      /* (lo, hi) per record */

@@ -134,7 +134,8 @@ __device__ inline float q2f(uint32_t f) { return (float)f / 100000 - 180; } /* u
 
 __global__ void k_set_oid(uint32_t n, const uint32_t *__restrict__ same_pid_count, const uint4 *__restrict__ o_boxs,
                           uint16_t *__restrict__ sids, uint64_t *__restrict__ mid, uint32_t *__restrict__ oids,
-                          uint8_t *__restrict__ part_tail, double area_threshold, FlushCounters *fc) {
+                          uint8_t *__restrict__ part_tail, double area_threshold, uint32_t sticky,
+                          FlushCounters *fc) {
     uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= n) return;
     oids[id] = id;
@@ -143,7 +144,7 @@ __global__ void k_set_oid(uint32_t n, const uint32_t *__restrict__ same_pid_coun
         uint4 ob = o_boxs[id];
         float low0 = q2f(ob.x), high0 = q2f(ob.z), low1 = q2f(ob.y), high1 = q2f(ob.w);
         float area = (high1 - low1) * (high0 - low0);
-        if ((double)area > area_threshold) {
+        if ((sticky && sids[id] == 1) || (double)area > area_threshold) {
             atomicAdd(&fc->oversize_oid_count, 1u);
             sids[id] = 1;
         } else {
@@ -605,7 +606,7 @@ int stc_flush_run(FlushWork *w, const FlushArgs *a, FlushOut *out, hipStream_t s
 
     /* F1 */
     hipLaunchKernelGGL(k_set_oid, dim3(blocks_for(n, B)), dim3(B), 0, s, n, a->same_pid_count, a->o_boxs, a->sids,
-                       w->mid, w->oids, w->part_tail, a->area_threshold, fc);
+                       w->mid, w->oids, w->part_tail, a->area_threshold, a->sticky_oversize, fc);
     hipLaunchKernelGGL(k_str_prepare, dim3(1), dim3(1), 0, s, fc, n, split);
     FCK(hipGetLastError());
     /* F2: STR over the full 64-bit key (x high, y low), stable from oid order */

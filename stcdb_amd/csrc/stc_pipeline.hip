@@ -619,13 +619,7 @@ __global__ __launch_bounds__(256) void k_refine(
 }
 
 /* float_to_uint (cuda_util.cuh:115-119), saturating as the reference GPU converts */
-__device__ inline uint32_t f2q(float xy) {
-    xy += 180;
-    float v = xy * 100000;
-    if (!(v > 0.0f)) return 0;
-    if (v >= 4294967296.0f) return 0xFFFFFFFFu;
-    return (uint32_t)v;
-}
+
 
 struct MemtableDev {
     uint64_t *keys;        /* the reference's 128-bit key as (lo, hi) words:
@@ -1667,7 +1661,7 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
     uint32_t flushed = 0;
     if (c->kv_count > c->cfg.kv_restriction) {
         RunHost run;
-        FlushArgs fa;
+        FlushArgs fa{};
         fa.n = n;
         fa.ctf_count = c->cfg.ctf_count;
         fa.kv_count = c->kv_count;
@@ -2020,6 +2014,125 @@ int stc_drop_runs(void *vc) {
         }
     }
     c->runs.clear();
+    return STC_OK;
+}
+
+/* Merges runs [first, first + count) into one run in their place. Every record
+ * of the inputs (CTF records decoded as compaction.cpp:320-336 does, oversize
+ * records as stored) forms a memtable whose end offsets are relative to the
+ * earliest input base (compaction.cpp:483-487); it is flushed by the step's
+ * own flush with cur_time = the latest input's. Objects oversize in any input
+ * stay oversize (compaction.cpp:259-268). */
+int stc_compact_runs(void *vc, uint32_t first, uint32_t count) {
+    StcCtx *c = (StcCtx *)vc;
+    if (!c || count == 0) return STC_E_ARG;
+    if ((uint64_t)first + count > c->runs.size()) return fail(c, STC_E_ARG, "run range out of bounds");
+    if (count == 1) return STC_OK;
+    const uint32_t n = c->n;
+    uint64_t total = 0;
+    uint32_t base = 0xFFFFFFFFu, tmax = 0;
+    std::vector<uint64_t> in_ctf(count, 0); /* records held in each input's CTFs */
+    for (uint32_t r = first; r < first + count; r++) {
+        const RunHost &R = c->runs[r];
+        if (!R.dev) return fail(c, STC_E_STATE, "compaction reads runs kept in HBM (stc_config.device_runs)");
+        for (const auto &m : R.ctfs) in_ctf[r - first] += m.kv_capacity;
+        total += in_ctf[r - first] + R.info.oversize_count;
+        base = std::min(base, R.info.end_time_min);
+        tmax = std::max(tmax, R.info.cur_time);
+    }
+    hipStream_t s = c->s;
+    uint64_t *keys = nullptr;
+    float4 *boxes = nullptr;
+    uint32_t *spc = nullptr, *dcount = nullptr;
+    uint4 *obox = nullptr;
+    uint16_t *sids = nullptr;
+    auto release = [&]() {
+        (void)hipStreamSynchronize(s);
+        for (void *p : {(void *)keys, (void *)boxes, (void *)spc, (void *)dcount, (void *)obox, (void *)sids})
+            if (p) (void)hipFree(p);
+    };
+    if (hipMalloc((void **)&keys, std::max<uint64_t>(total, 1) * 16) ||
+        hipMalloc((void **)&boxes, std::max<uint64_t>(total, 1) * 16) || hipMalloc((void **)&spc, 4ull * n) ||
+        hipMalloc((void **)&dcount, 8) || hipMalloc((void **)&obox, 16ull * n) || hipMalloc((void **)&sids, 2ull * n)) {
+        release();
+        return fail(c, STC_E_CAPACITY, "compaction buffers");
+    }
+    int rc = STC_OK;
+    uint32_t h_count[2] = {0, 0}; /* objects with records, bad ids */
+    if (hipMemsetAsync(spc, 0, 4ull * n, s) || hipMemsetAsync(sids, 0, 2ull * n, s) ||
+        hipMemsetAsync(dcount, 0, 8, s))
+        rc = STC_E_HIP;
+    if (!rc) {
+        hipLaunchKernelGGL(k_fill_obox, dim3(blocks_for(n, 256)), dim3(256), 0, s, obox, n);
+        if (hipGetLastError() != hipSuccess) rc = STC_E_HIP;
+    }
+    uint64_t out0 = 0;
+    for (uint32_t r = first; r < first + count && !rc; r++) {
+        const RunHost &R = c->runs[r];
+        rc = stc_compact_decode(R.qdev, in_ctf[r - first], n, base, out0, keys, boxes, spc, obox, sids, dcount + 1,
+                                s);
+        out0 += in_ctf[r - first] + R.info.oversize_count;
+    }
+    if (!rc) rc = stc_count_objects(spc, n, dcount, s);
+    if (!rc && (hipMemcpyAsync(h_count, dcount, 8, hipMemcpyDeviceToHost, s) || hipStreamSynchronize(s)))
+        rc = STC_E_HIP;
+    if (rc) {
+        release();
+        return fail(c, rc, std::string("compaction decode: ") + g_last_fail);
+    }
+    if (h_count[1]) {
+        release();
+        return fail(c, STC_E_STATE, "an input run holds a record whose object id is not below num_objects");
+    }
+    FlushArgs fa{};
+    fa.n = n;
+    fa.ctf_count = c->cfg.ctf_count;
+    fa.kv_count = total;
+    fa.kv_restriction = (uint32_t)std::min<uint64_t>((uint64_t)c->cfg.kv_restriction * count, 0xFFFFFFFFu);
+    fa.end_time_min = base;
+    fa.cur_time = tmax;
+    fa.max_meet_time = c->cfg.max_meet_time;
+    fa.area_threshold = c->cfg.area_threshold;
+    fa.compat = c->cfg.compat;
+    fa.keep_sorted = (c->cfg.keep_pairs & 2) ? 1 : 0;
+    fa.sticky_oversize = 1;
+    fa.map[0] = c->map.lo0; fa.map[1] = c->map.lo1; fa.map[2] = c->map.hi0; fa.map[3] = c->map.hi1;
+    fa.sid_count = h_count[0];
+    fa.oversize_oid_count = 0; /* counted by the flush: objects with records that are oversize */
+    fa.rec_keys = keys;
+    fa.rec_boxes = boxes;
+    fa.same_pid_count = spc;
+    fa.o_boxs = obox;
+    fa.sids = sids;
+    FlushOut fo;
+    fo.acquire = pool_acquire;
+    fo.acquire_ctx = &c->pinned;
+    rc = stc_flush_run(&c->flushw, &fa, &fo, s);
+    if (rc) {
+        c->pinned.release(fo.blk);
+        release();
+        return fail(c, rc, std::string("compaction flush: ") + g_last_fail);
+    }
+    RunHost run;
+    run.info = fo.info;
+    run.ctfs.swap(fo.ctfs);
+    run.lay = fo.lay;
+    run.has_sorted = fo.has_sorted;
+    run.blk = fo.blk;
+    rc = keep_run_on_device(c, run);
+    release();
+    if (rc) {
+        c->pinned.release(run.blk);
+        if (run.dev) (void)hipFree(run.dev);
+        return rc;
+    }
+    CK(hipStreamSynchronize(s));
+    for (uint32_t r = first; r < first + count; r++) {
+        c->pinned.release(c->runs[r].blk);
+        (void)hipFree(c->runs[r].dev);
+    }
+    c->runs.erase(c->runs.begin() + first, c->runs.begin() + first + count);
+    c->runs.insert(c->runs.begin() + first, std::move(run));
     return STC_OK;
 }
 

@@ -55,6 +55,19 @@ QueryBufs *stc_query_bufs_new();
 void stc_query_bufs_free(QueryBufs *b);
 int stc_query_run(QueryBufs *b, const QueryState &st, const stc_query *q, uint32_t nq, stc_hit *hits, uint64_t cap,
                   uint64_t *n_hits, uint64_t *counts, hipStream_t s);
+/* Compaction input: every record of run R (the kv_count records of its CTFs,
+ * kv_count = the sum of their kv_capacity, then the oversize buffer) in
+ * memtable form at keys/boxes[out0 ..], ends re-based on `base`, boxes decoded
+ * as CTF::new_parse_mbr_f_box (step_merge.cpp:381-401); per object: record
+ * counts, quantised MBRs, and sids[o] = 1 where R had o oversize. A record
+ * whose ids are not below num_objects sets *bad and is not counted. */
+int stc_compact_decode(const QueryRunDev &R, uint64_t kv_count, uint32_t num_objects, uint32_t base, uint64_t out0,
+                       uint64_t *keys, float4 *boxes, uint32_t *same_pid_count, uint4 *o_boxs, uint16_t *sids,
+                       uint32_t *bad, hipStream_t s);
+
+/* number of objects with at least one record, into *d_count (device) */
+int stc_count_objects(const uint32_t *same_pid_count, uint32_t n, uint32_t *d_count, hipStream_t s);
+
 /* decode every record of a run kept in HBM once and summarise each CTF */
 int stc_query_ctf_stats(const QueryRunDev &run, std::vector<CtfStats> &out, hipStream_t s);
 

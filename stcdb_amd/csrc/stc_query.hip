@@ -426,6 +426,114 @@ static uint32_t blocks(uint64_t n, uint32_t b) { return (uint32_t)std::max<uint6
         if (e_ != hipSuccess) return stc_hip_fail(e_, #expr, __FILE__, __LINE__);                   \
     } while (0)
 
+/* CTF::new_parse_mbr_f_box: the double expressions of new_parse_mbr stored to
+ * float, the high corner computed from the stored (float) low corner */
+__device__ inline float4 decode_fbox(const stc_ctf_meta &c, uint64_t vm) {
+    const uint64_t mask_edge = (1ull << c.edge_bit) - 1, mask_low_y = (1ull << c.low_y_bit) - 1,
+                   mask_low_x = (1ull << c.low_x_bit) - 1;
+    uint64_t y = vm & mask_edge;
+    vm >>= c.edge_bit;
+    uint64_t x = vm & mask_edge;
+    vm >>= c.edge_bit;
+    uint64_t low1 = vm & mask_low_y;
+    vm >>= c.low_y_bit;
+    uint64_t low0 = vm & mask_low_x;
+    const float w0 = c.mbr[2] - c.mbr[0], w1 = c.mbr[3] - c.mbr[1];
+    const float longer = w0 > w1 ? w0 : w1;
+    float4 b;
+    b.x = (float)((double)c.mbr[0] + (double)low0 / (double)mask_low_x * (double)w0);
+    b.y = (float)((double)c.mbr[1] + (double)low1 / (double)mask_low_y * (double)w1);
+    b.z = (float)((double)b.x + (double)x / (double)mask_edge * (double)longer);
+    b.w = (float)((double)b.y + (double)y / (double)mask_edge * (double)longer);
+    return b;
+}
+
+__global__ __launch_bounds__(256) void k_compact_decode(QueryRunDev R, uint64_t kv_count, uint32_t num_objects,
+                                                        uint32_t base, uint64_t out0, uint64_t *__restrict__ keys,
+                                                        float4 *__restrict__ boxes, uint32_t *__restrict__ same_pid_count,
+                                                        uint4 *__restrict__ o_boxs, uint32_t *__restrict__ bad) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= kv_count + R.n_tail) return;
+    uint32_t pid, target, dur, end;
+    float4 b;
+    if (i < kv_count) {
+        /* the record's CTF: the last c with rec_base[c] <= i */
+        uint32_t lo = 0, hi = R.ctf_count;
+        while (hi - lo > 1) {
+            uint32_t mid = (lo + hi) / 2;
+            if (R.rec_base[mid] <= i) lo = mid;
+            else hi = mid;
+        }
+        const stc_ctf_meta c = R.ctfs[lo];
+        const uint32_t kb = c.key_bit / 8;
+        unsigned __int128 k = load_packed(R.key_bytes + R.byte_base[lo] + (i - R.rec_base[lo]) * kb, kb);
+        const uint64_t vm = (uint64_t)(k & (((unsigned __int128)1 << c.mbr_bit) - 1));
+        unsigned __int128 r = k >> c.mbr_bit;
+        end = (uint32_t)(r & ((1ull << c.end_bit) - 1)) + c.end_time_min;
+        r >>= c.end_bit;
+        dur = (uint32_t)(r & ((1ull << c.duration_bit) - 1));
+        r >>= c.duration_bit;
+        target = (uint32_t)(r & ((1ull << c.id_bit) - 1));
+        r >>= c.id_bit;
+        pid = (uint32_t)(r & ((1ull << c.id_bit) - 1));
+        b = decode_fbox(c, vm);
+    } else {
+        const uint64_t j = i - kv_count;
+        const uint64_t lo = R.o_keys[2 * j], hi = R.o_keys[2 * j + 1];
+        pid = (uint32_t)(hi >> 26) & MASK26;
+        target = (uint32_t)hi & MASK26;
+        end = (uint32_t)lo + R.end_time_min;
+        dur = (uint32_t)(lo >> 32);
+        b = R.o_boxes[j];
+    }
+    keys[2 * (out0 + i)] = ((uint64_t)dur << 32) | (uint32_t)(end - base);
+    keys[2 * (out0 + i) + 1] = ((uint64_t)pid << 26) | target;
+    boxes[out0 + i] = b;
+    if (pid >= num_objects || target >= num_objects) { /* not a key this context wrote */
+        atomicOr(bad, 1u);
+        return;
+    }
+    atomicAdd(&same_pid_count[pid], 1u);
+    atomicMin(&o_boxs[pid].x, f2q(b.x));
+    atomicMax(&o_boxs[pid].z, f2q(b.z));
+    atomicMin(&o_boxs[pid].y, f2q(b.y));
+    atomicMax(&o_boxs[pid].w, f2q(b.w));
+}
+
+__global__ void k_compact_sids(const uint16_t *__restrict__ run_sids, uint32_t n, uint16_t *__restrict__ sids) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && run_sids[i] == 1) sids[i] = 1;
+}
+
+int stc_compact_decode(const QueryRunDev &R, uint64_t kv_count, uint32_t num_objects, uint32_t base, uint64_t out0,
+                       uint64_t *keys, float4 *boxes, uint32_t *same_pid_count, uint4 *o_boxs, uint16_t *sids,
+                       uint32_t *bad, hipStream_t s) {
+    if (kv_count + R.n_tail)
+        hipLaunchKernelGGL(k_compact_decode, dim3(blocks(kv_count + R.n_tail, 256)), dim3(256), 0, s, R, kv_count,
+                           num_objects, base, out0, keys, boxes, same_pid_count, o_boxs, bad);
+    QCK(hipGetLastError());
+    hipLaunchKernelGGL(k_compact_sids, dim3(blocks(num_objects, 256)), dim3(256), 0, s, R.sids, num_objects, sids);
+    QCK(hipGetLastError());
+    return STC_OK;
+}
+
+__global__ __launch_bounds__(256) void k_count_objects(const uint32_t *__restrict__ same_pid_count, uint32_t n,
+                                                       uint32_t *__restrict__ d_count) {
+    uint32_t mine = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        mine += same_pid_count[i] ? 1u : 0u;
+    for (int o = STC_WAVE / 2; o > 0; o >>= 1) mine += __shfl_xor(mine, o, STC_WAVE);
+    if ((threadIdx.x & (STC_WAVE - 1)) == 0 && mine) atomicAdd(d_count, mine);
+}
+
+int stc_count_objects(const uint32_t *same_pid_count, uint32_t n, uint32_t *d_count, hipStream_t s) {
+    QCK(hipMemsetAsync(d_count, 0, 4, s));
+    hipLaunchKernelGGL(k_count_objects, dim3(std::max<uint32_t>(std::min<uint32_t>(blocks(n, 256), 1024), 1)),
+                       dim3(256), 0, s, same_pid_count, n, d_count);
+    QCK(hipGetLastError());
+    return STC_OK;
+}
+
 int stc_query_ctf_stats(const QueryRunDev &run, std::vector<CtfStats> &out, hipStream_t s) {
     out.assign(run.ctf_count, CtfStats());
     if (!run.ctf_count) return STC_OK;

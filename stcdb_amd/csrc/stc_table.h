@@ -41,4 +41,14 @@ __host__ __device__ inline float o2f(uint32_t o) {
     return f;
 }
 
+/* float_to_uint (src/cuda/cuda_util.cuh:115-119): the quantised coordinate of
+ * the per-object MBRs, saturating as the reference GPU converts */
+__host__ __device__ inline uint32_t f2q(float xy) {
+    xy += 180;
+    float v = xy * 100000;
+    if (!(v > 0.0f)) return 0;
+    if (v >= 4294967296.0f) return 0xFFFFFFFFu;
+    return (uint32_t)v;
+}
+
 #endif /* STC_TABLE_H_ */

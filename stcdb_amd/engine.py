@@ -221,6 +221,10 @@ class Pipeline:
         """write run i in the reference's file layout (stc_dump_run)"""
         _check(self.lib.stc_dump_run(self.ctx, i, directory.encode(), prefix.encode(), ctb), self.ctx)
 
+    def compact_runs(self, first, count):
+        """merge runs [first, first + count) into one run in their place (stc_compact_runs)"""
+        _check(self.lib.stc_compact_runs(self.ctx, first, count), self.ctx)
+
     def drop_runs(self):
         _check(self.lib.stc_drop_runs(self.ctx), self.ctx)
 

@@ -106,6 +106,8 @@ def load():
         lib.orc_query_batch.argtypes = [V, V, C.c_uint32, V, C.c_uint64, V]
         lib.orc_end_time_min.restype = C.c_uint32
         lib.orc_end_time_min.argtypes = [V]
+        lib.orc_compact_runs.restype = C.c_int
+        lib.orc_compact_runs.argtypes = [V, C.c_uint32, C.c_uint32]
         lib.orc_brute_pairs.restype = C.c_uint64
         lib.orc_brute_pairs.argtypes = [V, C.c_uint32, C.c_double, C.c_int, V, C.c_uint64]
         _lib = lib
@@ -258,6 +260,11 @@ class Oracle:
 
     def end_time_min(self):
         return self.lib.orc_end_time_min(self.h)
+
+    def compact_runs(self, first, count):
+        """merge runs [first, first + count) into one run in their place"""
+        if self.lib.orc_compact_runs(self.h, first, count) != 0:
+            raise RuntimeError("oracle: " + self.lib.orc_last_error(self.h).decode())
 
     def query(self, q):
         """answer a batch of queries (QUERY_DTYPE); returns (hits sorted, counts)"""
