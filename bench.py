@@ -202,6 +202,55 @@ def serve(args):
     trace.close()
 
 
+def compact(args):
+    """compaction of runs kept in HBM: the workload's steps run until --compact
+    runs exist, then one stc_compact_runs merges them; reports records merged
+    per second"""
+    import torch
+    from stcdb_amd import DeviceTrace, Pipeline, workload
+    over = {"num_objects": args.objects} if args.objects else {}
+    w = workload(args.workload, **over)
+    cfg = dict(w["cfg"])
+    cfg["device_runs"] = 1
+    n = cfg["num_objects"]
+    pipe = Pipeline(cfg, w["box"])
+    trace = DeviceTrace(w["gen"])
+    t = 0
+    while pipe.num_runs() < args.compact:
+        r = pipe.step_device(trace.next(t), t)
+        print(f"[compact] step {t}: {r['ms_total']:.1f} ms, runs {pipe.num_runs()}", file=sys.stderr, flush=True)
+        t += 1
+    trace.close()
+    k = args.compact
+    infos = [pipe.run_info(i) for i in range(k)]
+    records = sum(i["kv_count"] + i["oversize_count"] for i in infos)
+    in_bytes = sum(i["key_bytes"] for i in infos)
+    pipe.sync_runs()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    pipe.compact_runs(0, k)
+    pipe.sync_runs()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    out = pipe.run_info(0)
+    assert out["kv_count"] + out["oversize_count"] == records and pipe.num_runs() == 1
+    line = {
+        "metric": f"records/s merged by compaction of {k} runs at {n // 1000000}M objects",
+        "value": round(records / elapsed, 1),
+        "unit": "records/s",
+        "ms": round(elapsed * 1000, 3),
+        "records": records,
+        "input_key_bytes": in_bytes,
+        "output_key_bytes": out["key_bytes"],
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "data": "synthetic (runs flushed from the seeded street-grid walk)",
+        "config": {"workload": args.workload, "objects": n, "runs": k, "ctf_count": cfg["ctf_count"]},
+    }
+    print(json.dumps(line), flush=True)
+    pipe.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -217,6 +266,8 @@ def main():
                     help="serving mode: 5M objects ingested at 1 Hz plus 2,000 queries/s; prints p50/p99 latency")
     ap.add_argument("--qps", type=float, default=2000.0)
     ap.add_argument("--serve-seconds", type=int, default=10)
+    ap.add_argument("--compact", type=int, default=0,
+                    help="compaction mode: flush this many runs into HBM, then time merging them into one")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="copy a buffer of this size once before the steps (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -226,6 +277,9 @@ def main():
         sys.exit(2)
     if args.serve:
         serve(args)
+        return
+    if args.compact:
+        compact(args)
         return
 
     import torch

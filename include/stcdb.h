@@ -154,6 +154,29 @@ const char *stc_last_error(void *ctx);
 int stc_step(void *ctx, const double *xy_host, uint32_t t, stc_step_out *out);
 int stc_step_device(void *ctx, const double *xy_dev, uint32_t t, stc_step_out *out);
 
+/* ---- per-second meeting persistence ----
+ * The reference can save the pairs each step finds and replay them later in
+ * place of the positions (save_meetings_pers / load_meetings_pers:
+ * geometry.cu:456-466, 1609-1622, 1718-1723; files by workbench::dump_meetings
+ * and load_meetings, workbench.cpp:446-470). One unit per pair and step, the
+ * reference's 32-byte meeting_unit (workbench.h:62-83). */
+typedef struct stc_meeting_unit {
+    uint64_t key;              /* Cantor key of (min pid, max pid): (a + b)(a + b + 1)/2 + b */
+    uint32_t start, end;       /* the step (a replay ignores them, as insert_meetings_to_buckets does) */
+    float mbr[4];              /* float bbox of the two positions at the step: low0, low1, high0, high1 */
+} stc_meeting_unit;
+
+/* the last step's pairs as units, in pair order; needs keep_pairs bit 0 */
+int stc_get_step_meetings(void *ctx, stc_meeting_unit *units, uint64_t cap, uint64_t *n);
+/* A step from units instead of positions (insert_meetings_to_buckets,
+ * geometry.cu:544-565, without its 5-probe drop): each pair is inserted or
+ * extended with its MBR, then meetings are identified and the memtable
+ * flushed as in stc_step. Each pair at most once per call (STC_E_INPUT
+ * otherwise, and for keys naming no two distinct objects below num_objects);
+ * fewer than 2^32 - 1 units. No tree is built, so the step reports no nodes,
+ * leaves or candidates. */
+int stc_step_meetings(void *ctx, const stc_meeting_unit *units, uint64_t n, uint32_t t, stc_step_out *out);
+
 /* ---- inspection of the current state (copies to host memory) ----
  * Each getter writes at most `cap` items and stores the total count in *n. */
 int stc_get_pairs(void *ctx, uint64_t *keys, uint64_t cap, uint64_t *n);   /* last step, sorted, oid1<<32|oid2 */

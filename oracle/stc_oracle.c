@@ -124,6 +124,7 @@ typedef struct orc {
     uint16_t *sids;
     uint32_t sid_count, oversize_oid_count;
     int sticky;                 /* compaction flush: objects entering with sids 1 stay oversize */
+    uint32_t last_t;            /* the last step */
 
     run *runs; uint32_t n_runs;
 } orc;
@@ -983,9 +984,77 @@ int orc_step(void *vo, const double *xy, uint32_t t) {
     o->counts[ORC_C_NODES] = o->n_nodes;
     o->counts[ORC_C_LEAVES] = o->n_leaves;
     if (find_pairs(o, xy)) return -1;
+    o->last_t = t;
     if (update_meetings(o, t)) return -1;
     o->counts[ORC_C_KV_COUNT] = o->n_recs;
     if (o->n_recs > o->cfg.kv_restriction) { /* geometry.cu:1752 */
+        if (flush(o, t)) return -1;
+        o->counts[ORC_C_FLUSHED] = 1;
+    }
+    return 0;
+}
+
+/* Persisted meeting units (workbench::dump_meetings / load_meetings,
+ * workbench.cpp:446-470): one 32 B meeting_unit per pair found at a step,
+ * key = Cantor(min pid, max pid) (geometry.cu:453-455), start = end = step,
+ * mbr = float bbox of the two positions (geometry.cu:456-466). */
+static uint64_t cantor(uint32_t a, uint32_t b) { return ((uint64_t)a + b) * ((uint64_t)a + b + 1) / 2 + b; }
+
+/* inverse by exact integer search around the double estimate (cuda_util.cuh:55-70) */
+static int uncantor(uint64_t z, uint32_t n, uint32_t *a, uint32_t *b) {
+    uint64_t w = (uint64_t)floor((sqrt(8.0 * (double)z + 1) - 1) / 2);
+    while (w > 0 && (u128)w * (w + 1) / 2 > z) w--;
+    while ((u128)(w + 1) * (w + 2) / 2 <= z) w++;
+    uint64_t y = z - (uint64_t)((u128)w * (w + 1) / 2), x = w - y;
+    if (!(x < y && y < n)) return -1;
+    *a = (uint32_t)x;
+    *b = (uint32_t)y;
+    return 0;
+}
+
+void orc_get_step_meetings(void *vo, orc_meeting_unit *u) {
+    orc *o = (orc *)vo;
+    for (uint64_t i = 0; i < o->n_pairs; i++) {
+        uint32_t a = (uint32_t)(o->pairs[i].key >> 32), b = (uint32_t)o->pairs[i].key;
+        u[i].key = cantor(a, b);
+        u[i].start = u[i].end = o->last_t;
+        memcpy(u[i].mbr, o->pairs[i].mbr, sizeof(u[i].mbr));
+    }
+}
+
+/* a step from persisted units instead of positions (geometry.cu:1609-1622):
+ * every unit's pair is inserted or extended with the unit's MBR */
+int orc_step_meetings(void *vo, const orc_meeting_unit *u, uint64_t n, uint32_t t) {
+    orc *o = (orc *)vo;
+    memset(o->counts, 0, sizeof(o->counts));
+    pairhit *p = (pairhit *)malloc(sizeof(pairhit) * (n ? n : 1));
+    if (!p) { set_err(o, "out of memory"); return -1; }
+    for (uint64_t i = 0; i < n; i++) {
+        uint32_t a, b;
+        if (uncantor(u[i].key, o->cfg.num_objects, &a, &b)) {
+            free(p);
+            set_err(o, "meeting unit key is not the Cantor key of two distinct objects below num_objects");
+            return -1;
+        }
+        p[i].key = ((uint64_t)a << 32) | b;
+        memcpy(p[i].mbr, u[i].mbr, sizeof(p[i].mbr));
+    }
+    qsort(p, n, sizeof(pairhit), cmp_pairhit);
+    for (uint64_t i = 1; i < n; i++)
+        if (p[i].key == p[i - 1].key) {
+            free(p);
+            set_err(o, "a pair appears twice among the step's meeting units");
+            return -1;
+        }
+    free(o->pairs);
+    o->pairs = p;
+    o->n_pairs = n;
+    o->cap_pairs = n ? n : 1;
+    o->counts[ORC_C_PAIRS] = n;
+    o->last_t = t;
+    if (update_meetings(o, t)) return -1;
+    o->counts[ORC_C_KV_COUNT] = o->n_recs;
+    if (o->n_recs > o->cfg.kv_restriction) {
         if (flush(o, t)) return -1;
         o->counts[ORC_C_FLUSHED] = 1;
     }

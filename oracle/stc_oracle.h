@@ -149,6 +149,16 @@ typedef struct orc_hit {
 } orc_hit;
 uint64_t orc_query_batch(void *o, const orc_query *q, uint32_t nq, orc_hit *hits, uint64_t cap, uint64_t *counts);
 uint32_t orc_end_time_min(void *o); /* time base of the memtable's end offsets */
+/* meeting_unit (src/tracing/workbench.h:62-83) as persisted per second */
+typedef struct orc_meeting_unit {
+    uint64_t key;                /* Cantor(min pid, max pid) */
+    uint32_t start, end;
+    float mbr[4];
+} orc_meeting_unit;
+/* the last step's pairs as units, in pair order (orc_num_pairs of them) */
+void orc_get_step_meetings(void *o, orc_meeting_unit *u);
+/* a step from persisted units instead of positions */
+int orc_step_meetings(void *o, const orc_meeting_unit *u, uint64_t n, uint32_t t);
 /* merge runs [first, first + count) into one run in their place (tools/compaction.cpp) */
 int orc_compact_runs(void *o, uint32_t first, uint32_t count);
 

@@ -171,6 +171,8 @@ SIGNATURES = {
     "stc_step": (C.c_int, [P, P, C.c_uint32, C.POINTER(StcStepOut)]),
     "stc_step_device": (C.c_int, [P, P, C.c_uint32, C.POINTER(StcStepOut)]),
     "stc_get_pairs": (C.c_int, [P, P, C.c_uint64, U64P]),
+    "stc_get_step_meetings": (C.c_int, [P, P, C.c_uint64, U64P]),
+    "stc_step_meetings": (C.c_int, [P, P, C.c_uint64, C.c_uint32, C.POINTER(StcStepOut)]),
     "stc_get_active": (C.c_int, [P, P, P, P, P, C.c_uint64, U64P]),
     "stc_get_records": (C.c_int, [P, P, P, C.c_uint64, U64P]),
     "stc_get_leaves": (C.c_int, [P, P, P, C.c_uint64, U64P]),

@@ -513,6 +513,43 @@ __global__ void k_table_insert(Bucket *t, uint64_t nb, const PairHit *__restrict
     if (rc == 2) atomicAdd(&cnt->continuing, 1ull);
 }
 
+/* Persisted meeting units back to pair keys: the Cantor key of
+ * (min pid, max pid) (geometry.cu:453-455) inverted exactly, by integer search
+ * around the double estimate of getpid1/getpid2 (cuda_util.cuh:55-70). A key
+ * that does not name two distinct objects below n sets bad. */
+__global__ void k_units_decode(const stc_meeting_unit *__restrict__ u, uint32_t nu, uint32_t n,
+                               uint64_t *__restrict__ keys, uint32_t *__restrict__ idx, Counters *cnt) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nu) return;
+    const uint64_t z = u[i].key;
+    uint64_t w = (uint64_t)floor((sqrt(8.0 * (double)z + 1.0) - 1.0) / 2.0);
+    while (w > 0 && (unsigned __int128)w * (w + 1) / 2 > z) w--;
+    while ((unsigned __int128)(w + 1) * (w + 2) / 2 <= z) w++;
+    const uint64_t y = z - (uint64_t)((unsigned __int128)w * (w + 1) / 2), x = w - y;
+    if (!(x < y && y < n)) {
+        atomicAdd(&cnt->bad_points, 1u);
+        keys[i] = 0;
+    } else {
+        keys[i] = (x << 32) | y;
+    }
+    idx[i] = i;
+}
+
+/* the units in pair order as PairHits; a pair named twice sets table_fail's
+ * neighbour flag (bad_points) */
+__global__ void k_units_gather(const uint64_t *__restrict__ keys, const uint32_t *__restrict__ idx,
+                               const stc_meeting_unit *__restrict__ u, uint32_t nu, PairHit *__restrict__ out,
+                               Counters *cnt) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nu) return;
+    if (i > 0 && keys[i] == keys[i - 1]) atomicAdd(&cnt->bad_points, 1u);
+    PairHit h;
+    h.key = keys[i];
+    const stc_meeting_unit m = u[idx[i]];
+    h.mbr[0] = m.mbr[0]; h.mbr[1] = m.mbr[1]; h.mbr[2] = m.mbr[2]; h.mbr[3] = m.mbr[3];
+    out[i] = h;
+}
+
 /* ------------------------------------------------------------------ */
 /* refinement fused with the table insert                               */
 /* ------------------------------------------------------------------ */
@@ -1161,6 +1198,9 @@ struct StcCtx {
     uint64_t last_pairs = 0;
     std::vector<PairHit> host_pairs_extra;
     DBuf<PairHit> amb_pairs;
+    DBuf<stc_meeting_unit> units_in;  /* replayed meeting units and their pair keys */
+    DBuf<uint64_t> ukeys, ukeys_alt;
+    DBuf<uint32_t> uidx, uidx_alt, utemp;
     DBuf<uint32_t> chunk_count, chunk_offset;
     DBuf<Bucket> stage;
     std::vector<RunHost> runs;
@@ -1425,6 +1465,8 @@ int stc_destroy(void *vc) {
     c->lkeys.release(); c->ucount.release(); c->uoffset.release(); c->units.release(); c->pairs.release();
     c->amb.release();
     c->amb_pairs.release();
+    c->units_in.release(); c->ukeys.release(); c->ukeys_alt.release();
+    c->uidx.release(); c->uidx_alt.release(); c->utemp.release();
     c->chunk_count.release();
     c->chunk_offset.release();
     c->stage.release();
@@ -1463,6 +1505,13 @@ const char *stc_last_error(void *vc) {
 static double host_dpm_lon(double lat) {
     return 360.0 / (std::sin((90 - std::fabs(lat)) * STC_PI_REF / 180) * 40076.0 * 1000.0);
 }
+
+struct StepGeom {
+    uint32_t nodes, leaves;
+    uint64_t units, pairs, ambiguous;
+    uint32_t rebuilt;
+};
+static int step_tail(StcCtx *c, uint32_t t, StepGeom &g, stc_step_out *out);
 
 static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out) {
     const uint32_t n = c->n;
@@ -1625,9 +1674,20 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
     }
     c->last_pairs = n_pairs;
     CK(hipEventRecord(c->ev[3], s));
+    StepGeom g{n_nodes, n_leaves, n_units, n_pairs, n_amb, rebuilt};
+    return step_tail(c, t, g, out);
+}
 
+/* Steps 9-10, shared by the position step and the replay of persisted
+ * meeting units: identify, rebuild when needed, flush, report. */
+static int step_tail(StcCtx *c, uint32_t t, StepGeom &g, stc_step_out *out) {
+    const uint32_t n = c->n;
+    hipStream_t s = c->s;
+    Counters *cnt = c->dcnt;
+    uint32_t rebuilt = g.rebuilt;
     /* 9: identify: meetings live at t - 1 and not continued now expire */
     CKRC(sync_counters(c));
+    Counters h = *c->hcnt;
     uint64_t h_cont = c->hcnt->continuing;
     uint64_t expiring = c->last_live > h_cont ? c->last_live - h_cont : 0;
     uint64_t need_rec = c->kv_count + 2 * expiring + 2;
@@ -1713,17 +1773,17 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
     c->last_t = t;
     if (out) {
         memset(out, 0, sizeof(*out));
-        out->nodes = n_nodes;
-        out->leaves = n_leaves;
+        out->nodes = g.nodes;
+        out->leaves = g.leaves;
         out->border_points = h.border;
-        out->cross_units = n_units;
+        out->cross_units = g.units;
         out->candidates = h.candidates;
-        out->pairs = n_pairs;
+        out->pairs = g.pairs;
         out->active_meetings = h.live;
         out->expired = h.expired;
         out->kv_count = h.kv_count;
         out->overflow_points = h.overflow;
-        out->ambiguous_pairs = n_amb;
+        out->ambiguous_pairs = g.ambiguous;
         out->ambiguous_nodes = h.ambiguous_nodes;
         out->table_buckets = c->table_buckets;
         out->table_occupied = occupied;
@@ -1759,6 +1819,109 @@ int stc_step(void *vc, const double *xy_host, uint32_t t, stc_step_out *out) {
     if (!c || !xy_host) return STC_E_ARG;
     CK(hipMemcpyAsync(c->xy_in.p, xy_host, 16ull * c->n, hipMemcpyHostToDevice, c->s));
     return stc_step_device(c, (const double *)c->xy_in.p, t, out);
+}
+
+/* A step from persisted meeting units (the reference's load_meetings_pers
+ * mode, geometry.cu:1609-1622): the units' pairs are inserted or extended in
+ * the meeting table with their MBRs, then steps 9-10 run as after refinement. */
+int stc_step_meetings(void *vc, const stc_meeting_unit *units, uint64_t nu64, uint32_t t, stc_step_out *out) {
+    StcCtx *c = (StcCtx *)vc;
+    if (!c || (!units && nu64)) return STC_E_ARG;
+    if (t >= 0xFFFFFFFEu) return fail(c, STC_E_ARG, "timestep out of range");
+    if (c->stepped && t <= c->last_t) return fail(c, STC_E_STATE, "timesteps must increase");
+    if (nu64 >= 0xFFFFFFFFull) return fail(c, STC_E_ARG, "too many meeting units for one step");
+    const uint32_t nu = (uint32_t)nu64;
+    const uint64_t kv_before = c->kv_count;
+    hipStream_t s = c->s;
+    Counters *cnt = c->dcnt;
+    memset(c->hinit, 0, sizeof(Counters));
+    c->hinit->kv_count = c->kv_count;
+    c->hinit->sid_count = c->sid_count;
+    c->hinit->oversize_oid_count = c->oversize_oid_count;
+    CK(hipMemcpyAsync(cnt, c->hinit, sizeof(Counters), hipMemcpyHostToDevice, s));
+    CK(hipEventRecord(c->ev[0], s));
+    CK(hipEventRecord(c->ev[1], s));
+    CK(hipEventRecord(c->ev[2], s));
+    if (c->units_in.ensure(nu + 1ull) || c->ukeys.ensure(nu + 1ull) || c->ukeys_alt.ensure(nu + 1ull) ||
+        c->uidx.ensure(nu + 1ull) || c->uidx_alt.ensure(nu + 1ull) ||
+        c->utemp.ensure(stc_radix_temp_words(nu + 1) + 64) || c->amb_pairs.ensure(nu + 1ull))
+        return fail(c, STC_E_CAPACITY, "meeting unit buffers");
+    if (nu) {
+        CK(hipMemcpyAsync(c->units_in.p, units, (size_t)nu * sizeof(stc_meeting_unit), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_units_decode, dim3(blocks_for(nu, 256)), dim3(256), 0, s, c->units_in.p, nu, c->n,
+                           c->ukeys.p, c->uidx.p, cnt);
+        CK(hipGetLastError());
+        CKRC(stc_radix_sort_u64(c->ukeys.p, c->uidx.p, c->ukeys_alt.p, c->uidx_alt.p, nu, 0,
+                                32 + std::max(1, ilog2_ceil(c->n)), c->utemp.p, s));
+        hipLaunchKernelGGL(k_units_gather, dim3(blocks_for(nu, 256)), dim3(256), 0, s, c->ukeys.p, c->uidx.p,
+                           c->units_in.p, nu, c->amb_pairs.p, cnt);
+        CK(hipGetLastError());
+    }
+    CKRC(sync_counters(c));
+    if (c->hcnt->bad_points)
+        return fail(c, STC_E_INPUT, "meeting units must name distinct pairs of objects below num_objects "
+                                    "by the Cantor key of (min, max)");
+    uint32_t rebuilt = 0;
+    if ((c->occupied + nu) * 10 > c->table_buckets * 6) {
+        CKRC(table_rehash(c, t - 1, c->last_live + nu));
+        rebuilt = 1;
+    }
+    Counters h{};
+    for (int attempt = 0;; attempt++) {
+        CK(hipMemsetAsync(&cnt->new_keys, 0, 2 * sizeof(unsigned long long), s)); /* new_keys, continuing */
+        CK(hipMemsetAsync(&cnt->table_fail, 0, sizeof(uint32_t), s));
+        CK(hipEventRecord(c->ev[6], s));
+        if (nu)
+            hipLaunchKernelGGL(k_table_insert, dim3(blocks_for(nu, 256)), dim3(256), 0, s, c->table,
+                               c->table_buckets, c->amb_pairs.p, (uint64_t)nu, t, cnt);
+        CK(hipGetLastError());
+        CK(hipEventRecord(c->ev[7], s));
+        CKRC(sync_counters(c));
+        h = *c->hcnt;
+        if (!h.table_fail) break;
+        if (attempt >= 8) return fail(c, STC_E_CAPACITY, "meeting table cannot hold the step's pairs");
+        CKRC(table_rehash(c, t - 1, c->last_live + nu + 1024)); /* reinserting is idempotent */
+        rebuilt = 1;
+    }
+    c->last_new = h.new_keys;
+    c->occupied += h.new_keys;
+    c->host_pairs_extra.clear();
+    c->last_listed = 0;
+    if ((c->cfg.keep_pairs & 1) && nu) {
+        if (c->pairs.ensure(nu)) return fail(c, STC_E_CAPACITY, "pair list allocation failed");
+        CK(hipMemcpyAsync(c->pairs.p, c->amb_pairs.p, (size_t)nu * sizeof(PairHit), hipMemcpyDeviceToDevice, s));
+        c->last_listed = nu;
+    }
+    c->last_pairs = nu;
+    CK(hipEventRecord(c->ev[3], s));
+    StepGeom g{0, 0, 0, nu, 0, rebuilt};
+    int rc = step_tail(c, t, g, out);
+    if (rc == STC_OK && out) out->new_records = out->kv_count - kv_before;
+    return rc;
+}
+
+/* The last step's pairs as the reference persists them per second
+ * (geometry.cu:456-466, 1718-1723), in pair order. */
+int stc_get_step_meetings(void *vc, stc_meeting_unit *units, uint64_t cap, uint64_t *n) {
+    StcCtx *c = (StcCtx *)vc;
+    if (!c || !n) return STC_E_ARG;
+    if (!c->stepped) return fail(c, STC_E_STATE, "no step processed");
+    if (!(c->cfg.keep_pairs & 1)) return fail(c, STC_E_STATE, "pairs are kept only with keep_pairs bit 0 set");
+    std::vector<PairHit> ph(c->last_listed);
+    if (c->last_listed) CK(hipMemcpy(ph.data(), c->pairs.p, c->last_listed * sizeof(PairHit), hipMemcpyDeviceToHost));
+    for (const PairHit &e : c->host_pairs_extra) ph.push_back(e);
+    std::sort(ph.begin(), ph.end(), [](const PairHit &a, const PairHit &b) { return a.key < b.key; });
+    ph.erase(std::unique(ph.begin(), ph.end(), [](const PairHit &a, const PairHit &b) { return a.key == b.key; }),
+             ph.end());
+    *n = ph.size();
+    if (units)
+        for (uint64_t i = 0; i < std::min<uint64_t>(cap, ph.size()); i++) {
+            const uint64_t a = ph[i].key >> 32, b = ph[i].key & 0xFFFFFFFFull;
+            units[i].key = (a + b) * (a + b + 1) / 2 + b;
+            units[i].start = units[i].end = c->last_t;
+            memcpy(units[i].mbr, ph[i].mbr, sizeof(units[i].mbr));
+        }
+    return STC_OK;
 }
 
 int stc_get_pairs(void *vc, uint64_t *keys, uint64_t cap, uint64_t *n) {

@@ -37,6 +37,9 @@ QUERY_DTYPE = np.dtype([("kind", "<u4"), ("oid", "<u4"), ("oid2", "<u4"), ("t0",
 HIT_DTYPE = np.dtype([("query", "<u4"), ("pid", "<u4"), ("target", "<u4"), ("start", "<u4"), ("end", "<u4"),
                       ("source", "<u4"), ("mbr", "<f8", (4,))])
 Q_OID, Q_RANGE, Q_PAIR = 1, 2, 3
+# stc_meeting_unit: the reference's 32-byte meeting_unit (src/tracing/workbench.h:62-83)
+MEETING_UNIT_DTYPE = np.dtype([("key", "<u8"), ("start", "<u4"), ("end", "<u4"), ("mbr", "<f4", (4,))])
+assert MEETING_UNIT_DTYPE.itemsize == 32
 
 
 class HostTrace:
@@ -129,6 +132,21 @@ class Pipeline:
         _check(self.lib.stc_step_device(self.ctx, xy_dev, t, C.byref(out)), self.ctx)
         return out.as_dict()
 
+    def step_meetings(self, units, t):
+        """one timestep from persisted meeting units (MEETING_UNIT_DTYPE) instead of positions"""
+        units = np.ascontiguousarray(units, dtype=MEETING_UNIT_DTYPE)
+        out = L.StcStepOut()
+        _check(self.lib.stc_step_meetings(self.ctx, _ptr(units), len(units), t, C.byref(out)), self.ctx)
+        return out.as_dict()
+
+    def step_meeting_units(self):
+        """the last step's pairs as persisted meeting units (needs keep_pairs bit 0)"""
+        n = self._count(self.lib.stc_get_step_meetings, None)
+        u = np.empty(n, dtype=MEETING_UNIT_DTYPE)
+        m = C.c_uint64()
+        _check(self.lib.stc_get_step_meetings(self.ctx, _ptr(u), n, C.byref(m)), self.ctx)
+        return u
+
     def _count(self, fn, *args):
         n = C.c_uint64()
         _check(fn(self.ctx, *args, 0, C.byref(n)), self.ctx)
@@ -172,6 +190,12 @@ class Pipeline:
         n = C.c_uint32()
         _check(self.lib.stc_num_runs(self.ctx, C.byref(n)), self.ctx)
         return n.value
+
+    def run_info(self, i):
+        """stc_get_run_info as a dict (no wait for the run's host copy)"""
+        info = L.StcRunInfo()
+        _check(self.lib.stc_get_run_info(self.ctx, i, C.byref(info)), self.ctx)
+        return {k: getattr(info, k) for k, _ in info._fields_}
 
     def run(self, i):
         info = L.StcRunInfo()

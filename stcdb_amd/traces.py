@@ -61,3 +61,50 @@ def iter_trace(prefix, num_objects, steps):
             t += 1
         if len(chunk) < STEPS_PER_CHUNK and t < steps:
             raise EOFError(f"trace ends at step {t}")
+
+
+def meetings_path(prefix, st, num_objects):
+    """workbench::dump_meetings names the file <trace_path>meeting<st>_<num_objects>.tr"""
+    return f"{prefix}meeting{st}_{num_objects}.tr"
+
+
+def write_meetings(prefix, st, num_objects, steps):
+    """Per-second meeting units of up to 100 timesteps starting at st, in the
+    layout of workbench::dump_meetings (src/tracing/workbench.cpp:446-455): 100
+    u32 counts indexed by timestep % 100, then the 32-byte units of each
+    timestep, blocks in that index order. steps[i] holds the units of
+    timestep st + i (MEETING_UNIT_DTYPE, as Pipeline.step_meeting_units
+    returns them). The reference's own writer never advances its offset
+    (geometry.cu:1720 writes every second at h_meetings_count, which stays 0);
+    this writes the layout its reader expects."""
+    from .engine import MEETING_UNIT_DTYPE
+    assert len(steps) <= STEPS_PER_CHUNK
+    counts = np.zeros(STEPS_PER_CHUNK, dtype="<u4")
+    blocks = [np.zeros(0, dtype=MEETING_UNIT_DTYPE)] * STEPS_PER_CHUNK
+    for i, u in enumerate(steps):
+        k = (st + i) % STEPS_PER_CHUNK
+        blocks[k] = np.ascontiguousarray(u, dtype=MEETING_UNIT_DTYPE)
+        counts[k] = len(u)
+    path = meetings_path(prefix, st, num_objects)
+    with open(path, "wb") as f:
+        f.write(counts.tobytes())
+        for b in blocks:
+            f.write(b.tobytes())
+    return path
+
+
+def read_meetings(prefix, st, num_objects):
+    """the 100 unit blocks of a meetings file, indexed by timestep % 100
+    (workbench::load_meetings, workbench.cpp:457-470; the block of timestep t
+    starts after the blocks of the indices below t % 100 — the reference's
+    loader, geometry.cu:1612-1615, sums one index short)"""
+    from .engine import MEETING_UNIT_DTYPE
+    path = meetings_path(prefix, st, num_objects)
+    raw = np.fromfile(path, dtype=np.uint8)
+    counts = raw[:4 * STEPS_PER_CHUNK].view("<u4")
+    units = raw[4 * STEPS_PER_CHUNK:]
+    if units.size != 32 * int(counts.sum()):
+        raise ValueError(f"{path}: {units.size} unit bytes for {int(counts.sum())} units")
+    units = units.view(MEETING_UNIT_DTYPE)
+    off = np.concatenate([[0], np.cumsum(counts, dtype=np.int64)])
+    return [units[off[k]:off[k + 1]] for k in range(STEPS_PER_CHUNK)]

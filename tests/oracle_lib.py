@@ -106,6 +106,9 @@ def load():
         lib.orc_query_batch.argtypes = [V, V, C.c_uint32, V, C.c_uint64, V]
         lib.orc_end_time_min.restype = C.c_uint32
         lib.orc_end_time_min.argtypes = [V]
+        lib.orc_get_step_meetings.argtypes = [V, V]
+        lib.orc_step_meetings.restype = C.c_int
+        lib.orc_step_meetings.argtypes = [V, V, C.c_uint64, C.c_uint32]
         lib.orc_compact_runs.restype = C.c_int
         lib.orc_compact_runs.argtypes = [V, C.c_uint32, C.c_uint32]
         lib.orc_brute_pairs.restype = C.c_uint64
@@ -176,6 +179,21 @@ class Oracle:
         if rc != 0:
             raise RuntimeError("oracle: " + self.lib.orc_last_error(self.h).decode())
         return self.counts()
+
+    def step_meetings(self, units, t):
+        """a step from persisted meeting units instead of positions"""
+        from stcdb_amd.engine import MEETING_UNIT_DTYPE
+        units = np.ascontiguousarray(units, dtype=MEETING_UNIT_DTYPE)
+        if self.lib.orc_step_meetings(self.h, _p(units), len(units), t) != 0:
+            raise RuntimeError("oracle: " + self.lib.orc_last_error(self.h).decode())
+        return self.counts()
+
+    def step_meeting_units(self):
+        """the last step's pairs as meeting units, in pair order"""
+        from stcdb_amd.engine import MEETING_UNIT_DTYPE
+        u = np.empty(self.lib.orc_num_pairs(self.h), dtype=MEETING_UNIT_DTYPE)
+        self.lib.orc_get_step_meetings(self.h, _p(u))
+        return u
 
     def counts(self):
         a = np.zeros(len(ORC_C), dtype=np.uint64)
