@@ -24,17 +24,16 @@
  *               for leaves within reach on their top or right (cuda_filtering,
  *               geometry.cu:295-392): a counting pass, a prefix sum, and an
  *               emitting pass, so no global atomics and a deterministic order.
- *  7. refine    one wavefront per check unit; lanes stride over the target
- *               leaf's members with coalesced SoA loads; hits are compacted with
- *               a ballot and one atomic per wavefront (cuda_refinement,
- *               geometry.cu:424-497).
- *  8. meetings  open-addressing table keyed by (oid1 << 32 | oid2) with 64-bit
- *               CAS: insert or extend, MBR by ordered-integer atomic min/max.
- *  9. identify  full-table scan: meetings not seen this step expire; those that
- *               lasted at least min_meet_time and stay inside the map emit two
- *               records each (cuda_identify_meetings, geometry.cu:595-737).
- *               Expired entries keep their key (so probe chains stay intact)
- *               until the table is rehashed.
+ *  7. refine    the combinations of the check units (cuda_refinement,
+ *               geometry.cu:424-497) regrouped by the owner of their pair (its
+ *               smaller id): one wavefront per owner counts its pairs, a prefix
+ *               sum gives each owner a segment, and a second pass writes them.
+ *  8. meetings  the live meetings are key-sorted segments, one per owner; a
+ *  9. identify  merge per owner joins the step's pairs with the previous
+ *               step's segment (insert/extend, geometry.cu:469-494) and turns
+ *               meetings without a pair into records when they lasted at least
+ *               min_meet_time and stay inside the map (cuda_identify_meetings,
+ *               geometry.cu:595-737).
  * 10. flush     when the memtable exceeds kv_restriction (stc_flush.hip).
  */
 #include "stc_internal.h"
@@ -89,13 +88,12 @@ struct PairHit {              /* 24 B: key and the float MBR of the two position
 struct Counters {             /* device-side counters, copied back once per sync point */
     unsigned long long units;
     unsigned long long candidates;
-    unsigned long long pairs;
+    unsigned long long pairs;          /* slots reserved by the pair count (pairs and band combinations) */
     unsigned long long ambiguous;
-    unsigned long long pairs_listed;
     unsigned long long new_keys;
     unsigned long long continuing;
     unsigned long long ambiguous_nodes;
-    unsigned long long live, expired, retired, freed, tombs;
+    unsigned long long live, expired;
     unsigned long long kv_count;
     uint32_t nodes, leaves;
     uint32_t frontier[NLEVELS + 1];
@@ -104,6 +102,7 @@ struct Counters {             /* device-side counters, copied back once per sync
     uint32_t overflow;
     uint32_t sid_count, oversize_oid_count;
     uint32_t table_fail;
+    uint32_t big, huge;                /* segments left to the block sort and to the global-scratch sort */
 };
 
 
@@ -122,16 +121,6 @@ __device__ inline uint32_t compact1by1(uint32_t x) {
     x = (x | (x >> 4)) & 0x00FF00FF;
     x = (x | (x >> 8)) & 0x0000FFFF;
     return x;
-}
-
-/* home slot of a key in a table of nb buckets: a 64-bit mix of the key
- * mapped onto [0, nb) by a multiply-shift (nb need not be a power of two) */
-__device__ inline uint64_t hash_slot(unsigned long long key, uint64_t nb) {
-    uint64_t z = key + 0x9E3779B97F4A7C15ull;
-    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-    z ^= z >> 31;
-    return (uint64_t)(((unsigned __int128)z * nb) >> 64);
 }
 
 /* ------------------------------------------------------------------ */
@@ -330,7 +319,8 @@ __global__ void k_iota(uint32_t *a, uint32_t n) {
 __global__ void k_gather(const double2 *__restrict__ xy, const double *__restrict__ dlon,
                          const uint32_t *__restrict__ lp, const uint32_t *__restrict__ pt_leaf,
                          const uint32_t *__restrict__ leaf_start, uint32_t n, double *__restrict__ lx,
-                         double *__restrict__ ly, double *__restrict__ ld, uint32_t *__restrict__ pt_rank) {
+                         double *__restrict__ ly, double *__restrict__ ld, uint32_t *__restrict__ pt_rank,
+                         uint32_t *__restrict__ lleaf) {
     uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
     uint32_t p = lp[j];
@@ -338,7 +328,18 @@ __global__ void k_gather(const double2 *__restrict__ xy, const double *__restric
     lx[j] = q.x;
     ly[j] = q.y;
     ld[j] = dlon[p];
-    pt_rank[p] = j - leaf_start[pt_leaf[p]];
+    const uint32_t L = pt_leaf[p];
+    pt_rank[p] = j - leaf_start[L];
+    lleaf[j] = L;
+}
+
+/* each point's cross units in leaf order (first unit, count) */
+__global__ void k_units_lorder(const uint32_t *__restrict__ lp, const uint32_t *__restrict__ uoffset,
+                               const uint32_t *__restrict__ ucount, uint32_t n, uint2 *__restrict__ lunits) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const uint32_t p = lp[j];
+    lunits[j] = make_uint2(uoffset[p], ucount[p]);
 }
 
 __global__ void k_overflow(const uint32_t *leaf_count, uint32_t nl, uint32_t buf, Counters *cnt) {
@@ -398,119 +399,466 @@ __global__ void k_lookup(const double2 *__restrict__ xy, const double *__restric
 }
 
 /* ------------------------------------------------------------------ */
-/* meeting table                                                       */
+/* meetings: pairs grouped by owner, merged with the live meetings      */
 /* ------------------------------------------------------------------ */
-
-#define MAX_PROBES 1024u
+/*
+ * The live meetings after a step are an array of 32 B buckets sorted by key
+ * (oid1 << 32 | oid2): one segment per owner oid1 (the pair's smaller id), in
+ * owner order, each segment's live buckets followed by tombstones for the
+ * step's duplicate or rejected pairs. A step finds its pairs owner by owner,
+ * twice (count, then write into the owner's segment), and one merge per owner
+ * joins the new segment with the previous step's: a pair continues a meeting
+ * of the previous step or opens one (insert/extend, geometry.cu:469-494), a
+ * meeting of the previous step without a pair now expires and, when it lasted
+ * long enough, becomes two memtable records (identify, geometry.cu:595-737).
+ * Every pass streams whole segments instead of touching one random bucket per
+ * pair.
+ */
 
 __device__ inline unsigned long long wave_sum_u64(unsigned long long v) {
     for (int o = STC_WAVE / 2; o > 0; o >>= 1) v += __shfl_down(v, o, STC_WAVE);
     return v;
 }
 
-__global__ void k_table_clear(Bucket *t, uint64_t nb) {
-    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nb) return;
-    Bucket b;
-    b.key = EMPTY_KEY;
-    b.start = 0;
-    b.end = 0xFFFFFFFFu;
-    b.mbr[0] = b.mbr[1] = 0xFFFFFFFFu;
-    b.mbr[2] = b.mbr[3] = 0u;
-    t[i] = b;
-}
+/* geometry of the step that the pair passes read */
+struct PairGeo {
+    const double2 *xy;
+    const double *dlon, *idlon, *lx, *ly;
+    const uint32_t *pt_leaf, *pt_rank, *leaf_start, *leaf_count, *lp;
+    const double *ld;                    /* dpm_lon in leaf order */
+    const uint32_t *lleaf;               /* leaf of each point, in leaf order */
+    const uint2 *lunits;                 /* (first cross unit, count) of each point, in leaf order */
+    const uint2 *units;                  /* cross units (pid, leaf), point after point */
+    const uint32_t *uoffset, *ucount;    /* a point's cross units: units[uoffset[p] ..] */
+    const uint32_t *rev_start, *rev_pid; /* per leaf, by pid: the border points with a cross unit to it */
+    const double2 *rev_xy;               /* their positions, dpm_lon and its inverse, in rev_pid order */
+    const double *rev_dl, *rev_il;
+    uint32_t n, buf;
+    double r, band, r2_lo, r2_hi;
+};
 
-/* linear probing from the key's hash slot; claims an empty bucket with a
- * 64-bit CAS. Returns null after MAX_PROBES probes (table too full). */
-__device__ inline Bucket *table_find_or_claim(Bucket *t, uint64_t nb, unsigned long long key, bool *claimed) {
-    uint64_t h = hash_slot(key, nb);
-    for (uint32_t probe = 0; probe < MAX_PROBES; probe++) {
-        Bucket *b = &t[h];
-        unsigned long long k = __atomic_load_n(&b->key, __ATOMIC_RELAXED);
-        if (k == key) { *claimed = false; return b; }
-        if (key_free(k)) {
-            unsigned long long prev = atomicCAS(&b->key, k, key);
-            if (prev == k) { *claimed = true; return b; }
-            if (prev == key) { *claimed = false; return b; }
-        }
-        h = (h + 1 == nb) ? 0 : h + 1;
+/* The T1 decision for a probe point and another point: 1 within r, 0 beyond,
+ * 2 inside the guard band around r (decided on the host with the C library's
+ * sin, geometry.h:40-48). The squared distance with reciprocal multiplies is
+ * within 1.2e-15 (relative) of the formula's, so only a band of 1e-12 around
+ * r^2 needs the exact divisions and square root (cuda_util.cuh:41-53). */
+__device__ inline int pair_decide(double2 pp, double pd, double pinv, double qx, double qy, const PairGeo &g) {
+    const double ax = (pp.x - qx) * pinv;
+    const double ay = (pp.y - qy) * STC_INV_DPM_LAT;
+    const double s2 = ax * ax + ay * ay;
+    if (s2 < g.r2_lo) return 1;
+    if (s2 <= g.r2_hi) {
+        const double d = stc_point_dist(pp.x, pp.y, qx, qy, pd);
+        if (fabs(d - g.r) <= g.band) return 2;
+        return d <= g.r ? 1 : 0;
     }
-    return nullptr;
+    return 0;
 }
 
-/* Insert or extend one pair seen at `now` (geometry.cu:469-494, without the
- * 5-probe drop). Within a step a key is updated only by the threads that
- * found this very pair, and all of them carry the same two positions, so the
- * bucket is read once and rewritten with plain stores: concurrent writers
- * store identical values and the result is the same in any order. Only
- * claiming an empty bucket needs a CAS. Returns -1 when the table is full,
- * 1 for a new key, 2 when it continues a meeting seen at now - 1, else 0. */
-__device__ inline int table_upsert(Bucket *t, uint64_t nb, unsigned long long key, const float *mbr, uint32_t now) {
-    uint64_t h = hash_slot(key, nb);
-    for (uint32_t probe = 0; probe < MAX_PROBES; probe++) {
-        Bucket *b = &t[h];
-        Bucket cur; /* the key half first; the MBR half only for the matching bucket */
-        {
-            const uint4 lo4 = *(const uint4 *)b; /* key, start, end */
-            cur.key = ((unsigned long long)lo4.y << 32) | lo4.x;
-            cur.start = lo4.z;
-            cur.end = lo4.w;
+__device__ inline float4 pair_mbr(float ax, float ay, float bx, float by) {
+    return make_float4(fminf(ax, bx), fminf(ay, by), fmaxf(ax, bx), fmaxf(ay, by));
+}
+
+/* Every (unit, member) combination of the step whose pair is owned by a,
+ * visited by the 64 lanes of a wave, at most one per lane per call of
+ * f(valid, other, decision, probe, mbr):
+ *   (i)   a against the later (larger-id) kept members of its own leaf (the
+ *         inside unit of geometry.cu:258; probe a);
+ *   (ii)  a against the kept members above a of the leaves of a's cross units
+ *         (geometry.cu:295-392; probe a);
+ *   (iii) the border points above a that have a cross unit to a's leaf, when a
+ *         is a kept member of it (probe: the border point).
+ * Each combination of the reference's refinement (geometry.cu:424-497) is
+ * visited exactly once, by the owner of its pair. Wave-uniform. */
+/* first index i in [0, n) with arr[i] > v (n when none) for ascending arr: the
+ * wave's lanes probe 64 evenly spaced entries per round, so a search takes one
+ * or two dependent loads where a binary search takes log2(n). Wave-uniform. */
+__device__ inline uint32_t wave_upper_bound(const uint32_t *arr, uint32_t n, uint32_t v, int lane) {
+    uint32_t lo = 0, hi = n; /* the answer lies in [lo, hi] */
+    while (lo < hi) {
+        const uint32_t span = hi - lo;
+        if (span <= STC_WAVE) {
+            const bool gt = (uint32_t)lane < span ? arr[lo + lane] > v : true;
+            const uint64_t m = __ballot(gt);
+            return m ? lo + (uint32_t)__ffsll((long long)m) - 1 : hi;
         }
-        if (cur.key == key) {
-            const uint4 hi4 = *((const uint4 *)b + 1);
-            cur.mbr[0] = hi4.x; cur.mbr[1] = hi4.y; cur.mbr[2] = hi4.z; cur.mbr[3] = hi4.w;
+        const uint32_t step = (span + STC_WAVE - 1) / STC_WAVE;
+        const uint32_t idx = lo + (uint32_t)lane * step;
+        const bool gt = idx < hi ? arr[idx] > v : true;
+        const uint64_t m = __ballot(gt);
+        const uint32_t j = m ? (uint32_t)__ffsll((long long)m) - 1 : STC_WAVE;
+        if (j == 0) return lo;
+        const uint32_t nlo = lo + (j - 1) * step + 1, nhi = min(lo + j * step, hi);
+        lo = nlo;
+        hi = nhi;
+    }
+    return lo;
+}
+
+template <class F>
+__device__ inline void owner_combos_by_source(uint32_t w, uint32_t a, const PairGeo &g, int lane, F &f) {
+    const uint32_t L = g.lleaf[w];
+    uint32_t s = g.leaf_count[L];
+    if (s > g.buf) s = g.buf;
+    const uint32_t st = g.leaf_start[L], rank = w - st;
+    const double2 pa = make_double2(g.lx[w], g.ly[w]);
+    const double pda = g.ld[w], pia = 1.0 / pda; /* the values of dlon[a] and idlon[a] (k_prep) */
+    const float afx = (float)pa.x, afy = (float)pa.y;
+    /* (i) */
+    for (uint32_t m0 = rank + 1; m0 < s; m0 += STC_WAVE) {
+        const uint32_t m = m0 + lane;
+        const bool v = m < s;
+        int dec = 0;
+        uint32_t q = 0;
+        float4 mb = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (v) {
+            const double qx = g.lx[st + m], qy = g.ly[st + m];
+            dec = pair_decide(pa, pda, pia, qx, qy, g);
+            q = g.lp[st + m];
+            mb = pair_mbr(afx, afy, (float)qx, (float)qy);
         }
-        bool claimed = false, raced = false;
-        if (key_free(cur.key)) {
-            unsigned long long prev = atomicCAS(&b->key, cur.key, key);
-            if (prev == cur.key) {
-                claimed = true;
-            } else if (prev == key) { /* another thread claimed this very pair this step */
-                cur = *b;
-                raced = true;
-            } else {
-                h = (h + 1 == nb) ? 0 : h + 1;
-                continue;
+        f(v, q, dec, a, mb);
+    }
+    /* (ii) */
+    const uint2 un = g.lunits[w];
+    const uint32_t u0 = un.x, nu = un.y;
+    for (uint32_t k = 0; k < nu; k++) {
+        const uint32_t L2 = g.units[u0 + k].y;
+        uint32_t s2 = g.leaf_count[L2];
+        if (s2 > g.buf) s2 = g.buf;
+        const uint32_t st2 = g.leaf_start[L2];
+        for (uint32_t m0 = wave_upper_bound(g.lp + st2, s2, a, lane); m0 < s2; m0 += STC_WAVE) {
+            const uint32_t m = m0 + lane;
+            const bool v = m < s2;
+            int dec = 0;
+            uint32_t q = 0;
+            float4 mb = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (v) {
+                const double qx = g.lx[st2 + m], qy = g.ly[st2 + m];
+                dec = pair_decide(pa, pda, pia, qx, qy, g);
+                q = g.lp[st2 + m];
+                mb = pair_mbr(afx, afy, (float)qx, (float)qy);
             }
-        } else if (cur.key != key) {
-            h = (h + 1 == nb) ? 0 : h + 1;
+            f(v, q, dec, a, mb);
+        }
+    }
+    /* (iii) */
+    if (rank < s) {
+        const uint32_t r0 = g.rev_start[L], r1 = g.rev_start[L + 1];
+        const double qx = g.lx[st + rank], qy = g.ly[st + rank];
+        const float qfx = (float)qx, qfy = (float)qy;
+        for (uint32_t i0 = r0 + wave_upper_bound(g.rev_pid + r0, r1 - r0, a, lane); i0 < r1; i0 += STC_WAVE) {
+            const uint32_t i = i0 + lane;
+            const bool v = i < r1;
+            int dec = 0;
+            uint32_t p = 0;
+            float4 mb = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (v) {
+                p = g.rev_pid[i];
+                const double2 pp = g.rev_xy[i];
+                dec = pair_decide(pp, g.rev_dl[i], g.rev_il[i], qx, qy, g);
+                mb = pair_mbr((float)pp.x, (float)pp.y, qfx, qfy);
+            }
+            f(v, p, dec, p, mb);
+        }
+    }
+}
+
+/* The same combinations as owner_combos_by_source, with the owner's sources
+ * laid end to end so that every call of f but the last fills all 64 lanes:
+ * lane j describes source j (j = 0: (i); 1 .. nu: the cross units; nu + 1:
+ * (iii)) by its first index and length, and each lane finds the source of its
+ * combination from the prefix of the lengths. Owners with more than 62 cross
+ * units take the per-source loops. Wave-uniform. */
+template <class F>
+__device__ inline void owner_combos(uint32_t w, uint32_t a, const PairGeo &g, int lane, F &f) {
+    const uint2 un = g.lunits[w];
+    const uint32_t u0 = un.x, nu = un.y;
+    if (nu + 2 > STC_WAVE) {
+        owner_combos_by_source(w, a, g, lane, f);
+        return;
+    }
+    const uint32_t L = g.lleaf[w];
+    uint32_t s = g.leaf_count[L];
+    if (s > g.buf) s = g.buf;
+    const uint32_t st = g.leaf_start[L], rank = w - st;
+    const double2 pa = make_double2(g.lx[w], g.ly[w]);
+    const double pda = g.ld[w], pia = 1.0 / pda; /* the values of dlon[a] and idlon[a] (k_prep) */
+    const float afx = (float)pa.x, afy = (float)pa.y;
+    /* source table: first index (into the leaf-ordered arrays, or rev_pid for
+     * the last source) and length, held by lane j for source j */
+    uint32_t sbase = 0, slen = 0;
+    if (lane == 0 && rank + 1 < s) {
+        sbase = st + rank + 1;
+        slen = s - rank - 1;
+    }
+    for (uint32_t k = 0; k < nu; k++) {
+        const uint32_t L2 = g.units[u0 + k].y;
+        uint32_t s2 = g.leaf_count[L2];
+        if (s2 > g.buf) s2 = g.buf;
+        const uint32_t st2 = g.leaf_start[L2];
+        const uint32_t m0 = wave_upper_bound(g.lp + st2, s2, a, lane);
+        if ((uint32_t)lane == k + 1) {
+            sbase = st2 + m0;
+            slen = s2 - m0;
+        }
+    }
+    if (rank < s) {
+        const uint32_t r0 = g.rev_start[L], r1 = g.rev_start[L + 1];
+        const uint32_t i0 = r0 + wave_upper_bound(g.rev_pid + r0, r1 - r0, a, lane);
+        if ((uint32_t)lane == nu + 1) {
+            sbase = i0;
+            slen = r1 - i0;
+        }
+    }
+    /* exclusive prefix of the lengths */
+    uint32_t scum = slen;
+    for (int o = 1; o < STC_WAVE; o <<= 1) {
+        const uint32_t y = __shfl_up(scum, o, STC_WAVE);
+        if (lane >= o) scum += y;
+    }
+    const uint32_t total = __shfl(scum, STC_WAVE - 1, STC_WAVE);
+    scum -= slen;
+    const uint32_t nseg = nu + 2;
+    const double qax = pa.x, qay = pa.y; /* a's position as a member of its leaf */
+    for (uint32_t c0 = 0; c0 < total; c0 += STC_WAVE) {
+        const uint32_t c = c0 + lane;
+        const bool v = c < total;
+        uint32_t j = 0;
+        for (uint32_t k = 1; k < nseg; k++)
+            if (__shfl(scum, (int)k, STC_WAVE) <= c) j = k;
+        const uint32_t jb = __shfl(sbase, (int)j, STC_WAVE), jc = __shfl(scum, (int)j, STC_WAVE);
+        int dec = 0;
+        uint32_t q = 0, probe = a;
+        float4 mb = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (v) {
+            const uint32_t idx = jb + (c - jc);
+            if (j + 1 < nseg) { /* a member of the owner's leaf or of a cross unit's leaf; probe a */
+                const double qx = g.lx[idx], qy = g.ly[idx];
+                dec = pair_decide(pa, pda, pia, qx, qy, g);
+                q = g.lp[idx];
+                mb = pair_mbr(afx, afy, (float)qx, (float)qy);
+            } else { /* a border point with a cross unit to a's leaf; probe the border point */
+                const uint32_t p = g.rev_pid[idx];
+                const double2 pp = g.rev_xy[idx];
+                dec = pair_decide(pp, g.rev_dl[idx], g.rev_il[idx], qax, qay, g);
+                q = p;
+                probe = p;
+                mb = pair_mbr((float)pp.x, (float)pp.y, (float)qax, (float)qay);
+            }
+        }
+        f(v, q, dec, probe, mb);
+    }
+}
+
+/* split the cross units into (leaf, pid) key/value arrays for the reverse lists */
+__global__ void k_units_split(const uint2 *__restrict__ units, uint32_t nu, uint32_t *__restrict__ leaf,
+                              uint32_t *__restrict__ pid, uint32_t *__restrict__ per_leaf) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nu) return;
+    const uint2 u = units[i];
+    leaf[i] = u.y;
+    pid[i] = u.x;
+    atomicAdd(&per_leaf[u.y], 1u);
+}
+
+/* the border points' data in reverse-list order, so that owners read it coalesced */
+__global__ void k_rev_gather(const uint32_t *__restrict__ rev_pid, uint32_t nr, const double2 *__restrict__ xy,
+                             const double *__restrict__ dlon, const double *__restrict__ idlon,
+                             double2 *__restrict__ rxy, double *__restrict__ rdl, double *__restrict__ ril) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nr) return;
+    const uint32_t p = rev_pid[i];
+    rxy[i] = xy[p];
+    rdl[i] = dlon[p];
+    ril[i] = idlon[p];
+}
+
+/* Pass 1: per owner, the combinations that are pairs or lie in the guard band
+ * (each reserves a slot of the owner's segment); band combinations are listed
+ * as (probe, other) for the host. One wave per owner, owners in leaf order. */
+__global__ __launch_bounds__(256) void k_pair_count(PairGeo g, uint32_t *__restrict__ pcount,
+                                                    uint2 *__restrict__ amb, uint32_t amb_cap, Counters *cnt) {
+    const int lane = threadIdx.x & (STC_WAVE - 1);
+    const uint64_t wid = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / STC_WAVE;
+    const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) / STC_WAVE;
+    unsigned long long cand = 0, total = 0;
+    for (uint64_t w = wid; w < g.n; w += nw) {
+        const uint32_t a = g.lp[w];
+        unsigned long long c = 0;
+        auto f = [&](bool v, uint32_t q, int dec, uint32_t probe, float4) {
+            if (!v) return;
+            cand++;
+            if (dec) c++;
+            if (dec == 2) {
+                const unsigned long long k = atomicAdd(&cnt->ambiguous, 1ull);
+                if (k < amb_cap) amb[k] = make_uint2(probe, probe == a ? q : a);
+            }
+        };
+        owner_combos((uint32_t)w, a, g, lane, f);
+        c = wave_sum_u64(c);
+        if (lane == 0) {
+            pcount[a] = (uint32_t)c;
+            total += c;
+        }
+    }
+    cand = wave_sum_u64(cand);
+    if (lane == 0) {
+        if (cand) atomicAdd(&cnt->candidates, cand);
+        if (total) atomicAdd(&cnt->pairs, total);
+    }
+}
+
+__device__ inline bool sorted_contains(const unsigned long long *a, uint32_t n, unsigned long long v) {
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (a[mid] < v) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo < n && a[lo] == v;
+}
+
+#define DEAD_B 0xFFFFFFFFu /* partner word of a dead slot or tombstone: sorts after the owner's live pairs */
+#define WCAP 256u          /* slots of a segment the write pass assembles and sorts in its wave's LDS */
+
+__device__ inline uint64_t lanemask_lt(int lane) { return lane == 0 ? 0ull : (~0ull >> (STC_WAVE - lane)); }
+
+/* ascending bitonic sort of s[0, P) (P a power of two) by one wave */
+__device__ inline void wave_bitonic(unsigned long long *s, uint32_t P, int lane) {
+    if (P <= STC_WAVE) { /* one element per lane, in registers */
+        unsigned long long x = (uint32_t)lane < P ? s[lane] : ~0ull;
+        for (uint32_t k = 2; k <= STC_WAVE; k <<= 1)
+            for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+                const unsigned long long y = __shfl_xor(x, (int)j, STC_WAVE);
+                const bool asc = (lane & k) == 0, lower = (lane & j) == 0;
+                x = (asc == lower) ? (x < y ? x : y) : (x < y ? y : x);
+            }
+        if ((uint32_t)lane < P) s[lane] = x;
+        __builtin_amdgcn_wave_barrier();
+        return;
+    }
+    for (uint32_t k = 2; k <= P; k <<= 1)
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = lane; i < P; i += STC_WAVE) {
+                const uint32_t ixj = i ^ j;
+                if (ixj > i) {
+                    const unsigned long long x = s[i], y = s[ixj];
+                    if ((x > y) == ((i & k) == 0)) {
+                        s[i] = y;
+                        s[ixj] = x;
+                    }
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+}
+
+/* Pass 2: the same combinations, written to pn as each owner's segment sorted
+ * by partner, its live pairs first and its dead slots (band combinations the
+ * host rejected, repeats of a pair) behind them, so that pn is sorted by key.
+ * A segment of at most WCAP slots is assembled and sorted in the wave's LDS;
+ * up to 2 * WCAP slots go through ptmp with only their sort keys in LDS; a
+ * longer one is left unsorted in ptmp and listed in `big` for k_seg_sort. The
+ * pass repeats pass 1's arithmetic exactly; a slot beyond the counted ones
+ * would be dropped and reported through table_fail. */
+__global__ __launch_bounds__(256) void k_pair_write(PairGeo g, const uint32_t *__restrict__ poff,
+                                                    const uint32_t *__restrict__ pcount,
+                                                    const unsigned long long *__restrict__ acc, uint32_t n_acc,
+                                                    PairHit *__restrict__ ptmp, PairHit *__restrict__ pn,
+                                                    uint32_t *__restrict__ big, Counters *cnt) {
+    /* per wave: WCAP sort keys and WCAP boxes, or 2 * WCAP sort keys */
+    __shared__ __align__(16) unsigned long long wraw[256 / STC_WAVE][3 * WCAP];
+    const int lane = threadIdx.x & (STC_WAVE - 1);
+    unsigned long long *keys = wraw[threadIdx.x / STC_WAVE];
+    float4 *mbrs = (float4 *)(keys + WCAP);
+    const uint64_t wid = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / STC_WAVE;
+    const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) / STC_WAVE;
+    const uint64_t lt = lanemask_lt(lane);
+    for (uint64_t w = wid; w < g.n; w += nw) {
+        const uint32_t a = g.lp[w];
+        const uint32_t base = poff[a], slots = pcount[a];
+        const bool small = slots <= WCAP, mid = !small && slots <= 2 * WCAP;
+        uint32_t fill = 0;
+        auto f = [&](bool v, uint32_t q, int dec, uint32_t, float4 mb) {
+            const bool take = v && dec != 0;
+            const uint64_t m = __ballot(take);
+            if (take) {
+                uint32_t bq = q;
+                if (dec == 2 && !sorted_contains(acc, n_acc, ((unsigned long long)a << 32) | q)) bq = DEAD_B;
+                const uint32_t at = fill + (uint32_t)__popcll(m & lt);
+                if (at >= slots) {
+                    atomicOr(&cnt->table_fail, 1u);
+                } else if (small) {
+                    keys[at] = ((unsigned long long)bq << 32) | at;
+                    mbrs[at] = mb;
+                } else {
+                    PairHit h;
+                    h.key = ((unsigned long long)a << 32) | bq;
+                    h.mbr[0] = mb.x; h.mbr[1] = mb.y; h.mbr[2] = mb.z; h.mbr[3] = mb.w;
+                    ptmp[base + at] = h;
+                    if (mid) keys[at] = ((unsigned long long)bq << 32) | at;
+                }
+            }
+            fill += (uint32_t)__popcll(m);
+        };
+        owner_combos((uint32_t)w, a, g, lane, f);
+        if (!small && !mid) {
+            if (lane == 0) big[atomicAdd(&cnt->big, 1u)] = a; /* the list holds every owner */
             continue;
         }
-        if (claimed) {
-            b->start = now;
-            b->end = now;
-            b->mbr[0] = f2o(mbr[0]);
-            b->mbr[1] = f2o(mbr[1]);
-            b->mbr[2] = f2o(mbr[2]);
-            b->mbr[3] = f2o(mbr[3]);
-            return 1;
+        if (slots == 0) continue;
+        if (mid) __threadfence_block(); /* the staged slots are read back by other lanes of the wave */
+        uint32_t P = 1;
+        while (P < slots) P <<= 1;
+        for (uint32_t i = slots + lane; i < P; i += STC_WAVE) keys[i] = ~0ull;
+        __builtin_amdgcn_wave_barrier();
+        wave_bitonic(keys, P, lane);
+        /* live pairs in partner order, then the dead slots */
+        uint32_t nlive = 0;
+        for (uint32_t k0 = 0; k0 < slots; k0 += STC_WAVE) {
+            const uint32_t k = k0 + lane;
+            bool live = false;
+            if (k < slots) {
+                const uint32_t b = (uint32_t)(keys[k] >> 32);
+                live = b != DEAD_B && (k == 0 || (uint32_t)(keys[k - 1] >> 32) != b);
+            }
+            nlive += (uint32_t)__popcll(__ballot(live));
         }
-        uint32_t old_end = cur.end;
-        if (old_end != now && old_end != now - 1) b->start = now; /* restarted (retired key) or racing a claim */
-        b->end = now;
-        {
-            const uint32_t n0 = min(cur.mbr[0], f2o(mbr[0])), n1 = min(cur.mbr[1], f2o(mbr[1]));
-            const uint32_t n2 = max(cur.mbr[2], f2o(mbr[2])), n3 = max(cur.mbr[3], f2o(mbr[3]));
-            if (n0 != cur.mbr[0] || n1 != cur.mbr[1] || n2 != cur.mbr[2] || n3 != cur.mbr[3])
-                *((uint4 *)b + 1) = make_uint4(n0, n1, n2, n3); /* only a grown MBR is written */
+        uint32_t lr = 0, dr = 0;
+        for (uint32_t k0 = 0; k0 < slots; k0 += STC_WAVE) {
+            const uint32_t k = k0 + lane;
+            bool live = false;
+            uint32_t b = DEAD_B;
+            if (k < slots) {
+                b = (uint32_t)(keys[k] >> 32);
+                live = b != DEAD_B && (k == 0 || (uint32_t)(keys[k - 1] >> 32) != b);
+            }
+            const uint64_t ml = __ballot(live), md = __ballot(k < slots && !live);
+            if (k < slots) {
+                PairHit e;
+                if (mid) {
+                    e = ptmp[base + (uint32_t)keys[k]];
+                } else {
+                    const float4 mb = mbrs[(uint32_t)keys[k]];
+                    e.mbr[0] = mb.x; e.mbr[1] = mb.y; e.mbr[2] = mb.z; e.mbr[3] = mb.w;
+                }
+                e.key = ((unsigned long long)a << 32) | (live ? b : DEAD_B);
+                const uint32_t dest = live ? lr + (uint32_t)__popcll(ml & lt) : nlive + dr + (uint32_t)__popcll(md & lt);
+                pn[base + dest] = e;
+            }
+            lr += (uint32_t)__popcll(ml);
+            dr += (uint32_t)__popcll(md);
         }
-        /* a free bucket's end field reads 0xFFFFFFFF until its claimer writes it: at
-         * now == 0 that equals now - 1, so nothing continues into step 0 */
-        return (!raced && now != 0 && old_end == now - 1) ? 2 : 0;
+        __builtin_amdgcn_wave_barrier(); /* the stage is reused by the wave's next owner */
     }
-    return -1;
 }
 
-/* insert a list of pairs (host-resolved ambiguous pairs) */
-__global__ void k_table_insert(Bucket *t, uint64_t nb, const PairHit *__restrict__ pairs, uint64_t np, uint32_t now,
-                               Counters *cnt) {
-    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+/* replayed pairs (sorted by key): the segment length of each owner */
+__global__ void k_owner_hist(const PairHit *__restrict__ pn, uint32_t np, uint32_t *__restrict__ pcount) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= np) return;
-    PairHit h = pairs[i];
-    int rc = table_upsert(t, nb, h.key, h.mbr, now);
-    if (rc < 0) atomicAdd(&cnt->table_fail, 1u);
-    if (rc == 1) atomicAdd(&cnt->new_keys, 1ull);
-    if (rc == 2) atomicAdd(&cnt->continuing, 1ull);
+    atomicAdd(&pcount[(uint32_t)(pn[i].key >> 32)], 1u);
 }
 
 /* Persisted meeting units back to pair keys: the Cantor key of
@@ -535,8 +883,7 @@ __global__ void k_units_decode(const stc_meeting_unit *__restrict__ u, uint32_t 
     idx[i] = i;
 }
 
-/* the units in pair order as PairHits; a pair named twice sets table_fail's
- * neighbour flag (bad_points) */
+/* the units in pair order as PairHits; a pair named twice sets bad_points */
 __global__ void k_units_gather(const uint64_t *__restrict__ keys, const uint32_t *__restrict__ idx,
                                const stc_meeting_unit *__restrict__ u, uint32_t nu, PairHit *__restrict__ out,
                                Counters *cnt) {
@@ -550,114 +897,6 @@ __global__ void k_units_gather(const uint64_t *__restrict__ keys, const uint32_t
     out[i] = h;
 }
 
-/* ------------------------------------------------------------------ */
-/* refinement fused with the table insert                               */
-/* ------------------------------------------------------------------ */
-
-__global__ __launch_bounds__(256) void k_refine(
-    const double2 *__restrict__ xy, const double *__restrict__ dlon, const double *__restrict__ idlon, uint32_t n,
-    const uint32_t *__restrict__ pt_leaf, const uint32_t *__restrict__ pt_rank, const uint32_t *__restrict__ leaf_start,
-    const uint32_t *__restrict__ leaf_count, const uint32_t *__restrict__ lp, const double *__restrict__ lx,
-    const double *__restrict__ ly, const uint2 *__restrict__ units, uint64_t n_units_total, uint32_t buf, double r,
-    double band, double r2_lo, double r2_hi, Bucket *table, uint64_t nb, uint32_t now,
-    PairHit *__restrict__ pairs, uint64_t pair_cap,
-    uint2 *__restrict__ amb, uint32_t amb_cap, Counters *cnt) {
-    const int lane = threadIdx.x & (STC_WAVE - 1);
-    const uint64_t wid = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / STC_WAVE;
-    const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) / STC_WAVE;
-    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (STC_WAVE - lane));
-    unsigned long long cand = 0, hits = 0, fresh = 0, fails = 0, cont = 0;
-    for (uint64_t u = wid; u < n_units_total; u += nw) {
-        uint32_t p, leaf, begin;
-        if (u < n) {
-            p = (uint32_t)u;
-            leaf = pt_leaf[p];
-            begin = pt_rank[p] + 1;
-        } else {
-            uint2 un = units[u - n];
-            p = un.x;
-            leaf = un.y;
-            begin = 0;
-        }
-        uint32_t s = leaf_count[leaf];
-        if (s > buf) s = buf;
-        if (begin >= s) continue;
-        uint32_t st = leaf_start[leaf];
-        double2 pp = xy[p];
-        double pd = dlon[p];
-        double pinv = idlon[p];
-        float pfx = (float)pp.x, pfy = (float)pp.y;
-        for (uint32_t m0 = begin; m0 < s; m0 += STC_WAVE) {
-            uint32_t m = m0 + lane;
-            bool hit = false, ambg = false;
-            uint32_t q = 0;
-            double qx = 0, qy = 0;
-            if (m < s) {
-                qx = lx[st + m];
-                qy = ly[st + m];
-                /* squared distance with reciprocal multiplies; it is within
-                 * 1.2e-15 (relative) of the formula's, so only a band of 1e-12
-                 * around r^2 needs the exact divisions and square root */
-                double ax = (pp.x - qx) * pinv;
-                double ay = (pp.y - qy) * STC_INV_DPM_LAT;
-                double s2 = ax * ax + ay * ay;
-                if (s2 < r2_lo) {
-                    hit = true;
-                } else if (s2 <= r2_hi) {
-                    double d = stc_point_dist(pp.x, pp.y, qx, qy, pd);
-                    if (fabs(d - r) <= band) ambg = true;
-                    else hit = d <= r;
-                }
-                if (hit || ambg) q = lp[st + m];
-                cand++;
-            }
-            if (hit) {
-                uint32_t a = p < q ? p : q, b = p < q ? q : p;
-                unsigned long long key = ((unsigned long long)a << 32) | b;
-                float qfx = (float)qx, qfy = (float)qy;
-                float mbr[4] = {fminf(pfx, qfx), fminf(pfy, qfy), fmaxf(pfx, qfx), fmaxf(pfy, qfy)};
-                /* once the table has overflowed this pass only counts; the host
-                 * grows the table and repeats the pass */
-                int rc = __atomic_load_n(&cnt->table_fail, __ATOMIC_RELAXED) ? -1
-                                                                             : table_upsert(table, nb, key, mbr, now);
-                hits++;
-                if (rc == 1) fresh++;
-                if (rc == 2) cont++;
-                if (rc < 0) fails++;
-                if (pairs) { /* inspection copy of the step's pairs */
-                    unsigned long long k = atomicAdd(&cnt->pairs_listed, 1ull);
-                    if (k < pair_cap) {
-                        PairHit h;
-                        h.key = key;
-                        h.mbr[0] = mbr[0]; h.mbr[1] = mbr[1]; h.mbr[2] = mbr[2]; h.mbr[3] = mbr[3];
-                        pairs[k] = h;
-                    }
-                }
-            }
-            if (ambg) {
-                unsigned long long k = atomicAdd(&cnt->ambiguous, 1ull);
-                if (k < amb_cap) amb[k] = make_uint2(p, q);
-            }
-        }
-    }
-    (void)lt;
-    cand = wave_sum_u64(cand);
-    hits = wave_sum_u64(hits);
-    fresh = wave_sum_u64(fresh);
-    fails = wave_sum_u64(fails);
-    cont = wave_sum_u64(cont);
-    if (lane == 0) {
-        if (cont) atomicAdd(&cnt->continuing, cont);
-        if (cand) atomicAdd(&cnt->candidates, cand);
-        if (hits) atomicAdd(&cnt->pairs, hits);
-        if (fresh) atomicAdd(&cnt->new_keys, fresh);
-        if (fails) atomicAdd(&cnt->table_fail, (uint32_t)fails);
-    }
-}
-
-/* float_to_uint (cuda_util.cuh:115-119), saturating as the reference GPU converts */
-
-
 struct MemtableDev {
     uint64_t *keys;        /* the reference's 128-bit key as (lo, hi) words:
                               hi = pid << 26 | target, lo = duration << 32 | end offset */
@@ -668,390 +907,328 @@ struct MemtableDev {
     uint16_t *sids;
 };
 
-#define ID_STAGE 128 /* per-wavefront staging of records to emit */
-
-/* write the two memtable records of each staged meeting (geometry.cu:641-705) */
-__device__ inline void emit_staged(const Bucket *st, uint32_t staged, int lane, uint32_t max_meet,
-                                   uint32_t end_time_min, MemtableDev mt, Counters *cnt) {
-    unsigned long long kbase = 0;
-    if (lane == 0) kbase = atomicAdd(&cnt->kv_count, 2ull * staged);
-    kbase = __shfl(kbase, 0, STC_WAVE);
-    for (uint32_t e = lane; e < staged; e += STC_WAVE) {
-        Bucket b = st[e];
-        float lo0 = o2f(b.mbr[0]), lo1 = o2f(b.mbr[1]), hi0 = o2f(b.mbr[2]), hi1 = o2f(b.mbr[3]);
-        uint32_t duration = b.end - b.start;
-        uint32_t pid = (uint32_t)(b.key >> 32), target = (uint32_t)b.key;
-        if (duration > max_meet) {
-            atomicAdd(&cnt->oversize_oid_count, 1u);
-            mt.sids[pid] = 1;
-            mt.sids[target] = 1;
-        }
-        unsigned long long k = kbase + 2ull * e;
-        uint32_t q0 = f2q(lo0), q1 = f2q(lo1), q2 = f2q(hi0), q3 = f2q(hi1);
-        for (int sdir = 0; sdir < 2; sdir++) {
-            uint32_t a = sdir ? target : pid, c = sdir ? pid : target;
-            if (k + sdir < mt.cap) {
-                mt.keys[2 * (k + sdir)] = ((uint64_t)duration << 32) | (uint32_t)(b.end - end_time_min);
-                mt.keys[2 * (k + sdir) + 1] = ((uint64_t)a << 26) | (uint64_t)c;
-                mt.boxes[k + sdir] = make_float4(lo0, lo1, hi0, hi1);
-            }
-            if (atomicAdd(&mt.same_pid_count[a], 1u) == 0) atomicAdd(&cnt->sid_count, 1u);
-            atomicMin(&mt.o_boxs[a].x, q0);
-            atomicMax(&mt.o_boxs[a].z, q2);
-            atomicMin(&mt.o_boxs[a].y, q1);
-            atomicMax(&mt.o_boxs[a].w, q3);
-        }
-    }
-}
-
-__device__ inline uint64_t slot_next(uint64_t j, uint64_t nb) { return (j + 1 == nb) ? 0 : j + 1; }
-
-__device__ inline void write_marker(Bucket *b, unsigned long long key) {
-    Bucket x;
-    x.key = key;
-    x.start = 0;
-    x.end = 0xFFFFFFFFu;
-    x.mbr[0] = x.mbr[1] = 0xFFFFFFFFu;
-    x.mbr[2] = x.mbr[3] = 0u;
-    *b = x;
-}
-
-/* Stage the buckets of the lanes with `emit` set, writing the staged records
- * out first when the stage could overflow. Wave-uniform. */
-__device__ inline void stage_push(Bucket *st, uint32_t &staged, bool emit, const Bucket &b, int lane, uint64_t lt,
-                                  uint32_t max_meet, uint32_t end_time_min, MemtableDev mt, Counters *cnt) {
-    uint64_t em = __ballot(emit);
-    if (!em) return;
-    if (staged + (uint32_t)__popcll(em) > ID_STAGE) {
-        __builtin_amdgcn_wave_barrier();
-        emit_staged(st, staged, lane, max_meet, end_time_min, mt, cnt);
-        staged = 0;
-        __builtin_amdgcn_wave_barrier();
-    }
-    if (emit) st[staged + __popcll(em & lt)] = b;
-    staged += (uint32_t)__popcll(em);
-}
-
-struct IdentifyArgs {
+struct MergeArgs {
     uint32_t now, min_meet, max_meet, end_time_min;
     MapBox m;
 };
 
-__device__ inline bool meeting_emits(const Bucket &b, const IdentifyArgs &a) {
+/* a meeting that expires at `now` is recorded when it lasted long enough and
+ * its MBR lies in the map (geometry.cu:665-670) */
+__device__ inline bool meeting_emits(const Bucket &b, const MergeArgs &a) {
     float lo0 = o2f(b.mbr[0]), lo1 = o2f(b.mbr[1]), hi0 = o2f(b.mbr[2]), hi1 = o2f(b.mbr[3]);
     return a.now - b.start >= a.min_meet + 1 && (double)lo0 >= a.m.lo0 && (double)hi0 <= a.m.hi0 &&
            (double)lo1 >= a.m.lo1 && (double)hi1 <= a.m.hi1;
 }
 
-struct IdentifyTally {
-    unsigned long long live = 0, exp = 0, ret = 0, freed = 0, tomb = 0;
-};
-
-/* Backward-shift deletion over one probe cluster that starts after the free
- * bucket `owner`, walked bucket by bucket from global memory by the lanes with
- * `walking` set (wave-uniform call). Holes are kept in a 64-slot window behind
- * the walker; a hole that leaves the window becomes a tombstone, an occupied
- * bucket that matches no key. Used for clusters longer than the identify
- * window, which are rare. */
-__device__ void walk_cluster(Bucket *t, uint64_t nb, uint64_t owner, bool walking, const IdentifyArgs &a,
-                             Bucket *st, uint32_t &staged, int lane, uint64_t lt, MemtableDev mt, Counters *cnt,
-                             IdentifyTally &tl) {
-    const unsigned long long free_now = FREE_KEY(a.now), tomb = TOMB_KEY(a.now);
-    uint64_t j = slot_next(owner, nb), wb = j, holes = 0;
-    uint32_t rel = 0;
-    while (__ballot(walking)) {
-        bool emit = false;
-        Bucket b;
-        if (walking) {
-            b = t[j];
-            if (key_free(b.key)) {
-                while (holes) {
-                    int q = __builtin_ctzll(holes);
-                    holes &= holes - 1;
-                    uint64_t dst = wb + q;
-                    if (dst >= nb) dst -= nb;
-                    write_marker(&t[dst], free_now);
-                    tl.freed++;
-                }
-                walking = false;
-            } else {
-                if (rel == STC_WAVE) {
-                    if (holes & 1ull) {
-                        write_marker(&t[wb], tomb);
-                        tl.tomb++;
-                    }
-                    holes >>= 1;
-                    wb = slot_next(wb, nb);
-                    rel--;
-                }
-                if (b.end == a.now && !key_tomb(b.key)) {
-                    tl.live++;
-                    if (holes) {
-                        uint64_t h = hash_slot(b.key, nb);
-                        uint64_t back = (j >= h) ? j - h : j + nb - h;
-                        uint64_t mask = holes;
-                        if (back < rel) mask &= ~0ull << (rel - back);
-                        if (mask) {
-                            int q = __builtin_ctzll(mask);
-                            uint64_t dst = wb + q;
-                            if (dst >= nb) dst -= nb;
-                            t[dst] = b;
-                            holes ^= 1ull << q;
-                            holes |= 1ull << rel;
-                        }
-                    }
-                } else {
-                    if (!key_tomb(b.key) && b.end == a.now - 1) {
-                        tl.exp++;
-                        emit = meeting_emits(b, a);
-                    } else {
-                        tl.ret++;
-                    }
-                    holes |= 1ull << rel;
-                }
-                j = slot_next(j, nb);
-                rel++;
-            }
-        }
-        stage_push(st, staged, emit, b, lane, lt, a.max_meet, a.end_time_min, mt, cnt);
-    }
-}
-
-/* identify completed meetings (geometry.cu:595-737) and delete every bucket
- * that is not live at `now` from the table.
- *
- * Deletion is linear probing's backward shift, applied per probe cluster (a
- * maximal run of occupied buckets): each live bucket moves to the first hole
- * at or after its home slot, holes being dead buckets and the slots that moved
- * buckets vacated, so every remaining key keeps an unbroken probe chain; the
- * holes left at the end become free buckets.
- *
- * Each wavefront owns a contiguous range of 64-bucket chunks and the clusters
- * that start in them (after a bucket that was free when the pass began; slots
- * freed by the pass carry the tag `now` and start nothing). A chunk is handled
- * with the next chunk as lookahead, one bucket per lane and half: ballots give
- * the free, live and dead masks, each lane finds its cluster from them, and
- * the hole-filling runs as a scalar loop over 128-bit masks that visits only
- * live buckets with a hole before them in their cluster. The rare cluster
- * that leaves the 128-bucket window is walked from memory (walk_cluster).
- * Clusters are disjoint, so no two lanes ever write the same bucket, and a
- * cluster's buckets are read before its owner writes any of them.
- *
- * Expiring buckets that emit records are staged per wavefront and written in
- * batches, so the memtable counter sees one atomic per batch. */
-__global__ __launch_bounds__(256) void k_identify(Bucket *t, uint64_t nb, IdentifyArgs a, MemtableDev mt,
-                                                  Counters *cnt) {
-    __shared__ Bucket stage[256 / STC_WAVE][ID_STAGE];
-    const int lane = threadIdx.x & (STC_WAVE - 1), w = threadIdx.x / STC_WAVE;
-    const uint64_t lt = (1ull << lane) - 1;           /* lanes below */
-    const uint64_t gt = ~((2ull << lane) - 1);        /* lanes above */
-    const unsigned long long free_now = FREE_KEY(a.now);
-    const uint64_t nchunks = nb / STC_WAVE;
-    const uint64_t nwaves = (uint64_t)gridDim.x * (256 / STC_WAVE);
-    const uint64_t wid = (uint64_t)blockIdx.x * (256 / STC_WAVE) + w;
-    const uint64_t c_begin = wid * nchunks / nwaves, c_end = (wid + 1) * nchunks / nwaves;
-    Bucket *st = stage[w];
-    IdentifyTally tl;
-    uint32_t staged = 0;
-    if (c_begin < c_end) {
-        auto at = [&](uint64_t chunk) -> uint64_t {
-            uint64_t p = chunk * STC_WAVE + lane;
-            return p >= nb ? p - nb : p;
-        };
-        Bucket A = t[at(c_begin)], B = t[at(c_begin + 1)];
-        for (uint64_t c = c_begin; c < c_end; c++) {
-            Bucket N;
-            if (c + 1 < c_end) N = t[at(c + 2)]; /* prefetch */
-            const uint64_t pa = at(c), pb = at(c + 1);
-            const bool fA = key_free(A.key), fB = key_free(B.key);
-            const bool lA = !fA && !key_tomb(A.key) && A.end == a.now;
-            const bool lB = !fB && !key_tomb(B.key) && B.end == a.now;
-            const uint64_t FA = __ballot(fA), FB = __ballot(fB);
-            const uint64_t OA = __ballot(fA && A.key != free_now);
-            /* A-half lane: owned when the nearest free slot below it is an original free slot of this chunk */
-            bool ownA = false, ownB = false, longA = false, longB = false, pastHoleA = false, pastHoleB = false;
-            uint64_t below = FA & lt;
-            int pf = 63 - __builtin_clzll(below | 1ull); /* valid when below != 0 */
-            if (!fA && below && ((OA >> pf) & 1ull)) {
-                ownA = true;
-                longA = (FA & gt) == 0 && FB == 0;
-            }
-            /* B-half lane: owned when no free slot lies between the last free slot of the chunk and it */
-            int pl = 63 - __builtin_clzll(FA | 1ull);
-            if (!fB && (FB & lt) == 0 && FA && ((OA >> pl) & 1ull)) {
-                ownB = true;
-                longB = (FB & gt) == 0;
-            }
-            const bool partA = ownA && !longA, partB = ownB && !longB;
-            const bool dA = partA && !lA, dB = partB && !lB;
-            uint64_t HA = __ballot(dA), HB = __ballot(dB); /* holes: the owned dead buckets */
-            if (partA && lA) pastHoleA = (HA & lt & ~((2ull << pf) - 1)) != 0;
-            if (partB && lB) pastHoleB = (HB & lt) != 0 || (HA & ~((2ull << pl) - 1)) != 0;
-            /* tallies and emission */
-            bool emitA = false, emitB = false;
-            if (partA) {
-                if (lA) tl.live++;
-                else if (!key_tomb(A.key) && A.end == a.now - 1) { tl.exp++; emitA = meeting_emits(A, a); }
-                else tl.ret++;
-            }
-            if (partB) {
-                if (lB) tl.live++;
-                else if (!key_tomb(B.key) && B.end == a.now - 1) { tl.exp++; emitB = meeting_emits(B, a); }
-                else tl.ret++;
-            }
-            stage_push(st, staged, emitA, A, lane, lt, a.max_meet, a.end_time_min, mt, cnt);
-            stage_push(st, staged, emitB, B, lane, lt, a.max_meet, a.end_time_min, mt, cnt);
-            /* home slots, relative to the window, of the live buckets that may move;
-             * a bucket in its home slot never moves */
-            int hA = 0, hB = 0;
-            if (pastHoleA) {
-                uint64_t h = hash_slot(A.key, nb);
-                uint64_t back = (pa >= h) ? pa - h : pa + nb - h;
-                hA = back > (uint64_t)lane ? 0 : lane - (int)back;
-                pastHoleA = back != 0;
-            }
-            if (pastHoleB) {
-                uint64_t h = hash_slot(B.key, nb);
-                uint64_t back = (pb >= h) ? pb - h : pb + nb - h;
-                hB = back > (uint64_t)(64 + lane) ? 0 : 64 + lane - (int)back;
-                pastHoleB = back != 0;
-            }
-            uint64_t MA = __ballot(pastHoleA), MB = __ballot(pastHoleB);
-            int dstA = -1, dstB = -1;
-            while (MA) { /* scalar: first hole at or after the home slot */
-                int x = __builtin_ctzll(MA);
-                MA &= MA - 1;
-                int hx = __builtin_amdgcn_readlane(hA, x);
-                uint64_t e = HA & (~0ull << hx) & ((1ull << x) - 1);
-                if (e) {
-                    int q = __builtin_ctzll(e);
-                    HA ^= 1ull << q;
-                    HA |= 1ull << x;
-                    if (lane == x) dstA = q;
-                }
-            }
-            while (MB) {
-                int x = __builtin_ctzll(MB);
-                MB &= MB - 1;
-                int hx = __builtin_amdgcn_readlane(hB, x);
-                uint64_t eA = hx < 64 ? (HA & (~0ull << hx)) : 0ull;
-                uint64_t eB = HB & (~0ull << (hx < 64 ? 0 : hx - 64)) & ((1ull << x) - 1);
-                if (eA) {
-                    int q = __builtin_ctzll(eA);
-                    HA ^= 1ull << q;
-                    HB |= 1ull << x;
-                    if (lane == x) dstB = q;
-                } else if (eB) {
-                    int q = __builtin_ctzll(eB);
-                    HB ^= 1ull << q;
-                    HB |= 1ull << x;
-                    if (lane == x) dstB = 64 + q;
-                }
-            }
-            const uint64_t c0 = c * STC_WAVE;
-            auto slot = [&](int r) -> uint64_t {
-                uint64_t p = c0 + (uint64_t)r;
-                return p >= nb ? p - nb : p;
-            };
-            if (dstA >= 0) t[slot(dstA)] = A;
-            if (dstB >= 0) t[slot(dstB)] = B;
-            if ((HA >> lane) & 1ull) {
-                write_marker(&t[pa], free_now);
-                tl.freed++;
-            }
-            if ((HB >> lane) & 1ull) {
-                write_marker(&t[pb], free_now);
-                tl.freed++;
-            }
-            /* the cluster after the chunk's last free slot, when it leaves the window */
-            const uint64_t LG = __ballot(longA || longB);
-            const bool long_owner = LG != 0 && lane == pl;
-            walk_cluster(t, nb, pa, long_owner, a, st, staged, lane, lt, mt, cnt, tl);
-            A = B;
-            B = N;
+/* the two records of an expired meeting at k, k + 1 (geometry.cu:717-734) */
+__device__ inline void write_meeting_records(const Bucket &b, unsigned long long k, uint32_t end_time_min,
+                                             const MemtableDev &mt) {
+    const float4 box = make_float4(o2f(b.mbr[0]), o2f(b.mbr[1]), o2f(b.mbr[2]), o2f(b.mbr[3]));
+    const uint32_t duration = b.end - b.start;
+    const uint32_t pid = (uint32_t)(b.key >> 32), target = (uint32_t)b.key;
+    for (int sdir = 0; sdir < 2; sdir++) {
+        const uint32_t a = sdir ? target : pid, c = sdir ? pid : target;
+        if (k + sdir < mt.cap) {
+            mt.keys[2 * (k + sdir)] = ((uint64_t)duration << 32) | (uint32_t)(b.end - end_time_min);
+            mt.keys[2 * (k + sdir) + 1] = ((uint64_t)a << 26) | (uint64_t)c;
+            mt.boxes[k + sdir] = box;
         }
     }
-    if (staged) {
-        __builtin_amdgcn_wave_barrier();
-        emit_staged(st, staged, lane, a.max_meet, a.end_time_min, mt, cnt);
-    }
-    unsigned long long v0 = wave_sum_u64(tl.live), v1 = wave_sum_u64(tl.exp), v2 = wave_sum_u64(tl.ret),
-                       v3 = wave_sum_u64(tl.freed), v4 = wave_sum_u64(tl.tomb);
-    if (lane == 0) {
-        if (v0) atomicAdd(&cnt->live, v0);
-        if (v1) atomicAdd(&cnt->expired, v1);
-        if (v2) atomicAdd(&cnt->retired, v2);
-        if (v3) atomicAdd(&cnt->freed, v3);
-        if (v4) atomicAdd(&cnt->tombs, v4);
-    }
 }
 
-/* Rehash in three passes over the old table, chunked by workgroup:
- * count the kept buckets per chunk, prefix-sum the counts, and compact the
- * kept buckets in order into a staging array; the old table is then freed
- * and the staged buckets are inserted into the new one. */
-#define RH_CHUNK 8192u
-
-__device__ inline bool keep_bucket(const Bucket &b, uint32_t keep_from) {
-    return !key_free(b.key) && !key_tomb(b.key) && b.end != 0xFFFFFFFFu && b.end >= keep_from;
+/* per-object memtable state of the records [from, to) (geometry.cu:705-733):
+ * record counts, objects with records, quantised MBRs, and oversize meetings
+ * (one per record pair) marking both objects */
+__global__ void k_records_account(const uint64_t *__restrict__ keys, const float4 *__restrict__ boxes, uint64_t from,
+                                  uint64_t to, uint32_t max_meet, MemtableDev mt, Counters *cnt) {
+    const uint64_t i = from + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= to) return;
+    const uint64_t lo = keys[2 * i], hi = keys[2 * i + 1];
+    const uint32_t a = (uint32_t)(hi >> 26) & 0x3FFFFFFu, c = (uint32_t)hi & 0x3FFFFFFu;
+    const uint32_t duration = (uint32_t)(lo >> 32);
+    if (((i - from) & 1) == 0 && duration > max_meet) {
+        atomicAdd(&cnt->oversize_oid_count, 1u);
+        mt.sids[a] = 1;
+        mt.sids[c] = 1;
+    }
+    const float4 b = boxes[i];
+    if (atomicAdd(&mt.same_pid_count[a], 1u) == 0) atomicAdd(&cnt->sid_count, 1u);
+    atomicMin(&mt.o_boxs[a].x, f2q(b.x));
+    atomicMax(&mt.o_boxs[a].z, f2q(b.z));
+    atomicMin(&mt.o_boxs[a].y, f2q(b.y));
+    atomicMax(&mt.o_boxs[a].w, f2q(b.w));
 }
 
-__global__ __launch_bounds__(256) void k_stage_count(const Bucket *src, uint64_t nsrc, uint32_t keep_from,
-                                                     uint32_t *chunk_count) {
-    __shared__ uint32_t wsum[4];
-    uint64_t lo = (uint64_t)blockIdx.x * RH_CHUNK, hi = min(lo + RH_CHUNK, nsrc);
-    uint32_t c = 0;
-    for (uint64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) c += keep_bucket(src[i], keep_from);
-    for (int o = STC_WAVE / 2; o > 0; o >>= 1) c += __shfl_down(c, o, STC_WAVE);
-    if ((threadIdx.x & (STC_WAVE - 1)) == 0) wsum[threadIdx.x / STC_WAVE] = c;
+/* ascending bitonic sort of s[0, P) (P a power of two) by a block */
+template <int THREADS>
+__device__ inline void block_bitonic(unsigned long long *s, uint32_t P) {
+    for (uint32_t k = 2; k <= P; k <<= 1)
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = threadIdx.x; i < P; i += THREADS) {
+                const uint32_t ixj = i ^ j;
+                if (ixj > i) {
+                    const unsigned long long x = s[i], y = s[ixj];
+                    if ((x > y) == ((i & k) == 0)) {
+                        s[i] = y;
+                        s[ixj] = x;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+}
+
+/* rank of the set flags across the block, in thread order; *total = all set */
+template <int THREADS>
+__device__ inline uint32_t block_rank(bool f, uint32_t *wsum, uint32_t *total) {
+    const int lane = threadIdx.x & (STC_WAVE - 1), wv = threadIdx.x / STC_WAVE;
+    const uint64_t m = __ballot(f);
+    const uint32_t r = (uint32_t)__popcll(m & lanemask_lt(lane));
+    if (lane == 0) wsum[wv] = (uint32_t)__popcll(m);
     __syncthreads();
-    if (threadIdx.x == 0) chunk_count[blockIdx.x] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    uint32_t off = 0, tot = 0;
+    for (int w = 0; w < THREADS / STC_WAVE; w++) {
+        const uint32_t x = wsum[w];
+        if (w < wv) off += x;
+        tot += x;
+    }
+    __syncthreads();
+    *total = tot;
+    return off + r;
 }
 
-__global__ __launch_bounds__(256) void k_stage_write(const Bucket *src, uint64_t nsrc, uint32_t keep_from,
-                                                     const uint32_t *chunk_offset, Bucket *stage) {
-    __shared__ uint32_t wbase[5];
-    const int lane = threadIdx.x & (STC_WAVE - 1), w = threadIdx.x / STC_WAVE;
-    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (STC_WAVE - lane));
-    uint64_t lo = (uint64_t)blockIdx.x * RH_CHUNK, hi = min(lo + RH_CHUNK, nsrc);
-    uint64_t out = chunk_offset[blockIdx.x];
-    for (uint64_t base = lo; base < hi; base += blockDim.x) {
-        uint64_t i = base + threadIdx.x;
-        Bucket b;
-        bool k = false;
-        if (i < hi) {
-            b = src[i];
-            k = keep_bucket(b, keep_from);
+/* The segments the write pass left unsorted (longer than WCAP): the same
+ * ordering by a block, with the sort keys in LDS (LCAP entries) or, with
+ * GLOBAL, in a per-block slice of gcap entries of global scratch. Longer
+ * segments go to `spill` (or set table_fail when spill is null). */
+template <int THREADS, int LCAP, bool GLOBAL>
+__global__ __launch_bounds__(THREADS) void k_seg_sort(const uint32_t *__restrict__ owners, uint32_t n_owners,
+                                                      const uint32_t *__restrict__ poff, const PairHit *__restrict__ ptmp,
+                                                      PairHit *__restrict__ pn, uint32_t *__restrict__ spill,
+                                                      uint32_t *spill_count, unsigned long long *gsrt, uint32_t gcap,
+                                                      Counters *cnt) {
+    __shared__ unsigned long long lsrt[GLOBAL ? 1 : LCAP];
+    __shared__ uint32_t wsum[THREADS / STC_WAVE];
+    unsigned long long *keys = GLOBAL ? gsrt + (size_t)blockIdx.x * gcap : lsrt;
+    const uint32_t cap = GLOBAL ? gcap : (uint32_t)LCAP;
+    const int tid = threadIdx.x;
+    for (uint32_t oi = blockIdx.x; oi < n_owners; oi += gridDim.x) {
+        const uint32_t a = owners[oi];
+        const uint32_t base = poff[a], slots = poff[a + 1] - base;
+        if (slots > cap) {
+            if (tid == 0) {
+                if (spill) spill[atomicAdd(spill_count, 1u)] = a;
+                else atomicOr(&cnt->table_fail, 1u);
+            }
+            continue;
         }
-        uint64_t m = __ballot(k);
-        if (lane == 0) wbase[w] = (uint32_t)__popcll(m);
+        uint32_t P = 1;
+        while (P < slots) P <<= 1;
+        for (uint32_t i = tid; i < P; i += THREADS)
+            keys[i] = i < slots ? (((unsigned long long)(uint32_t)ptmp[base + i].key << 32) | i) : ~0ull;
         __syncthreads();
-        if (threadIdx.x == 0) {
-            uint32_t acc = 0;
-            for (int q = 0; q < 4; q++) { uint32_t v = wbase[q]; wbase[q] = acc; acc += v; }
-            wbase[4] = acc;
+        block_bitonic<THREADS>(keys, P);
+        uint32_t nlive = 0;
+        for (uint32_t k0 = 0; k0 < slots; k0 += THREADS) {
+            const uint32_t k = k0 + tid;
+            bool live = false;
+            if (k < slots) {
+                const uint32_t b = (uint32_t)(keys[k] >> 32);
+                live = b != DEAD_B && (k == 0 || (uint32_t)(keys[k - 1] >> 32) != b);
+            }
+            uint32_t total;
+            (void)block_rank<THREADS>(live, wsum, &total);
+            nlive += total;
         }
-        __syncthreads();
-        if (k) stage[out + wbase[w] + __popcll(m & lt)] = b;
-        out += wbase[4];
+        uint32_t lr = 0, dr = 0;
+        for (uint32_t k0 = 0; k0 < slots; k0 += THREADS) {
+            const uint32_t k = k0 + tid;
+            bool live = false;
+            if (k < slots) {
+                const uint32_t b = (uint32_t)(keys[k] >> 32);
+                live = b != DEAD_B && (k == 0 || (uint32_t)(keys[k - 1] >> 32) != b);
+            }
+            uint32_t tl, td;
+            const uint32_t rl = block_rank<THREADS>(live, wsum, &tl);
+            const uint32_t rd = block_rank<THREADS>(k < slots && !live, wsum, &td);
+            if (k < slots) {
+                PairHit e = ptmp[base + (uint32_t)keys[k]];
+                uint32_t dest;
+                if (live) {
+                    dest = lr + rl;
+                } else {
+                    dest = nlive + dr + rd;
+                    e.key = ((unsigned long long)a << 32) | DEAD_B;
+                }
+                pn[base + dest] = e;
+            }
+            lr += tl;
+            dr += td;
+        }
         __syncthreads();
     }
 }
 
-__global__ void k_table_unstage(const Bucket *stage, uint64_t n, Bucket *dst, uint64_t ndst, Counters *cnt) {
-    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    Bucket b = stage[i];
-    bool claimed;
-    Bucket *d = table_find_or_claim(dst, ndst, b.key, &claimed);
-    if (!d) { atomicAdd(&cnt->table_fail, 1u); return; }
-    d->start = b.start;
-    d->end = b.end;
-    d->mbr[0] = b.mbr[0];
-    d->mbr[1] = b.mbr[1];
-    d->mbr[2] = b.mbr[2];
-    d->mbr[3] = b.mbr[3];
+#define MTILE 1024u  /* new pairs per merge tile */
+#define OCAP 2048u   /* previous meetings of a tile held in LDS */
+#define MSTAGE 256u  /* expired meetings a merge block stages before reserving their records */
+
+/* write the staged meetings' records: one counter reservation per batch */
+__device__ inline void flush_stage(const Bucket *stage, uint32_t staged, unsigned long long *kb_sh, uint32_t end_time_min,
+                                   const MemtableDev &mt, Counters *cnt) {
+    if (threadIdx.x == 0) *kb_sh = atomicAdd(&cnt->kv_count, 2ull * staged);
+    __syncthreads();
+    const unsigned long long kb = *kb_sh;
+    for (uint32_t e = threadIdx.x; e < staged; e += blockDim.x)
+        write_meeting_records(stage[e], kb + 2ull * e, end_time_min, mt);
+    __syncthreads();
 }
+
+/* Start of each merge tile's range in the previous meeting array: the number
+ * of previous keys below the tile's first new key (found in that key's owner
+ * segment). tb[n_tiles] = n_o. */
+__global__ void k_tile_bounds(const PairHit *__restrict__ pn, uint64_t n_p, const Bucket *__restrict__ mold,
+                              const uint32_t *__restrict__ ooff, uint64_t n_o, uint32_t n_tiles,
+                              uint32_t *__restrict__ tb) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t > n_tiles) return;
+    if (t == 0 || t == n_tiles || (uint64_t)t * MTILE >= n_p) {
+        tb[t] = t == 0 ? 0u : (uint32_t)n_o;
+        return;
+    }
+    const unsigned long long key = pn[(uint64_t)t * MTILE].key;
+    const uint32_t a = (uint32_t)(key >> 32);
+    uint32_t lo = ooff[a], hi = ooff[a + 1];
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (mold[mid].key < key) lo = mid + 1;
+        else hi = mid;
+    }
+    tb[t] = lo;
+}
+
+__device__ inline uint32_t lower_bound_u64(const unsigned long long *s, uint32_t n, unsigned long long v) {
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (s[mid] < v) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__device__ inline uint32_t lower_bound_bucket(const Bucket *s, uint32_t n, unsigned long long v) {
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (s[mid].key < v) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+/* Merge of the step's pairs (pn, sorted by key, dead slots behind each owner's
+ * live pairs) with the previous step's meetings (mold, sorted the same way),
+ * tile by tile: the tile's new keys and its range of previous keys (tb) are
+ * staged in LDS. A live pair continues the previous meeting of its key (start
+ * kept, MBR widened) or opens one (insert/extend, geometry.cu:469-494); a dead
+ * slot becomes a tombstone; a previous meeting whose key has no pair expires
+ * and, when it lasted long enough inside the map, becomes two records
+ * (identify, geometry.cu:595-737). mnew[i] is built from pn[i]. */
+__global__ __launch_bounds__(256) void k_meet_merge(const PairHit *__restrict__ pn, uint64_t n_p,
+                                                    const Bucket *__restrict__ mold, const uint32_t *__restrict__ tb,
+                                                    uint32_t n_tiles, Bucket *__restrict__ mnew, MergeArgs ma,
+                                                    MemtableDev mt, Counters *cnt) {
+    __shared__ unsigned long long pk[MTILE];
+    __shared__ unsigned long long ok[OCAP];
+    __shared__ Bucket stage[MSTAGE];
+    __shared__ uint32_t wsum[256 / STC_WAVE];
+    __shared__ unsigned long long kb_sh;
+    const int tid = threadIdx.x, lane = tid & (STC_WAVE - 1);
+    uint32_t staged = 0;
+    unsigned long long n_live = 0, n_exp = 0, n_fresh = 0, n_cont = 0;
+    for (uint32_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+        const uint64_t i0 = (uint64_t)t * MTILE;
+        const uint32_t np = i0 < n_p ? (uint32_t)min<uint64_t>(MTILE, n_p - i0) : 0u;
+        const uint32_t o0 = tb[t], no = tb[t + 1] - o0;
+        const bool oin = no <= OCAP;
+        for (uint32_t i = tid; i < np; i += 256) pk[i] = pn[i0 + i].key;
+        if (oin)
+            for (uint32_t j = tid; j < no; j += 256) ok[j] = mold[o0 + j].key;
+        __syncthreads();
+        for (uint32_t i = tid; i < np; i += 256) {
+            const unsigned long long key = pk[i];
+            Bucket nb;
+            nb.key = key;
+            if ((uint32_t)key == DEAD_B) {
+                nb.start = 0;
+                nb.end = 0xFFFFFFFFu;
+                nb.mbr[0] = nb.mbr[1] = nb.mbr[2] = nb.mbr[3] = 0;
+            } else {
+                const PairHit e = pn[i0 + i];
+                const uint32_t q0 = f2o(e.mbr[0]), q1 = f2o(e.mbr[1]), q2 = f2o(e.mbr[2]), q3 = f2o(e.mbr[3]);
+                nb.end = ma.now;
+                const uint32_t j = oin ? lower_bound_u64(ok, no, key) : lower_bound_bucket(mold + o0, no, key);
+                if (j < no && (oin ? ok[j] : mold[o0 + j].key) == key) {
+                    const Bucket ob = mold[o0 + j];
+                    nb.start = ob.start;
+                    nb.mbr[0] = min(ob.mbr[0], q0);
+                    nb.mbr[1] = min(ob.mbr[1], q1);
+                    nb.mbr[2] = max(ob.mbr[2], q2);
+                    nb.mbr[3] = max(ob.mbr[3], q3);
+                    n_cont++;
+                } else {
+                    nb.start = ma.now;
+                    nb.mbr[0] = q0; nb.mbr[1] = q1; nb.mbr[2] = q2; nb.mbr[3] = q3;
+                    n_fresh++;
+                }
+                n_live++;
+            }
+            mnew[i0 + i] = nb;
+        }
+        for (uint32_t j0 = 0; j0 < no; j0 += 256) {
+            const uint32_t j = j0 + tid;
+            bool emit = false;
+            Bucket ob;
+            if (j < no) {
+                const unsigned long long key = oin ? ok[j] : mold[o0 + j].key;
+                if ((uint32_t)key != DEAD_B) {
+                    const uint32_t k = lower_bound_u64(pk, np, key);
+                    if (!(k < np && pk[k] == key)) {
+                        n_exp++;
+                        ob = mold[o0 + j];
+                        emit = meeting_emits(ob, ma);
+                    }
+                }
+            }
+            uint32_t total;
+            const uint32_t pos = block_rank<256>(emit, wsum, &total);
+            if (total) {
+                if (staged + total > MSTAGE) {
+                    flush_stage(stage, staged, &kb_sh, ma.end_time_min, mt, cnt);
+                    staged = 0;
+                }
+                if (emit) stage[staged + pos] = ob;
+                staged += total;
+                __syncthreads();
+            }
+        }
+        __syncthreads();
+    }
+    if (staged) flush_stage(stage, staged, &kb_sh, ma.end_time_min, mt, cnt);
+    n_live = wave_sum_u64(n_live);
+    n_exp = wave_sum_u64(n_exp);
+    n_fresh = wave_sum_u64(n_fresh);
+    n_cont = wave_sum_u64(n_cont);
+    if (lane == 0) {
+        if (n_live) atomicAdd(&cnt->live, n_live);
+        if (n_exp) atomicAdd(&cnt->expired, n_exp);
+        if (n_fresh) atomicAdd(&cnt->new_keys, n_fresh);
+        if (n_cont) atomicAdd(&cnt->continuing, n_cont);
+    }
+}
+
+#define SEG_CAP 4096u  /* LDS sort keys of the four-wave segment sort */
 
 __global__ void k_fill_u32(uint32_t *a, uint64_t n, uint32_t v) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1168,13 +1345,28 @@ struct StcCtx {
     DBuf<FrontierItem> frontier_a, frontier_b;
     DBuf<uint32_t> leaf_count, leaf_start, leaf_node;
     DBuf<uint32_t> pt_leaf, pt_lv, pt_rank, lp, lkeys;
+    DBuf<uint32_t> lleaf;                /* leaf of each point, in leaf order */
+    DBuf<uint2> lunits;                  /* first cross unit and count of each point, in leaf order */
     DBuf<uint32_t> ucount, uoffset;
     DBuf<uint2> units;
-    DBuf<PairHit> pairs;
-    DBuf<uint2> amb;
-    Bucket *table = nullptr;
-    uint64_t table_buckets = 0;
-    int log2b = 0;
+    DBuf<uint2> amb;                     /* band combinations (probe, other) for the host */
+    /* per leaf, by pid: the border points with a cross unit to it */
+    DBuf<uint32_t> rev_leaf, rev_pid, rev_leaf_alt, rev_pid_alt, rev_count, rev_start;
+    DBuf<double2> rev_xy;
+    DBuf<double> rev_dl, rev_il;
+    /* the step's pairs by owner segment, and the live meetings: two arrays of
+     * segments (offsets [n + 1], live buckets per segment [n]) used in turn */
+    DBuf<uint32_t> pcount;
+    DBuf<PairHit> ptmp, pnew;            /* the pair slots as found, and sorted by key */
+    DBuf<Bucket> meet[2];
+    DBuf<uint32_t> moff[2];
+    DBuf<uint32_t> tbounds;              /* merge tiles' ranges in the previous meetings */
+    int cur = 0;                         /* meet[cur] holds the meetings of the last step */
+    uint64_t mlen = 0;                   /* buckets of meet[cur] (live and tombstones) */
+    uint64_t plen = 0;                   /* entries of pnew (the last step's pair slots) */
+    DBuf<unsigned long long> acc;        /* band pairs the host accepted, sorted */
+    DBuf<uint32_t> spill_big, spill_huge;
+    DBuf<unsigned long long> gsrt;       /* global sort keys of very large segments */
     /* memtable */
     uint64_t *rec_keys = nullptr;
     float4 *rec_boxes = nullptr;
@@ -1189,20 +1381,13 @@ struct StcCtx {
     uint32_t end_time_min = 0;
     uint64_t kv_count = 0;
     uint64_t last_live = 0;
-    uint64_t occupied = 0;
-    uint64_t last_new = 0;
-    uint64_t last_listed = 0;
+    uint64_t last_expired = 0;
     uint32_t last_t = 0;
     bool stepped = false;
     uint32_t last_nodes = 0, last_leaves = 0;
-    uint64_t last_pairs = 0;
-    std::vector<PairHit> host_pairs_extra;
-    DBuf<PairHit> amb_pairs;
     DBuf<stc_meeting_unit> units_in;  /* replayed meeting units and their pair keys */
     DBuf<uint64_t> ukeys, ukeys_alt;
     DBuf<uint32_t> uidx, uidx_alt, utemp;
-    DBuf<uint32_t> chunk_count, chunk_offset;
-    DBuf<Bucket> stage;
     std::vector<RunHost> runs;
     PinnedPool pinned;
     QueryBufs *qb = nullptr;
@@ -1236,18 +1421,6 @@ static int sync_counters(StcCtx *c) {
     return 0;
 }
 
-/* the identify pass works on 64-bucket chunks: table sizes are multiples of 64 */
-static uint64_t round_buckets(uint64_t nb) { return (nb + 63) & ~63ull; }
-
-static int alloc_table(StcCtx *c, uint64_t nb, Bucket **out) {
-    Bucket *t = nullptr;
-    CK(hipMalloc((void **)&t, nb * sizeof(Bucket)));
-    hipLaunchKernelGGL(k_table_clear, dim3(blocks_for(nb, 256)), dim3(256), 0, c->s, t, nb);
-    CK(hipGetLastError());
-    *out = t;
-    return 0;
-}
-
 static int ensure_records(StcCtx *c, uint64_t need) {
     if (need <= c->rec_cap) return 0;
     uint64_t nc = std::max<uint64_t>(need + need / 2, 1 << 16);
@@ -1274,12 +1447,16 @@ static int ilog2_ceil(uint64_t v) {
     return l;
 }
 
-/* Move the entries last seen at or after keep_from into a table sized for
- * expected_keys at a load of at most 1/3 (see the kernels above): fewer
- * probes per insert make the refine pass faster than the larger table makes
- * the identify scan slower. The kept entries go to a staging buffer that
- * stays allocated between rehashes; the table is refilled in place unless it
- * must grow. */
+/* the last step's pairs (its pair slots without the dead ones), sorted by key */
+static int step_pairs(StcCtx *c, std::vector<PairHit> &ph) {
+    ph.resize(c->plen);
+    if (c->plen) CK(hipMemcpy(ph.data(), c->pnew.p, c->plen * sizeof(PairHit), hipMemcpyDeviceToHost));
+    ph.erase(std::remove_if(ph.begin(), ph.end(), [](const PairHit &e) { return (uint32_t)e.key == DEAD_B; }),
+             ph.end());
+    std::sort(ph.begin(), ph.end(), [](const PairHit &a, const PairHit &b) { return a.key < b.key; });
+    return 0;
+}
+
 /* Copy a just-flushed run out of the flush buffers into its own HBM block
  * (packed keys, CTF metadata and bases, sids, oversize buffer) for queries. */
 static int keep_run_on_device(StcCtx *c, RunHost &run) {
@@ -1318,50 +1495,6 @@ static int keep_run_on_device(StcCtx *c, RunHost &run) {
     q.source = 0; /* set per query from the run's position */
     CK(hipStreamSynchronize(s)); /* the host metadata vector is read by the copy above */
     return stc_query_ctf_stats(q, run.stats, s) ? fail(c, STC_E_HIP, std::string("run statistics: ") + g_last_fail) : 0;
-}
-
-static int table_rehash(StcCtx *c, uint32_t keep_from, uint64_t expected_keys) {
-    hipStream_t s = c->s;
-    uint64_t nold = c->table_buckets;
-    uint32_t chunks = (uint32_t)((nold + RH_CHUNK - 1) / RH_CHUNK);
-    if (c->chunk_count.ensure(chunks + 1) || c->chunk_offset.ensure(chunks + 1))
-        return fail(c, STC_E_CAPACITY, "rehash chunk counts");
-    hipLaunchKernelGGL(k_stage_count, dim3(chunks), dim3(256), 0, s, c->table, nold, keep_from, c->chunk_count.p);
-    CK(hipGetLastError());
-    CKRC(stc_exclusive_scan_u32(c->chunk_count.p, c->chunk_offset.p, chunks, c->chunk_offset.p + chunks, c->temp.p,
-                                s));
-    uint32_t kept = 0;
-    CK(hipMemcpyAsync(&kept, c->chunk_offset.p + chunks, 4, hipMemcpyDeviceToHost, s));
-    CK(hipStreamSynchronize(s));
-    if (kept) {
-        if (c->stage.ensure(kept)) return fail(c, STC_E_CAPACITY, "rehash staging buffer");
-        hipLaunchKernelGGL(k_stage_write, dim3(chunks), dim3(256), 0, s, c->table, nold, keep_from,
-                           c->chunk_offset.p, c->stage.p);
-        CK(hipGetLastError());
-    }
-    uint64_t want = round_buckets(std::max<uint64_t>(expected_keys, kept) * 3 + 1024);
-    uint64_t nb = nold;
-    if (want > nold) {
-        CK(hipStreamSynchronize(s));
-        CK(hipFree(c->table));
-        c->table = nullptr;
-        Bucket *nt = nullptr;
-        CKRC(alloc_table(c, want, &nt));
-        c->table = nt;
-        nb = want;
-    } else {
-        hipLaunchKernelGGL(k_table_clear, dim3(blocks_for(nb, 256)), dim3(256), 0, s, c->table, nb);
-        CK(hipGetLastError());
-    }
-    c->table_buckets = nb;
-    if (kept) {
-        hipLaunchKernelGGL(k_table_unstage, dim3(blocks_for(kept, 256)), dim3(256), 0, s, c->stage.p,
-                           (uint64_t)kept, c->table, nb, c->dcnt);
-        CK(hipGetLastError());
-    }
-    CK(hipStreamSynchronize(s));
-    c->occupied = kept;
-    return 0;
 }
 
 extern "C" {
@@ -1429,8 +1562,11 @@ int stc_create(const stc_config *cfg, const stc_box *mbr, void **out) {
         c->idx_alt.ensure(n) || c->dlon.ensure(n) || c->idlon.ensure(n) || c->lx.ensure(n) || c->ly.ensure(n) || c->ld.ensure(n) ||
         c->nodes.ensure(node_cap) || c->frontier_a.ensure(front_cap) || c->frontier_b.ensure(front_cap) ||
         c->pt_leaf.ensure(n) || c->pt_lv.ensure(n) || c->pt_rank.ensure(n) || c->lp.ensure(n) ||
+        c->lleaf.ensure(n) || c->lunits.ensure(n) ||
         c->lkeys.ensure(n) || c->ucount.ensure(n) || c->uoffset.ensure(n) || c->units.ensure(4ull * n) ||
-        c->pairs.ensure(std::max<size_t>(4ull * n, 1 << 16)) || c->amb.ensure(1 << 16) ||
+        c->amb.ensure(1 << 16) || c->pcount.ensure(n) || c->moff[0].ensure(n + 1ull) || c->moff[1].ensure(n + 1ull) ||
+        c->spill_big.ensure(n) || c->spill_huge.ensure(n) ||
+        c->rev_count.ensure(node_cap + 1) || c->rev_start.ensure(node_cap + 1) ||
         c->leaf_count.ensure(node_cap) || c->leaf_start.ensure(node_cap) || c->leaf_node.ensure(node_cap) ||
         c->temp.ensure(stc_radix_temp_words((uint32_t)std::max<size_t>(n, node_cap)) + 64))
         return bad("device allocation failed");
@@ -1438,9 +1574,11 @@ int stc_create(const stc_config *cfg, const stc_box *mbr, void **out) {
     if (hipHostMalloc((void **)&c->hcnt, sizeof(Counters), hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void **)&c->hinit, sizeof(Counters), hipHostMallocDefault) != hipSuccess)
         return bad("pinned");
-    uint64_t nb = cfg->num_meeting_buckets ? cfg->num_meeting_buckets : 4ull * n;
-    c->table_buckets = round_buckets(std::max<uint64_t>(nb, 1024));
-    if (alloc_table(c, c->table_buckets, &c->table)) return bad("table");
+    /* no meetings yet: every segment of the current array is empty */
+    uint64_t nb = cfg->num_meeting_buckets ? cfg->num_meeting_buckets : 2ull * n;
+    if (c->ptmp.ensure(nb) || c->pnew.ensure(nb) || c->meet[0].ensure(nb) || c->meet[1].ensure(nb))
+        return bad("meeting arrays");
+    if (hipMemsetAsync(c->moff[0].p, 0, 4ull * (n + 1), c->s) != hipSuccess) return bad("meeting offsets");
     if (hipMalloc((void **)&c->same_pid_count, 4ull * n) != hipSuccess ||
         hipMalloc((void **)&c->o_boxs, 16ull * n) != hipSuccess || hipMalloc((void **)&c->sids, 2ull * n) != hipSuccess)
         return bad("per-object state");
@@ -1462,15 +1600,18 @@ int stc_destroy(void *vc) {
     c->temp.release(); c->dlon.release(); c->idlon.release(); c->lx.release(); c->ly.release(); c->ld.release(); c->nodes.release();
     c->frontier_a.release(); c->frontier_b.release(); c->leaf_count.release(); c->leaf_start.release();
     c->leaf_node.release(); c->pt_leaf.release(); c->pt_lv.release(); c->pt_rank.release(); c->lp.release();
-    c->lkeys.release(); c->ucount.release(); c->uoffset.release(); c->units.release(); c->pairs.release();
+    c->lleaf.release(); c->lunits.release();
+    c->lkeys.release(); c->ucount.release(); c->uoffset.release(); c->units.release();
     c->amb.release();
-    c->amb_pairs.release();
+    c->rev_leaf.release(); c->rev_pid.release(); c->rev_leaf_alt.release(); c->rev_pid_alt.release();
+    c->rev_count.release(); c->rev_start.release();
+    c->rev_xy.release(); c->rev_dl.release(); c->rev_il.release();
+    c->pcount.release(); c->pnew.release(); c->acc.release();
+    for (int i = 0; i < 2; i++) { c->meet[i].release(); c->moff[i].release(); }
+    c->ptmp.release(); c->tbounds.release();
+    c->spill_big.release(); c->spill_huge.release(); c->gsrt.release();
     c->units_in.release(); c->ukeys.release(); c->ukeys_alt.release();
     c->uidx.release(); c->uidx_alt.release(); c->utemp.release();
-    c->chunk_count.release();
-    c->chunk_offset.release();
-    c->stage.release();
-    if (c->table) hipFree(c->table);
     if (c->rec_keys) hipFree(c->rec_keys);
     if (c->rec_boxes) hipFree(c->rec_boxes);
     if (c->same_pid_count) hipFree(c->same_pid_count);
@@ -1569,7 +1710,7 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
     CKRC(stc_radix_sort_u32(c->lkeys.p, c->lp.p, c->codes_alt.p, c->idx_alt.p, n,
                             std::max(1, ilog2_ceil(n_leaves)), c->temp.p, s));
     hipLaunchKernelGGL(k_gather, dim3(blocks_for(n, 256)), dim3(256), 0, s, xy, c->dlon.p, c->lp.p, c->pt_leaf.p,
-                       c->leaf_start.p, n, c->lx.p, c->ly.p, c->ld.p, c->pt_rank.p);
+                       c->leaf_start.p, n, c->lx.p, c->ly.p, c->ld.p, c->pt_rank.p, c->lleaf.p);
     hipLaunchKernelGGL(k_overflow, dim3(blocks_for(n_leaves, 256)), dim3(256), 0, s, c->leaf_count.p, n_leaves,
                        c->buf, cnt);
     CK(hipGetLastError());
@@ -1591,90 +1732,115 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
     CK(hipGetLastError());
     CK(hipEventRecord(c->ev[2], s));
 
-    /* 7-8: refinement fused with the meeting-table insert. First make room:
-     * after this step's new keys at most half of the buckets hold keys. */
-    uint32_t rebuilt = 0;
-    uint64_t est_new = c->stepped ? c->last_new / 2 + 1024 : (uint64_t)n;
-    if ((c->occupied + est_new) * 10 > c->table_buckets * 6) {
-        CKRC(table_rehash(c, t - 1, c->last_live + est_new));
-        rebuilt = 1;
-    }
-    const bool keep = (c->cfg.keep_pairs & 1) != 0;
-    uint64_t total_units = (uint64_t)n + n_units;
-    uint32_t grid = (uint32_t)std::min<uint64_t>(blocks_for(total_units * STC_WAVE, 256), 256u * 64u);
-    for (int attempt = 0;; attempt++) {
-        CK(hipMemsetAsync(&cnt->candidates, 0, 6 * sizeof(unsigned long long), s)); /* candidates .. continuing */
-        CK(hipMemsetAsync(&cnt->table_fail, 0, sizeof(uint32_t), s));
-        CK(hipEventRecord(c->ev[6], s));
-        hipLaunchKernelGGL(k_refine, dim3(grid), dim3(256), 0, s, xy, c->dlon.p, c->idlon.p, n, c->pt_leaf.p,
-                           c->pt_rank.p, c->leaf_start.p, c->leaf_count.p, c->lp.p, c->lx.p, c->ly.p, c->units.p,
-                           total_units, c->buf, r, c->band, r * r * (1.0 - 1e-12), r * r * (1.0 + 1e-12), c->table,
-                           c->table_buckets, t, keep ? c->pairs.p : (PairHit *)nullptr,
-                           (uint64_t)c->pairs.cap, c->amb.p, (uint32_t)c->amb.cap, cnt);
+    /* 7: reverse cross units: per leaf, the border points with a cross unit to it, by pid */
+    if (c->rev_leaf.ensure(n_units + 1) || c->rev_pid.ensure(n_units + 1) || c->rev_leaf_alt.ensure(n_units + 1) ||
+        c->rev_xy.ensure(n_units + 1) || c->rev_dl.ensure(n_units + 1) || c->rev_il.ensure(n_units + 1) ||
+        c->rev_pid_alt.ensure(n_units + 1) || c->rev_count.ensure(n_leaves + 1ull) ||
+        c->rev_start.ensure(n_leaves + 1ull) ||
+        c->temp.ensure(stc_radix_temp_words((uint32_t)std::max<uint64_t>(std::max<uint64_t>(n_units, n),
+                                                                          c->nodes.cap)) + 64))
+        return fail(c, STC_E_CAPACITY, "reverse unit buffers");
+    CK(hipMemsetAsync(c->rev_count.p, 0, 4ull * n_leaves, s));
+    if (n_units) {
+        hipLaunchKernelGGL(k_units_split, dim3(blocks_for(n_units, 256)), dim3(256), 0, s, c->units.p,
+                           (uint32_t)n_units, c->rev_leaf.p, c->rev_pid.p, c->rev_count.p);
         CK(hipGetLastError());
-        CK(hipEventRecord(c->ev[7], s));
+        CKRC(stc_radix_sort_u32(c->rev_leaf.p, c->rev_pid.p, c->rev_leaf_alt.p, c->rev_pid_alt.p, (uint32_t)n_units,
+                                std::max(1, ilog2_ceil(n_leaves)), c->temp.p, s));
+        hipLaunchKernelGGL(k_rev_gather, dim3(blocks_for(n_units, 256)), dim3(256), 0, s, (const uint32_t *)c->rev_pid.p,
+                           (uint32_t)n_units, xy, (const double *)c->dlon.p, (const double *)c->idlon.p, c->rev_xy.p,
+                           c->rev_dl.p, c->rev_il.p);
+        CK(hipGetLastError());
+    }
+    CKRC(stc_exclusive_scan_u32(c->rev_count.p, c->rev_start.p, n_leaves, c->rev_start.p + n_leaves, c->temp.p, s));
+
+    /* 8: the pairs owner by owner: count (a band combination reserves a slot),
+     * the band decided on the host, segment offsets, write */
+    hipLaunchKernelGGL(k_units_lorder, dim3(blocks_for(n, 256)), dim3(256), 0, s, (const uint32_t *)c->lp.p,
+                       (const uint32_t *)c->uoffset.p, (const uint32_t *)c->ucount.p, n, c->lunits.p);
+    CK(hipGetLastError());
+    const PairGeo pg{xy, c->dlon.p, c->idlon.p, c->lx.p, c->ly.p, c->pt_leaf.p, c->pt_rank.p, c->leaf_start.p,
+                     c->leaf_count.p, c->lp.p, c->ld.p, c->lleaf.p, c->lunits.p, c->units.p, c->uoffset.p,
+                     c->ucount.p, c->rev_start.p, c->rev_pid.p,
+                     c->rev_xy.p, c->rev_dl.p, c->rev_il.p, n, c->buf, r, c->band, r * r * (1.0 - 1e-12),
+                     r * r * (1.0 + 1e-12)};
+    const uint32_t pgrid = (uint32_t)std::min<uint64_t>(blocks_for((uint64_t)n * STC_WAVE, 256), 256u * 64u);
+    CK(hipEventRecord(c->ev[6], s));
+    for (int attempt = 0;; attempt++) {
+        CK(hipMemsetAsync(&cnt->candidates, 0, 3 * sizeof(unsigned long long), s)); /* candidates, pairs, ambiguous */
+        hipLaunchKernelGGL(k_pair_count, dim3(pgrid), dim3(256), 0, s, pg, c->pcount.p, c->amb.p,
+                           (uint32_t)std::min<size_t>(c->amb.cap, 0xFFFFFFFFu), cnt);
+        CK(hipGetLastError());
         CKRC(sync_counters(c));
         h = *c->hcnt;
-        bool redo = false;
-        if (h.table_fail) {
-            /* grow, keeping the keys seen last step and this step; reinserting is idempotent */
-            CKRC(table_rehash(c, t - 1, c->last_live + h.pairs + 1024));
-            rebuilt = 1;
-            redo = true;
-            CK(hipMemsetAsync(&cnt->table_fail, 0, sizeof(uint32_t), s));
-        }
-        if (keep && h.pairs_listed > c->pairs.cap) {
-            if (c->pairs.ensure(h.pairs_listed + 1024)) return fail(c, STC_E_CAPACITY, "pair list allocation failed");
-            redo = true;
-        }
-        if (h.ambiguous > c->amb.cap) {
-            if (c->amb.ensure(h.ambiguous + 1024)) return fail(c, STC_E_CAPACITY, "ambiguous list allocation failed");
-            redo = true;
-        }
-        if (!redo) break;
-        if (attempt >= 8) return fail(c, STC_E_CAPACITY, "meeting table cannot hold the step's pairs");
+        if (h.ambiguous <= c->amb.cap) break;
+        if (attempt >= 2 || c->amb.ensure(h.ambiguous + 1024))
+            return fail(c, STC_E_CAPACITY, "ambiguous list allocation failed");
     }
-    uint64_t n_pairs = h.pairs;
-    uint64_t n_amb = h.ambiguous;
-    c->last_new = h.new_keys;
-    c->occupied += h.new_keys;
-    c->last_listed = keep ? h.pairs_listed : 0;
-    c->host_pairs_extra.clear();
+    const uint64_t total = h.pairs, n_amb = h.ambiguous;
+    if (total >= 0xFFFFFFFFull) return fail(c, STC_E_CAPACITY, "more than 2^32 - 2 pairs in one step");
+    std::vector<unsigned long long> accepted;
     if (n_amb) {
-        /* decide the band with the C library's sin and insert the accepted pairs */
+        /* decide the band with the C library's sin (geometry.h:40-48) */
         std::vector<uint2> ab(n_amb);
         CK(hipMemcpy(ab.data(), c->amb.p, n_amb * sizeof(uint2), hipMemcpyDeviceToHost));
-        std::vector<PairHit> extra;
         for (const uint2 &e : ab) {
             double2 pp, qq;
             CK(hipMemcpy(&pp, xy + e.x, sizeof(double2), hipMemcpyDeviceToHost));
             CK(hipMemcpy(&qq, xy + e.y, sizeof(double2), hipMemcpyDeviceToHost));
             double dx = (pp.x - qq.x) / host_dpm_lon(pp.y);
             double dy = (pp.y - qq.y) / STC_DPM_LAT;
-            if (std::sqrt(dx * dx + dy * dy) <= r) {
-                PairHit ph;
-                uint32_t a = std::min(e.x, e.y), b = std::max(e.x, e.y);
-                ph.key = ((unsigned long long)a << 32) | b;
-                float pfx = (float)pp.x, pfy = (float)pp.y, qfx = (float)qq.x, qfy = (float)qq.y;
-                ph.mbr[0] = std::min(pfx, qfx); ph.mbr[1] = std::min(pfy, qfy);
-                ph.mbr[2] = std::max(pfx, qfx); ph.mbr[3] = std::max(pfy, qfy);
-                extra.push_back(ph);
-            }
+            if (std::sqrt(dx * dx + dy * dy) <= r)
+                accepted.push_back(((unsigned long long)std::min(e.x, e.y) << 32) | std::max(e.x, e.y));
         }
-        if (!extra.empty()) {
-            if (c->amb_pairs.ensure(extra.size())) return fail(c, STC_E_CAPACITY, "ambiguous pair buffer");
-            CK(hipMemcpy(c->amb_pairs.p, extra.data(), extra.size() * sizeof(PairHit), hipMemcpyHostToDevice));
-            hipLaunchKernelGGL(k_table_insert, dim3(blocks_for(extra.size(), 256)), dim3(256), 0, s, c->table,
-                               c->table_buckets, c->amb_pairs.p, (uint64_t)extra.size(), t, cnt);
-            CK(hipGetLastError());
-            n_pairs += extra.size();
-            c->occupied += extra.size();
-            c->host_pairs_extra = extra;
-        }
+        std::sort(accepted.begin(), accepted.end());
+        accepted.erase(std::unique(accepted.begin(), accepted.end()), accepted.end());
     }
-    c->last_pairs = n_pairs;
+    if (c->acc.ensure(accepted.size() + 1)) return fail(c, STC_E_CAPACITY, "band pair list");
+    if (!accepted.empty())
+        CK(hipMemcpy(c->acc.p, accepted.data(), accepted.size() * sizeof(unsigned long long), hipMemcpyHostToDevice));
+    const int nx = c->cur ^ 1;
+    CKRC(stc_exclusive_scan_u32(c->pcount.p, c->moff[nx].p, n, c->moff[nx].p + n, c->temp.p, s));
+    if (c->ptmp.ensure(total + 1) || c->pnew.ensure(total + 1) || c->meet[nx].ensure(total + 1))
+        return fail(c, STC_E_CAPACITY, "pair and meeting arrays");
+    CK(hipMemsetAsync(&cnt->table_fail, 0, sizeof(uint32_t), s));
+    CK(hipMemsetAsync(&cnt->big, 0, 2 * sizeof(uint32_t), s)); /* big, huge */
+    hipLaunchKernelGGL(k_pair_write, dim3(pgrid), dim3(256), 0, s, pg, (const uint32_t *)c->moff[nx].p,
+                       (const uint32_t *)c->pcount.p, c->acc.p, (uint32_t)accepted.size(), c->ptmp.p, c->pnew.p,
+                       c->spill_big.p, cnt);
+    CK(hipGetLastError());
+    CK(hipEventRecord(c->ev[7], s));
+    CKRC(sync_counters(c));
+    h = *c->hcnt;
+    if (h.big) { /* segments longer than the write pass sorts: by blocks, in LDS or in global scratch */
+        hipLaunchKernelGGL((k_seg_sort<256, SEG_CAP, false>), dim3(std::min<uint32_t>(h.big, 4096)), dim3(256), 0, s,
+                           (const uint32_t *)c->spill_big.p, h.big, (const uint32_t *)c->moff[nx].p,
+                           (const PairHit *)c->ptmp.p, c->pnew.p, c->spill_huge.p, &cnt->huge,
+                           (unsigned long long *)nullptr, 0u, cnt);
+        CK(hipGetLastError());
+        CKRC(sync_counters(c));
+        h = *c->hcnt;
+    }
+    if (h.huge) {
+        std::vector<uint32_t> owners(h.huge), seg(2);
+        CK(hipMemcpy(owners.data(), c->spill_huge.p, 4ull * h.huge, hipMemcpyDeviceToHost));
+        uint32_t longest = 1;
+        for (uint32_t o : owners) {
+            CK(hipMemcpy(seg.data(), c->moff[nx].p + o, 8, hipMemcpyDeviceToHost));
+            longest = std::max(longest, seg[1] - seg[0]);
+        }
+        uint32_t gcap = 1;
+        while (gcap < longest) gcap <<= 1;
+        const uint32_t hgrid = std::min<uint32_t>(h.huge, 64);
+        if (c->gsrt.ensure((size_t)gcap * hgrid)) return fail(c, STC_E_CAPACITY, "sort scratch of a large segment");
+        hipLaunchKernelGGL((k_seg_sort<256, 1, true>), dim3(hgrid), dim3(256), 0, s, (const uint32_t *)c->spill_huge.p,
+                           h.huge, (const uint32_t *)c->moff[nx].p, (const PairHit *)c->ptmp.p, c->pnew.p,
+                           (uint32_t *)nullptr, (uint32_t *)nullptr, c->gsrt.p, gcap, cnt);
+        CK(hipGetLastError());
+    }
+    c->plen = total;
     CK(hipEventRecord(c->ev[3], s));
-    StepGeom g{n_nodes, n_leaves, n_units, n_pairs, n_amb, rebuilt};
+    StepGeom g{n_nodes, n_leaves, n_units, 0, n_amb, 0};
     return step_tail(c, t, g, out);
 }
 
@@ -1684,37 +1850,63 @@ static int step_tail(StcCtx *c, uint32_t t, StepGeom &g, stc_step_out *out) {
     const uint32_t n = c->n;
     hipStream_t s = c->s;
     Counters *cnt = c->dcnt;
-    uint32_t rebuilt = g.rebuilt;
-    /* 9: identify: meetings live at t - 1 and not continued now expire */
+    /* 9: merge the step's pairs with the previous step's meetings; meetings
+     * without a pair expire and the long ones become records */
+    const int cu = c->cur, nx = cu ^ 1;
+    const uint64_t kv_before = c->kv_count;
+    if (ensure_records(c, kv_before + 2 * std::max<uint64_t>(c->last_expired + c->last_expired / 4, 1 << 16) + 2))
+        return fail(c, STC_E_CAPACITY, "record buffer allocation failed");
+    const MergeArgs ma{t, c->cfg.min_meet_time, c->cfg.max_meet_time, c->end_time_min, c->map};
     CKRC(sync_counters(c));
     Counters h = *c->hcnt;
-    uint64_t h_cont = c->hcnt->continuing;
-    uint64_t expiring = c->last_live > h_cont ? c->last_live - h_cont : 0;
-    uint64_t need_rec = c->kv_count + 2 * expiring + 2;
-    if (ensure_records(c, need_rec)) return fail(c, STC_E_CAPACITY, "record buffer allocation failed");
-    MemtableDev mt{c->rec_keys, c->rec_boxes, c->rec_cap, c->same_pid_count, c->o_boxs, c->sids};
-    /* one wavefront per range of at least 16 chunks */
-    uint32_t id_grid = (uint32_t)std::max<uint64_t>(std::min<uint64_t>(c->table_buckets / (64 * 16 * 4), 4096), 1);
-    IdentifyArgs ia{t, c->cfg.min_meet_time, c->cfg.max_meet_time, c->end_time_min, c->map};
-    CK(hipEventRecord(c->ev[8], s));
-    hipLaunchKernelGGL(k_identify, dim3(id_grid), dim3(256), 0, s, c->table, c->table_buckets, ia, mt, cnt);
-    CK(hipEventRecord(c->ev[9], s));
+    if (h.table_fail) return fail(c, STC_E_HIP, "the pair write pass found more pairs than its count");
+    const Counters before = h;
+    /* tiles of the step's pairs and their ranges in the previous meetings */
+    const uint32_t n_tiles = (uint32_t)std::max<uint64_t>((c->plen + MTILE - 1) / MTILE, 1);
+    if (c->tbounds.ensure(n_tiles + 1ull)) return fail(c, STC_E_CAPACITY, "merge tile bounds");
+    hipLaunchKernelGGL(k_tile_bounds, dim3(blocks_for(n_tiles + 1ull, 256)), dim3(256), 0, s,
+                       (const PairHit *)c->pnew.p, c->plen, (const Bucket *)c->meet[cu].p,
+                       (const uint32_t *)c->moff[cu].p, c->mlen, n_tiles, c->tbounds.p);
     CK(hipGetLastError());
-    CKRC(sync_counters(c));
-    h = *c->hcnt;
-    if (h.table_fail) return fail(c, STC_E_CAPACITY, "meeting table full");
+    const uint32_t mgrid = std::min<uint32_t>(n_tiles, 8192);
+    for (int attempt = 0;; attempt++) {
+        /* the merge rewrites the same buckets, so a re-run with more record room is exact */
+        *c->hinit = before;
+        c->hinit->new_keys = c->hinit->continuing = 0;
+        c->hinit->live = c->hinit->expired = 0;
+        c->hinit->kv_count = kv_before;
+        CK(hipMemcpyAsync(cnt, c->hinit, sizeof(Counters), hipMemcpyHostToDevice, s));
+        const MemtableDev mt{c->rec_keys, c->rec_boxes, c->rec_cap, c->same_pid_count, c->o_boxs, c->sids};
+        CK(hipEventRecord(c->ev[8], s));
+        hipLaunchKernelGGL(k_meet_merge, dim3(mgrid), dim3(256), 0, s, (const PairHit *)c->pnew.p, c->plen,
+                           (const Bucket *)c->meet[cu].p, (const uint32_t *)c->tbounds.p, n_tiles, c->meet[nx].p, ma,
+                           mt, cnt);
+        CK(hipGetLastError());
+        CK(hipEventRecord(c->ev[9], s));
+        CKRC(sync_counters(c));
+        h = *c->hcnt;
+        if (h.kv_count <= c->rec_cap) break;
+        if (attempt >= 2 || ensure_records(c, h.kv_count + h.kv_count / 4 + 2))
+            return fail(c, STC_E_CAPACITY, "record buffer allocation failed");
+    }
+    /* per-object memtable state of the new records */
+    if (h.kv_count > kv_before) {
+        const MemtableDev mt{c->rec_keys, c->rec_boxes, c->rec_cap, c->same_pid_count, c->o_boxs, c->sids};
+        hipLaunchKernelGGL(k_records_account, dim3(blocks_for(h.kv_count - kv_before, 256)), dim3(256), 0, s,
+                           (const uint64_t *)c->rec_keys, (const float4 *)c->rec_boxes, kv_before, (uint64_t)h.kv_count,
+                           c->cfg.max_meet_time, mt, cnt);
+        CK(hipGetLastError());
+        CKRC(sync_counters(c));
+        h = *c->hcnt;
+    }
     c->kv_count = h.kv_count;
     c->sid_count = h.sid_count;
     c->oversize_oid_count = h.oversize_oid_count;
     c->last_live = h.live;
-    uint64_t occupied = h.live + h.tombs;
-    c->occupied = occupied;
-    /* rebuild once live keys and tombstones fill 60% of the table */
-    if (occupied * 10 > c->table_buckets * 6) {
-        CKRC(table_rehash(c, t, h.live));
-        rebuilt = 1;
-        occupied = c->occupied;
-    }
+    c->last_expired = h.expired;
+    c->cur = nx;
+    c->mlen = c->plen;
+    g.pairs = h.live;
     CK(hipEventRecord(c->ev[4], s));
 
     /* 10: flush */
@@ -1785,10 +1977,10 @@ static int step_tail(StcCtx *c, uint32_t t, StepGeom &g, stc_step_out *out) {
         out->overflow_points = h.overflow;
         out->ambiguous_pairs = g.ambiguous;
         out->ambiguous_nodes = h.ambiguous_nodes;
-        out->table_buckets = c->table_buckets;
-        out->table_occupied = occupied;
+        out->table_buckets = c->mlen;
+        out->table_occupied = c->mlen;
         out->flushed = flushed;
-        out->rebuilt = rebuilt;
+        out->rebuilt = 0;
         hipEventElapsedTime(&out->ms_partition, c->ev[0], c->ev[1]);
         hipEventElapsedTime(&out->ms_filter, c->ev[1], c->ev[2]);
         hipEventElapsedTime(&out->ms_refine, c->ev[2], c->ev[3]);
@@ -1797,7 +1989,7 @@ static int step_tail(StcCtx *c, uint32_t t, StepGeom &g, stc_step_out *out) {
         hipEventElapsedTime(&out->ms_total, c->ev[0], c->ev[5]);
         hipEventElapsedTime(&out->ms_refine_kernel, c->ev[6], c->ev[7]);
         hipEventElapsedTime(&out->ms_identify_kernel, c->ev[8], c->ev[9]);
-        out->continuing = h_cont;
+        out->continuing = h.continuing;
         out->new_keys = h.new_keys;
     }
     return STC_OK;
@@ -1844,7 +2036,8 @@ int stc_step_meetings(void *vc, const stc_meeting_unit *units, uint64_t nu64, ui
     CK(hipEventRecord(c->ev[2], s));
     if (c->units_in.ensure(nu + 1ull) || c->ukeys.ensure(nu + 1ull) || c->ukeys_alt.ensure(nu + 1ull) ||
         c->uidx.ensure(nu + 1ull) || c->uidx_alt.ensure(nu + 1ull) ||
-        c->utemp.ensure(stc_radix_temp_words(nu + 1) + 64) || c->amb_pairs.ensure(nu + 1ull))
+        c->utemp.ensure(stc_radix_temp_words(std::max<uint32_t>(nu + 1, c->n)) + 64) || c->pnew.ensure(nu + 1ull) ||
+        c->meet[c->cur ^ 1].ensure(nu + 1ull))
         return fail(c, STC_E_CAPACITY, "meeting unit buffers");
     if (nu) {
         CK(hipMemcpyAsync(c->units_in.p, units, (size_t)nu * sizeof(stc_meeting_unit), hipMemcpyHostToDevice, s));
@@ -1854,47 +2047,27 @@ int stc_step_meetings(void *vc, const stc_meeting_unit *units, uint64_t nu64, ui
         CKRC(stc_radix_sort_u64(c->ukeys.p, c->uidx.p, c->ukeys_alt.p, c->uidx_alt.p, nu, 0,
                                 32 + std::max(1, ilog2_ceil(c->n)), c->utemp.p, s));
         hipLaunchKernelGGL(k_units_gather, dim3(blocks_for(nu, 256)), dim3(256), 0, s, c->ukeys.p, c->uidx.p,
-                           c->units_in.p, nu, c->amb_pairs.p, cnt);
+                           c->units_in.p, nu, c->pnew.p, cnt);
         CK(hipGetLastError());
     }
     CKRC(sync_counters(c));
     if (c->hcnt->bad_points)
         return fail(c, STC_E_INPUT, "meeting units must name distinct pairs of objects below num_objects "
                                     "by the Cantor key of (min, max)");
-    uint32_t rebuilt = 0;
-    if ((c->occupied + nu) * 10 > c->table_buckets * 6) {
-        CKRC(table_rehash(c, t - 1, c->last_live + nu));
-        rebuilt = 1;
-    }
-    Counters h{};
-    for (int attempt = 0;; attempt++) {
-        CK(hipMemsetAsync(&cnt->new_keys, 0, 2 * sizeof(unsigned long long), s)); /* new_keys, continuing */
-        CK(hipMemsetAsync(&cnt->table_fail, 0, sizeof(uint32_t), s));
-        CK(hipEventRecord(c->ev[6], s));
-        if (nu)
-            hipLaunchKernelGGL(k_table_insert, dim3(blocks_for(nu, 256)), dim3(256), 0, s, c->table,
-                               c->table_buckets, c->amb_pairs.p, (uint64_t)nu, t, cnt);
+    /* the sorted pairs are already in owner segments: only the offsets are needed */
+    const int nx = c->cur ^ 1;
+    CK(hipEventRecord(c->ev[6], s));
+    CK(hipMemsetAsync(c->pcount.p, 0, 4ull * c->n, s));
+    if (nu) {
+        hipLaunchKernelGGL(k_owner_hist, dim3(blocks_for(nu, 256)), dim3(256), 0, s, (const PairHit *)c->pnew.p, nu,
+                           c->pcount.p);
         CK(hipGetLastError());
-        CK(hipEventRecord(c->ev[7], s));
-        CKRC(sync_counters(c));
-        h = *c->hcnt;
-        if (!h.table_fail) break;
-        if (attempt >= 8) return fail(c, STC_E_CAPACITY, "meeting table cannot hold the step's pairs");
-        CKRC(table_rehash(c, t - 1, c->last_live + nu + 1024)); /* reinserting is idempotent */
-        rebuilt = 1;
     }
-    c->last_new = h.new_keys;
-    c->occupied += h.new_keys;
-    c->host_pairs_extra.clear();
-    c->last_listed = 0;
-    if ((c->cfg.keep_pairs & 1) && nu) {
-        if (c->pairs.ensure(nu)) return fail(c, STC_E_CAPACITY, "pair list allocation failed");
-        CK(hipMemcpyAsync(c->pairs.p, c->amb_pairs.p, (size_t)nu * sizeof(PairHit), hipMemcpyDeviceToDevice, s));
-        c->last_listed = nu;
-    }
-    c->last_pairs = nu;
+    CKRC(stc_exclusive_scan_u32(c->pcount.p, c->moff[nx].p, c->n, c->moff[nx].p + c->n, c->utemp.p, s));
+    CK(hipEventRecord(c->ev[7], s));
+    c->plen = nu;
     CK(hipEventRecord(c->ev[3], s));
-    StepGeom g{0, 0, 0, nu, 0, rebuilt};
+    StepGeom g{0, 0, 0, 0, 0, 0};
     int rc = step_tail(c, t, g, out);
     if (rc == STC_OK && out) out->new_records = out->kv_count - kv_before;
     return rc;
@@ -1907,10 +2080,8 @@ int stc_get_step_meetings(void *vc, stc_meeting_unit *units, uint64_t cap, uint6
     if (!c || !n) return STC_E_ARG;
     if (!c->stepped) return fail(c, STC_E_STATE, "no step processed");
     if (!(c->cfg.keep_pairs & 1)) return fail(c, STC_E_STATE, "pairs are kept only with keep_pairs bit 0 set");
-    std::vector<PairHit> ph(c->last_listed);
-    if (c->last_listed) CK(hipMemcpy(ph.data(), c->pairs.p, c->last_listed * sizeof(PairHit), hipMemcpyDeviceToHost));
-    for (const PairHit &e : c->host_pairs_extra) ph.push_back(e);
-    std::sort(ph.begin(), ph.end(), [](const PairHit &a, const PairHit &b) { return a.key < b.key; });
+    std::vector<PairHit> ph;
+    CKRC(step_pairs(c, ph));
     ph.erase(std::unique(ph.begin(), ph.end(), [](const PairHit &a, const PairHit &b) { return a.key == b.key; }),
              ph.end());
     *n = ph.size();
@@ -1929,12 +2100,10 @@ int stc_get_pairs(void *vc, uint64_t *keys, uint64_t cap, uint64_t *n) {
     if (!c || !n) return STC_E_ARG;
     if (!c->stepped) return fail(c, STC_E_STATE, "no step processed");
     if (!(c->cfg.keep_pairs & 1)) return fail(c, STC_E_STATE, "pairs are kept only with keep_pairs bit 0 set");
-    std::vector<PairHit> ph(c->last_listed);
-    if (c->last_listed) CK(hipMemcpy(ph.data(), c->pairs.p, c->last_listed * sizeof(PairHit), hipMemcpyDeviceToHost));
+    std::vector<PairHit> ph;
+    CKRC(step_pairs(c, ph));
     std::vector<uint64_t> k(ph.size());
     for (size_t i = 0; i < ph.size(); i++) k[i] = ph[i].key;
-    for (const PairHit &e : c->host_pairs_extra) k.push_back(e.key);
-    std::sort(k.begin(), k.end());
     k.erase(std::unique(k.begin(), k.end()), k.end());
     *n = k.size();
     if (keys) memcpy(keys, k.data(), std::min<uint64_t>(cap, k.size()) * 8);
@@ -1945,12 +2114,11 @@ int stc_get_active(void *vc, uint64_t *keys, uint32_t *start, uint32_t *end, flo
     StcCtx *c = (StcCtx *)vc;
     if (!c || !n) return STC_E_ARG;
     if (!c->stepped) return fail(c, STC_E_STATE, "no step processed");
-    std::vector<Bucket> tb(c->table_buckets);
-    CK(hipMemcpy(tb.data(), c->table, c->table_buckets * sizeof(Bucket), hipMemcpyDeviceToHost));
-    std::vector<Bucket> live;
+    std::vector<Bucket> tb(c->mlen);
+    if (c->mlen) CK(hipMemcpy(tb.data(), c->meet[c->cur].p, c->mlen * sizeof(Bucket), hipMemcpyDeviceToHost));
+    std::vector<Bucket> live; /* the array is sorted by key; tombstones trail each segment */
     for (const Bucket &b : tb)
         if (!key_free(b.key) && !key_tomb(b.key) && b.end == c->last_t) live.push_back(b);
-    std::sort(live.begin(), live.end(), [](const Bucket &a, const Bucket &b) { return a.key < b.key; });
     *n = live.size();
     for (uint64_t i = 0; i < std::min<uint64_t>(cap, live.size()); i++) {
         if (keys) keys[i] = live[i].key;
@@ -2308,8 +2476,8 @@ int stc_query_batch(void *vc, const stc_query *q, uint32_t nq, stc_hit *hits, ui
         if (q[i].kind < STC_Q_OID || q[i].kind > STC_Q_PAIR) return fail(c, STC_E_ARG, "unknown query kind");
     if (!c->qb) c->qb = stc_query_bufs_new();
     QueryState st;
-    st.table = c->table;
-    st.nb = c->table_buckets;
+    st.table = c->meet[c->cur].p;
+    st.nb = c->mlen;
     st.last_t = c->last_t;
     st.stepped = 1;
     st.rec_keys = c->rec_keys;
