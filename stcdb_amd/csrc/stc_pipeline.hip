@@ -459,9 +459,13 @@ __device__ inline float4 pair_mbr(float ax, float ay, float bx, float by) {
     return make_float4(fminf(ax, bx), fminf(ay, by), fmaxf(ax, bx), fmaxf(ay, by));
 }
 
+#define SRC_REV 0x80000000u /* source index tag: an index into the reverse lists, not the leaf-ordered arrays */
+
 /* Every (unit, member) combination of the step whose pair is owned by a,
  * visited by the 64 lanes of a wave, at most one per lane per call of
- * f(valid, other, decision, probe, mbr):
+ * f(valid, other, decision, probe, mbr, source) (source: the other point's
+ * index in the leaf-ordered arrays, or SRC_REV | its index in the reverse
+ * lists):
  *   (i)   a against the later (larger-id) kept members of its own leaf (the
  *         inside unit of geometry.cu:258; probe a);
  *   (ii)  a against the kept members above a of the leaves of a's cross units
@@ -517,7 +521,7 @@ __device__ inline void owner_combos_by_source(uint32_t w, uint32_t a, const Pair
             q = g.lp[st + m];
             mb = pair_mbr(afx, afy, (float)qx, (float)qy);
         }
-        f(v, q, dec, a, mb);
+        f(v, q, dec, a, mb, st + m);
     }
     /* (ii) */
     const uint2 un = g.lunits[w];
@@ -539,7 +543,7 @@ __device__ inline void owner_combos_by_source(uint32_t w, uint32_t a, const Pair
                 q = g.lp[st2 + m];
                 mb = pair_mbr(afx, afy, (float)qx, (float)qy);
             }
-            f(v, q, dec, a, mb);
+            f(v, q, dec, a, mb, st2 + m);
         }
     }
     /* (iii) */
@@ -559,7 +563,7 @@ __device__ inline void owner_combos_by_source(uint32_t w, uint32_t a, const Pair
                 dec = pair_decide(pp, g.rev_dl[i], g.rev_il[i], qx, qy, g);
                 mb = pair_mbr((float)pp.x, (float)pp.y, qfx, qfy);
             }
-            f(v, p, dec, p, mb);
+            f(v, p, dec, p, mb, SRC_REV | i);
         }
     }
 }
@@ -629,10 +633,11 @@ __device__ inline void owner_combos(uint32_t w, uint32_t a, const PairGeo &g, in
             if (__shfl(scum, (int)k, STC_WAVE) <= c) j = k;
         const uint32_t jb = __shfl(sbase, (int)j, STC_WAVE), jc = __shfl(scum, (int)j, STC_WAVE);
         int dec = 0;
-        uint32_t q = 0, probe = a;
+        uint32_t q = 0, probe = a, src = 0;
         float4 mb = make_float4(0.f, 0.f, 0.f, 0.f);
         if (v) {
             const uint32_t idx = jb + (c - jc);
+            src = idx;
             if (j + 1 < nseg) { /* a member of the owner's leaf or of a cross unit's leaf; probe a */
                 const double qx = g.lx[idx], qy = g.ly[idx];
                 dec = pair_decide(pa, pda, pia, qx, qy, g);
@@ -644,10 +649,11 @@ __device__ inline void owner_combos(uint32_t w, uint32_t a, const PairGeo &g, in
                 dec = pair_decide(pp, g.rev_dl[idx], g.rev_il[idx], qax, qay, g);
                 q = p;
                 probe = p;
+                src = SRC_REV | idx;
                 mb = pair_mbr((float)pp.x, (float)pp.y, (float)qax, (float)qay);
             }
         }
-        f(v, q, dec, probe, mb);
+        f(v, q, dec, probe, mb, src);
     }
 }
 
@@ -686,7 +692,7 @@ __global__ __launch_bounds__(256) void k_pair_count(PairGeo g, uint32_t *__restr
     for (uint64_t w = wid; w < g.n; w += nw) {
         const uint32_t a = g.lp[w];
         unsigned long long c = 0;
-        auto f = [&](bool v, uint32_t q, int dec, uint32_t probe, float4) {
+        auto f = [&](bool v, uint32_t q, int dec, uint32_t probe, float4, uint32_t) {
             if (!v) return;
             cand++;
             if (dec) c++;
@@ -720,67 +726,120 @@ __device__ inline bool sorted_contains(const unsigned long long *a, uint32_t n, 
 }
 
 #define DEAD_B 0xFFFFFFFFu /* partner word of a dead slot or tombstone: sorts after the owner's live pairs */
-#define WCAP 256u          /* slots of a segment the write pass assembles and sorts in its wave's LDS */
+#define WCAP 512u          /* slots of a segment the write pass sorts in its wave's LDS */
 
 __device__ inline uint64_t lanemask_lt(int lane) { return lane == 0 ? 0ull : (~0ull >> (STC_WAVE - lane)); }
 
-/* ascending bitonic sort of s[0, P) (P a power of two) by one wave */
-__device__ inline void wave_bitonic(unsigned long long *s, uint32_t P, int lane) {
-    if (P <= STC_WAVE) { /* one element per lane, in registers */
-        unsigned long long x = (uint32_t)lane < P ? s[lane] : ~0ull;
-        for (uint32_t k = 2; k <= STC_WAVE; k <<= 1)
-            for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-                const unsigned long long y = __shfl_xor(x, (int)j, STC_WAVE);
-                const bool asc = (lane & k) == 0, lower = (lane & j) == 0;
-                x = (asc == lower) ? (x < y ? x : y) : (x < y ? y : x);
-            }
-        if ((uint32_t)lane < P) s[lane] = x;
-        __builtin_amdgcn_wave_barrier();
-        return;
-    }
-    for (uint32_t k = 2; k <= P; k <<= 1)
-        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-            for (uint32_t i = lane; i < P; i += STC_WAVE) {
-                const uint32_t ixj = i ^ j;
-                if (ixj > i) {
-                    const unsigned long long x = s[i], y = s[ixj];
-                    if ((x > y) == ((i & k) == 0)) {
-                        s[i] = y;
-                        s[ixj] = x;
+
+/* ascending bitonic sort of 64 * K keys held by a wave, element r * 64 + lane
+ * in x[r] of the lane: stages across lanes exchange through shuffles, stages
+ * across registers swap within the lane */
+template <int K>
+__device__ inline void reg_bitonic(unsigned long long (&x)[K], int lane) {
+#pragma unroll
+    for (int k = 2; k <= STC_WAVE * K; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            if (j >= STC_WAVE) {
+#pragma unroll
+                for (int r = 0; r < K; r++) {
+                    const int r2 = r ^ (j / STC_WAVE);
+                    if (r2 > r) {
+                        const bool asc = (((r * STC_WAVE + lane) & k) == 0);
+                        const unsigned long long u = x[r], v = x[r2];
+                        const bool sw = asc ? (u > v) : (u < v);
+                        x[r] = sw ? v : u;
+                        x[r2] = sw ? u : v;
                     }
                 }
+            } else {
+#pragma unroll
+                for (int r = 0; r < K; r++) {
+                    const unsigned long long y = __shfl_xor(x[r], j, STC_WAVE);
+                    const bool asc = (((r * STC_WAVE + lane) & k) == 0), lower = (lane & j) == 0;
+                    x[r] = (asc == lower) ? (x[r] < y ? x[r] : y) : (x[r] < y ? y : x[r]);
+                }
             }
-            __builtin_amdgcn_wave_barrier();
         }
+    }
+}
+
+/* Sort an owner's slots (keys[0, slots), slots <= 64 * K) in registers and
+ * write its segment: live pairs in partner order, then the dead slots. */
+template <int K>
+__device__ inline void write_sorted_segment(const unsigned long long *keys, uint32_t slots, uint32_t a, uint32_t w,
+                                            const PairGeo &g, PairHit *__restrict__ seg, int lane) {
+    const uint64_t lt = lanemask_lt(lane);
+    unsigned long long x[K];
+#pragma unroll
+    for (int r = 0; r < K; r++) {
+        const uint32_t i = r * STC_WAVE + lane;
+        x[r] = i < slots ? keys[i] : ~0ull;
+    }
+    reg_bitonic<K>(x, lane);
+    bool live[K];
+    uint32_t nlive = 0;
+#pragma unroll
+    for (int r = 0; r < K; r++) {
+        const unsigned long long up = __shfl_up(x[r], 1, STC_WAVE);
+        const unsigned long long last = r > 0 ? __shfl(x[r > 0 ? r - 1 : 0], STC_WAVE - 1, STC_WAVE) : ~0ull;
+        const unsigned long long prev = lane > 0 ? up : last;
+        const uint32_t i = r * STC_WAVE + lane;
+        const uint32_t b = (uint32_t)(x[r] >> 32);
+        live[r] = i < slots && b != DEAD_B && (i == 0 || (uint32_t)(prev >> 32) != b);
+        nlive += (uint32_t)__popcll(__ballot(live[r]));
+    }
+    const float afx = (float)g.lx[w], afy = (float)g.ly[w];
+    uint32_t lr = 0, dr = 0;
+#pragma unroll
+    for (int r = 0; r < K; r++) {
+        const uint32_t i = r * STC_WAVE + lane;
+        const bool valid = i < slots;
+        const uint64_t ml = __ballot(live[r]), md = __ballot(valid && !live[r]);
+        if (valid) {
+            const uint32_t src = (uint32_t)x[r];
+            float4 mb;
+            if (src & SRC_REV) { /* found by the border point: its position first, as pass 1 forms it */
+                const double2 pp = g.rev_xy[src & ~SRC_REV];
+                mb = pair_mbr((float)pp.x, (float)pp.y, afx, afy);
+            } else {
+                mb = pair_mbr(afx, afy, (float)g.lx[src], (float)g.ly[src]);
+            }
+            PairHit e;
+            e.key = ((unsigned long long)a << 32) | (live[r] ? (uint32_t)(x[r] >> 32) : DEAD_B);
+            e.mbr[0] = mb.x; e.mbr[1] = mb.y; e.mbr[2] = mb.z; e.mbr[3] = mb.w;
+            seg[live[r] ? lr + (uint32_t)__popcll(ml & lt) : nlive + dr + (uint32_t)__popcll(md & lt)] = e;
+        }
+        lr += (uint32_t)__popcll(ml);
+        dr += (uint32_t)__popcll(md);
+    }
 }
 
 /* Pass 2: the same combinations, written to pn as each owner's segment sorted
  * by partner, its live pairs first and its dead slots (band combinations the
  * host rejected, repeats of a pair) behind them, so that pn is sorted by key.
- * A segment of at most WCAP slots is assembled and sorted in the wave's LDS;
- * up to 2 * WCAP slots go through ptmp with only their sort keys in LDS; a
- * longer one is left unsorted in ptmp and listed in `big` for k_seg_sort. The
- * pass repeats pass 1's arithmetic exactly; a slot beyond the counted ones
- * would be dropped and reported through table_fail. */
+ * A segment of at most WCAP slots is sorted in the wave's LDS as (partner,
+ * source) keys, each pair's MBR recomputed from its source's position when
+ * written; a longer one is written unsorted to ptmp and listed in `big` for
+ * k_seg_sort. The pass repeats pass 1's arithmetic exactly; a slot beyond the
+ * counted ones would be dropped and reported through table_fail. */
 __global__ __launch_bounds__(256) void k_pair_write(PairGeo g, const uint32_t *__restrict__ poff,
                                                     const uint32_t *__restrict__ pcount,
                                                     const unsigned long long *__restrict__ acc, uint32_t n_acc,
                                                     PairHit *__restrict__ ptmp, PairHit *__restrict__ pn,
                                                     uint32_t *__restrict__ big, Counters *cnt) {
-    /* per wave: WCAP sort keys and WCAP boxes, or 2 * WCAP sort keys */
-    __shared__ __align__(16) unsigned long long wraw[256 / STC_WAVE][3 * WCAP];
+    __shared__ unsigned long long wk[256 / STC_WAVE][WCAP];
     const int lane = threadIdx.x & (STC_WAVE - 1);
-    unsigned long long *keys = wraw[threadIdx.x / STC_WAVE];
-    float4 *mbrs = (float4 *)(keys + WCAP);
+    unsigned long long *keys = wk[threadIdx.x / STC_WAVE];
     const uint64_t wid = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / STC_WAVE;
     const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) / STC_WAVE;
     const uint64_t lt = lanemask_lt(lane);
     for (uint64_t w = wid; w < g.n; w += nw) {
         const uint32_t a = g.lp[w];
         const uint32_t base = poff[a], slots = pcount[a];
-        const bool small = slots <= WCAP, mid = !small && slots <= 2 * WCAP;
+        const bool small = slots <= WCAP;
         uint32_t fill = 0;
-        auto f = [&](bool v, uint32_t q, int dec, uint32_t, float4 mb) {
+        auto f = [&](bool v, uint32_t q, int dec, uint32_t, float4 mb, uint32_t src) {
             const bool take = v && dec != 0;
             const uint64_t m = __ballot(take);
             if (take) {
@@ -790,67 +849,28 @@ __global__ __launch_bounds__(256) void k_pair_write(PairGeo g, const uint32_t *_
                 if (at >= slots) {
                     atomicOr(&cnt->table_fail, 1u);
                 } else if (small) {
-                    keys[at] = ((unsigned long long)bq << 32) | at;
-                    mbrs[at] = mb;
+                    keys[at] = ((unsigned long long)bq << 32) | src;
                 } else {
                     PairHit h;
                     h.key = ((unsigned long long)a << 32) | bq;
                     h.mbr[0] = mb.x; h.mbr[1] = mb.y; h.mbr[2] = mb.z; h.mbr[3] = mb.w;
                     ptmp[base + at] = h;
-                    if (mid) keys[at] = ((unsigned long long)bq << 32) | at;
                 }
             }
             fill += (uint32_t)__popcll(m);
         };
         owner_combos((uint32_t)w, a, g, lane, f);
-        if (!small && !mid) {
+        if (!small) {
             if (lane == 0) big[atomicAdd(&cnt->big, 1u)] = a; /* the list holds every owner */
             continue;
         }
         if (slots == 0) continue;
-        if (mid) __threadfence_block(); /* the staged slots are read back by other lanes of the wave */
-        uint32_t P = 1;
-        while (P < slots) P <<= 1;
-        for (uint32_t i = slots + lane; i < P; i += STC_WAVE) keys[i] = ~0ull;
         __builtin_amdgcn_wave_barrier();
-        wave_bitonic(keys, P, lane);
-        /* live pairs in partner order, then the dead slots */
-        uint32_t nlive = 0;
-        for (uint32_t k0 = 0; k0 < slots; k0 += STC_WAVE) {
-            const uint32_t k = k0 + lane;
-            bool live = false;
-            if (k < slots) {
-                const uint32_t b = (uint32_t)(keys[k] >> 32);
-                live = b != DEAD_B && (k == 0 || (uint32_t)(keys[k - 1] >> 32) != b);
-            }
-            nlive += (uint32_t)__popcll(__ballot(live));
-        }
-        uint32_t lr = 0, dr = 0;
-        for (uint32_t k0 = 0; k0 < slots; k0 += STC_WAVE) {
-            const uint32_t k = k0 + lane;
-            bool live = false;
-            uint32_t b = DEAD_B;
-            if (k < slots) {
-                b = (uint32_t)(keys[k] >> 32);
-                live = b != DEAD_B && (k == 0 || (uint32_t)(keys[k - 1] >> 32) != b);
-            }
-            const uint64_t ml = __ballot(live), md = __ballot(k < slots && !live);
-            if (k < slots) {
-                PairHit e;
-                if (mid) {
-                    e = ptmp[base + (uint32_t)keys[k]];
-                } else {
-                    const float4 mb = mbrs[(uint32_t)keys[k]];
-                    e.mbr[0] = mb.x; e.mbr[1] = mb.y; e.mbr[2] = mb.z; e.mbr[3] = mb.w;
-                }
-                e.key = ((unsigned long long)a << 32) | (live ? b : DEAD_B);
-                const uint32_t dest = live ? lr + (uint32_t)__popcll(ml & lt) : nlive + dr + (uint32_t)__popcll(md & lt);
-                pn[base + dest] = e;
-            }
-            lr += (uint32_t)__popcll(ml);
-            dr += (uint32_t)__popcll(md);
-        }
-        __builtin_amdgcn_wave_barrier(); /* the stage is reused by the wave's next owner */
+        if (slots <= STC_WAVE) write_sorted_segment<1>(keys, slots, a, (uint32_t)w, g, pn + base, lane);
+        else if (slots <= 2 * STC_WAVE) write_sorted_segment<2>(keys, slots, a, (uint32_t)w, g, pn + base, lane);
+        else if (slots <= 4 * STC_WAVE) write_sorted_segment<4>(keys, slots, a, (uint32_t)w, g, pn + base, lane);
+        else write_sorted_segment<8>(keys, slots, a, (uint32_t)w, g, pn + base, lane);
+        __builtin_amdgcn_wave_barrier(); /* the keys are reused by the wave's next owner */
     }
 }
 
