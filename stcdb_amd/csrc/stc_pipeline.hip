@@ -102,7 +102,7 @@ struct Counters {             /* device-side counters, copied back once per sync
     uint32_t overflow;
     uint32_t sid_count, oversize_oid_count;
     uint32_t table_fail;
-    uint32_t big, huge;                /* segments left to the block sort and to the global-scratch sort */
+    uint32_t big, huge, giant;         /* segments left to the wave sort, the block sort, the global-scratch sort */
 };
 
 
@@ -428,6 +428,9 @@ struct PairGeo {
     const double *ld;                    /* dpm_lon in leaf order */
     const uint32_t *lleaf;               /* leaf of each point, in leaf order */
     const uint2 *lunits;                 /* (first cross unit, count) of each point, in leaf order */
+    const uint2 *urange;                 /* per cross unit: its leaf's kept members above the unit's point (first, count) */
+    const uint4 *prange;                 /* per point in leaf order: its later leaf members and the border
+                                            points above it with a unit to its leaf (first, count, first, count) */
     const uint2 *units;                  /* cross units (pid, leaf), point after point */
     const uint32_t *uoffset, *ucount;    /* a point's cross units: units[uoffset[p] ..] */
     const uint32_t *rev_start, *rev_pid; /* per leaf, by pid: the border points with a cross unit to it */
@@ -582,38 +585,23 @@ __device__ inline void owner_combos(uint32_t w, uint32_t a, const PairGeo &g, in
         owner_combos_by_source(w, a, g, lane, f);
         return;
     }
-    const uint32_t L = g.lleaf[w];
-    uint32_t s = g.leaf_count[L];
-    if (s > g.buf) s = g.buf;
-    const uint32_t st = g.leaf_start[L], rank = w - st;
+    const uint4 pr = g.prange[w];
     const double2 pa = make_double2(g.lx[w], g.ly[w]);
     const double pda = g.ld[w], pia = 1.0 / pda; /* the values of dlon[a] and idlon[a] (k_prep) */
     const float afx = (float)pa.x, afy = (float)pa.y;
     /* source table: first index (into the leaf-ordered arrays, or rev_pid for
      * the last source) and length, held by lane j for source j */
     uint32_t sbase = 0, slen = 0;
-    if (lane == 0 && rank + 1 < s) {
-        sbase = st + rank + 1;
-        slen = s - rank - 1;
-    }
-    for (uint32_t k = 0; k < nu; k++) {
-        const uint32_t L2 = g.units[u0 + k].y;
-        uint32_t s2 = g.leaf_count[L2];
-        if (s2 > g.buf) s2 = g.buf;
-        const uint32_t st2 = g.leaf_start[L2];
-        const uint32_t m0 = wave_upper_bound(g.lp + st2, s2, a, lane);
-        if ((uint32_t)lane == k + 1) {
-            sbase = st2 + m0;
-            slen = s2 - m0;
-        }
-    }
-    if (rank < s) {
-        const uint32_t r0 = g.rev_start[L], r1 = g.rev_start[L + 1];
-        const uint32_t i0 = r0 + wave_upper_bound(g.rev_pid + r0, r1 - r0, a, lane);
-        if ((uint32_t)lane == nu + 1) {
-            sbase = i0;
-            slen = r1 - i0;
-        }
+    if (lane == 0) {
+        sbase = pr.x;
+        slen = pr.y;
+    } else if ((uint32_t)lane <= nu) {
+        const uint2 ur = g.urange[u0 + lane - 1];
+        sbase = ur.x;
+        slen = ur.y;
+    } else if ((uint32_t)lane == nu + 1) {
+        sbase = pr.z;
+        slen = pr.w;
     }
     /* exclusive prefix of the lengths */
     uint32_t scum = slen;
@@ -678,6 +666,47 @@ __global__ void k_rev_gather(const uint32_t *__restrict__ rev_pid, uint32_t nr, 
     rxy[i] = xy[p];
     rdl[i] = dlon[p];
     ril[i] = idlon[p];
+}
+
+/* per cross unit (p, leaf): the leaf's kept members above p, as an index range
+ * into the leaf-ordered arrays */
+__global__ void k_unit_ranges(const uint2 *__restrict__ units, uint32_t nu, const uint32_t *__restrict__ leaf_start,
+                              const uint32_t *__restrict__ leaf_count, const uint32_t *__restrict__ lp, uint32_t buf,
+                              uint2 *__restrict__ urange) {
+    const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= nu) return;
+    const uint2 un = units[u];
+    uint32_t s2 = leaf_count[un.y];
+    if (s2 > buf) s2 = buf;
+    const uint32_t st2 = leaf_start[un.y];
+    const uint32_t m0 = lower_bound_u32(lp + st2, s2, un.x + 1);
+    urange[u] = make_uint2(st2 + m0, s2 - m0);
+}
+
+/* per point in leaf order: its later kept members in its leaf, and (when it is
+ * kept) the border points above it with a cross unit to its leaf */
+__global__ void k_point_ranges(const uint32_t *__restrict__ lp, const uint32_t *__restrict__ lleaf, uint32_t n,
+                               const uint32_t *__restrict__ leaf_start, const uint32_t *__restrict__ leaf_count,
+                               const uint32_t *__restrict__ rev_start, const uint32_t *__restrict__ rev_pid,
+                               uint32_t buf, uint4 *__restrict__ prange) {
+    const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= n) return;
+    const uint32_t a = lp[w], L = lleaf[w];
+    uint32_t s = leaf_count[L];
+    if (s > buf) s = buf;
+    const uint32_t st = leaf_start[L], rank = w - st;
+    uint4 r = make_uint4(0, 0, 0, 0);
+    if (rank + 1 < s) {
+        r.x = w + 1;
+        r.y = s - rank - 1;
+    }
+    if (rank < s) {
+        const uint32_t r0 = rev_start[L], r1 = rev_start[L + 1];
+        const uint32_t i0 = r0 + lower_bound_u32(rev_pid + r0, r1 - r0, a + 1);
+        r.z = i0;
+        r.w = r1 - i0;
+    }
+    prange[w] = r;
 }
 
 /* Pass 1: per owner, the combinations that are pairs or lie in the guard band
@@ -812,6 +841,68 @@ __device__ inline void write_sorted_segment(const unsigned long long *keys, uint
         }
         lr += (uint32_t)__popcll(ml);
         dr += (uint32_t)__popcll(md);
+    }
+}
+
+/* the same for a segment staged unsorted in tmp (MBRs stored with the slots) */
+template <int K>
+__device__ inline void write_sorted_from_tmp(const PairHit *__restrict__ tmp, uint32_t slots, uint32_t a,
+                                             PairHit *__restrict__ seg, int lane) {
+    const uint64_t lt = lanemask_lt(lane);
+    unsigned long long x[K];
+#pragma unroll
+    for (int r = 0; r < K; r++) {
+        const uint32_t i = r * STC_WAVE + lane;
+        x[r] = i < slots ? (((unsigned long long)(uint32_t)tmp[i].key << 32) | i) : ~0ull;
+    }
+    reg_bitonic<K>(x, lane);
+    bool live[K];
+    uint32_t nlive = 0;
+#pragma unroll
+    for (int r = 0; r < K; r++) {
+        const unsigned long long up = __shfl_up(x[r], 1, STC_WAVE);
+        const unsigned long long last = r > 0 ? __shfl(x[r > 0 ? r - 1 : 0], STC_WAVE - 1, STC_WAVE) : ~0ull;
+        const unsigned long long prev = lane > 0 ? up : last;
+        const uint32_t i = r * STC_WAVE + lane;
+        const uint32_t b = (uint32_t)(x[r] >> 32);
+        live[r] = i < slots && b != DEAD_B && (i == 0 || (uint32_t)(prev >> 32) != b);
+        nlive += (uint32_t)__popcll(__ballot(live[r]));
+    }
+    uint32_t lr = 0, dr = 0;
+#pragma unroll
+    for (int r = 0; r < K; r++) {
+        const uint32_t i = r * STC_WAVE + lane;
+        const bool valid = i < slots;
+        const uint64_t ml = __ballot(live[r]), md = __ballot(valid && !live[r]);
+        if (valid) {
+            PairHit e = tmp[(uint32_t)x[r]];
+            e.key = ((unsigned long long)a << 32) | (live[r] ? (uint32_t)(x[r] >> 32) : DEAD_B);
+            seg[live[r] ? lr + (uint32_t)__popcll(ml & lt) : nlive + dr + (uint32_t)__popcll(md & lt)] = e;
+        }
+        lr += (uint32_t)__popcll(ml);
+        dr += (uint32_t)__popcll(md);
+    }
+}
+
+#define WSORT_CAP 1024u /* slots a wave sorts in registers when the write pass could not */
+
+/* segments the write pass left in ptmp: one wave each, up to WSORT_CAP slots;
+ * longer ones go to `spill` for the block sort */
+__global__ __launch_bounds__(256) void k_seg_sort_wave(const uint32_t *__restrict__ owners, uint32_t n_owners,
+                                                       const uint32_t *__restrict__ poff,
+                                                       const PairHit *__restrict__ ptmp, PairHit *__restrict__ pn,
+                                                       uint32_t *__restrict__ spill, uint32_t *spill_count) {
+    const int lane = threadIdx.x & (STC_WAVE - 1);
+    const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / STC_WAVE;
+    const uint32_t nw = (gridDim.x * blockDim.x) / STC_WAVE;
+    for (uint32_t oi = wid; oi < n_owners; oi += nw) {
+        const uint32_t a = owners[oi];
+        const uint32_t base = poff[a], slots = poff[a + 1] - base;
+        if (slots > WSORT_CAP) {
+            if (lane == 0) spill[atomicAdd(spill_count, 1u)] = a;
+            continue;
+        }
+        write_sorted_from_tmp<WSORT_CAP / STC_WAVE>(ptmp + base, slots, a, pn + base, lane);
     }
 }
 
@@ -1089,19 +1180,18 @@ __global__ __launch_bounds__(THREADS) void k_seg_sort(const uint32_t *__restrict
     }
 }
 
-#define MTILE 1024u  /* new pairs per merge tile */
-#define OCAP 2048u   /* previous meetings of a tile held in LDS */
-#define MSTAGE 256u  /* expired meetings a merge block stages before reserving their records */
+#define MTILE 256u   /* new pairs per merge tile (one wave) */
+#define OCAP 512u    /* previous meetings of a tile held in the wave's LDS */
+#define MSTAGE 64u   /* expired meetings a merge wave stages before reserving their records */
 
-/* write the staged meetings' records: one counter reservation per batch */
-__device__ inline void flush_stage(const Bucket *stage, uint32_t staged, unsigned long long *kb_sh, uint32_t end_time_min,
-                                   const MemtableDev &mt, Counters *cnt) {
-    if (threadIdx.x == 0) *kb_sh = atomicAdd(&cnt->kv_count, 2ull * staged);
-    __syncthreads();
-    const unsigned long long kb = *kb_sh;
-    for (uint32_t e = threadIdx.x; e < staged; e += blockDim.x)
-        write_meeting_records(stage[e], kb + 2ull * e, end_time_min, mt);
-    __syncthreads();
+/* write the wave's staged meetings' records: one counter reservation per batch */
+__device__ inline void flush_stage(const Bucket *stage, uint32_t staged, uint32_t end_time_min, const MemtableDev &mt,
+                                   Counters *cnt, int lane) {
+    unsigned long long kb = 0;
+    if (lane == 0) kb = atomicAdd(&cnt->kv_count, 2ull * staged);
+    kb = __shfl(kb, 0, STC_WAVE);
+    for (uint32_t e = lane; e < staged; e += STC_WAVE) write_meeting_records(stage[e], kb + 2ull * e, end_time_min, mt);
+    __builtin_amdgcn_wave_barrier();
 }
 
 /* Start of each merge tile's range in the previous meeting array: the number
@@ -1149,34 +1239,38 @@ __device__ inline uint32_t lower_bound_bucket(const Bucket *s, uint32_t n, unsig
 
 /* Merge of the step's pairs (pn, sorted by key, dead slots behind each owner's
  * live pairs) with the previous step's meetings (mold, sorted the same way),
- * tile by tile: the tile's new keys and its range of previous keys (tb) are
- * staged in LDS. A live pair continues the previous meeting of its key (start
- * kept, MBR widened) or opens one (insert/extend, geometry.cu:469-494); a dead
- * slot becomes a tombstone; a previous meeting whose key has no pair expires
- * and, when it lasted long enough inside the map, becomes two records
- * (identify, geometry.cu:595-737). mnew[i] is built from pn[i]. */
+ * tile by tile, one wave per tile: the tile's new keys and its range of
+ * previous keys (tb) are staged in the wave's LDS. A live pair continues the
+ * previous meeting of its key (start kept, MBR widened) or opens one
+ * (insert/extend, geometry.cu:469-494); a dead slot becomes a tombstone; a
+ * previous meeting whose key has no pair expires and, when it lasted long
+ * enough inside the map, becomes two records (identify, geometry.cu:595-737).
+ * mnew[i] is built from pn[i]. */
 __global__ __launch_bounds__(256) void k_meet_merge(const PairHit *__restrict__ pn, uint64_t n_p,
                                                     const Bucket *__restrict__ mold, const uint32_t *__restrict__ tb,
                                                     uint32_t n_tiles, Bucket *__restrict__ mnew, MergeArgs ma,
                                                     MemtableDev mt, Counters *cnt) {
-    __shared__ unsigned long long pk[MTILE];
-    __shared__ unsigned long long ok[OCAP];
-    __shared__ Bucket stage[MSTAGE];
-    __shared__ uint32_t wsum[256 / STC_WAVE];
-    __shared__ unsigned long long kb_sh;
-    const int tid = threadIdx.x, lane = tid & (STC_WAVE - 1);
+    __shared__ unsigned long long wpk[256 / STC_WAVE][MTILE];
+    __shared__ unsigned long long wok[256 / STC_WAVE][OCAP];
+    __shared__ Bucket wstage[256 / STC_WAVE][MSTAGE];
+    const int lane = threadIdx.x & (STC_WAVE - 1), wv = threadIdx.x / STC_WAVE;
+    unsigned long long *pk = wpk[wv], *ok = wok[wv];
+    Bucket *stage = wstage[wv];
+    const uint64_t lt = lanemask_lt(lane);
+    const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / STC_WAVE;
+    const uint32_t nw = (gridDim.x * blockDim.x) / STC_WAVE;
     uint32_t staged = 0;
     unsigned long long n_live = 0, n_exp = 0, n_fresh = 0, n_cont = 0;
-    for (uint32_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+    for (uint32_t t = wid; t < n_tiles; t += nw) {
         const uint64_t i0 = (uint64_t)t * MTILE;
         const uint32_t np = i0 < n_p ? (uint32_t)min<uint64_t>(MTILE, n_p - i0) : 0u;
         const uint32_t o0 = tb[t], no = tb[t + 1] - o0;
         const bool oin = no <= OCAP;
-        for (uint32_t i = tid; i < np; i += 256) pk[i] = pn[i0 + i].key;
+        for (uint32_t i = lane; i < np; i += STC_WAVE) pk[i] = pn[i0 + i].key;
         if (oin)
-            for (uint32_t j = tid; j < no; j += 256) ok[j] = mold[o0 + j].key;
-        __syncthreads();
-        for (uint32_t i = tid; i < np; i += 256) {
+            for (uint32_t j = lane; j < no; j += STC_WAVE) ok[j] = mold[o0 + j].key;
+        __builtin_amdgcn_wave_barrier();
+        for (uint32_t i = lane; i < np; i += STC_WAVE) {
             const unsigned long long key = pk[i];
             Bucket nb;
             nb.key = key;
@@ -1206,8 +1300,8 @@ __global__ __launch_bounds__(256) void k_meet_merge(const PairHit *__restrict__ 
             }
             mnew[i0 + i] = nb;
         }
-        for (uint32_t j0 = 0; j0 < no; j0 += 256) {
-            const uint32_t j = j0 + tid;
+        for (uint32_t j0 = 0; j0 < no; j0 += STC_WAVE) {
+            const uint32_t j = j0 + lane;
             bool emit = false;
             Bucket ob;
             if (j < no) {
@@ -1221,21 +1315,24 @@ __global__ __launch_bounds__(256) void k_meet_merge(const PairHit *__restrict__ 
                     }
                 }
             }
-            uint32_t total;
-            const uint32_t pos = block_rank<256>(emit, wsum, &total);
-            if (total) {
+            const uint64_t em = __ballot(emit);
+            if (em) {
+                const uint32_t total = (uint32_t)__popcll(em);
                 if (staged + total > MSTAGE) {
-                    flush_stage(stage, staged, &kb_sh, ma.end_time_min, mt, cnt);
+                    __builtin_amdgcn_wave_barrier();
+                    flush_stage(stage, staged, ma.end_time_min, mt, cnt, lane);
                     staged = 0;
                 }
-                if (emit) stage[staged + pos] = ob;
+                if (emit) stage[staged + (uint32_t)__popcll(em & lt)] = ob;
                 staged += total;
-                __syncthreads();
             }
         }
-        __syncthreads();
+        __builtin_amdgcn_wave_barrier(); /* the wave's LDS is reused by its next tile */
     }
-    if (staged) flush_stage(stage, staged, &kb_sh, ma.end_time_min, mt, cnt);
+    if (staged) {
+        __builtin_amdgcn_wave_barrier();
+        flush_stage(stage, staged, ma.end_time_min, mt, cnt, lane);
+    }
     n_live = wave_sum_u64(n_live);
     n_exp = wave_sum_u64(n_exp);
     n_fresh = wave_sum_u64(n_fresh);
@@ -1367,6 +1464,8 @@ struct StcCtx {
     DBuf<uint32_t> pt_leaf, pt_lv, pt_rank, lp, lkeys;
     DBuf<uint32_t> lleaf;                /* leaf of each point, in leaf order */
     DBuf<uint2> lunits;                  /* first cross unit and count of each point, in leaf order */
+    DBuf<uint2> urange;                  /* per cross unit: member range above its point */
+    DBuf<uint4> prange;                  /* per point in leaf order: own-leaf and reverse-list ranges */
     DBuf<uint32_t> ucount, uoffset;
     DBuf<uint2> units;
     DBuf<uint2> amb;                     /* band combinations (probe, other) for the host */
@@ -1385,7 +1484,7 @@ struct StcCtx {
     uint64_t mlen = 0;                   /* buckets of meet[cur] (live and tombstones) */
     uint64_t plen = 0;                   /* entries of pnew (the last step's pair slots) */
     DBuf<unsigned long long> acc;        /* band pairs the host accepted, sorted */
-    DBuf<uint32_t> spill_big, spill_huge;
+    DBuf<uint32_t> spill_big, spill_huge, spill_giant;
     DBuf<unsigned long long> gsrt;       /* global sort keys of very large segments */
     /* memtable */
     uint64_t *rec_keys = nullptr;
@@ -1582,10 +1681,10 @@ int stc_create(const stc_config *cfg, const stc_box *mbr, void **out) {
         c->idx_alt.ensure(n) || c->dlon.ensure(n) || c->idlon.ensure(n) || c->lx.ensure(n) || c->ly.ensure(n) || c->ld.ensure(n) ||
         c->nodes.ensure(node_cap) || c->frontier_a.ensure(front_cap) || c->frontier_b.ensure(front_cap) ||
         c->pt_leaf.ensure(n) || c->pt_lv.ensure(n) || c->pt_rank.ensure(n) || c->lp.ensure(n) ||
-        c->lleaf.ensure(n) || c->lunits.ensure(n) ||
+        c->lleaf.ensure(n) || c->lunits.ensure(n) || c->prange.ensure(n) ||
         c->lkeys.ensure(n) || c->ucount.ensure(n) || c->uoffset.ensure(n) || c->units.ensure(4ull * n) ||
         c->amb.ensure(1 << 16) || c->pcount.ensure(n) || c->moff[0].ensure(n + 1ull) || c->moff[1].ensure(n + 1ull) ||
-        c->spill_big.ensure(n) || c->spill_huge.ensure(n) ||
+        c->spill_big.ensure(n) || c->spill_huge.ensure(n) || c->spill_giant.ensure(n) ||
         c->rev_count.ensure(node_cap + 1) || c->rev_start.ensure(node_cap + 1) ||
         c->leaf_count.ensure(node_cap) || c->leaf_start.ensure(node_cap) || c->leaf_node.ensure(node_cap) ||
         c->temp.ensure(stc_radix_temp_words((uint32_t)std::max<size_t>(n, node_cap)) + 64))
@@ -1620,7 +1719,7 @@ int stc_destroy(void *vc) {
     c->temp.release(); c->dlon.release(); c->idlon.release(); c->lx.release(); c->ly.release(); c->ld.release(); c->nodes.release();
     c->frontier_a.release(); c->frontier_b.release(); c->leaf_count.release(); c->leaf_start.release();
     c->leaf_node.release(); c->pt_leaf.release(); c->pt_lv.release(); c->pt_rank.release(); c->lp.release();
-    c->lleaf.release(); c->lunits.release();
+    c->lleaf.release(); c->lunits.release(); c->urange.release(); c->prange.release();
     c->lkeys.release(); c->ucount.release(); c->uoffset.release(); c->units.release();
     c->amb.release();
     c->rev_leaf.release(); c->rev_pid.release(); c->rev_leaf_alt.release(); c->rev_pid_alt.release();
@@ -1629,7 +1728,7 @@ int stc_destroy(void *vc) {
     c->pcount.release(); c->pnew.release(); c->acc.release();
     for (int i = 0; i < 2; i++) { c->meet[i].release(); c->moff[i].release(); }
     c->ptmp.release(); c->tbounds.release();
-    c->spill_big.release(); c->spill_huge.release(); c->gsrt.release();
+    c->spill_big.release(); c->spill_huge.release(); c->spill_giant.release(); c->gsrt.release();
     c->units_in.release(); c->ukeys.release(); c->ukeys_alt.release();
     c->uidx.release(); c->uidx_alt.release(); c->utemp.release();
     if (c->rec_keys) hipFree(c->rec_keys);
@@ -1778,9 +1877,19 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
      * the band decided on the host, segment offsets, write */
     hipLaunchKernelGGL(k_units_lorder, dim3(blocks_for(n, 256)), dim3(256), 0, s, (const uint32_t *)c->lp.p,
                        (const uint32_t *)c->uoffset.p, (const uint32_t *)c->ucount.p, n, c->lunits.p);
+    if (c->urange.ensure(n_units + 1)) return fail(c, STC_E_CAPACITY, "unit ranges");
+    if (n_units)
+        hipLaunchKernelGGL(k_unit_ranges, dim3(blocks_for(n_units, 256)), dim3(256), 0, s, (const uint2 *)c->units.p,
+                           (uint32_t)n_units, (const uint32_t *)c->leaf_start.p, (const uint32_t *)c->leaf_count.p,
+                           (const uint32_t *)c->lp.p, c->buf, c->urange.p);
+    hipLaunchKernelGGL(k_point_ranges, dim3(blocks_for(n, 256)), dim3(256), 0, s, (const uint32_t *)c->lp.p,
+                       (const uint32_t *)c->lleaf.p, n, (const uint32_t *)c->leaf_start.p,
+                       (const uint32_t *)c->leaf_count.p, (const uint32_t *)c->rev_start.p,
+                       (const uint32_t *)c->rev_pid.p, c->buf, c->prange.p);
     CK(hipGetLastError());
     const PairGeo pg{xy, c->dlon.p, c->idlon.p, c->lx.p, c->ly.p, c->pt_leaf.p, c->pt_rank.p, c->leaf_start.p,
-                     c->leaf_count.p, c->lp.p, c->ld.p, c->lleaf.p, c->lunits.p, c->units.p, c->uoffset.p,
+                     c->leaf_count.p, c->lp.p, c->ld.p, c->lleaf.p, c->lunits.p, c->urange.p, c->prange.p,
+                     c->units.p, c->uoffset.p,
                      c->ucount.p, c->rev_start.p, c->rev_pid.p,
                      c->rev_xy.p, c->rev_dl.p, c->rev_il.p, n, c->buf, r, c->band, r * r * (1.0 - 1e-12),
                      r * r * (1.0 + 1e-12)};
@@ -1824,7 +1933,7 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
     if (c->ptmp.ensure(total + 1) || c->pnew.ensure(total + 1) || c->meet[nx].ensure(total + 1))
         return fail(c, STC_E_CAPACITY, "pair and meeting arrays");
     CK(hipMemsetAsync(&cnt->table_fail, 0, sizeof(uint32_t), s));
-    CK(hipMemsetAsync(&cnt->big, 0, 2 * sizeof(uint32_t), s)); /* big, huge */
+    CK(hipMemsetAsync(&cnt->big, 0, 3 * sizeof(uint32_t), s)); /* big, huge, giant */
     hipLaunchKernelGGL(k_pair_write, dim3(pgrid), dim3(256), 0, s, pg, (const uint32_t *)c->moff[nx].p,
                        (const uint32_t *)c->pcount.p, c->acc.p, (uint32_t)accepted.size(), c->ptmp.p, c->pnew.p,
                        c->spill_big.p, cnt);
@@ -1832,18 +1941,26 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
     CK(hipEventRecord(c->ev[7], s));
     CKRC(sync_counters(c));
     h = *c->hcnt;
-    if (h.big) { /* segments longer than the write pass sorts: by blocks, in LDS or in global scratch */
-        hipLaunchKernelGGL((k_seg_sort<256, SEG_CAP, false>), dim3(std::min<uint32_t>(h.big, 4096)), dim3(256), 0, s,
+    if (h.big) { /* segments longer than the write pass sorts: by waves, by blocks, in global scratch */
+        hipLaunchKernelGGL(k_seg_sort_wave, dim3(blocks_for((uint64_t)h.big * STC_WAVE, 256)), dim3(256), 0, s,
                            (const uint32_t *)c->spill_big.p, h.big, (const uint32_t *)c->moff[nx].p,
-                           (const PairHit *)c->ptmp.p, c->pnew.p, c->spill_huge.p, &cnt->huge,
-                           (unsigned long long *)nullptr, 0u, cnt);
+                           (const PairHit *)c->ptmp.p, c->pnew.p, c->spill_huge.p, &cnt->huge);
         CK(hipGetLastError());
         CKRC(sync_counters(c));
         h = *c->hcnt;
     }
     if (h.huge) {
-        std::vector<uint32_t> owners(h.huge), seg(2);
-        CK(hipMemcpy(owners.data(), c->spill_huge.p, 4ull * h.huge, hipMemcpyDeviceToHost));
+        hipLaunchKernelGGL((k_seg_sort<256, SEG_CAP, false>), dim3(std::min<uint32_t>(h.huge, 4096)), dim3(256), 0, s,
+                           (const uint32_t *)c->spill_huge.p, h.huge, (const uint32_t *)c->moff[nx].p,
+                           (const PairHit *)c->ptmp.p, c->pnew.p, c->spill_giant.p, &cnt->giant,
+                           (unsigned long long *)nullptr, 0u, cnt);
+        CK(hipGetLastError());
+        CKRC(sync_counters(c));
+        h = *c->hcnt;
+    }
+    if (h.giant) {
+        std::vector<uint32_t> owners(h.giant), seg(2);
+        CK(hipMemcpy(owners.data(), c->spill_giant.p, 4ull * h.giant, hipMemcpyDeviceToHost));
         uint32_t longest = 1;
         for (uint32_t o : owners) {
             CK(hipMemcpy(seg.data(), c->moff[nx].p + o, 8, hipMemcpyDeviceToHost));
@@ -1851,10 +1968,10 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
         }
         uint32_t gcap = 1;
         while (gcap < longest) gcap <<= 1;
-        const uint32_t hgrid = std::min<uint32_t>(h.huge, 64);
+        const uint32_t hgrid = std::min<uint32_t>(h.giant, 64);
         if (c->gsrt.ensure((size_t)gcap * hgrid)) return fail(c, STC_E_CAPACITY, "sort scratch of a large segment");
-        hipLaunchKernelGGL((k_seg_sort<256, 1, true>), dim3(hgrid), dim3(256), 0, s, (const uint32_t *)c->spill_huge.p,
-                           h.huge, (const uint32_t *)c->moff[nx].p, (const PairHit *)c->ptmp.p, c->pnew.p,
+        hipLaunchKernelGGL((k_seg_sort<256, 1, true>), dim3(hgrid), dim3(256), 0, s, (const uint32_t *)c->spill_giant.p,
+                           h.giant, (const uint32_t *)c->moff[nx].p, (const PairHit *)c->ptmp.p, c->pnew.p,
                            (uint32_t *)nullptr, (uint32_t *)nullptr, c->gsrt.p, gcap, cnt);
         CK(hipGetLastError());
     }
@@ -1888,7 +2005,7 @@ static int step_tail(StcCtx *c, uint32_t t, StepGeom &g, stc_step_out *out) {
                        (const PairHit *)c->pnew.p, c->plen, (const Bucket *)c->meet[cu].p,
                        (const uint32_t *)c->moff[cu].p, c->mlen, n_tiles, c->tbounds.p);
     CK(hipGetLastError());
-    const uint32_t mgrid = std::min<uint32_t>(n_tiles, 8192);
+    const uint32_t mgrid = std::max<uint32_t>(std::min<uint32_t>((n_tiles + 3) / 4, 16384), 1);
     for (int attempt = 0;; attempt++) {
         /* the merge rewrites the same buckets, so a re-run with more record room is exact */
         *c->hinit = before;
