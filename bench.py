@@ -332,8 +332,8 @@ def main():
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     for i, r in enumerate(outs):
-        print(f"[bench] step {args.warmup + i}: {r['ms_total']:.1f} ms (refine {r['ms_refine_kernel']:.1f}, "
-              f"identify {r['ms_identify_kernel']:.1f}, flush {r['ms_flush']:.1f}) "
+        print(f"[bench] step {args.warmup + i}: {r['ms_total']:.1f} ms (count {r['ms_count_kernel']:.1f}, "
+              f"write {r['ms_refine_kernel']:.1f}, merge {r['ms_identify_kernel']:.1f}, flush {r['ms_flush']:.1f}) "
               f"pairs {r['pairs']} records {r['new_records']} flushed {r['flushed']}", file=sys.stderr, flush=True)
 
     value = n * args.steps / elapsed
@@ -357,14 +357,17 @@ def main():
         return float(np.mean([r[k] for r in outs]))
 
     pairs = mean("pairs")
+    slots = mean("table_buckets")  # pair slots of a step = buckets of its meeting array
     kernels = {
-        # pair/refine (coords 16N + pids 4N in, pair keys 8P out) fused with the
-        # hash insert/extend (one 32 B bucket read and one 32 B write per pair)
-        "k_refine": ("pair refinement fused with the meeting-table insert/extend", mean("ms_refine_kernel"),
-                     20.0 * n + 72.0 * pairs),
-        # expire/identify: 32 B per bucket scanned, 2 records x 32 B per completed meeting
-        "k_identify": ("meeting expiry scan and emission", mean("ms_identify_kernel"),
-                       32.0 * mean("table_buckets") + 32.0 * mean("new_records")),
+        # count pass: the N points in (16 B position + 4 B id), a 4 B count per owner out
+        "k_pair_count": ("pair count pass, owner by owner", mean("ms_count_kernel"), 24.0 * n),
+        # write pass: the N points in, one 24 B slot (key + float MBR) per pair out
+        "k_pair_write": ("pair write pass: owner segments sorted by partner", mean("ms_refine_kernel"),
+                         20.0 * n + 24.0 * slots),
+        # merge: the step's slots (24 B) and the previous meetings (32 B) in, the new
+        # meetings (32 B) and 2 records (32 B each) per recorded meeting out
+        "k_meet_merge": ("merge with the previous step's meetings, expiry, records", mean("ms_identify_kernel"),
+                         24.0 * slots + 32.0 * slots + 32.0 * slots + 32.0 * mean("new_records")),
     }
     kname = max(kernels, key=lambda k: kernels[k][1])
     desc, ms, alg = kernels[kname]
@@ -372,7 +375,8 @@ def main():
     roof = dict(bound="hbm", kernel=f"{kname} ({desc})", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(achieved / HBM_PEAK_GBS, 4), traffic=measured_traffic(kname, args.workload, n),
                 alg_bytes_per_launch=alg, avg_ms=round(ms, 3),
-                kernels_ms={k: round(v[1], 3) for k, v in kernels.items()})
+                kernels={k: dict(ms=round(v[1], 3), alg_bytes=v[2], gbs=round(v[2] / (v[1] * 1e-3) / 1e9, 1))
+                         for k, v in kernels.items()})
 
     cpu = None
     if not args.no_cpu:

@@ -79,21 +79,22 @@ typedef struct stc_step_out {
     uint64_t overflow_points;  /* points beyond the leaf buffer (not stored as targets) */
     uint64_t ambiguous_pairs;  /* pair decisions re-evaluated on the host (distance within 16 ulp of r) */
     uint64_t ambiguous_nodes;  /* box-distance decisions within 16 ulp of r (decided on the device) */
-    uint64_t table_buckets;    /* meeting-table size */
-    uint64_t table_occupied;   /* buckets holding a key (live or retired) */
+    uint64_t table_buckets;    /* buckets of the meeting array: live meetings and tombstones */
+    uint64_t table_occupied;   /* the same (the array holds no free buckets) */
     uint32_t flushed;          /* 1 if this step flushed a run */
-    uint32_t rebuilt;          /* 1 if the meeting table was rehashed this step */
+    uint32_t rebuilt;          /* 0 (every step's merge writes a new meeting array) */
     float ms_partition;        /* QTree build + partition */
     float ms_filter;           /* border lookup */
-    float ms_refine;           /* pair refinement */
-    float ms_meetings;         /* table insert + identify */
+    float ms_refine;           /* pair refinement: reverse lists, count, write, segment sorts */
+    float ms_meetings;         /* merge with the previous meetings + records */
     float ms_flush;            /* flush (0 when no flush) */
     float ms_total;
-    float ms_refine_kernel;    /* the fused refinement + table-insert kernel (last launch) */
-    float ms_identify_kernel;  /* the identify kernel */
-    uint64_t continuing;       /* pairs that continued a meeting seen at the previous step (a pair found
-                                  twice in one step, an ulp sliver between leaves, may count twice) */
-    uint64_t new_keys;         /* pairs that opened a new table entry */
+    float ms_refine_kernel;    /* the pair write pass (owner segments, sorted) */
+    float ms_identify_kernel;  /* the merge with the previous meetings (last launch) */
+    uint64_t continuing;       /* pairs that continued a meeting of the previous step */
+    uint64_t new_keys;         /* pairs that opened a meeting */
+    float ms_count_kernel;     /* the pair count pass (last launch) */
+    float ms_reserved;
 } stc_step_out;
 
 /* CTF metadata of a flushed run: the fields of the reference's CTF

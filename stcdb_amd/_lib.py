@@ -71,6 +71,8 @@ class StcStepOut(C.Structure):
         ("ms_identify_kernel", C.c_float),
         ("continuing", C.c_uint64),
         ("new_keys", C.c_uint64),
+        ("ms_count_kernel", C.c_float),
+        ("ms_reserved", C.c_float),
     ]
 
     def as_dict(self):

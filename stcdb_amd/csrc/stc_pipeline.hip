@@ -1180,9 +1180,9 @@ __global__ __launch_bounds__(THREADS) void k_seg_sort(const uint32_t *__restrict
     }
 }
 
-#define MTILE 256u   /* new pairs per merge tile (one wave) */
-#define OCAP 512u    /* previous meetings of a tile held in the wave's LDS */
-#define MSTAGE 64u   /* expired meetings a merge wave stages before reserving their records */
+#define MTILE 128u   /* new pairs per merge tile (one wave) */
+#define OCAP 256u    /* previous meetings of a tile held whole in the wave's LDS */
+#define MSTAGE 32u   /* expired meetings a merge wave stages before reserving their records */
 
 /* write the wave's staged meetings' records: one counter reservation per batch */
 __device__ inline void flush_stage(const Bucket *stage, uint32_t staged, uint32_t end_time_min, const MemtableDev &mt,
@@ -1239,22 +1239,24 @@ __device__ inline uint32_t lower_bound_bucket(const Bucket *s, uint32_t n, unsig
 
 /* Merge of the step's pairs (pn, sorted by key, dead slots behind each owner's
  * live pairs) with the previous step's meetings (mold, sorted the same way),
- * tile by tile, one wave per tile: the tile's new keys and its range of
- * previous keys (tb) are staged in the wave's LDS. A live pair continues the
- * previous meeting of its key (start kept, MBR widened) or opens one
- * (insert/extend, geometry.cu:469-494); a dead slot becomes a tombstone; a
- * previous meeting whose key has no pair expires and, when it lasted long
- * enough inside the map, becomes two records (identify, geometry.cu:595-737).
- * mnew[i] is built from pn[i]. */
+ * tile by tile, one wave per tile: the tile's range of previous meetings (tb)
+ * is staged whole in the wave's LDS, then each new slot is read once and
+ * joined with it. A live pair continues the previous meeting of its key
+ * (start kept, MBR widened) or opens one (insert/extend, geometry.cu:469-494);
+ * a dead slot becomes a tombstone; a previous meeting whose key has no pair
+ * expires and, when it lasted long enough inside the map, becomes two records
+ * (identify, geometry.cu:595-737). mnew[i] is built from pn[i]. A range
+ * longer than OCAP is searched in global memory. */
 __global__ __launch_bounds__(256) void k_meet_merge(const PairHit *__restrict__ pn, uint64_t n_p,
                                                     const Bucket *__restrict__ mold, const uint32_t *__restrict__ tb,
                                                     uint32_t n_tiles, Bucket *__restrict__ mnew, MergeArgs ma,
                                                     MemtableDev mt, Counters *cnt) {
     __shared__ unsigned long long wpk[256 / STC_WAVE][MTILE];
-    __shared__ unsigned long long wok[256 / STC_WAVE][OCAP];
+    __shared__ Bucket wob[256 / STC_WAVE][OCAP];
     __shared__ Bucket wstage[256 / STC_WAVE][MSTAGE];
     const int lane = threadIdx.x & (STC_WAVE - 1), wv = threadIdx.x / STC_WAVE;
-    unsigned long long *pk = wpk[wv], *ok = wok[wv];
+    unsigned long long *pk = wpk[wv];
+    Bucket *ob_l = wob[wv];
     Bucket *stage = wstage[wv];
     const uint64_t lt = lanemask_lt(lane);
     const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / STC_WAVE;
@@ -1266,12 +1268,14 @@ __global__ __launch_bounds__(256) void k_meet_merge(const PairHit *__restrict__ 
         const uint32_t np = i0 < n_p ? (uint32_t)min<uint64_t>(MTILE, n_p - i0) : 0u;
         const uint32_t o0 = tb[t], no = tb[t + 1] - o0;
         const bool oin = no <= OCAP;
-        for (uint32_t i = lane; i < np; i += STC_WAVE) pk[i] = pn[i0 + i].key;
+        const Bucket *obk = oin ? ob_l : mold + o0; /* the tile's previous meetings */
         if (oin)
-            for (uint32_t j = lane; j < no; j += STC_WAVE) ok[j] = mold[o0 + j].key;
+            for (uint32_t j = lane; j < no; j += STC_WAVE) ob_l[j] = mold[o0 + j];
         __builtin_amdgcn_wave_barrier();
         for (uint32_t i = lane; i < np; i += STC_WAVE) {
-            const unsigned long long key = pk[i];
+            const PairHit e = pn[i0 + i];
+            const unsigned long long key = e.key;
+            pk[i] = key;
             Bucket nb;
             nb.key = key;
             if ((uint32_t)key == DEAD_B) {
@@ -1279,12 +1283,11 @@ __global__ __launch_bounds__(256) void k_meet_merge(const PairHit *__restrict__ 
                 nb.end = 0xFFFFFFFFu;
                 nb.mbr[0] = nb.mbr[1] = nb.mbr[2] = nb.mbr[3] = 0;
             } else {
-                const PairHit e = pn[i0 + i];
                 const uint32_t q0 = f2o(e.mbr[0]), q1 = f2o(e.mbr[1]), q2 = f2o(e.mbr[2]), q3 = f2o(e.mbr[3]);
                 nb.end = ma.now;
-                const uint32_t j = oin ? lower_bound_u64(ok, no, key) : lower_bound_bucket(mold + o0, no, key);
-                if (j < no && (oin ? ok[j] : mold[o0 + j].key) == key) {
-                    const Bucket ob = mold[o0 + j];
+                const uint32_t j = lower_bound_bucket(obk, no, key);
+                if (j < no && obk[j].key == key) {
+                    const Bucket ob = obk[j];
                     nb.start = ob.start;
                     nb.mbr[0] = min(ob.mbr[0], q0);
                     nb.mbr[1] = min(ob.mbr[1], q1);
@@ -1300,17 +1303,18 @@ __global__ __launch_bounds__(256) void k_meet_merge(const PairHit *__restrict__ 
             }
             mnew[i0 + i] = nb;
         }
+        __builtin_amdgcn_wave_barrier();
         for (uint32_t j0 = 0; j0 < no; j0 += STC_WAVE) {
             const uint32_t j = j0 + lane;
             bool emit = false;
             Bucket ob;
             if (j < no) {
-                const unsigned long long key = oin ? ok[j] : mold[o0 + j].key;
+                const unsigned long long key = obk[j].key;
                 if ((uint32_t)key != DEAD_B) {
                     const uint32_t k = lower_bound_u64(pk, np, key);
                     if (!(k < np && pk[k] == key)) {
                         n_exp++;
-                        ob = mold[o0 + j];
+                        ob = obk[j];
                         emit = meeting_emits(ob, ma);
                     }
                 }
@@ -1323,8 +1327,18 @@ __global__ __launch_bounds__(256) void k_meet_merge(const PairHit *__restrict__ 
                     flush_stage(stage, staged, ma.end_time_min, mt, cnt, lane);
                     staged = 0;
                 }
-                if (emit) stage[staged + (uint32_t)__popcll(em & lt)] = ob;
-                staged += total;
+                const uint32_t put = staged + (uint32_t)__popcll(em & lt);
+                if (emit && put < MSTAGE) stage[put] = ob;
+                /* a batch beyond the stage (more than MSTAGE at once) is written directly */
+                if (total > MSTAGE) {
+                    unsigned long long kb = 0;
+                    if (lane == 0) kb = atomicAdd(&cnt->kv_count, 2ull * (total - MSTAGE));
+                    kb = __shfl(kb, 0, STC_WAVE);
+                    if (emit && put >= MSTAGE) write_meeting_records(ob, kb + 2ull * (put - MSTAGE), ma.end_time_min, mt);
+                    staged = MSTAGE;
+                } else {
+                    staged += total;
+                }
             }
         }
         __builtin_amdgcn_wave_barrier(); /* the wave's LDS is reused by its next tile */
@@ -1511,7 +1525,7 @@ struct StcCtx {
     PinnedPool pinned;
     QueryBufs *qb = nullptr;
     FlushWork flushw;
-    hipEvent_t ev[10];
+    hipEvent_t ev[12];
 };
 
 static int fail(StcCtx *c, int code, const std::string &msg) {
@@ -1673,7 +1687,7 @@ int stc_create(const stc_config *cfg, const stc_box *mbr, void **out) {
         return STC_E_HIP;
     };
     if (hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking) != hipSuccess) return bad("stream");
-    for (int i = 0; i < 10; i++) hipEventCreate(&c->ev[i]);
+    for (int i = 0; i < 12; i++) hipEventCreate(&c->ev[i]);
     uint32_t n = c->n;
     size_t node_cap = 1 + 4ull * NLEVELS * (n / (c->cap + 1) + 1);
     size_t front_cap = n / (c->cap + 1) + 2;
@@ -1748,7 +1762,7 @@ int stc_destroy(void *vc) {
     c->pinned.clear();
     stc_query_bufs_free(c->qb);
     c->qb = nullptr;
-    for (int i = 0; i < 10; i++) if (c->ev[i]) hipEventDestroy(c->ev[i]);
+    for (int i = 0; i < 12; i++) if (c->ev[i]) hipEventDestroy(c->ev[i]);
     if (c->s) hipStreamDestroy(c->s);
     delete c;
     return STC_OK;
@@ -1894,12 +1908,13 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
                      c->rev_xy.p, c->rev_dl.p, c->rev_il.p, n, c->buf, r, c->band, r * r * (1.0 - 1e-12),
                      r * r * (1.0 + 1e-12)};
     const uint32_t pgrid = (uint32_t)std::min<uint64_t>(blocks_for((uint64_t)n * STC_WAVE, 256), 256u * 64u);
-    CK(hipEventRecord(c->ev[6], s));
     for (int attempt = 0;; attempt++) {
         CK(hipMemsetAsync(&cnt->candidates, 0, 3 * sizeof(unsigned long long), s)); /* candidates, pairs, ambiguous */
+        CK(hipEventRecord(c->ev[10], s));
         hipLaunchKernelGGL(k_pair_count, dim3(pgrid), dim3(256), 0, s, pg, c->pcount.p, c->amb.p,
                            (uint32_t)std::min<size_t>(c->amb.cap, 0xFFFFFFFFu), cnt);
         CK(hipGetLastError());
+        CK(hipEventRecord(c->ev[11], s));
         CKRC(sync_counters(c));
         h = *c->hcnt;
         if (h.ambiguous <= c->amb.cap) break;
@@ -1934,6 +1949,7 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
         return fail(c, STC_E_CAPACITY, "pair and meeting arrays");
     CK(hipMemsetAsync(&cnt->table_fail, 0, sizeof(uint32_t), s));
     CK(hipMemsetAsync(&cnt->big, 0, 3 * sizeof(uint32_t), s)); /* big, huge, giant */
+    CK(hipEventRecord(c->ev[6], s));
     hipLaunchKernelGGL(k_pair_write, dim3(pgrid), dim3(256), 0, s, pg, (const uint32_t *)c->moff[nx].p,
                        (const uint32_t *)c->pcount.p, c->acc.p, (uint32_t)accepted.size(), c->ptmp.p, c->pnew.p,
                        c->spill_big.p, cnt);
@@ -2128,6 +2144,7 @@ static int step_tail(StcCtx *c, uint32_t t, StepGeom &g, stc_step_out *out) {
         hipEventElapsedTime(&out->ms_identify_kernel, c->ev[8], c->ev[9]);
         out->continuing = h.continuing;
         out->new_keys = h.new_keys;
+        if (g.units || g.nodes) hipEventElapsedTime(&out->ms_count_kernel, c->ev[10], c->ev[11]);
     }
     return STC_OK;
 }
