@@ -94,7 +94,7 @@ def measured_traffic(kernel, workload, n):
     """HBM bytes per launch of a kernel, from the committed rocprofv3 counter
     summary for this workload (FETCH_SIZE and WRITE_SIZE, calibrated with a copy
     of known size); None when no summary covers it."""
-    path = os.path.join(PROFILES, f"{kernel}_{workload}_traffic.md")
+    path = os.path.join(PROFILES, f"pair_merge_{workload}_traffic.md")
     if not os.path.exists(path):
         return None
     vals = {}
@@ -105,7 +105,7 @@ def measured_traffic(kernel, workload, n):
     try:
         if int(vals["objects"]) != n:
             return None
-        return int(vals["traffic_bytes_per_launch"])
+        return int(vals[f"{kernel}.traffic_bytes_per_launch"])
     except (KeyError, ValueError):
         return None
 
@@ -375,7 +375,8 @@ def main():
     roof = dict(bound="hbm", kernel=f"{kname} ({desc})", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(achieved / HBM_PEAK_GBS, 4), traffic=measured_traffic(kname, args.workload, n),
                 alg_bytes_per_launch=alg, avg_ms=round(ms, 3),
-                kernels={k: dict(ms=round(v[1], 3), alg_bytes=v[2], gbs=round(v[2] / (v[1] * 1e-3) / 1e9, 1))
+                kernels={k: dict(ms=round(v[1], 3), alg_bytes=v[2], gbs=round(v[2] / (v[1] * 1e-3) / 1e9, 1),
+                                 traffic=measured_traffic(k, args.workload, n))
                          for k, v in kernels.items()})
 
     cpu = None
