@@ -617,8 +617,8 @@ __device__ inline void owner_combos(uint32_t w, uint32_t a, const PairGeo &g, in
         const uint32_t c = c0 + lane;
         const bool v = c < total;
         uint32_t j = 0;
-        for (uint32_t k = 1; k < nseg; k++)
-            if (__shfl(scum, (int)k, STC_WAVE) <= c) j = k;
+        for (uint32_t k = 1; k < nseg; k++) /* k is wave-uniform: a lane read, no exchange */
+            if ((uint32_t)__builtin_amdgcn_readlane((int)scum, (int)k) <= c) j = k;
         const uint32_t jb = __shfl(sbase, (int)j, STC_WAVE), jc = __shfl(scum, (int)j, STC_WAVE);
         int dec = 0;
         uint32_t q = 0, probe = a, src = 0;
@@ -760,37 +760,70 @@ __device__ inline bool sorted_contains(const unsigned long long *a, uint32_t n, 
 __device__ inline uint64_t lanemask_lt(int lane) { return lane == 0 ? 0ull : (~0ull >> (STC_WAVE - lane)); }
 
 
-/* ascending bitonic sort of 64 * K keys held by a wave, element r * 64 + lane
- * in x[r] of the lane: stages across lanes exchange through shuffles, stages
- * across registers swap within the lane */
-template <int K>
-__device__ inline void reg_bitonic(unsigned long long (&x)[K], int lane) {
+/* the value that lane (lane ^ J) holds, J a power of two below 64: quad
+ * permutes (DPP) for J = 1, 2, the bit-mask swizzle within 32 lanes for
+ * J = 4 .. 16, a lane permute for J = 32 */
+template <int J>
+__device__ inline uint32_t xor_lane32(uint32_t v, int lane) {
+    static_assert(J >= 1 && J < STC_WAVE && (J & (J - 1)) == 0, "lane distance");
+    if constexpr (J == 1) return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, false); /* [1,0,3,2] */
+    else if constexpr (J == 2) return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xF, 0xF, false); /* [2,3,0,1] */
+    else if constexpr (J < 32) return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x1F | (J << 10));
+    else return (uint32_t)__builtin_amdgcn_ds_bpermute((lane ^ J) << 2, (int)v);
+}
+
+template <int J>
+__device__ inline unsigned long long xor_lane(unsigned long long v, int lane) {
+    const uint32_t lo = xor_lane32<J>((uint32_t)v, lane), hi = xor_lane32<J>((uint32_t)(v >> 32), lane);
+    return ((unsigned long long)hi << 32) | lo;
+}
+
+/* one compare-exchange step of a bitonic stage k at lane distance J: each
+ * lane keeps the smaller key when its block of k ascends and it is the lower
+ * lane of its pair (or neither), the larger otherwise */
+template <int K, int k, int J>
+__device__ inline void bitonic_lane_step(unsigned long long (&x)[K], int lane) {
 #pragma unroll
-    for (int k = 2; k <= STC_WAVE * K; k <<= 1) {
+    for (int r = 0; r < K; r++) {
+        const unsigned long long y = xor_lane<J>(x[r], lane);
+        const bool keep_min = (((r * STC_WAVE + lane) & k) == 0) == ((lane & J) == 0);
+        const bool le = x[r] <= y;
+        x[r] = le == keep_min ? x[r] : y;
+    }
+}
+
+template <int K, int k, int J>
+__device__ inline void bitonic_stage_steps(unsigned long long (&x)[K], int lane) {
+    if constexpr (J >= STC_WAVE) { /* across registers of the lane */
 #pragma unroll
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            if (j >= STC_WAVE) {
-#pragma unroll
-                for (int r = 0; r < K; r++) {
-                    const int r2 = r ^ (j / STC_WAVE);
-                    if (r2 > r) {
-                        const bool asc = (((r * STC_WAVE + lane) & k) == 0);
-                        const unsigned long long u = x[r], v = x[r2];
-                        const bool sw = asc ? (u > v) : (u < v);
-                        x[r] = sw ? v : u;
-                        x[r2] = sw ? u : v;
-                    }
-                }
-            } else {
-#pragma unroll
-                for (int r = 0; r < K; r++) {
-                    const unsigned long long y = __shfl_xor(x[r], j, STC_WAVE);
-                    const bool asc = (((r * STC_WAVE + lane) & k) == 0), lower = (lane & j) == 0;
-                    x[r] = (asc == lower) ? (x[r] < y ? x[r] : y) : (x[r] < y ? y : x[r]);
-                }
+        for (int r = 0; r < K; r++) {
+            const int r2 = r ^ (J / STC_WAVE);
+            if (r2 > r) {
+                const bool asc = ((r * STC_WAVE) & k) == 0;
+                const unsigned long long u = x[r], v = x[r2];
+                const bool sw = asc ? (u > v) : (u < v);
+                x[r] = sw ? v : u;
+                x[r2] = sw ? u : v;
             }
         }
+    } else {
+        bitonic_lane_step<K, k, J>(x, lane);
     }
+    if constexpr (J > 1) bitonic_stage_steps<K, k, J / 2>(x, lane);
+}
+
+template <int K, int k>
+__device__ inline void bitonic_stages(unsigned long long (&x)[K], int lane) {
+    bitonic_stage_steps<K, k, k / 2>(x, lane);
+    if constexpr (k < STC_WAVE * K) bitonic_stages<K, 2 * k>(x, lane);
+}
+
+/* ascending bitonic sort of 64 * K keys held by a wave, element r * 64 + lane
+ * in x[r] of the lane: stages across lanes exchange through DPP, swizzles and
+ * permutes, stages across registers swap within the lane */
+template <int K>
+__device__ inline void reg_bitonic(unsigned long long (&x)[K], int lane) {
+    bitonic_stages<K, 2>(x, lane);
 }
 
 /* Sort an owner's slots (keys[0, slots), slots <= 64 * K) in registers and
@@ -1053,21 +1086,30 @@ __device__ inline void write_meeting_records(const Bucket &b, unsigned long long
 __global__ void k_records_account(const uint64_t *__restrict__ keys, const float4 *__restrict__ boxes, uint64_t from,
                                   uint64_t to, uint32_t max_meet, MemtableDev mt, Counters *cnt) {
     const uint64_t i = from + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= to) return;
-    const uint64_t lo = keys[2 * i], hi = keys[2 * i + 1];
-    const uint32_t a = (uint32_t)(hi >> 26) & 0x3FFFFFFu, c = (uint32_t)hi & 0x3FFFFFFu;
-    const uint32_t duration = (uint32_t)(lo >> 32);
-    if (((i - from) & 1) == 0 && duration > max_meet) {
-        atomicAdd(&cnt->oversize_oid_count, 1u);
-        mt.sids[a] = 1;
-        mt.sids[c] = 1;
+    const int lane = threadIdx.x & (STC_WAVE - 1);
+    bool oversize = false, first = false;
+    if (i < to) {
+        const uint64_t lo = keys[2 * i], hi = keys[2 * i + 1];
+        const uint32_t a = (uint32_t)(hi >> 26) & 0x3FFFFFFu, c = (uint32_t)hi & 0x3FFFFFFu;
+        const uint32_t duration = (uint32_t)(lo >> 32);
+        if (((i - from) & 1) == 0 && duration > max_meet) {
+            oversize = true;
+            mt.sids[a] = 1;
+            mt.sids[c] = 1;
+        }
+        const float4 b = boxes[i];
+        first = atomicAdd(&mt.same_pid_count[a], 1u) == 0;
+        atomicMin(&mt.o_boxs[a].x, f2q(b.x));
+        atomicMax(&mt.o_boxs[a].z, f2q(b.z));
+        atomicMin(&mt.o_boxs[a].y, f2q(b.y));
+        atomicMax(&mt.o_boxs[a].w, f2q(b.w));
     }
-    const float4 b = boxes[i];
-    if (atomicAdd(&mt.same_pid_count[a], 1u) == 0) atomicAdd(&cnt->sid_count, 1u);
-    atomicMin(&mt.o_boxs[a].x, f2q(b.x));
-    atomicMax(&mt.o_boxs[a].z, f2q(b.z));
-    atomicMin(&mt.o_boxs[a].y, f2q(b.y));
-    atomicMax(&mt.o_boxs[a].w, f2q(b.w));
+    /* one counter update per wave: a counter updated by every thread serialises on its address */
+    const uint64_t fm = __ballot(first), om = __ballot(oversize);
+    if (lane == 0) {
+        if (fm) atomicAdd(&cnt->sid_count, (uint32_t)__popcll(fm));
+        if (om) atomicAdd(&cnt->oversize_oid_count, (uint32_t)__popcll(om));
+    }
 }
 
 /* ascending bitonic sort of s[0, P) (P a power of two) by a block */
@@ -1266,7 +1308,12 @@ __global__ __launch_bounds__(256) void k_meet_merge(const PairHit *__restrict__ 
     for (uint32_t t = wid; t < n_tiles; t += nw) {
         const uint64_t i0 = (uint64_t)t * MTILE;
         const uint32_t np = i0 < n_p ? (uint32_t)min<uint64_t>(MTILE, n_p - i0) : 0u;
-        const uint32_t o0 = tb[t], no = tb[t + 1] - o0;
+        const uint32_t o0 = tb[t], o1 = tb[t + 1];
+        if (o1 < o0) { /* the pairs were not sorted by key: report it, read nothing */
+            if (lane == 0) atomicOr(&cnt->table_fail, 2u);
+            continue;
+        }
+        const uint32_t no = o1 - o0;
         const bool oin = no <= OCAP;
         const Bucket *obk = oin ? ob_l : mold + o0; /* the tile's previous meetings */
         if (oin)
@@ -2038,6 +2085,7 @@ static int step_tail(StcCtx *c, uint32_t t, StepGeom &g, stc_step_out *out) {
         CK(hipEventRecord(c->ev[9], s));
         CKRC(sync_counters(c));
         h = *c->hcnt;
+        if (h.table_fail) return fail(c, STC_E_HIP, "merge: the step's pairs are not sorted by key");
         if (h.kv_count <= c->rec_cap) break;
         if (attempt >= 2 || ensure_records(c, h.kv_count + h.kv_count / 4 + 2))
             return fail(c, STC_E_CAPACITY, "record buffer allocation failed");

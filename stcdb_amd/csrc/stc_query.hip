@@ -131,7 +131,14 @@ __device__ inline void q_emit(const QDev &Q, unsigned int *lc, uint32_t qi, uint
     if (Q.lds) atomicAdd(&lc[qi], 1u);
     else atomicAdd(&Q.counts[qi], 1ull);
     if (!Q.cap) return;
-    unsigned long long k = atomicAdd(Q.n_hits, 1ull);
+    /* one reservation per call for all lanes that emit together: a counter
+     * updated by every hit serialises on its address */
+    const uint64_t act = __ballot(1);
+    const int lane = threadIdx.x & (STC_WAVE - 1), leader = __ffsll((long long)act) - 1;
+    unsigned long long base = 0;
+    if (lane == leader) base = atomicAdd(Q.n_hits, (unsigned long long)__popcll(act));
+    base = __shfl(base, leader, STC_WAVE);
+    const unsigned long long k = base + (unsigned long long)__popcll(act & ((1ull << lane) - 1ull));
     if (k < Q.cap) {
         stc_hit h;
         h.query = qi;
