@@ -49,9 +49,10 @@ struct FlushArgs {
     uint64_t kv_count;
     uint32_t kv_restriction;
     uint32_t end_time_min, cur_time, max_meet_time;
-    uint32_t sid_count, oversize_oid_count;
+    uint32_t oversize_oid_count;
     uint32_t compat;
     uint32_t keep_sorted;    /* copy the sorted CTF records back to the host */
+    uint32_t reset_memtable;  /* clear the per-object memtable state once the run is built, before its copies start */
     uint32_t sticky_oversize; /* compaction: objects entering with sids 1 stay oversize (compaction.cpp:259-268) */
     double area_threshold;
     double map[4];

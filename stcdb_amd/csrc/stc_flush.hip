@@ -157,6 +157,17 @@ __global__ void k_set_oid(uint32_t n, const uint32_t *__restrict__ same_pid_coun
     part_tail[id] = (sids[id] == 1) ? 1 : 0; /* partition-time oversize flag (geometry.cu:1777-1780) */
 }
 
+/* sid_count: the objects with at least one record (geometry.cu:726-729 counts
+ * them as their first record arrives) */
+__global__ __launch_bounds__(256) void k_count_sids(const uint32_t *__restrict__ same_pid_count, uint32_t n,
+                                                    FlushCounters *fc) {
+    uint32_t mine = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        mine += same_pid_count[i] ? 1u : 0u;
+    for (int o = STC_WAVE / 2; o > 0; o >>= 1) mine += __shfl_xor(mine, o, STC_WAVE);
+    if ((threadIdx.x & (STC_WAVE - 1)) == 0 && mine) atomicAdd(&fc->sid_count, mine);
+}
+
 __global__ void k_str_prepare(FlushCounters *fc, uint32_t n, uint32_t split) {
     if (threadIdx.x || blockIdx.x) return;
     uint32_t z = (n - fc->sid_count) + fc->oversize_oid_count;
@@ -598,8 +609,10 @@ int stc_flush_run(FlushWork *w, const FlushArgs *a, FlushOut *out, hipStream_t s
     FlushCounters init;
     memset(&init, 0, sizeof(init));
     init.oversize_oid_count = a->oversize_oid_count;
-    init.sid_count = a->sid_count;
+    init.sid_count = 0; /* counted below */
     FCK(hipMemcpyAsync(fc, &init, sizeof(init), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_count_sids, dim3(std::max<uint32_t>(std::min<uint32_t>(blocks_for(n, 256), 1024), 1)),
+                       dim3(256), 0, s, a->same_pid_count, n, fc);
     FCK(hipMemsetAsync(w->capacity, 0, (C + 1) * 4, s));
     FCK(hipMemsetAsync(w->bitmaps, 0, (size_t)C * 8192, s));
     const uint32_t B = 256;
@@ -710,6 +723,11 @@ int stc_flush_run(FlushWork *w, const FlushArgs *a, FlushOut *out, hipStream_t s
     out->blk = out->acquire(out->acquire_ctx, L.total);
     if (!out->blk.p) return stc_hip_fail(hipErrorOutOfMemory, "pinned run buffer", __FILE__, __LINE__);
     uint8_t *dst = out->blk.p;
+    if (a->reset_memtable) { /* geometry.cu:2111-2131; ahead of the host copies, which slow concurrent writes */
+        FCK(hipMemsetAsync(a->same_pid_count, 0, 4ull * n, s));
+        FCK(hipMemsetAsync(a->sids, 0, 2ull * n, s));
+        hipLaunchKernelGGL(k_fill_obox2, dim3(blocks_for(n, 256)), dim3(256), 0, s, a->o_boxs, n);
+    }
     FCK(hipEventRecord(w->ready, s));
     FCK(hipStreamWaitEvent(w->cs, w->ready, 0));
     if (byte_base[C]) FCK(hipMemcpyAsync(dst + L.key_bytes, w->key_bytes, byte_base[C], hipMemcpyDeviceToHost, w->cs));

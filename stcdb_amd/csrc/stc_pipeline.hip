@@ -100,7 +100,7 @@ struct Counters {             /* device-side counters, copied back once per sync
     uint32_t bad_points;
     uint32_t border;
     uint32_t overflow;
-    uint32_t sid_count, oversize_oid_count;
+    uint32_t oversize_oid_count;
     uint32_t table_fail;
     uint32_t big, huge, giant;         /* segments left to the wave sort, the block sort, the global-scratch sort */
 };
@@ -1081,13 +1081,16 @@ __device__ inline void write_meeting_records(const Bucket &b, unsigned long long
 }
 
 /* per-object memtable state of the records [from, to) (geometry.cu:705-733):
- * record counts, objects with records, quantised MBRs, and oversize meetings
- * (one per record pair) marking both objects */
+ * record counts, quantised MBRs, and oversize meetings (one per record pair)
+ * marking both objects. The objects with records are counted by the flush.
+ * A box bound is updated only when the record moves it past the value read
+ * first: within a memtable the bounds only shrink (low) or grow (high), so a
+ * value read before another thread's update still proves an update redundant. */
 __global__ void k_records_account(const uint64_t *__restrict__ keys, const float4 *__restrict__ boxes, uint64_t from,
                                   uint64_t to, uint32_t max_meet, MemtableDev mt, Counters *cnt) {
     const uint64_t i = from + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & (STC_WAVE - 1);
-    bool oversize = false, first = false;
+    bool oversize = false;
     if (i < to) {
         const uint64_t lo = keys[2 * i], hi = keys[2 * i + 1];
         const uint32_t a = (uint32_t)(hi >> 26) & 0x3FFFFFFu, c = (uint32_t)hi & 0x3FFFFFFu;
@@ -1098,18 +1101,17 @@ __global__ void k_records_account(const uint64_t *__restrict__ keys, const float
             mt.sids[c] = 1;
         }
         const float4 b = boxes[i];
-        first = atomicAdd(&mt.same_pid_count[a], 1u) == 0;
-        atomicMin(&mt.o_boxs[a].x, f2q(b.x));
-        atomicMax(&mt.o_boxs[a].z, f2q(b.z));
-        atomicMin(&mt.o_boxs[a].y, f2q(b.y));
-        atomicMax(&mt.o_boxs[a].w, f2q(b.w));
+        atomicAdd(&mt.same_pid_count[a], 1u);
+        const uint4 cur = mt.o_boxs[a];
+        const uint32_t q0 = f2q(b.x), q1 = f2q(b.y), q2 = f2q(b.z), q3 = f2q(b.w);
+        if (q0 < cur.x) atomicMin(&mt.o_boxs[a].x, q0);
+        if (q2 > cur.z) atomicMax(&mt.o_boxs[a].z, q2);
+        if (q1 < cur.y) atomicMin(&mt.o_boxs[a].y, q1);
+        if (q3 > cur.w) atomicMax(&mt.o_boxs[a].w, q3);
     }
     /* one counter update per wave: a counter updated by every thread serialises on its address */
-    const uint64_t fm = __ballot(first), om = __ballot(oversize);
-    if (lane == 0) {
-        if (fm) atomicAdd(&cnt->sid_count, (uint32_t)__popcll(fm));
-        if (om) atomicAdd(&cnt->oversize_oid_count, (uint32_t)__popcll(om));
-    }
+    const uint64_t om = __ballot(oversize);
+    if (lane == 0 && om) atomicAdd(&cnt->oversize_oid_count, (uint32_t)__popcll(om));
 }
 
 /* ascending bitonic sort of s[0, P) (P a power of two) by a block */
@@ -1557,7 +1559,7 @@ struct StcCtx {
     Counters *dcnt = nullptr;
     Counters *hcnt = nullptr;  /* pinned */
     Counters *hinit = nullptr; /* pinned: initial counter values of a step */
-    uint32_t sid_count = 0, oversize_oid_count = 0;
+    uint32_t oversize_oid_count = 0;
     uint32_t end_time_min = 0;
     uint64_t kv_count = 0;
     uint64_t last_live = 0;
@@ -1840,7 +1842,6 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
     Counters *cnt = c->dcnt;
     memset(c->hinit, 0, sizeof(Counters));
     c->hinit->kv_count = c->kv_count;
-    c->hinit->sid_count = c->sid_count;
     c->hinit->oversize_oid_count = c->oversize_oid_count;
     CK(hipMemcpyAsync(cnt, c->hinit, sizeof(Counters), hipMemcpyHostToDevice, s));
     CK(hipEventRecord(c->ev[0], s));
@@ -2101,7 +2102,6 @@ static int step_tail(StcCtx *c, uint32_t t, StepGeom &g, stc_step_out *out) {
         h = *c->hcnt;
     }
     c->kv_count = h.kv_count;
-    c->sid_count = h.sid_count;
     c->oversize_oid_count = h.oversize_oid_count;
     c->last_live = h.live;
     c->last_expired = h.expired;
@@ -2126,13 +2126,13 @@ static int step_tail(StcCtx *c, uint32_t t, StepGeom &g, stc_step_out *out) {
         fa.compat = c->cfg.compat;
         fa.keep_sorted = (c->cfg.keep_pairs & 2) ? 1 : 0;
         fa.map[0] = c->map.lo0; fa.map[1] = c->map.lo1; fa.map[2] = c->map.hi0; fa.map[3] = c->map.hi1;
-        fa.sid_count = c->sid_count;
         fa.oversize_oid_count = c->oversize_oid_count;
         fa.rec_keys = c->rec_keys;
         fa.rec_boxes = c->rec_boxes;
         fa.same_pid_count = c->same_pid_count;
         fa.o_boxs = c->o_boxs;
         fa.sids = c->sids;
+        fa.reset_memtable = 1;
         FlushOut fo;
         fo.acquire = pool_acquire;
         fo.acquire_ctx = &c->pinned;
@@ -2148,13 +2148,8 @@ static int step_tail(StcCtx *c, uint32_t t, StepGeom &g, stc_step_out *out) {
         run.blk = fo.blk;
         if (c->cfg.device_runs) CKRC(keep_run_on_device(c, run));
         c->runs.push_back(std::move(run));
-        /* reset the memtable (geometry.cu:2111-2131) */
-        CK(hipMemsetAsync(c->same_pid_count, 0, 4ull * n, s));
-        CK(hipMemsetAsync(c->sids, 0, 2ull * n, s));
-        hipLaunchKernelGGL(k_fill_obox, dim3(blocks_for(n, 256)), dim3(256), 0, s, c->o_boxs, n);
-        CK(hipGetLastError());
+        /* the memtable's per-object state was reset inside the flush (reset_memtable) */
         c->kv_count = 0;
-        c->sid_count = 0;
         c->oversize_oid_count = 0;
         c->end_time_min = t; /* process.cpp:271 with end_time_max = cur_time */
         flushed = 1;
@@ -2230,7 +2225,6 @@ int stc_step_meetings(void *vc, const stc_meeting_unit *units, uint64_t nu64, ui
     Counters *cnt = c->dcnt;
     memset(c->hinit, 0, sizeof(Counters));
     c->hinit->kv_count = c->kv_count;
-    c->hinit->sid_count = c->sid_count;
     c->hinit->oversize_oid_count = c->oversize_oid_count;
     CK(hipMemcpyAsync(cnt, c->hinit, sizeof(Counters), hipMemcpyHostToDevice, s));
     CK(hipEventRecord(c->ev[0], s));
@@ -2606,7 +2600,6 @@ int stc_compact_runs(void *vc, uint32_t first, uint32_t count) {
                                 s);
         out0 += in_ctf[r - first] + R.info.oversize_count;
     }
-    if (!rc) rc = stc_count_objects(spc, n, dcount, s);
     if (!rc && (hipMemcpyAsync(h_count, dcount, 8, hipMemcpyDeviceToHost, s) || hipStreamSynchronize(s)))
         rc = STC_E_HIP;
     if (rc) {
@@ -2630,7 +2623,6 @@ int stc_compact_runs(void *vc, uint32_t first, uint32_t count) {
     fa.keep_sorted = (c->cfg.keep_pairs & 2) ? 1 : 0;
     fa.sticky_oversize = 1;
     fa.map[0] = c->map.lo0; fa.map[1] = c->map.lo1; fa.map[2] = c->map.hi0; fa.map[3] = c->map.hi1;
-    fa.sid_count = h_count[0];
     fa.oversize_oid_count = 0; /* counted by the flush: objects with records that are oversize */
     fa.rec_keys = keys;
     fa.rec_boxes = boxes;

@@ -65,9 +65,6 @@ int stc_compact_decode(const QueryRunDev &R, uint64_t kv_count, uint32_t num_obj
                        uint64_t *keys, float4 *boxes, uint32_t *same_pid_count, uint4 *o_boxs, uint16_t *sids,
                        uint32_t *bad, hipStream_t s);
 
-/* number of objects with at least one record, into *d_count (device) */
-int stc_count_objects(const uint32_t *same_pid_count, uint32_t n, uint32_t *d_count, hipStream_t s);
-
 /* decode every record of a run kept in HBM once and summarise each CTF */
 int stc_query_ctf_stats(const QueryRunDev &run, std::vector<CtfStats> &out, hipStream_t s);
 

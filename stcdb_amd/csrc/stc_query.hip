@@ -524,23 +524,6 @@ int stc_compact_decode(const QueryRunDev &R, uint64_t kv_count, uint32_t num_obj
     return STC_OK;
 }
 
-__global__ __launch_bounds__(256) void k_count_objects(const uint32_t *__restrict__ same_pid_count, uint32_t n,
-                                                       uint32_t *__restrict__ d_count) {
-    uint32_t mine = 0;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-        mine += same_pid_count[i] ? 1u : 0u;
-    for (int o = STC_WAVE / 2; o > 0; o >>= 1) mine += __shfl_xor(mine, o, STC_WAVE);
-    if ((threadIdx.x & (STC_WAVE - 1)) == 0 && mine) atomicAdd(d_count, mine);
-}
-
-int stc_count_objects(const uint32_t *same_pid_count, uint32_t n, uint32_t *d_count, hipStream_t s) {
-    QCK(hipMemsetAsync(d_count, 0, 4, s));
-    hipLaunchKernelGGL(k_count_objects, dim3(std::max<uint32_t>(std::min<uint32_t>(blocks(n, 256), 1024), 1)),
-                       dim3(256), 0, s, same_pid_count, n, d_count);
-    QCK(hipGetLastError());
-    return STC_OK;
-}
-
 int stc_query_ctf_stats(const QueryRunDev &run, std::vector<CtfStats> &out, hipStream_t s) {
     out.assign(run.ctf_count, CtfStats());
     if (!run.ctf_count) return STC_OK;
