@@ -284,12 +284,18 @@ __global__ void k_leaf_nodes(const Node *nodes, uint32_t n_nodes, uint32_t *leaf
     if (nodes[i].type == T_LEAF) leaf_node[nodes[i].leaf] = i;
 }
 
-__global__ void k_partition(const double2 *__restrict__ xy, uint32_t n, const Node *__restrict__ nodes, double xbuf,
-                            double ybuf, uint32_t *__restrict__ pt_leaf, uint32_t *__restrict__ pt_lv,
-                            uint32_t *__restrict__ leaf_count, Counters *cnt) {
-    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+/* One thread per point, points in Morton order (midx): a wave's points descend
+ * the same subtrees, and a leaf's points are consecutive, so each run of
+ * equal leaves in a wave adds its length to leaf_count once. */
+__global__ void k_partition(const double2 *__restrict__ xy, const uint32_t *__restrict__ midx, uint32_t n,
+                            const Node *__restrict__ nodes, double xbuf, double ybuf, uint32_t *__restrict__ pt_leaf,
+                            uint32_t *__restrict__ pt_lv, uint32_t *__restrict__ leaf_count, Counters *cnt) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & (STC_WAVE - 1);
     bool border = false;
-    if (p < n) {
+    uint32_t leaf = 0xFFFFFFFFu;
+    if (j < n) {
+        const uint32_t p = midx[j];
         double2 q = xy[p];
         uint32_t cur = 0, lv = 0;
         for (int depth = 0; depth < 64; depth++) {
@@ -300,14 +306,22 @@ __global__ void k_partition(const double2 *__restrict__ xy, uint32_t n, const No
             if (q.x + xbuf < c.hi0 && q.y + ybuf < c.hi1) lv = cur;
             if (c.type == T_LEAF) break;
         }
-        uint32_t leaf = nodes[cur].leaf;
+        leaf = nodes[cur].leaf;
         pt_leaf[p] = leaf;
         border = (lv != cur);
         pt_lv[p] = border ? lv : 0xFFFFFFFFu;
-        atomicAdd(&leaf_count[leaf], 1u);
+    }
+    /* runs of equal leaves: the head of each run adds the run's length */
+    const uint32_t prev = __shfl_up(leaf, 1, STC_WAVE);
+    const bool head = j < n && (lane == 0 || prev != leaf);
+    const uint64_t heads = __ballot(head), valid = __ballot(j < n);
+    if (head) {
+        const uint64_t later = lane == STC_WAVE - 1 ? 0ull : (heads >> (lane + 1));
+        const uint32_t end = later ? (uint32_t)(lane + __ffsll((long long)later)) : (uint32_t)__popcll(valid);
+        atomicAdd(&leaf_count[leaf], end - (uint32_t)lane);
     }
     uint64_t b = __ballot(border);
-    if ((threadIdx.x & (STC_WAVE - 1)) == 0 && b) atomicAdd(&cnt->border, (uint32_t)__popcll(b));
+    if (lane == 0 && b) atomicAdd(&cnt->border, (uint32_t)__popcll(b));
 }
 
 __global__ void k_iota(uint32_t *a, uint32_t n) {
@@ -353,20 +367,25 @@ __global__ void k_overflow(const uint32_t *leaf_count, uint32_t nl, uint32_t buf
 /* border lookup (two passes)                                          */
 /* ------------------------------------------------------------------ */
 
+/* One thread per point, points in leaf order: a wave's points share a leaf
+ * and mostly the same subtrees, so their descents stay together. Outputs are
+ * per point id (count[p], units from offset[p]). */
 template <bool EMIT>
-__global__ void k_lookup(const double2 *__restrict__ xy, const double *__restrict__ dlon, uint32_t n,
+__global__ void k_lookup(const uint32_t *__restrict__ lp, const double *__restrict__ lx,
+                         const double *__restrict__ ly, const double *__restrict__ ld, uint32_t n,
                          const Node *__restrict__ nodes, const uint32_t *__restrict__ pt_lv, double r, double band,
                          uint32_t *__restrict__ count, const uint32_t *__restrict__ offset,
                          uint2 *__restrict__ units, Counters *cnt) {
-    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const uint32_t p = lp[j];
     uint32_t lv = pt_lv[p];
     if (lv == 0xFFFFFFFFu) {
         if (!EMIT) count[p] = 0;
         return;
     }
-    double2 q = xy[p];
-    double dl = dlon[p];
+    const double2 q = make_double2(lx[j], ly[j]);
+    const double dl = ld[j];
     uint32_t stk[STACK_DEPTH];
     int sp = 0;
     stk[sp++] = lv;
@@ -1881,8 +1900,8 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
     CK(hipMemsetAsync(c->leaf_count.p, 0, 4ull * n_leaves, s));
     hipLaunchKernelGGL(k_leaf_nodes, dim3(blocks_for(n_nodes, 256)), dim3(256), 0, s, c->nodes.p, n_nodes,
                        c->leaf_node.p);
-    hipLaunchKernelGGL(k_partition, dim3(blocks_for(n, 256)), dim3(256), 0, s, xy, n, c->nodes.p, c->x_buffer,
-                       c->y_buffer, c->pt_leaf.p, c->pt_lv.p, c->leaf_count.p, cnt);
+    hipLaunchKernelGGL(k_partition, dim3(blocks_for(n, 256)), dim3(256), 0, s, xy, (const uint32_t *)c->idx.p, n,
+                       c->nodes.p, c->x_buffer, c->y_buffer, c->pt_leaf.p, c->pt_lv.p, c->leaf_count.p, cnt);
     CK(hipGetLastError());
     /* 5: leaves in id order */
     CKRC(stc_exclusive_scan_u32(c->leaf_count.p, c->leaf_start.p, n_leaves, nullptr, c->temp.p, s));
@@ -1899,7 +1918,8 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
 
     /* 6: border lookup (count, scan, emit) */
     double r = c->cfg.reach_distance;
-    hipLaunchKernelGGL(k_lookup<false>, dim3(blocks_for(n, 128)), dim3(128), 0, s, xy, c->dlon.p, n, c->nodes.p,
+    hipLaunchKernelGGL(k_lookup<false>, dim3(blocks_for(n, 128)), dim3(128), 0, s, (const uint32_t *)c->lp.p,
+                       (const double *)c->lx.p, (const double *)c->ly.p, (const double *)c->ld.p, n, c->nodes.p,
                        c->pt_lv.p, r, c->band, c->ucount.p, (const uint32_t *)nullptr, (uint2 *)nullptr, cnt);
     CK(hipGetLastError());
     uint32_t *units_total = (uint32_t *)((char *)cnt + offsetof(Counters, units));
@@ -1908,7 +1928,8 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
     h = *c->hcnt;
     uint64_t n_units = (uint32_t)h.units;
     if (c->units.ensure(n_units + 1)) return fail(c, STC_E_CAPACITY, "unit buffer allocation failed");
-    hipLaunchKernelGGL(k_lookup<true>, dim3(blocks_for(n, 128)), dim3(128), 0, s, xy, c->dlon.p, n, c->nodes.p,
+    hipLaunchKernelGGL(k_lookup<true>, dim3(blocks_for(n, 128)), dim3(128), 0, s, (const uint32_t *)c->lp.p,
+                       (const double *)c->lx.p, (const double *)c->ly.p, (const double *)c->ld.p, n, c->nodes.p,
                        c->pt_lv.p, r, c->band, c->ucount.p, (const uint32_t *)c->uoffset.p, c->units.p, cnt);
     CK(hipGetLastError());
     CK(hipEventRecord(c->ev[2], s));
