@@ -731,7 +731,7 @@ __global__ void k_point_ranges(const uint32_t *__restrict__ lp, const uint32_t *
 /* Pass 1: per owner, the combinations that are pairs or lie in the guard band
  * (each reserves a slot of the owner's segment); band combinations are listed
  * as (probe, other) for the host. One wave per owner, owners in leaf order. */
-__global__ __launch_bounds__(256) void k_pair_count(PairGeo g, uint32_t *__restrict__ pcount,
+__global__ __launch_bounds__(256, 8) void k_pair_count(PairGeo g, uint32_t *__restrict__ pcount,
                                                     uint2 *__restrict__ amb, uint32_t amb_cap, Counters *cnt) {
     const int lane = threadIdx.x & (STC_WAVE - 1);
     const uint64_t wid = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / STC_WAVE;
@@ -966,7 +966,7 @@ __global__ __launch_bounds__(256) void k_seg_sort_wave(const uint32_t *__restric
  * written; a longer one is written unsorted to ptmp and listed in `big` for
  * k_seg_sort. The pass repeats pass 1's arithmetic exactly; a slot beyond the
  * counted ones would be dropped and reported through table_fail. */
-__global__ __launch_bounds__(256) void k_pair_write(PairGeo g, const uint32_t *__restrict__ poff,
+__global__ __launch_bounds__(256, 7) void k_pair_write(PairGeo g, const uint32_t *__restrict__ poff,
                                                     const uint32_t *__restrict__ pcount,
                                                     const unsigned long long *__restrict__ acc, uint32_t n_acc,
                                                     PairHit *__restrict__ ptmp, PairHit *__restrict__ pn,
