@@ -34,14 +34,27 @@ struct FlushWork {
     uint64_t key_bytes_cap = 0;
     /* run hand-off: the payload is copied to pinned host memory on `cs`,
      * overlapping the following steps; the next flush waits for `copied`
-     * before it rewrites the buffers the copy reads */
+     * before it rewrites the buffers the copy reads. The copies are queued
+     * only when stc_flush_release is called (the caller picks a point where
+     * the device is busy computing rather than writing memory: copies to the
+     * host slow every concurrent writer) or at the next flush. */
     uint16_t *sids_out = nullptr;                  /* sids as of the flush */
     uint4 *chunks = nullptr;                       /* (ctf, first, count) work items of the key/bitmap pass */
     size_t chunks_cap = 0;
     uint8_t *o_bitmap_dev = nullptr;               /* oversize bitmap as of the flush */
     hipStream_t cs = nullptr;
     hipEvent_t ready = nullptr, copied = nullptr;
-    bool pending = false;
+    bool pending = false;                          /* copies queued on cs, maybe still running */
+    struct Job {                                   /* copies of the last flush not yet queued */
+        uint8_t *dst;
+        size_t key_bytes_at, bitmaps_at, o_bitmap_at, sids_at, o_keys_at, o_boxes_at, sorted_keys_at,
+            sorted_boxes_at, counters_at;
+        uint64_t key_bytes, nf, n_tail;
+        uint32_t ctf_count, n;
+        bool has_sorted;
+        hipEvent_t done;                           /* the pinned block's event */
+    } job{};
+    bool job_set = false;
 };
 
 struct FlushArgs {
@@ -93,6 +106,10 @@ uint32_t stc_flush_raster_capped(const void *counters);
 int stc_flush_init(FlushWork *w, uint32_t n, uint32_t ctf_count);
 void stc_flush_free(FlushWork *w);
 int stc_flush_run(FlushWork *w, const FlushArgs *a, FlushOut *out, hipStream_t s);
+/* queue the last flush's copies to its pinned block behind the work queued on s so far (no-op when none wait) */
+int stc_flush_release(FlushWork *w, hipStream_t s);
+/* forget the copies into block `dst` (its run was dropped before they were queued) */
+void stc_flush_cancel(FlushWork *w, const uint8_t *dst);
 
 /* CTF::get_ctf_bits (src/tracing/step_merge.cpp:140-186), host arithmetic */
 void stc_ctf_bits(stc_ctf_meta *c, const double *map, uint32_t num_objects, uint32_t max_meet_time);

@@ -570,6 +570,7 @@ int stc_flush_init(FlushWork *w, uint32_t n, uint32_t ctf_count) {
 }
 
 void stc_flush_free(FlushWork *w) {
+    w->job_set = false;
     if (w->cs) (void)hipStreamSynchronize(w->cs);
     if (w->cs) (void)hipStreamDestroy(w->cs);
     if (w->ready) (void)hipEventDestroy(w->ready);
@@ -580,6 +581,36 @@ void stc_flush_free(FlushWork *w) {
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     memset((void *)w, 0, sizeof(*w));
+}
+
+int stc_flush_release(FlushWork *w, hipStream_t s) {
+    if (!w->job_set) return STC_OK;
+    w->job_set = false;
+    const FlushWork::Job &j = w->job;
+    const FlushCounters *fc = (const FlushCounters *)w->flags;
+    FCK(hipEventRecord(w->ready, s));
+    FCK(hipStreamWaitEvent(w->cs, w->ready, 0));
+    if (j.key_bytes) FCK(hipMemcpyAsync(j.dst + j.key_bytes_at, w->key_bytes, j.key_bytes, hipMemcpyDeviceToHost, w->cs));
+    FCK(hipMemcpyAsync(j.dst + j.bitmaps_at, w->bitmaps, (size_t)j.ctf_count * 8192, hipMemcpyDeviceToHost, w->cs));
+    FCK(hipMemcpyAsync(j.dst + j.o_bitmap_at, w->o_bitmap_dev, 8192, hipMemcpyDeviceToHost, w->cs));
+    FCK(hipMemcpyAsync(j.dst + j.sids_at, w->sids_out, (size_t)j.n * 2, hipMemcpyDeviceToHost, w->cs));
+    if (j.n_tail) {
+        FCK(hipMemcpyAsync(j.dst + j.o_keys_at, w->keys_out + 2 * j.nf, j.n_tail * 16, hipMemcpyDeviceToHost, w->cs));
+        FCK(hipMemcpyAsync(j.dst + j.o_boxes_at, w->boxes_out + j.nf, j.n_tail * 16, hipMemcpyDeviceToHost, w->cs));
+    }
+    if (j.has_sorted) {
+        FCK(hipMemcpyAsync(j.dst + j.sorted_keys_at, w->keys_out, j.nf * 16, hipMemcpyDeviceToHost, w->cs));
+        FCK(hipMemcpyAsync(j.dst + j.sorted_boxes_at, w->boxes_out, j.nf * 16, hipMemcpyDeviceToHost, w->cs));
+    }
+    FCK(hipMemcpyAsync(j.dst + j.counters_at, fc, sizeof(FlushCounters), hipMemcpyDeviceToHost, w->cs));
+    FCK(hipEventRecord(w->copied, w->cs));
+    FCK(hipEventRecord(j.done, w->cs));
+    w->pending = true;
+    return STC_OK;
+}
+
+void stc_flush_cancel(FlushWork *w, const uint8_t *dst) {
+    if (w->job_set && w->job.dst == dst) w->job_set = false;
 }
 
 static int ensure_rec(FlushWork *w, uint64_t K) {
@@ -602,6 +633,7 @@ int stc_flush_run(FlushWork *w, const FlushArgs *a, FlushOut *out, hipStream_t s
     const uint32_t n = a->n, C = a->ctf_count;
     const uint32_t split = (uint32_t)std::sqrt((double)C);
     const uint64_t K = a->kv_count;
+    FRC(stc_flush_release(w, s));
     FRC(ensure_rec(w, K + 1));
     /* the previous run's copies read keys_out, key_bytes, bitmaps, sids_out and the counters */
     if (w->pending) FCK(hipStreamWaitEvent(s, w->copied, 0));
@@ -722,30 +754,26 @@ int stc_flush_run(FlushWork *w, const FlushArgs *a, FlushOut *out, hipStream_t s
     L.total = off;
     out->blk = out->acquire(out->acquire_ctx, L.total);
     if (!out->blk.p) return stc_hip_fail(hipErrorOutOfMemory, "pinned run buffer", __FILE__, __LINE__);
-    uint8_t *dst = out->blk.p;
-    if (a->reset_memtable) { /* geometry.cu:2111-2131; ahead of the host copies, which slow concurrent writes */
+    if (a->reset_memtable) { /* geometry.cu:2111-2131 */
         FCK(hipMemsetAsync(a->same_pid_count, 0, 4ull * n, s));
         FCK(hipMemsetAsync(a->sids, 0, 2ull * n, s));
         hipLaunchKernelGGL(k_fill_obox2, dim3(blocks_for(n, 256)), dim3(256), 0, s, a->o_boxs, n);
+        FCK(hipGetLastError());
     }
-    FCK(hipEventRecord(w->ready, s));
-    FCK(hipStreamWaitEvent(w->cs, w->ready, 0));
-    if (byte_base[C]) FCK(hipMemcpyAsync(dst + L.key_bytes, w->key_bytes, byte_base[C], hipMemcpyDeviceToHost, w->cs));
-    FCK(hipMemcpyAsync(dst + L.bitmaps, w->bitmaps, (size_t)C * 8192, hipMemcpyDeviceToHost, w->cs));
-    FCK(hipMemcpyAsync(dst + L.o_bitmap, w->o_bitmap_dev, 8192, hipMemcpyDeviceToHost, w->cs));
-    FCK(hipMemcpyAsync(dst + L.sids, w->sids_out, (size_t)n * 2, hipMemcpyDeviceToHost, w->cs));
-    if (n_tail) {
-        FCK(hipMemcpyAsync(dst + L.o_keys, w->keys_out + 2 * nf, n_tail * 16, hipMemcpyDeviceToHost, w->cs));
-        FCK(hipMemcpyAsync(dst + L.o_boxes, w->boxes_out + nf, n_tail * 16, hipMemcpyDeviceToHost, w->cs));
-    }
-    if (out->has_sorted) {
-        FCK(hipMemcpyAsync(dst + L.sorted_keys, w->keys_out, nf * 16, hipMemcpyDeviceToHost, w->cs));
-        FCK(hipMemcpyAsync(dst + L.sorted_boxes, w->boxes_out, nf * 16, hipMemcpyDeviceToHost, w->cs));
-    }
-    FCK(hipMemcpyAsync(dst + L.counters, fc, sizeof(FlushCounters), hipMemcpyDeviceToHost, w->cs));
-    FCK(hipEventRecord(w->copied, w->cs));
-    FCK(hipEventRecord(out->blk.ev, w->cs));
-    w->pending = true;
+    /* the copies to the pinned block wait for stc_flush_release */
+    FlushWork::Job &j = w->job;
+    j.dst = out->blk.p;
+    j.key_bytes_at = L.key_bytes; j.bitmaps_at = L.bitmaps; j.o_bitmap_at = L.o_bitmap; j.sids_at = L.sids;
+    j.o_keys_at = L.o_keys; j.o_boxes_at = L.o_boxes; j.sorted_keys_at = L.sorted_keys;
+    j.sorted_boxes_at = L.sorted_boxes; j.counters_at = L.counters;
+    j.key_bytes = byte_base[C];
+    j.nf = nf;
+    j.n_tail = n_tail;
+    j.ctf_count = C;
+    j.n = n;
+    j.has_sorted = out->has_sorted;
+    j.done = out->blk.ev;
+    w->job_set = true;
     stc_run_info &info = out->info;
     memset(&info, 0, sizeof(info));
     info.kv_count = (uint32_t)nf;

@@ -1658,6 +1658,12 @@ static int step_pairs(StcCtx *c, std::vector<PairHit> &ph) {
     return 0;
 }
 
+/* a run's pinned block back to the pool; copies into it that were never queued are dropped */
+static void release_block(StcCtx *c, PinnedBlock b) {
+    stc_flush_cancel(&c->flushw, b.p);
+    c->pinned.release(b);
+}
+
 /* Copy a just-flushed run out of the flush buffers into its own HBM block
  * (packed keys, CTF metadata and bases, sids, oversize buffer) for queries. */
 static int keep_run_on_device(StcCtx *c, RunHost &run) {
@@ -1823,7 +1829,7 @@ int stc_destroy(void *vc) {
     if (c->hinit) hipHostFree(c->hinit);
     stc_flush_free(&c->flushw);
     for (auto &r : c->runs) {
-        c->pinned.release(r.blk);
+        release_block(c, r.blk);
         if (r.dev) (void)hipFree(r.dev);
     }
     c->runs.clear();
@@ -1977,6 +1983,8 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
                      c->rev_xy.p, c->rev_dl.p, c->rev_il.p, n, c->buf, r, c->band, r * r * (1.0 - 1e-12),
                      r * r * (1.0 + 1e-12)};
     const uint32_t pgrid = (uint32_t)std::min<uint64_t>(blocks_for((uint64_t)n * STC_WAVE, 256), 256u * 64u);
+    /* the last run's host copies overlap the count pass, which computes more than it writes */
+    if (stc_flush_release(&c->flushw, s)) return fail(c, STC_E_HIP, std::string("run copies: ") + g_last_fail);
     for (int attempt = 0;; attempt++) {
         CK(hipMemsetAsync(&cnt->candidates, 0, 3 * sizeof(unsigned long long), s)); /* candidates, pairs, ambiguous */
         CK(hipEventRecord(c->ev[10], s));
@@ -2074,6 +2082,7 @@ static int step_tail(StcCtx *c, uint32_t t, StepGeom &g, stc_step_out *out) {
     Counters *cnt = c->dcnt;
     /* 9: merge the step's pairs with the previous step's meetings; meetings
      * without a pair expire and the long ones become records */
+    if (stc_flush_release(&c->flushw, s)) return fail(c, STC_E_HIP, std::string("run copies: ") + g_last_fail);
     const int cu = c->cur, nx = cu ^ 1;
     const uint64_t kv_before = c->kv_count;
     if (ensure_records(c, kv_before + 2 * std::max<uint64_t>(c->last_expired + c->last_expired / 4, 1 << 16) + 2))
@@ -2159,7 +2168,7 @@ static int step_tail(StcCtx *c, uint32_t t, StepGeom &g, stc_step_out *out) {
         fo.acquire_ctx = &c->pinned;
         int rc = stc_flush_run(&c->flushw, &fa, &fo, s);
         if (rc) {
-            c->pinned.release(fo.blk);
+            release_block(c, fo.blk);
             return fail(c, rc, std::string("flush: ") + g_last_fail);
         }
         run.info = fo.info;
@@ -2402,6 +2411,7 @@ int stc_get_run(void *vc, uint32_t run, stc_ctf_meta *ctfs, uint8_t *key_bytes, 
     StcCtx *c = (StcCtx *)vc;
     if (!c || run >= c->runs.size()) return STC_E_ARG;
     RunHost &R = c->runs[run];
+    if (stc_flush_release(&c->flushw, c->s)) return fail(c, STC_E_HIP, std::string("run copies: ") + g_last_fail);
     CK(hipEventSynchronize(R.blk.ev));
     const uint8_t *src = R.blk.p;
     const RunLayout &L = R.lay;
@@ -2426,6 +2436,7 @@ int stc_get_run(void *vc, uint32_t run, stc_ctf_meta *ctfs, uint8_t *key_bytes, 
 int stc_sync(void *vc) {
     StcCtx *c = (StcCtx *)vc;
     if (!c) return STC_E_ARG;
+    if (stc_flush_release(&c->flushw, c->s)) return fail(c, STC_E_HIP, std::string("run copies: ") + g_last_fail);
     CK(hipStreamSynchronize(c->s));
     if (c->flushw.cs) CK(hipStreamSynchronize(c->flushw.cs));
     return STC_OK;
@@ -2490,6 +2501,7 @@ int stc_dump_run(void *vc, uint32_t run, const char *dir, const char *prefix, ui
     StcCtx *c = (StcCtx *)vc;
     if (!c || !dir || !prefix || run >= c->runs.size()) return STC_E_ARG;
     RunHost &R = c->runs[run];
+    if (stc_flush_release(&c->flushw, c->s)) return fail(c, STC_E_HIP, std::string("run copies: ") + g_last_fail);
     CK(hipEventSynchronize(R.blk.ev));
     const uint8_t *src = R.blk.p;
     const RunLayout &L = R.lay;
@@ -2554,7 +2566,7 @@ int stc_drop_runs(void *vc) {
     StcCtx *c = (StcCtx *)vc;
     if (!c) return STC_E_ARG;
     for (auto &r : c->runs) {
-        c->pinned.release(r.blk);
+        release_block(c, r.blk);
         if (r.dev) {
             (void)hipStreamSynchronize(c->s);
             (void)hipFree(r.dev);
@@ -2655,7 +2667,7 @@ int stc_compact_runs(void *vc, uint32_t first, uint32_t count) {
     fo.acquire_ctx = &c->pinned;
     rc = stc_flush_run(&c->flushw, &fa, &fo, s);
     if (rc) {
-        c->pinned.release(fo.blk);
+        release_block(c, fo.blk);
         release();
         return fail(c, rc, std::string("compaction flush: ") + g_last_fail);
     }
@@ -2668,13 +2680,13 @@ int stc_compact_runs(void *vc, uint32_t first, uint32_t count) {
     rc = keep_run_on_device(c, run);
     release();
     if (rc) {
-        c->pinned.release(run.blk);
+        release_block(c, run.blk);
         if (run.dev) (void)hipFree(run.dev);
         return rc;
     }
     CK(hipStreamSynchronize(s));
     for (uint32_t r = first; r < first + count; r++) {
-        c->pinned.release(c->runs[r].blk);
+        release_block(c, c->runs[r].blk);
         (void)hipFree(c->runs[r].dev);
     }
     c->runs.erase(c->runs.begin() + first, c->runs.begin() + first + count);
