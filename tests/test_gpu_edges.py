@@ -1,8 +1,11 @@
 """Edge cases of the pipeline, GPU against the oracle: a single object,
 coincident objects (one leaf overflowing its buffer), points on QTree split
-lines and on the map border, ragged object counts, an input outside the map;
+lines and on the map border, pairs within a few ulps of the reach distance
+(decided on the host), ragged object counts, an input outside the map;
 and, at the full 10M-object size, invariants that tie each step's counters
 together."""
+import math
+
 import numpy as np
 import pytest
 
@@ -24,9 +27,11 @@ def run_frames(cfg, box, frames, tag):
     gpu = Pipeline(cfg, box)
     orc = O.Oracle(cfg, box)
     print(f"[edges {tag}] n={cfg['num_objects']} steps={len(frames)}", flush=True)
+    band = 0
     for t, xy in enumerate(frames):
         g = gpu.step(xy, t)
         o = orc.step(xy, t)
+        band += g["ambiguous_pairs"]
         for k, ko in [("leaves", "leaves"), ("nodes", "nodes"), ("overflow_points", "overflow"),
                       ("active_meetings", "active"), ("expired", "expired"), ("kv_count", "kv_count"),
                       ("flushed", "flushed")]:
@@ -45,6 +50,7 @@ def run_frames(cfg, box, frames, tag):
         assert np.array_equal(a["key_bytes"], b["key_bytes"]) and np.array_equal(a["sids"], b["sids"]), (tag, r)
     gpu.close()
     orc.close()
+    return band
 
 
 def walk(box, n, steps, seed, step_m=3.0, spread=0.3):
@@ -113,6 +119,38 @@ def test_points_on_split_lines_and_map_border():
         xy = np.clip(xy, [lo0, lo1], [np.nextafter(hi0, lo0), np.nextafter(hi1, lo1)])
         frames.append(xy)
     run_frames(cfg, box, frames, "split-lines")
+
+
+def dpm_lon(lat):
+    """degrees of longitude per metre at latitude `lat`, in the reference's
+    operation order (util.h:38,57) with the C library's sin"""
+    return 360.0 / (math.sin((90 - abs(lat)) * 3.14159265 / 180) * 40076.0 * 1000.0)
+
+
+def test_pairs_at_the_reach_distance():
+    """Pairs placed a few ulps from the reach distance: near the origin a
+    coordinate ulp is far below the 16-ulp guard band around r, so many pairs
+    fall inside it and are decided on the host; the pairs, meetings and runs
+    must still equal the oracle's, which evaluates every pair exactly."""
+    cfg, _ = config(600, grid_capacity=10)
+    box = [[0.001, 0.001], [0.011, 0.011]]
+    r = cfg["reach_distance"]
+    rng = np.random.default_rng(12)
+    frames = []
+    for t in range(8):
+        p = rng.uniform(0.002, 0.010, (300, 2))
+        th = rng.uniform(0, 2 * np.pi, 300)
+        q = np.empty_like(p)
+        for i in range(300):
+            dx = r * dpm_lon(p[i, 1]) * math.cos(th[i])
+            dy = r * (360.0 / (40076.0 * 1000.0)) * math.sin(th[i])
+            q[i] = p[i] + np.array([dx, dy])
+            for d in range(2):  # a few ulps either way
+                for _ in range(int(rng.integers(0, 6))):
+                    q[i, d] = np.nextafter(q[i, d], np.inf if rng.random() < 0.5 else -np.inf)
+        frames.append(np.concatenate([p, q]))
+    band = run_frames(cfg, box, frames, "reach-distance")
+    assert band > 0, "no pair fell inside the guard band"
 
 
 @pytest.mark.parametrize("n", [63, 65, 4097])
