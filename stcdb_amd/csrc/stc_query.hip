@@ -455,56 +455,84 @@ __device__ inline float4 decode_fbox(const stc_ctf_meta &c, uint64_t vm) {
     return b;
 }
 
+/* A run's records are grouped by object, so the lanes of a wave mostly share
+ * a few objects: their per-object counts and box bounds are combined across
+ * each run of equal ids first, and the run's first lane makes the atomic
+ * updates (64 updates of one address would be serialised). */
 __global__ __launch_bounds__(256) void k_compact_decode(QueryRunDev R, uint64_t kv_count, uint32_t num_objects,
                                                         uint32_t base, uint64_t out0, uint64_t *__restrict__ keys,
                                                         float4 *__restrict__ boxes, uint32_t *__restrict__ same_pid_count,
                                                         uint4 *__restrict__ o_boxs, uint32_t *__restrict__ bad) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= kv_count + R.n_tail) return;
-    uint32_t pid, target, dur, end;
-    float4 b;
-    if (i < kv_count) {
-        /* the record's CTF: the last c with rec_base[c] <= i */
-        uint32_t lo = 0, hi = R.ctf_count;
-        while (hi - lo > 1) {
-            uint32_t mid = (lo + hi) / 2;
-            if (R.rec_base[mid] <= i) lo = mid;
-            else hi = mid;
+    const int lane = threadIdx.x & (STC_WAVE - 1);
+    uint32_t pid = 0xFFFFFFFFu; /* no object: lanes past the end and bad records */
+    uint32_t cnt = 0;
+    uint4 q = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u);
+    if (i < kv_count + R.n_tail) {
+        uint32_t p, target, dur, end;
+        float4 b;
+        if (i < kv_count) {
+            /* the record's CTF: the last c with rec_base[c] <= i */
+            uint32_t lo = 0, hi = R.ctf_count;
+            while (hi - lo > 1) {
+                uint32_t mid = (lo + hi) / 2;
+                if (R.rec_base[mid] <= i) lo = mid;
+                else hi = mid;
+            }
+            const stc_ctf_meta c = R.ctfs[lo];
+            const uint32_t kb = c.key_bit / 8;
+            unsigned __int128 k = load_packed(R.key_bytes + R.byte_base[lo] + (i - R.rec_base[lo]) * kb, kb);
+            const uint64_t vm = (uint64_t)(k & (((unsigned __int128)1 << c.mbr_bit) - 1));
+            unsigned __int128 r = k >> c.mbr_bit;
+            end = (uint32_t)(r & ((1ull << c.end_bit) - 1)) + c.end_time_min;
+            r >>= c.end_bit;
+            dur = (uint32_t)(r & ((1ull << c.duration_bit) - 1));
+            r >>= c.duration_bit;
+            target = (uint32_t)(r & ((1ull << c.id_bit) - 1));
+            r >>= c.id_bit;
+            p = (uint32_t)(r & ((1ull << c.id_bit) - 1));
+            b = decode_fbox(c, vm);
+        } else {
+            const uint64_t j = i - kv_count;
+            const uint64_t lo = R.o_keys[2 * j], hi = R.o_keys[2 * j + 1];
+            p = (uint32_t)(hi >> 26) & MASK26;
+            target = (uint32_t)hi & MASK26;
+            end = (uint32_t)lo + R.end_time_min;
+            dur = (uint32_t)(lo >> 32);
+            b = R.o_boxes[j];
         }
-        const stc_ctf_meta c = R.ctfs[lo];
-        const uint32_t kb = c.key_bit / 8;
-        unsigned __int128 k = load_packed(R.key_bytes + R.byte_base[lo] + (i - R.rec_base[lo]) * kb, kb);
-        const uint64_t vm = (uint64_t)(k & (((unsigned __int128)1 << c.mbr_bit) - 1));
-        unsigned __int128 r = k >> c.mbr_bit;
-        end = (uint32_t)(r & ((1ull << c.end_bit) - 1)) + c.end_time_min;
-        r >>= c.end_bit;
-        dur = (uint32_t)(r & ((1ull << c.duration_bit) - 1));
-        r >>= c.duration_bit;
-        target = (uint32_t)(r & ((1ull << c.id_bit) - 1));
-        r >>= c.id_bit;
-        pid = (uint32_t)(r & ((1ull << c.id_bit) - 1));
-        b = decode_fbox(c, vm);
-    } else {
-        const uint64_t j = i - kv_count;
-        const uint64_t lo = R.o_keys[2 * j], hi = R.o_keys[2 * j + 1];
-        pid = (uint32_t)(hi >> 26) & MASK26;
-        target = (uint32_t)hi & MASK26;
-        end = (uint32_t)lo + R.end_time_min;
-        dur = (uint32_t)(lo >> 32);
-        b = R.o_boxes[j];
+        keys[2 * (out0 + i)] = ((uint64_t)dur << 32) | (uint32_t)(end - base);
+        keys[2 * (out0 + i) + 1] = ((uint64_t)p << 26) | target;
+        boxes[out0 + i] = b;
+        if (p >= num_objects || target >= num_objects) { /* not a key this context wrote */
+            atomicOr(bad, 1u);
+        } else {
+            pid = p;
+            cnt = 1;
+            q = make_uint4(f2q(b.x), f2q(b.y), f2q(b.z), f2q(b.w));
+        }
     }
-    keys[2 * (out0 + i)] = ((uint64_t)dur << 32) | (uint32_t)(end - base);
-    keys[2 * (out0 + i) + 1] = ((uint64_t)pid << 26) | target;
-    boxes[out0 + i] = b;
-    if (pid >= num_objects || target >= num_objects) { /* not a key this context wrote */
-        atomicOr(bad, 1u);
-        return;
+    /* lane l ends up holding the combination over lanes [l, end of its run of equal ids) */
+    for (int o = 1; o < STC_WAVE; o <<= 1) {
+        const uint32_t op = __shfl_down(pid, o, STC_WAVE), oc = __shfl_down(cnt, o, STC_WAVE);
+        const uint32_t o0 = __shfl_down(q.x, o, STC_WAVE), o1 = __shfl_down(q.y, o, STC_WAVE);
+        const uint32_t o2 = __shfl_down(q.z, o, STC_WAVE), o3 = __shfl_down(q.w, o, STC_WAVE);
+        if (lane + o < STC_WAVE && op == pid) {
+            cnt += oc;
+            q.x = min(q.x, o0);
+            q.y = min(q.y, o1);
+            q.z = max(q.z, o2);
+            q.w = max(q.w, o3);
+        }
     }
-    atomicAdd(&same_pid_count[pid], 1u);
-    atomicMin(&o_boxs[pid].x, f2q(b.x));
-    atomicMax(&o_boxs[pid].z, f2q(b.z));
-    atomicMin(&o_boxs[pid].y, f2q(b.y));
-    atomicMax(&o_boxs[pid].w, f2q(b.w));
+    const uint32_t prev = __shfl_up(pid, 1, STC_WAVE);
+    if (pid != 0xFFFFFFFFu && (lane == 0 || prev != pid)) {
+        atomicAdd(&same_pid_count[pid], cnt);
+        atomicMin(&o_boxs[pid].x, q.x);
+        atomicMax(&o_boxs[pid].z, q.z);
+        atomicMin(&o_boxs[pid].y, q.y);
+        atomicMax(&o_boxs[pid].w, q.w);
+    }
 }
 
 __global__ void k_compact_sids(const uint16_t *__restrict__ run_sids, uint32_t n, uint16_t *__restrict__ sids) {
