@@ -1701,7 +1701,7 @@ static int keep_run_on_device(StcCtx *c, RunHost &run) {
     q.end_time_min = run.info.end_time_min;
     q.source = 0; /* set per query from the run's position */
     CK(hipStreamSynchronize(s)); /* the host metadata vector is read by the copy above */
-    return stc_query_ctf_stats(q, run.stats, s) ? fail(c, STC_E_HIP, std::string("run statistics: ") + g_last_fail) : 0;
+    return stc_query_ctf_stats(q, run.ctfs.data(), run.stats, s) ? fail(c, STC_E_HIP, std::string("run statistics: ") + g_last_fail) : 0;
 }
 
 extern "C" {

@@ -66,6 +66,7 @@ int stc_compact_decode(const QueryRunDev &R, uint64_t kv_count, uint32_t num_obj
                        uint32_t *bad, hipStream_t s);
 
 /* decode every record of a run kept in HBM once and summarise each CTF */
-int stc_query_ctf_stats(const QueryRunDev &run, std::vector<CtfStats> &out, hipStream_t s);
+int stc_query_ctf_stats(const QueryRunDev &run, const stc_ctf_meta *host_ctfs, std::vector<CtfStats> &out,
+                        hipStream_t s);
 
 #endif /* STC_QUERY_H_ */

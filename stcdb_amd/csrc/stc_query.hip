@@ -367,16 +367,22 @@ __global__ __launch_bounds__(256) void k_q_run_range(const QueryRunDev *__restri
 }
 
 /* one block per CTF: exact union box, time span and forward-record count */
-__global__ __launch_bounds__(256) void k_ctf_stats(QueryRunDev R, CtfStats *out) {
+#define STATS_CHUNK 4096u /* records of a CTF one block summarises */
+
+/* one block per (CTF, chunk of STATS_CHUNK records): a partial summary,
+ * combined per CTF on the host */
+__global__ __launch_bounds__(256) void k_ctf_stats(QueryRunDev R, const uint2 *__restrict__ items,
+                                                   CtfStats *out) {
     __shared__ double sb[4][256];
     __shared__ uint32_t su[4][256];
-    const uint32_t c = blockIdx.x, tid = threadIdx.x;
+    const uint32_t c = items[blockIdx.x].x, first = items[blockIdx.x].y, tid = threadIdx.x;
     const stc_ctf_meta ctf = R.ctfs[c];
     const uint32_t kb = ctf.key_bit / 8;
     const uint8_t *base = R.key_bytes + R.byte_base[c];
+    const uint32_t last = min(ctf.kv_capacity, first + STATS_CHUNK);
     double b0 = INFINITY, b1 = INFINITY, b2 = -INFINITY, b3 = -INFINITY;
     uint32_t smin = 0xFFFFFFFFu, emax = 0, fwd = 0, irregular = 0;
-    for (uint32_t i = tid; i < ctf.kv_capacity; i += blockDim.x) {
+    for (uint32_t i = first + tid; i < last; i += blockDim.x) {
         uint32_t pid, target, start, end;
         double m[4];
         decode_packed(ctf, load_packed(base + (uint64_t)i * kb, kb), &pid, &target, &start, &end, m);
@@ -421,7 +427,7 @@ __global__ __launch_bounds__(256) void k_ctf_stats(QueryRunDev R, CtfStats *out)
         st.end_max = su[1][0];
         st.n_fwd = su[2][0];
         st.regular = !su[3][0];
-        out[c] = st;
+        out[blockIdx.x] = st;
     }
 }
 
@@ -552,17 +558,50 @@ int stc_compact_decode(const QueryRunDev &R, uint64_t kv_count, uint32_t num_obj
     return STC_OK;
 }
 
-int stc_query_ctf_stats(const QueryRunDev &run, std::vector<CtfStats> &out, hipStream_t s) {
+int stc_query_ctf_stats(const QueryRunDev &run, const stc_ctf_meta *host_ctfs, std::vector<CtfStats> &out,
+                        hipStream_t s) {
     out.assign(run.ctf_count, CtfStats());
     if (!run.ctf_count) return STC_OK;
+    /* work items: every CTF in chunks of STATS_CHUNK records (an empty CTF still gets one) */
+    std::vector<uint2> items;
+    std::vector<uint32_t> first_item(run.ctf_count);
+    for (uint32_t c = 0; c < run.ctf_count; c++) {
+        first_item[c] = (uint32_t)items.size();
+        const uint32_t cap = host_ctfs[c].kv_capacity;
+        for (uint32_t k = 0; k == 0 || k < cap; k += STATS_CHUNK) items.push_back(make_uint2(c, k));
+    }
+    const uint32_t ni = (uint32_t)items.size();
     CtfStats *d = nullptr;
-    QCK(hipMalloc((void **)&d, run.ctf_count * sizeof(CtfStats)));
-    hipLaunchKernelGGL(k_ctf_stats, dim3(run.ctf_count), dim3(256), 0, s, run, d);
-    hipError_t e = hipGetLastError();
-    if (e == hipSuccess) e = hipMemcpyAsync(out.data(), d, run.ctf_count * sizeof(CtfStats), hipMemcpyDeviceToHost, s);
+    uint2 *di = nullptr;
+    QCK(hipMalloc((void **)&d, ni * sizeof(CtfStats)));
+    hipError_t e = hipMalloc((void **)&di, ni * sizeof(uint2));
+    std::vector<CtfStats> part(ni);
+    if (e == hipSuccess) e = hipMemcpyAsync(di, items.data(), ni * sizeof(uint2), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_ctf_stats, dim3(ni), dim3(256), 0, s, run, (const uint2 *)di, d);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(part.data(), d, ni * sizeof(CtfStats), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     (void)hipFree(d);
+    if (di) (void)hipFree(di);
     if (e != hipSuccess) return stc_hip_fail(e, "CTF statistics", __FILE__, __LINE__);
+    for (uint32_t c = 0; c < run.ctf_count; c++) {
+        const uint32_t k1 = c + 1 < run.ctf_count ? first_item[c + 1] : ni;
+        CtfStats st = part[first_item[c]];
+        for (uint32_t k = first_item[c] + 1; k < k1; k++) {
+            const CtfStats &p = part[k];
+            st.box[0] = std::fmin(st.box[0], p.box[0]);
+            st.box[1] = std::fmin(st.box[1], p.box[1]);
+            st.box[2] = std::fmax(st.box[2], p.box[2]);
+            st.box[3] = std::fmax(st.box[3], p.box[3]);
+            st.start_min = std::min(st.start_min, p.start_min);
+            st.end_max = std::max(st.end_max, p.end_max);
+            st.n_fwd += p.n_fwd;
+            st.regular = st.regular && p.regular;
+        }
+        out[c] = st;
+    }
     return STC_OK;
 }
 
