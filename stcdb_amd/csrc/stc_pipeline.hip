@@ -284,13 +284,24 @@ __global__ void k_leaf_nodes(const Node *nodes, uint32_t n_nodes, uint32_t *leaf
     if (nodes[i].type == T_LEAF) leaf_node[nodes[i].leaf] = i;
 }
 
+/* Workgroups go to the 8 XCDs in turn (workgroup b to XCD b % 8), each XCD
+ * with its own L2. This id gives each XCD a contiguous eighth of the grid, so
+ * the owners that the waves of one XCD handle at a time are neighbours in leaf
+ * order and share their candidates' cache lines (with the plain id, each
+ * XCD's L2 would hold the neighbourhoods of the whole grid's owners). A grid
+ * that is not a multiple of 8 keeps the plain id. */
+__device__ inline uint32_t xcd_block_id() {
+    const uint32_t nb = gridDim.x, b = blockIdx.x;
+    return (nb & 7u) ? b : (b & 7u) * (nb >> 3) + (b >> 3);
+}
+
 /* One thread per point, points in Morton order (midx): a wave's points descend
  * the same subtrees, and a leaf's points are consecutive, so each run of
  * equal leaves in a wave adds its length to leaf_count once. */
 __global__ void k_partition(const double2 *__restrict__ xy, const uint32_t *__restrict__ midx, uint32_t n,
                             const Node *__restrict__ nodes, double xbuf, double ybuf, uint32_t *__restrict__ pt_leaf,
                             uint32_t *__restrict__ pt_lv, uint32_t *__restrict__ leaf_count, Counters *cnt) {
-    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t j = xcd_block_id() * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & (STC_WAVE - 1);
     bool border = false;
     uint32_t leaf = 0xFFFFFFFFu;
@@ -376,7 +387,7 @@ __global__ void k_lookup(const uint32_t *__restrict__ lp, const double *__restri
                          const Node *__restrict__ nodes, const uint32_t *__restrict__ pt_lv, double r, double band,
                          uint32_t *__restrict__ count, const uint32_t *__restrict__ offset,
                          uint2 *__restrict__ units, Counters *cnt) {
-    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t j = xcd_block_id() * blockDim.x + threadIdx.x;
     if (j >= n) return;
     const uint32_t p = lp[j];
     uint32_t lv = pt_lv[p];
@@ -734,7 +745,7 @@ __global__ void k_point_ranges(const uint32_t *__restrict__ lp, const uint32_t *
 __global__ __launch_bounds__(256, 8) void k_pair_count(PairGeo g, uint32_t *__restrict__ pcount,
                                                     uint2 *__restrict__ amb, uint32_t amb_cap, Counters *cnt) {
     const int lane = threadIdx.x & (STC_WAVE - 1);
-    const uint64_t wid = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / STC_WAVE;
+    const uint64_t wid = ((uint64_t)xcd_block_id() * blockDim.x + threadIdx.x) / STC_WAVE;
     const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) / STC_WAVE;
     unsigned long long cand = 0, total = 0;
     for (uint64_t w = wid; w < g.n; w += nw) {
@@ -974,7 +985,7 @@ __global__ __launch_bounds__(256, 7) void k_pair_write(PairGeo g, const uint32_t
     __shared__ unsigned long long wk[256 / STC_WAVE][WCAP];
     const int lane = threadIdx.x & (STC_WAVE - 1);
     unsigned long long *keys = wk[threadIdx.x / STC_WAVE];
-    const uint64_t wid = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / STC_WAVE;
+    const uint64_t wid = ((uint64_t)xcd_block_id() * blockDim.x + threadIdx.x) / STC_WAVE;
     const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) / STC_WAVE;
     const uint64_t lt = lanemask_lt(lane);
     for (uint64_t w = wid; w < g.n; w += nw) {
@@ -1906,7 +1917,7 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
     CK(hipMemsetAsync(c->leaf_count.p, 0, 4ull * n_leaves, s));
     hipLaunchKernelGGL(k_leaf_nodes, dim3(blocks_for(n_nodes, 256)), dim3(256), 0, s, c->nodes.p, n_nodes,
                        c->leaf_node.p);
-    hipLaunchKernelGGL(k_partition, dim3(blocks_for(n, 256)), dim3(256), 0, s, xy, (const uint32_t *)c->idx.p, n,
+    hipLaunchKernelGGL(k_partition, dim3((blocks_for(n, 256) + 7) & ~7u), dim3(256), 0, s, xy, (const uint32_t *)c->idx.p, n,
                        c->nodes.p, c->x_buffer, c->y_buffer, c->pt_leaf.p, c->pt_lv.p, c->leaf_count.p, cnt);
     CK(hipGetLastError());
     /* 5: leaves in id order */
@@ -1924,7 +1935,7 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
 
     /* 6: border lookup (count, scan, emit) */
     double r = c->cfg.reach_distance;
-    hipLaunchKernelGGL(k_lookup<false>, dim3(blocks_for(n, 128)), dim3(128), 0, s, (const uint32_t *)c->lp.p,
+    hipLaunchKernelGGL(k_lookup<false>, dim3((blocks_for(n, 128) + 7) & ~7u), dim3(128), 0, s, (const uint32_t *)c->lp.p,
                        (const double *)c->lx.p, (const double *)c->ly.p, (const double *)c->ld.p, n, c->nodes.p,
                        c->pt_lv.p, r, c->band, c->ucount.p, (const uint32_t *)nullptr, (uint2 *)nullptr, cnt);
     CK(hipGetLastError());
@@ -1934,7 +1945,7 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
     h = *c->hcnt;
     uint64_t n_units = (uint32_t)h.units;
     if (c->units.ensure(n_units + 1)) return fail(c, STC_E_CAPACITY, "unit buffer allocation failed");
-    hipLaunchKernelGGL(k_lookup<true>, dim3(blocks_for(n, 128)), dim3(128), 0, s, (const uint32_t *)c->lp.p,
+    hipLaunchKernelGGL(k_lookup<true>, dim3((blocks_for(n, 128) + 7) & ~7u), dim3(128), 0, s, (const uint32_t *)c->lp.p,
                        (const double *)c->lx.p, (const double *)c->ly.p, (const double *)c->ld.p, n, c->nodes.p,
                        c->pt_lv.p, r, c->band, c->ucount.p, (const uint32_t *)c->uoffset.p, c->units.p, cnt);
     CK(hipGetLastError());
@@ -1982,7 +1993,8 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
                      c->ucount.p, c->rev_start.p, c->rev_pid.p,
                      c->rev_xy.p, c->rev_dl.p, c->rev_il.p, n, c->buf, r, c->band, r * r * (1.0 - 1e-12),
                      r * r * (1.0 + 1e-12)};
-    const uint32_t pgrid = (uint32_t)std::min<uint64_t>(blocks_for((uint64_t)n * STC_WAVE, 256), 256u * 64u);
+    /* a multiple of 8 workgroups, for the XCD-aware owner order */
+    const uint32_t pgrid = (uint32_t)std::min<uint64_t>((blocks_for((uint64_t)n * STC_WAVE, 256) + 7) & ~7ull, 256u * 64u);
     /* the last run's host copies overlap the count pass, which computes more than it writes */
     if (stc_flush_release(&c->flushw, s)) return fail(c, STC_E_HIP, std::string("run copies: ") + g_last_fail);
     for (int attempt = 0;; attempt++) {
