@@ -301,7 +301,7 @@ __device__ inline uint32_t xcd_block_id() {
 __global__ void k_partition(const double2 *__restrict__ xy, const uint32_t *__restrict__ midx, uint32_t n,
                             const Node *__restrict__ nodes, double xbuf, double ybuf, uint32_t *__restrict__ pt_leaf,
                             uint32_t *__restrict__ pt_lv, uint32_t *__restrict__ leaf_count, Counters *cnt) {
-    const uint32_t j = xcd_block_id() * blockDim.x + threadIdx.x;
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & (STC_WAVE - 1);
     bool border = false;
     uint32_t leaf = 0xFFFFFFFFu;
@@ -387,7 +387,7 @@ __global__ void k_lookup(const uint32_t *__restrict__ lp, const double *__restri
                          const Node *__restrict__ nodes, const uint32_t *__restrict__ pt_lv, double r, double band,
                          uint32_t *__restrict__ count, const uint32_t *__restrict__ offset,
                          uint2 *__restrict__ units, Counters *cnt) {
-    const uint32_t j = xcd_block_id() * blockDim.x + threadIdx.x;
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
     const uint32_t p = lp[j];
     uint32_t lv = pt_lv[p];
@@ -1917,7 +1917,7 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
     CK(hipMemsetAsync(c->leaf_count.p, 0, 4ull * n_leaves, s));
     hipLaunchKernelGGL(k_leaf_nodes, dim3(blocks_for(n_nodes, 256)), dim3(256), 0, s, c->nodes.p, n_nodes,
                        c->leaf_node.p);
-    hipLaunchKernelGGL(k_partition, dim3((blocks_for(n, 256) + 7) & ~7u), dim3(256), 0, s, xy, (const uint32_t *)c->idx.p, n,
+    hipLaunchKernelGGL(k_partition, dim3(blocks_for(n, 256)), dim3(256), 0, s, xy, (const uint32_t *)c->idx.p, n,
                        c->nodes.p, c->x_buffer, c->y_buffer, c->pt_leaf.p, c->pt_lv.p, c->leaf_count.p, cnt);
     CK(hipGetLastError());
     /* 5: leaves in id order */
@@ -1935,7 +1935,7 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
 
     /* 6: border lookup (count, scan, emit) */
     double r = c->cfg.reach_distance;
-    hipLaunchKernelGGL(k_lookup<false>, dim3((blocks_for(n, 128) + 7) & ~7u), dim3(128), 0, s, (const uint32_t *)c->lp.p,
+    hipLaunchKernelGGL(k_lookup<false>, dim3(blocks_for(n, 128)), dim3(128), 0, s, (const uint32_t *)c->lp.p,
                        (const double *)c->lx.p, (const double *)c->ly.p, (const double *)c->ld.p, n, c->nodes.p,
                        c->pt_lv.p, r, c->band, c->ucount.p, (const uint32_t *)nullptr, (uint2 *)nullptr, cnt);
     CK(hipGetLastError());
@@ -1945,7 +1945,7 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
     h = *c->hcnt;
     uint64_t n_units = (uint32_t)h.units;
     if (c->units.ensure(n_units + 1)) return fail(c, STC_E_CAPACITY, "unit buffer allocation failed");
-    hipLaunchKernelGGL(k_lookup<true>, dim3((blocks_for(n, 128) + 7) & ~7u), dim3(128), 0, s, (const uint32_t *)c->lp.p,
+    hipLaunchKernelGGL(k_lookup<true>, dim3(blocks_for(n, 128)), dim3(128), 0, s, (const uint32_t *)c->lp.p,
                        (const double *)c->lx.p, (const double *)c->ly.p, (const double *)c->ld.p, n, c->nodes.p,
                        c->pt_lv.p, r, c->band, c->ucount.p, (const uint32_t *)c->uoffset.p, c->units.p, cnt);
     CK(hipGetLastError());

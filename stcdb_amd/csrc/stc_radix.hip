@@ -7,8 +7,9 @@
  *   scan      : exclusive prefix sum over that table (three-phase scan);
  *   scatter   : each workgroup ranks its tile stably, 256 keys per round, with
  *               per-wavefront peer masks built from 8 ballots (64-wide), then
- *               per-digit wave offsets in LDS, and writes every key/value to
- *               its global position.
+ *               per-digit wave offsets in LDS; 4-byte keys are placed in LDS
+ *               in digit order and written out in that order (coalesced),
+ *               8-byte keys straight to their global positions.
  * Keys of equal digit keep their input order, so chaining passes from the
  * least significant digit gives a stable sort, as thrust::sort_by_key on
  * primitive keys does in the reference (src/cuda/geometry.cu:1860, 1878-1886).
@@ -147,8 +148,89 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const K *keys, uint32_t 
     hist[(uint64_t)threadIdx.x * tiles + blockIdx.x] = h[threadIdx.x];
 }
 
+/* Each workgroup ranks its tile stably (256 keys per round, per-wavefront peer
+ * masks from 8 ballots, per-digit wave offsets in LDS), places the keys in
+ * LDS in digit order, and writes them out in that order: keys of one digit go
+ * to consecutive global positions, so the writes are coalesced. */
 template <typename K>
 __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const K *kin, const uint32_t *vin, K *kout,
+                                                            uint32_t *vout, uint32_t n, int shift,
+                                                            const uint32_t *offs, uint32_t tiles) {
+    __shared__ uint32_t run[RS_BINS];    /* keys of each digit ranked so far in this tile */
+    __shared__ uint32_t base[RS_BINS];   /* global position of the tile's first key of each digit */
+    __shared__ uint32_t lstart[RS_BINS]; /* position in the tile of its first key of each digit */
+    __shared__ uint32_t wcnt[RS_THREADS / STC_WAVE][RS_BINS];
+    __shared__ K sk[RS_TILE];
+    __shared__ uint32_t sv[RS_TILE];
+    __shared__ uint32_t tile_total;
+    const int tid = threadIdx.x, lane = tid & (STC_WAVE - 1), w = tid / STC_WAVE;
+    /* the tile's count of digit tid: the gap to the next entry of the scanned digit-major table */
+    const uint64_t slot = (uint64_t)tid * tiles + blockIdx.x;
+    const uint32_t b0 = offs[slot];
+    const uint32_t b1 = slot + 1 < (uint64_t)RS_BINS * tiles ? offs[slot + 1] : n;
+    base[tid] = b0;
+    run[tid] = 0;
+    lstart[tid] = block_excl_scan(b1 - b0, &tile_total);
+#pragma unroll
+    for (int q = 0; q < RS_THREADS / STC_WAVE; q++) wcnt[q][tid] = 0;
+    __syncthreads();
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (STC_WAVE - lane));
+    const uint64_t tile0 = (uint64_t)blockIdx.x * RS_TILE;
+    for (int r = 0; r < RS_ITEMS; r++) {
+        uint64_t i = tile0 + (uint64_t)r * RS_THREADS + tid;
+        bool valid = i < n;
+        K k = 0;
+        uint32_t v = 0, d = 0;
+        if (valid) {
+            k = kin[i];
+            v = vin[i];
+            d = (uint32_t)(k >> shift) & (RS_BINS - 1);
+        }
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            bool bit = (d >> b) & 1u;
+            uint64_t m = __ballot(bit);
+            peers &= bit ? m : ~m;
+        }
+        uint32_t rank = (uint32_t)__popcll(peers & lt);
+        if (valid && rank == 0) wcnt[w][d] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        {
+            uint32_t acc = run[tid];
+#pragma unroll
+            for (int q = 0; q < RS_THREADS / STC_WAVE; q++) {
+                uint32_t c = wcnt[q][tid];
+                wcnt[q][tid] = acc;
+                acc += c;
+            }
+            run[tid] = acc;
+        }
+        __syncthreads();
+        if (valid) {
+            const uint32_t lp = lstart[d] + wcnt[w][d] + rank;
+            sk[lp] = k;
+            sv[lp] = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < RS_THREADS / STC_WAVE; q++) wcnt[q][tid] = 0;
+        __syncthreads();
+    }
+    const uint32_t cnt = (uint32_t)min<uint64_t>(RS_TILE, n - tile0);
+    for (uint32_t p = tid; p < cnt; p += RS_THREADS) {
+        const K k = sk[p];
+        const uint32_t d = (uint32_t)(k >> shift) & (RS_BINS - 1);
+        const uint32_t pos = base[d] + (p - lstart[d]);
+        kout[pos] = k;
+        vout[pos] = sv[p];
+    }
+}
+
+/* The same ranking, each key written straight to its global position: for
+ * 8-byte keys, whose LDS tile would cost the kernel its occupancy. */
+template <typename K>
+__global__ __launch_bounds__(RS_THREADS) void k_rs_scatter_direct(const K *kin, const uint32_t *vin, K *kout,
                                                             uint32_t *vout, uint32_t n, int shift,
                                                             const uint32_t *offs, uint32_t tiles) {
     __shared__ uint32_t run[RS_BINS];
@@ -221,8 +303,12 @@ static int radix_sort(K *keys, uint32_t *vals, K *keys_alt, uint32_t *vals_alt, 
         STC_CHECK(hipGetLastError());
         int rc = stc_exclusive_scan_u32(hist, hist, RS_BINS * tiles, nullptr, scan_temp, s);
         if (rc) return rc;
-        hipLaunchKernelGGL(k_rs_scatter<K>, dim3(tiles), dim3(RS_THREADS), 0, s, (const K *)src_k,
-                           (const uint32_t *)src_v, dst_k, dst_v, n, shift, (const uint32_t *)hist, tiles);
+        if constexpr (sizeof(K) == 4)
+            hipLaunchKernelGGL(k_rs_scatter<K>, dim3(tiles), dim3(RS_THREADS), 0, s, (const K *)src_k,
+                               (const uint32_t *)src_v, dst_k, dst_v, n, shift, (const uint32_t *)hist, tiles);
+        else
+            hipLaunchKernelGGL(k_rs_scatter_direct<K>, dim3(tiles), dim3(RS_THREADS), 0, s, (const K *)src_k,
+                               (const uint32_t *)src_v, dst_k, dst_v, n, shift, (const uint32_t *)hist, tiles);
         STC_CHECK(hipGetLastError());
         K *tk = src_k; src_k = dst_k; dst_k = tk;
         uint32_t *tv = src_v; src_v = dst_v; dst_v = tv;
