@@ -677,13 +677,19 @@ __device__ inline void owner_combos(uint32_t w, uint32_t a, const PairGeo &g, in
 
 /* split the cross units into (leaf, pid) key/value arrays for the reverse lists */
 __global__ void k_units_split(const uint2 *__restrict__ units, uint32_t nu, uint32_t *__restrict__ leaf,
-                              uint32_t *__restrict__ pid, uint32_t *__restrict__ per_leaf) {
+                              uint32_t *__restrict__ pid) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nu) return;
     const uint2 u = units[i];
     leaf[i] = u.y;
     pid[i] = u.x;
-    atomicAdd(&per_leaf[u.y], 1u);
+}
+
+/* each leaf's first entry in the leaf-sorted reverse lists (rev_start[n_leaves] = nu) */
+__global__ void k_rev_starts(const uint32_t *__restrict__ sorted_leaf, uint32_t nu, uint32_t n_leaves,
+                             uint32_t *__restrict__ rev_start) {
+    const uint32_t L = blockIdx.x * blockDim.x + threadIdx.x;
+    if (L <= n_leaves) rev_start[L] = lower_bound_u32(sorted_leaf, nu, L);
 }
 
 /* the border points' data in reverse-list order, so that owners read it coalesced */
@@ -951,7 +957,7 @@ __device__ inline void write_sorted_from_tmp(const PairHit *__restrict__ tmp, ui
 
 /* segments the write pass left in ptmp: one wave each, up to WSORT_CAP slots;
  * longer ones go to `spill` for the block sort */
-__global__ __launch_bounds__(256) void k_seg_sort_wave(const uint32_t *__restrict__ owners, uint32_t n_owners,
+__global__ __launch_bounds__(256, 5) void k_seg_sort_wave(const uint32_t *__restrict__ owners, uint32_t n_owners,
                                                        const uint32_t *__restrict__ poff,
                                                        const PairHit *__restrict__ ptmp, PairHit *__restrict__ pn,
                                                        uint32_t *__restrict__ spill, uint32_t *spill_count) {
@@ -1563,7 +1569,7 @@ struct StcCtx {
     DBuf<uint2> units;
     DBuf<uint2> amb;                     /* band combinations (probe, other) for the host */
     /* per leaf, by pid: the border points with a cross unit to it */
-    DBuf<uint32_t> rev_leaf, rev_pid, rev_leaf_alt, rev_pid_alt, rev_count, rev_start;
+    DBuf<uint32_t> rev_leaf, rev_pid, rev_leaf_alt, rev_pid_alt, rev_start;
     DBuf<double2> rev_xy;
     DBuf<double> rev_dl, rev_il;
     /* the step's pairs by owner segment, and the live meetings: two arrays of
@@ -1784,7 +1790,7 @@ int stc_create(const stc_config *cfg, const stc_box *mbr, void **out) {
         c->lkeys.ensure(n) || c->ucount.ensure(n) || c->uoffset.ensure(n) || c->units.ensure(4ull * n) ||
         c->amb.ensure(1 << 16) || c->pcount.ensure(n) || c->moff[0].ensure(n + 1ull) || c->moff[1].ensure(n + 1ull) ||
         c->spill_big.ensure(n) || c->spill_huge.ensure(n) || c->spill_giant.ensure(n) ||
-        c->rev_count.ensure(node_cap + 1) || c->rev_start.ensure(node_cap + 1) ||
+        c->rev_start.ensure(node_cap + 1) ||
         c->leaf_count.ensure(node_cap) || c->leaf_start.ensure(node_cap) || c->leaf_node.ensure(node_cap) ||
         c->temp.ensure(stc_radix_temp_words((uint32_t)std::max<size_t>(n, node_cap)) + 64))
         return bad("device allocation failed");
@@ -1822,7 +1828,7 @@ int stc_destroy(void *vc) {
     c->lkeys.release(); c->ucount.release(); c->uoffset.release(); c->units.release();
     c->amb.release();
     c->rev_leaf.release(); c->rev_pid.release(); c->rev_leaf_alt.release(); c->rev_pid_alt.release();
-    c->rev_count.release(); c->rev_start.release();
+    c->rev_start.release();
     c->rev_xy.release(); c->rev_dl.release(); c->rev_il.release();
     c->pcount.release(); c->pnew.release(); c->acc.release();
     for (int i = 0; i < 2; i++) { c->meet[i].release(); c->moff[i].release(); }
@@ -1954,15 +1960,14 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
     /* 7: reverse cross units: per leaf, the border points with a cross unit to it, by pid */
     if (c->rev_leaf.ensure(n_units + 1) || c->rev_pid.ensure(n_units + 1) || c->rev_leaf_alt.ensure(n_units + 1) ||
         c->rev_xy.ensure(n_units + 1) || c->rev_dl.ensure(n_units + 1) || c->rev_il.ensure(n_units + 1) ||
-        c->rev_pid_alt.ensure(n_units + 1) || c->rev_count.ensure(n_leaves + 1ull) ||
+        c->rev_pid_alt.ensure(n_units + 1) ||
         c->rev_start.ensure(n_leaves + 1ull) ||
         c->temp.ensure(stc_radix_temp_words((uint32_t)std::max<uint64_t>(std::max<uint64_t>(n_units, n),
                                                                           c->nodes.cap)) + 64))
         return fail(c, STC_E_CAPACITY, "reverse unit buffers");
-    CK(hipMemsetAsync(c->rev_count.p, 0, 4ull * n_leaves, s));
     if (n_units) {
         hipLaunchKernelGGL(k_units_split, dim3(blocks_for(n_units, 256)), dim3(256), 0, s, c->units.p,
-                           (uint32_t)n_units, c->rev_leaf.p, c->rev_pid.p, c->rev_count.p);
+                           (uint32_t)n_units, c->rev_leaf.p, c->rev_pid.p);
         CK(hipGetLastError());
         CKRC(stc_radix_sort_u32(c->rev_leaf.p, c->rev_pid.p, c->rev_leaf_alt.p, c->rev_pid_alt.p, (uint32_t)n_units,
                                 std::max(1, ilog2_ceil(n_leaves)), c->temp.p, s));
@@ -1971,7 +1976,9 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
                            c->rev_dl.p, c->rev_il.p);
         CK(hipGetLastError());
     }
-    CKRC(stc_exclusive_scan_u32(c->rev_count.p, c->rev_start.p, n_leaves, c->rev_start.p + n_leaves, c->temp.p, s));
+    hipLaunchKernelGGL(k_rev_starts, dim3(blocks_for(n_leaves + 1ull, 256)), dim3(256), 0, s,
+                       (const uint32_t *)c->rev_leaf.p, (uint32_t)n_units, n_leaves, c->rev_start.p);
+    CK(hipGetLastError());
 
     /* 8: the pairs owner by owner: count (a band combination reserves a slot),
      * the band decided on the host, segment offsets, write */
