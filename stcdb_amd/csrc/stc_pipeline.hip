@@ -1276,19 +1276,54 @@ __device__ inline void flush_stage(const Bucket *stage, uint32_t staged, uint32_
 
 /* Start of each merge tile's range in the previous meeting array: the number
  * of previous keys below the tile's first new key (found in that key's owner
- * segment). tb[n_tiles] = n_o. */
+ * segment). tb[n_tiles] = n_o. The search starts where the key's place among
+ * the owner's new slots, scaled to the owner's previous segment, puts it (the
+ * two segments mostly hold the same partners) and widens by doubling steps
+ * until it brackets the key, so a few dependent reads usually suffice. */
 __global__ void k_tile_bounds(const PairHit *__restrict__ pn, uint64_t n_p, const Bucket *__restrict__ mold,
-                              const uint32_t *__restrict__ ooff, uint64_t n_o, uint32_t n_tiles,
-                              uint32_t *__restrict__ tb) {
+                              const uint32_t *__restrict__ ooff, const uint32_t *__restrict__ poff, uint64_t n_o,
+                              uint32_t n_tiles, uint32_t *__restrict__ tb) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t > n_tiles) return;
     if (t == 0 || t == n_tiles || (uint64_t)t * MTILE >= n_p) {
         tb[t] = t == 0 ? 0u : (uint32_t)n_o;
         return;
     }
-    const unsigned long long key = pn[(uint64_t)t * MTILE].key;
+    const uint64_t first = (uint64_t)t * MTILE;
+    const unsigned long long key = pn[first].key;
     const uint32_t a = (uint32_t)(key >> 32);
     uint32_t lo = ooff[a], hi = ooff[a + 1];
+    const uint32_t p0 = poff[a], np_a = poff[a + 1] - p0;
+    if (hi > lo && np_a) {
+        /* bracket [lo, hi) around the scaled guess g: keys below lo are < key, keys from hi on are >= key */
+        const uint32_t g = lo + (uint32_t)(((uint64_t)(first - p0) * (hi - lo)) / np_a);
+        if (g < hi && mold[g].key < key) {
+            uint32_t step = 1, l = g + 1;
+            while (l < hi) {
+                const uint32_t probe = min(l + step - 1, hi - 1);
+                if (mold[probe].key < key) {
+                    l = probe + 1;
+                    step <<= 1;
+                } else {
+                    hi = probe;
+                    break;
+                }
+            }
+            lo = l;
+        } else {
+            uint32_t step = 1, r = g; /* mold[r] >= key or r == hi */
+            while (r > lo) {
+                const uint32_t probe = r - min(step, r - lo);
+                if (mold[probe].key < key) {
+                    lo = probe + 1;
+                    break;
+                }
+                r = probe;
+                step <<= 1;
+            }
+            hi = r;
+        }
+    }
     while (lo < hi) {
         const uint32_t mid = (lo + hi) >> 1;
         if (mold[mid].key < key) lo = mid + 1;
@@ -2116,7 +2151,8 @@ static int step_tail(StcCtx *c, uint32_t t, StepGeom &g, stc_step_out *out) {
     if (c->tbounds.ensure(n_tiles + 1ull)) return fail(c, STC_E_CAPACITY, "merge tile bounds");
     hipLaunchKernelGGL(k_tile_bounds, dim3(blocks_for(n_tiles + 1ull, 256)), dim3(256), 0, s,
                        (const PairHit *)c->pnew.p, c->plen, (const Bucket *)c->meet[cu].p,
-                       (const uint32_t *)c->moff[cu].p, c->mlen, n_tiles, c->tbounds.p);
+                       (const uint32_t *)c->moff[cu].p, (const uint32_t *)c->moff[nx].p, c->mlen, n_tiles,
+                       c->tbounds.p);
     CK(hipGetLastError());
     const uint32_t mgrid = std::max<uint32_t>(std::min<uint32_t>((n_tiles + 3) / 4, 16384), 1);
     for (int attempt = 0;; attempt++) {
