@@ -639,7 +639,7 @@ __device__ inline void owner_combos(uint32_t w, uint32_t a, const PairGeo &g, in
         const uint32_t y = __shfl_up(scum, o, STC_WAVE);
         if (lane >= o) scum += y;
     }
-    const uint32_t total = __shfl(scum, STC_WAVE - 1, STC_WAVE);
+    const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)scum, STC_WAVE - 1);
     scum -= slen;
     const uint32_t nseg = nu + 2;
     const double qax = pa.x, qay = pa.y; /* a's position as a member of its leaf */
@@ -751,11 +751,12 @@ __global__ void k_point_ranges(const uint32_t *__restrict__ lp, const uint32_t *
 __global__ __launch_bounds__(256, 8) void k_pair_count(PairGeo g, uint32_t *__restrict__ pcount,
                                                     uint2 *__restrict__ amb, uint32_t amb_cap, Counters *cnt) {
     const int lane = threadIdx.x & (STC_WAVE - 1);
-    const uint64_t wid = ((uint64_t)xcd_block_id() * blockDim.x + threadIdx.x) / STC_WAVE;
+    /* wave-uniform values go through readfirstlane: scalar registers, scalar loads, scalar loop control */
+    const uint64_t wid = (uint32_t)__builtin_amdgcn_readfirstlane((int)((xcd_block_id() * blockDim.x + threadIdx.x) / STC_WAVE));
     const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) / STC_WAVE;
     unsigned long long cand = 0, total = 0;
     for (uint64_t w = wid; w < g.n; w += nw) {
-        const uint32_t a = g.lp[w];
+        const uint32_t a = (uint32_t)__builtin_amdgcn_readfirstlane((int)g.lp[w]);
         unsigned long long c = 0;
         auto f = [&](bool v, uint32_t q, int dec, uint32_t probe, float4, uint32_t) {
             if (!v) return;
@@ -991,11 +992,12 @@ __global__ __launch_bounds__(256, 7) void k_pair_write(PairGeo g, const uint32_t
     __shared__ unsigned long long wk[256 / STC_WAVE][WCAP];
     const int lane = threadIdx.x & (STC_WAVE - 1);
     unsigned long long *keys = wk[threadIdx.x / STC_WAVE];
-    const uint64_t wid = ((uint64_t)xcd_block_id() * blockDim.x + threadIdx.x) / STC_WAVE;
+    /* wave-uniform values go through readfirstlane: scalar registers, scalar loads, scalar loop control */
+    const uint64_t wid = (uint32_t)__builtin_amdgcn_readfirstlane((int)((xcd_block_id() * blockDim.x + threadIdx.x) / STC_WAVE));
     const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) / STC_WAVE;
     const uint64_t lt = lanemask_lt(lane);
     for (uint64_t w = wid; w < g.n; w += nw) {
-        const uint32_t a = g.lp[w];
+        const uint32_t a = (uint32_t)__builtin_amdgcn_readfirstlane((int)g.lp[w]);
         const uint32_t base = poff[a], slots = pcount[a];
         const bool small = slots <= WCAP;
         uint32_t fill = 0;
