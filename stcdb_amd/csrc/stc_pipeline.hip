@@ -1483,10 +1483,6 @@ __global__ __launch_bounds__(256) void k_meet_merge(const PairHit *__restrict__ 
 
 #define SEG_CAP 4096u  /* LDS sort keys of the four-wave segment sort */
 
-__global__ void k_fill_u32(uint32_t *a, uint64_t n, uint32_t v) {
-    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) a[i] = v;
-}
 __global__ void k_fill_obox(uint4 *a, uint32_t n) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) a[i] = make_uint4(100000000u, 100000000u, 0u, 0u); /* cuda_init_o_boxs */
