@@ -343,15 +343,14 @@ __global__ void k_iota(uint32_t *a, uint32_t n) {
 /* leaf-ordered SoA and each point's rank within its leaf */
 __global__ void k_gather(const double2 *__restrict__ xy, const double *__restrict__ dlon,
                          const uint32_t *__restrict__ lp, const uint32_t *__restrict__ pt_leaf,
-                         const uint32_t *__restrict__ leaf_start, uint32_t n, double *__restrict__ lx,
-                         double *__restrict__ ly, double *__restrict__ ld, uint32_t *__restrict__ pt_rank,
+                         const uint32_t *__restrict__ leaf_start, uint32_t n, double2 *__restrict__ lxy,
+                         double *__restrict__ ld, uint32_t *__restrict__ pt_rank,
                          uint32_t *__restrict__ lleaf) {
     uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
     uint32_t p = lp[j];
     double2 q = xy[p];
-    lx[j] = q.x;
-    ly[j] = q.y;
+    lxy[j] = q;
     ld[j] = dlon[p];
     const uint32_t L = pt_leaf[p];
     pt_rank[p] = j - leaf_start[L];
@@ -382,8 +381,8 @@ __global__ void k_overflow(const uint32_t *leaf_count, uint32_t nl, uint32_t buf
  * and mostly the same subtrees, so their descents stay together. Outputs are
  * per point id (count[p], units from offset[p]). */
 template <bool EMIT>
-__global__ void k_lookup(const uint32_t *__restrict__ lp, const double *__restrict__ lx,
-                         const double *__restrict__ ly, const double *__restrict__ ld, uint32_t n,
+__global__ void k_lookup(const uint32_t *__restrict__ lp, const double2 *__restrict__ lxy,
+                         const double *__restrict__ ld, uint32_t n,
                          const Node *__restrict__ nodes, const uint32_t *__restrict__ pt_lv, double r, double band,
                          uint32_t *__restrict__ count, const uint32_t *__restrict__ offset,
                          uint2 *__restrict__ units, Counters *cnt) {
@@ -395,7 +394,7 @@ __global__ void k_lookup(const uint32_t *__restrict__ lp, const double *__restri
         if (!EMIT) count[p] = 0;
         return;
     }
-    const double2 q = make_double2(lx[j], ly[j]);
+    const double2 q = lxy[j];
     const double dl = ld[j];
     uint32_t stk[STACK_DEPTH];
     int sp = 0;
@@ -453,7 +452,8 @@ __device__ inline unsigned long long wave_sum_u64(unsigned long long v) {
 /* geometry of the step that the pair passes read */
 struct PairGeo {
     const double2 *xy;
-    const double *dlon, *idlon, *lx, *ly;
+    const double *dlon, *idlon;
+    const double2 *lxy;                  /* positions in leaf order (one 16-byte read per candidate) */
     const uint32_t *pt_leaf, *pt_rank, *leaf_start, *leaf_count, *lp;
     const double *ld;                    /* dpm_lon in leaf order */
     const uint32_t *lleaf;               /* leaf of each point, in leaf order */
@@ -538,7 +538,7 @@ __device__ inline void owner_combos_by_source(uint32_t w, uint32_t a, const Pair
     uint32_t s = g.leaf_count[L];
     if (s > g.buf) s = g.buf;
     const uint32_t st = g.leaf_start[L], rank = w - st;
-    const double2 pa = make_double2(g.lx[w], g.ly[w]);
+    const double2 pa = g.lxy[w];
     const double pda = g.ld[w], pia = 1.0 / pda; /* the values of dlon[a] and idlon[a] (k_prep) */
     const float afx = (float)pa.x, afy = (float)pa.y;
     /* (i) */
@@ -549,7 +549,8 @@ __device__ inline void owner_combos_by_source(uint32_t w, uint32_t a, const Pair
         uint32_t q = 0;
         float4 mb = make_float4(0.f, 0.f, 0.f, 0.f);
         if (v) {
-            const double qx = g.lx[st + m], qy = g.ly[st + m];
+            const double2 qq = g.lxy[st + m];
+            const double qx = qq.x, qy = qq.y;
             dec = pair_decide(pa, pda, pia, qx, qy, g);
             q = g.lp[st + m];
             mb = pair_mbr(afx, afy, (float)qx, (float)qy);
@@ -571,7 +572,8 @@ __device__ inline void owner_combos_by_source(uint32_t w, uint32_t a, const Pair
             uint32_t q = 0;
             float4 mb = make_float4(0.f, 0.f, 0.f, 0.f);
             if (v) {
-                const double qx = g.lx[st2 + m], qy = g.ly[st2 + m];
+                const double2 qq = g.lxy[st2 + m];
+                const double qx = qq.x, qy = qq.y;
                 dec = pair_decide(pa, pda, pia, qx, qy, g);
                 q = g.lp[st2 + m];
                 mb = pair_mbr(afx, afy, (float)qx, (float)qy);
@@ -582,7 +584,7 @@ __device__ inline void owner_combos_by_source(uint32_t w, uint32_t a, const Pair
     /* (iii) */
     if (rank < s) {
         const uint32_t r0 = g.rev_start[L], r1 = g.rev_start[L + 1];
-        const double qx = g.lx[st + rank], qy = g.ly[st + rank];
+        const double qx = g.lxy[st + rank].x, qy = g.lxy[st + rank].y;
         const float qfx = (float)qx, qfy = (float)qy;
         for (uint32_t i0 = r0 + wave_upper_bound(g.rev_pid + r0, r1 - r0, a, lane); i0 < r1; i0 += STC_WAVE) {
             const uint32_t i = i0 + lane;
@@ -616,7 +618,7 @@ __device__ inline void owner_combos(uint32_t w, uint32_t a, const PairGeo &g, in
         return;
     }
     const uint4 pr = g.prange[w];
-    const double2 pa = make_double2(g.lx[w], g.ly[w]);
+    const double2 pa = g.lxy[w];
     const double pda = g.ld[w], pia = 1.0 / pda; /* the values of dlon[a] and idlon[a] (k_prep) */
     const float afx = (float)pa.x, afy = (float)pa.y;
     /* source table: first index (into the leaf-ordered arrays, or rev_pid for
@@ -657,7 +659,8 @@ __device__ inline void owner_combos(uint32_t w, uint32_t a, const PairGeo &g, in
             const uint32_t idx = jb + (c - jc);
             src = idx;
             if (j + 1 < nseg) { /* a member of the owner's leaf or of a cross unit's leaf; probe a */
-                const double qx = g.lx[idx], qy = g.ly[idx];
+                const double2 qq = g.lxy[idx];
+                const double qx = qq.x, qy = qq.y;
                 dec = pair_decide(pa, pda, pia, qx, qy, g);
                 q = g.lp[idx];
                 mb = pair_mbr(afx, afy, (float)qx, (float)qy);
@@ -888,7 +891,7 @@ __device__ inline void write_sorted_segment(const unsigned long long *keys, uint
         live[r] = i < slots && b != DEAD_B && (i == 0 || (uint32_t)(prev >> 32) != b);
         nlive += (uint32_t)__popcll(__ballot(live[r]));
     }
-    const float afx = (float)g.lx[w], afy = (float)g.ly[w];
+    const float afx = (float)g.lxy[w].x, afy = (float)g.lxy[w].y;
     uint32_t lr = 0, dr = 0;
 #pragma unroll
     for (int r = 0; r < K; r++) {
@@ -902,7 +905,8 @@ __device__ inline void write_sorted_segment(const unsigned long long *keys, uint
                 const double2 pp = g.rev_xy[src & ~SRC_REV];
                 mb = pair_mbr((float)pp.x, (float)pp.y, afx, afy);
             } else {
-                mb = pair_mbr(afx, afy, (float)g.lx[src], (float)g.ly[src]);
+                const double2 qq = g.lxy[src];
+                mb = pair_mbr(afx, afy, (float)qq.x, (float)qq.y);
             }
             PairHit e;
             e.key = ((unsigned long long)a << 32) | (live[r] ? (uint32_t)(x[r] >> 32) : DEAD_B);
@@ -1589,7 +1593,8 @@ struct StcCtx {
 
     DBuf<double2> xy_in;
     DBuf<uint32_t> codes, idx, codes_alt, idx_alt, temp;
-    DBuf<double> dlon, idlon, lx, ly, ld;
+    DBuf<double> dlon, idlon, ld;
+    DBuf<double2> lxy;
     DBuf<Node> nodes;
     DBuf<FrontierItem> frontier_a, frontier_b;
     DBuf<uint32_t> leaf_count, leaf_start, leaf_node;
@@ -1816,7 +1821,7 @@ int stc_create(const stc_config *cfg, const stc_box *mbr, void **out) {
     size_t node_cap = 1 + 4ull * NLEVELS * (n / (c->cap + 1) + 1);
     size_t front_cap = n / (c->cap + 1) + 2;
     if (c->xy_in.ensure(n) || c->codes.ensure(n) || c->idx.ensure(n) || c->codes_alt.ensure(n) ||
-        c->idx_alt.ensure(n) || c->dlon.ensure(n) || c->idlon.ensure(n) || c->lx.ensure(n) || c->ly.ensure(n) || c->ld.ensure(n) ||
+        c->idx_alt.ensure(n) || c->dlon.ensure(n) || c->idlon.ensure(n) || c->lxy.ensure(n) || c->ld.ensure(n) ||
         c->nodes.ensure(node_cap) || c->frontier_a.ensure(front_cap) || c->frontier_b.ensure(front_cap) ||
         c->pt_leaf.ensure(n) || c->pt_lv.ensure(n) || c->pt_rank.ensure(n) || c->lp.ensure(n) ||
         c->lleaf.ensure(n) || c->lunits.ensure(n) || c->prange.ensure(n) ||
@@ -1854,7 +1859,7 @@ int stc_destroy(void *vc) {
     if (!c) return STC_OK;
     if (c->s) hipStreamSynchronize(c->s);
     c->xy_in.release(); c->codes.release(); c->idx.release(); c->codes_alt.release(); c->idx_alt.release();
-    c->temp.release(); c->dlon.release(); c->idlon.release(); c->lx.release(); c->ly.release(); c->ld.release(); c->nodes.release();
+    c->temp.release(); c->dlon.release(); c->idlon.release(); c->lxy.release(); c->ld.release(); c->nodes.release();
     c->frontier_a.release(); c->frontier_b.release(); c->leaf_count.release(); c->leaf_start.release();
     c->leaf_node.release(); c->pt_leaf.release(); c->pt_lv.release(); c->pt_rank.release(); c->lp.release();
     c->lleaf.release(); c->lunits.release(); c->urange.release(); c->prange.release();
@@ -1966,7 +1971,7 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
     CKRC(stc_radix_sort_u32(c->lkeys.p, c->lp.p, c->codes_alt.p, c->idx_alt.p, n,
                             std::max(1, ilog2_ceil(n_leaves)), c->temp.p, s));
     hipLaunchKernelGGL(k_gather, dim3(blocks_for(n, 256)), dim3(256), 0, s, xy, c->dlon.p, c->lp.p, c->pt_leaf.p,
-                       c->leaf_start.p, n, c->lx.p, c->ly.p, c->ld.p, c->pt_rank.p, c->lleaf.p);
+                       c->leaf_start.p, n, c->lxy.p, c->ld.p, c->pt_rank.p, c->lleaf.p);
     hipLaunchKernelGGL(k_overflow, dim3(blocks_for(n_leaves, 256)), dim3(256), 0, s, c->leaf_count.p, n_leaves,
                        c->buf, cnt);
     CK(hipGetLastError());
@@ -1975,7 +1980,7 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
     /* 6: border lookup (count, scan, emit) */
     double r = c->cfg.reach_distance;
     hipLaunchKernelGGL(k_lookup<false>, dim3(blocks_for(n, 128)), dim3(128), 0, s, (const uint32_t *)c->lp.p,
-                       (const double *)c->lx.p, (const double *)c->ly.p, (const double *)c->ld.p, n, c->nodes.p,
+                       (const double2 *)c->lxy.p, (const double *)c->ld.p, n, c->nodes.p,
                        c->pt_lv.p, r, c->band, c->ucount.p, (const uint32_t *)nullptr, (uint2 *)nullptr, cnt);
     CK(hipGetLastError());
     uint32_t *units_total = (uint32_t *)((char *)cnt + offsetof(Counters, units));
@@ -1985,7 +1990,7 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
     uint64_t n_units = (uint32_t)h.units;
     if (c->units.ensure(n_units + 1)) return fail(c, STC_E_CAPACITY, "unit buffer allocation failed");
     hipLaunchKernelGGL(k_lookup<true>, dim3(blocks_for(n, 128)), dim3(128), 0, s, (const uint32_t *)c->lp.p,
-                       (const double *)c->lx.p, (const double *)c->ly.p, (const double *)c->ld.p, n, c->nodes.p,
+                       (const double2 *)c->lxy.p, (const double *)c->ld.p, n, c->nodes.p,
                        c->pt_lv.p, r, c->band, c->ucount.p, (const uint32_t *)c->uoffset.p, c->units.p, cnt);
     CK(hipGetLastError());
     CK(hipEventRecord(c->ev[2], s));
@@ -2027,7 +2032,7 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
                        (const uint32_t *)c->leaf_count.p, (const uint32_t *)c->rev_start.p,
                        (const uint32_t *)c->rev_pid.p, c->buf, c->prange.p);
     CK(hipGetLastError());
-    const PairGeo pg{xy, c->dlon.p, c->idlon.p, c->lx.p, c->ly.p, c->pt_leaf.p, c->pt_rank.p, c->leaf_start.p,
+    const PairGeo pg{xy, c->dlon.p, c->idlon.p, c->lxy.p, c->pt_leaf.p, c->pt_rank.p, c->leaf_start.p,
                      c->leaf_count.p, c->lp.p, c->ld.p, c->lleaf.p, c->lunits.p, c->urange.p, c->prange.p,
                      c->units.p, c->uoffset.p,
                      c->ucount.p, c->rev_start.p, c->rev_pid.p,
