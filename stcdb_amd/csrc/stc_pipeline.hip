@@ -450,6 +450,11 @@ __device__ inline unsigned long long wave_sum_u64(unsigned long long v) {
 }
 
 /* geometry of the step that the pair passes read */
+/* a border point of the reverse lists: position, 1/dpm_lon and dpm_lon in one 32-byte read */
+struct __align__(32) RevPt {
+    double x, y, il, dl;
+};
+
 struct PairGeo {
     const double2 *xy;
     const double *dlon, *idlon;
@@ -464,8 +469,7 @@ struct PairGeo {
     const uint2 *units;                  /* cross units (pid, leaf), point after point */
     const uint32_t *uoffset, *ucount;    /* a point's cross units: units[uoffset[p] ..] */
     const uint32_t *rev_start, *rev_pid; /* per leaf, by pid: the border points with a cross unit to it */
-    const double2 *rev_xy;               /* their positions, dpm_lon and its inverse, in rev_pid order */
-    const double *rev_dl, *rev_il;
+    const RevPt *rev;                    /* their positions, dpm_lon and its inverse, in rev_pid order */
     uint32_t n, buf;
     double r, band, r2_lo, r2_hi;
 };
@@ -594,8 +598,9 @@ __device__ inline void owner_combos_by_source(uint32_t w, uint32_t a, const Pair
             float4 mb = make_float4(0.f, 0.f, 0.f, 0.f);
             if (v) {
                 p = g.rev_pid[i];
-                const double2 pp = g.rev_xy[i];
-                dec = pair_decide(pp, g.rev_dl[i], g.rev_il[i], qx, qy, g);
+                const RevPt rp = g.rev[i];
+                const double2 pp = make_double2(rp.x, rp.y);
+                dec = pair_decide(pp, rp.dl, rp.il, qx, qy, g);
                 mb = pair_mbr((float)pp.x, (float)pp.y, qfx, qfy);
             }
             f(v, p, dec, p, mb, SRC_REV | i);
@@ -666,8 +671,9 @@ __device__ inline void owner_combos(uint32_t w, uint32_t a, const PairGeo &g, in
                 mb = pair_mbr(afx, afy, (float)qx, (float)qy);
             } else { /* a border point with a cross unit to a's leaf; probe the border point */
                 const uint32_t p = g.rev_pid[idx];
-                const double2 pp = g.rev_xy[idx];
-                dec = pair_decide(pp, g.rev_dl[idx], g.rev_il[idx], qax, qay, g);
+                const RevPt rp = g.rev[idx];
+                const double2 pp = make_double2(rp.x, rp.y);
+                dec = pair_decide(pp, rp.dl, rp.il, qax, qay, g);
                 q = p;
                 probe = p;
                 src = SRC_REV | idx;
@@ -698,13 +704,17 @@ __global__ void k_rev_starts(const uint32_t *__restrict__ sorted_leaf, uint32_t 
 /* the border points' data in reverse-list order, so that owners read it coalesced */
 __global__ void k_rev_gather(const uint32_t *__restrict__ rev_pid, uint32_t nr, const double2 *__restrict__ xy,
                              const double *__restrict__ dlon, const double *__restrict__ idlon,
-                             double2 *__restrict__ rxy, double *__restrict__ rdl, double *__restrict__ ril) {
+                             RevPt *__restrict__ rev) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nr) return;
     const uint32_t p = rev_pid[i];
-    rxy[i] = xy[p];
-    rdl[i] = dlon[p];
-    ril[i] = idlon[p];
+    const double2 q = xy[p];
+    RevPt r;
+    r.x = q.x;
+    r.y = q.y;
+    r.il = idlon[p];
+    r.dl = dlon[p];
+    rev[i] = r;
 }
 
 /* per cross unit (p, leaf): the leaf's kept members above p, as an index range
@@ -902,7 +912,8 @@ __device__ inline void write_sorted_segment(const unsigned long long *keys, uint
             const uint32_t src = (uint32_t)x[r];
             float4 mb;
             if (src & SRC_REV) { /* found by the border point: its position first, as pass 1 forms it */
-                const double2 pp = g.rev_xy[src & ~SRC_REV];
+                const RevPt &rp = g.rev[src & ~SRC_REV];
+                const double2 pp = make_double2(rp.x, rp.y);
                 mb = pair_mbr((float)pp.x, (float)pp.y, afx, afy);
             } else {
                 const double2 qq = g.lxy[src];
@@ -1608,8 +1619,7 @@ struct StcCtx {
     DBuf<uint2> amb;                     /* band combinations (probe, other) for the host */
     /* per leaf, by pid: the border points with a cross unit to it */
     DBuf<uint32_t> rev_leaf, rev_pid, rev_leaf_alt, rev_pid_alt, rev_start;
-    DBuf<double2> rev_xy;
-    DBuf<double> rev_dl, rev_il;
+    DBuf<RevPt> rev;
     /* the step's pairs by owner segment, and the live meetings: two arrays of
      * segments (offsets [n + 1], live buckets per segment [n]) used in turn */
     DBuf<uint32_t> pcount;
@@ -1867,7 +1877,7 @@ int stc_destroy(void *vc) {
     c->amb.release();
     c->rev_leaf.release(); c->rev_pid.release(); c->rev_leaf_alt.release(); c->rev_pid_alt.release();
     c->rev_start.release();
-    c->rev_xy.release(); c->rev_dl.release(); c->rev_il.release();
+    c->rev.release();
     c->pcount.release(); c->pnew.release(); c->acc.release();
     for (int i = 0; i < 2; i++) { c->meet[i].release(); c->moff[i].release(); }
     c->ptmp.release(); c->tbounds.release();
@@ -1997,7 +2007,7 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
 
     /* 7: reverse cross units: per leaf, the border points with a cross unit to it, by pid */
     if (c->rev_leaf.ensure(n_units + 1) || c->rev_pid.ensure(n_units + 1) || c->rev_leaf_alt.ensure(n_units + 1) ||
-        c->rev_xy.ensure(n_units + 1) || c->rev_dl.ensure(n_units + 1) || c->rev_il.ensure(n_units + 1) ||
+        c->rev.ensure(n_units + 1) ||
         c->rev_pid_alt.ensure(n_units + 1) ||
         c->rev_start.ensure(n_leaves + 1ull) ||
         c->temp.ensure(stc_radix_temp_words((uint32_t)std::max<uint64_t>(std::max<uint64_t>(n_units, n),
@@ -2010,8 +2020,7 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
         CKRC(stc_radix_sort_u32(c->rev_leaf.p, c->rev_pid.p, c->rev_leaf_alt.p, c->rev_pid_alt.p, (uint32_t)n_units,
                                 std::max(1, ilog2_ceil(n_leaves)), c->temp.p, s));
         hipLaunchKernelGGL(k_rev_gather, dim3(blocks_for(n_units, 256)), dim3(256), 0, s, (const uint32_t *)c->rev_pid.p,
-                           (uint32_t)n_units, xy, (const double *)c->dlon.p, (const double *)c->idlon.p, c->rev_xy.p,
-                           c->rev_dl.p, c->rev_il.p);
+                           (uint32_t)n_units, xy, (const double *)c->dlon.p, (const double *)c->idlon.p, c->rev.p);
         CK(hipGetLastError());
     }
     hipLaunchKernelGGL(k_rev_starts, dim3(blocks_for(n_leaves + 1ull, 256)), dim3(256), 0, s,
@@ -2036,7 +2045,7 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
                      c->leaf_count.p, c->lp.p, c->ld.p, c->lleaf.p, c->lunits.p, c->urange.p, c->prange.p,
                      c->units.p, c->uoffset.p,
                      c->ucount.p, c->rev_start.p, c->rev_pid.p,
-                     c->rev_xy.p, c->rev_dl.p, c->rev_il.p, n, c->buf, r, c->band, r * r * (1.0 - 1e-12),
+                     c->rev.p, n, c->buf, r, c->band, r * r * (1.0 - 1e-12),
                      r * r * (1.0 + 1e-12)};
     /* a multiple of 8 workgroups, for the XCD-aware owner order */
     const uint32_t pgrid = (uint32_t)std::min<uint64_t>((blocks_for((uint64_t)n * STC_WAVE, 256) + 7) & ~7ull, 256u * 64u);
