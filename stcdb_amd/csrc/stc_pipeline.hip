@@ -536,7 +536,7 @@ __device__ inline uint32_t wave_upper_bound(const uint32_t *arr, uint32_t n, uin
     return lo;
 }
 
-template <class F>
+template <bool OTHER, class F>
 __device__ inline void owner_combos_by_source(uint32_t w, uint32_t a, const PairGeo &g, int lane, F &f) {
     const uint32_t L = g.lleaf[w];
     uint32_t s = g.leaf_count[L];
@@ -556,7 +556,7 @@ __device__ inline void owner_combos_by_source(uint32_t w, uint32_t a, const Pair
             const double2 qq = g.lxy[st + m];
             const double qx = qq.x, qy = qq.y;
             dec = pair_decide(pa, pda, pia, qx, qy, g);
-            q = g.lp[st + m];
+            if (OTHER) q = g.lp[st + m];
             mb = pair_mbr(afx, afy, (float)qx, (float)qy);
         }
         f(v, q, dec, a, mb, st + m);
@@ -579,7 +579,7 @@ __device__ inline void owner_combos_by_source(uint32_t w, uint32_t a, const Pair
                 const double2 qq = g.lxy[st2 + m];
                 const double qx = qq.x, qy = qq.y;
                 dec = pair_decide(pa, pda, pia, qx, qy, g);
-                q = g.lp[st2 + m];
+                if (OTHER) q = g.lp[st2 + m];
                 mb = pair_mbr(afx, afy, (float)qx, (float)qy);
             }
             f(v, q, dec, a, mb, st2 + m);
@@ -597,7 +597,7 @@ __device__ inline void owner_combos_by_source(uint32_t w, uint32_t a, const Pair
             uint32_t p = 0;
             float4 mb = make_float4(0.f, 0.f, 0.f, 0.f);
             if (v) {
-                p = g.rev_pid[i];
+                if (OTHER) p = g.rev_pid[i];
                 const RevPt rp = g.rev[i];
                 const double2 pp = make_double2(rp.x, rp.y);
                 dec = pair_decide(pp, rp.dl, rp.il, qx, qy, g);
@@ -609,17 +609,19 @@ __device__ inline void owner_combos_by_source(uint32_t w, uint32_t a, const Pair
 }
 
 /* The same combinations as owner_combos_by_source, with the owner's sources
- * laid end to end so that every call of f but the last fills all 64 lanes:
+ * laid end to end so that every call of f but the last fills all 64 lanes
+ * (OTHER false: the other point's id and the probe are left out of the call,
+ * for a caller that rebuilds them from the source index when it needs them):
  * lane j describes source j (j = 0: (i); 1 .. nu: the cross units; nu + 1:
  * (iii)) by its first index and length, and each lane finds the source of its
  * combination from the prefix of the lengths. Owners with more than 62 cross
  * units take the per-source loops. Wave-uniform. */
-template <class F>
+template <bool OTHER, class F>
 __device__ inline void owner_combos(uint32_t w, uint32_t a, const PairGeo &g, int lane, F &f) {
     const uint2 un = g.lunits[w];
     const uint32_t u0 = un.x, nu = un.y;
     if (nu + 2 > STC_WAVE) {
-        owner_combos_by_source(w, a, g, lane, f);
+        owner_combos_by_source<OTHER>(w, a, g, lane, f);
         return;
     }
     const uint4 pr = g.prange[w];
@@ -667,10 +669,10 @@ __device__ inline void owner_combos(uint32_t w, uint32_t a, const PairGeo &g, in
                 const double2 qq = g.lxy[idx];
                 const double qx = qq.x, qy = qq.y;
                 dec = pair_decide(pa, pda, pia, qx, qy, g);
-                q = g.lp[idx];
+                if (OTHER) q = g.lp[idx];
                 mb = pair_mbr(afx, afy, (float)qx, (float)qy);
             } else { /* a border point with a cross unit to a's leaf; probe the border point */
-                const uint32_t p = g.rev_pid[idx];
+                const uint32_t p = OTHER ? g.rev_pid[idx] : 0u;
                 const RevPt rp = g.rev[idx];
                 const double2 pp = make_double2(rp.x, rp.y);
                 dec = pair_decide(pp, rp.dl, rp.il, qax, qay, g);
@@ -771,16 +773,18 @@ __global__ __launch_bounds__(256, 8) void k_pair_count(PairGeo g, uint32_t *__re
     for (uint64_t w = wid; w < g.n; w += nw) {
         const uint32_t a = (uint32_t)__builtin_amdgcn_readfirstlane((int)g.lp[w]);
         unsigned long long c = 0;
-        auto f = [&](bool v, uint32_t q, int dec, uint32_t probe, float4, uint32_t) {
+        auto f = [&](bool v, uint32_t, int dec, uint32_t, float4, uint32_t src) {
             if (!v) return;
             cand++;
             if (dec) c++;
-            if (dec == 2) {
+            if (dec == 2) { /* (probe, other): the border point probes in a reverse-list combination */
+                const bool rev = (src & SRC_REV) != 0;
+                const uint32_t o = rev ? g.rev_pid[src & ~SRC_REV] : g.lp[src];
                 const unsigned long long k = atomicAdd(&cnt->ambiguous, 1ull);
-                if (k < amb_cap) amb[k] = make_uint2(probe, probe == a ? q : a);
+                if (k < amb_cap) amb[k] = rev ? make_uint2(o, a) : make_uint2(a, o);
             }
         };
-        owner_combos((uint32_t)w, a, g, lane, f);
+        owner_combos<false>((uint32_t)w, a, g, lane, f);
         c = wave_sum_u64(c);
         if (lane == 0) {
             pcount[a] = (uint32_t)c;
@@ -1036,7 +1040,7 @@ __global__ __launch_bounds__(256, 7) void k_pair_write(PairGeo g, const uint32_t
             }
             fill += (uint32_t)__popcll(m);
         };
-        owner_combos((uint32_t)w, a, g, lane, f);
+        owner_combos<true>((uint32_t)w, a, g, lane, f);
         if (!small) {
             if (lane == 0) big[atomicAdd(&cnt->big, 1u)] = a; /* the list holds every owner */
             continue;
