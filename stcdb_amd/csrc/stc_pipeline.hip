@@ -450,9 +450,11 @@ __device__ inline unsigned long long wave_sum_u64(unsigned long long v) {
 }
 
 /* geometry of the step that the pair passes read */
-/* a border point of the reverse lists: position, 1/dpm_lon and dpm_lon in one 32-byte read */
+/* a border point of the reverse lists in one 32-byte read: position, 1/dpm_lon
+ * and id (its dpm_lon, needed only inside the guard band, is kept apart) */
 struct __align__(32) RevPt {
-    double x, y, il, dl;
+    double x, y, il;
+    uint32_t pid, pad;
 };
 
 struct PairGeo {
@@ -469,7 +471,8 @@ struct PairGeo {
     const uint2 *units;                  /* cross units (pid, leaf), point after point */
     const uint32_t *uoffset, *ucount;    /* a point's cross units: units[uoffset[p] ..] */
     const uint32_t *rev_start, *rev_pid; /* per leaf, by pid: the border points with a cross unit to it */
-    const RevPt *rev;                    /* their positions, dpm_lon and its inverse, in rev_pid order */
+    const RevPt *rev;                    /* their positions, 1/dpm_lon and ids, in rev_pid order */
+    const double *rev_dl;                /* their dpm_lon */
     uint32_t n, buf;
     double r, band, r2_lo, r2_hi;
 };
@@ -479,13 +482,13 @@ struct PairGeo {
  * sin, geometry.h:40-48). The squared distance with reciprocal multiplies is
  * within 1.2e-15 (relative) of the formula's, so only a band of 1e-12 around
  * r^2 needs the exact divisions and square root (cuda_util.cuh:41-53). */
-__device__ inline int pair_decide(double2 pp, double pd, double pinv, double qx, double qy, const PairGeo &g) {
+__device__ inline int pair_decide(double2 pp, const double *pd, double pinv, double qx, double qy, const PairGeo &g) {
     const double ax = (pp.x - qx) * pinv;
     const double ay = (pp.y - qy) * STC_INV_DPM_LAT;
     const double s2 = ax * ax + ay * ay;
     if (s2 < g.r2_lo) return 1;
     if (s2 <= g.r2_hi) {
-        const double d = stc_point_dist(pp.x, pp.y, qx, qy, pd);
+        const double d = stc_point_dist(pp.x, pp.y, qx, qy, *pd); /* dpm_lon of the probe: read only here */
         if (fabs(d - g.r) <= g.band) return 2;
         return d <= g.r ? 1 : 0;
     }
@@ -555,7 +558,7 @@ __device__ inline void owner_combos_by_source(uint32_t w, uint32_t a, const Pair
         if (v) {
             const double2 qq = g.lxy[st + m];
             const double qx = qq.x, qy = qq.y;
-            dec = pair_decide(pa, pda, pia, qx, qy, g);
+            dec = pair_decide(pa, &g.ld[w], pia, qx, qy, g);
             if (OTHER) q = g.lp[st + m];
             mb = pair_mbr(afx, afy, (float)qx, (float)qy);
         }
@@ -578,7 +581,7 @@ __device__ inline void owner_combos_by_source(uint32_t w, uint32_t a, const Pair
             if (v) {
                 const double2 qq = g.lxy[st2 + m];
                 const double qx = qq.x, qy = qq.y;
-                dec = pair_decide(pa, pda, pia, qx, qy, g);
+                dec = pair_decide(pa, &g.ld[w], pia, qx, qy, g);
                 if (OTHER) q = g.lp[st2 + m];
                 mb = pair_mbr(afx, afy, (float)qx, (float)qy);
             }
@@ -597,10 +600,10 @@ __device__ inline void owner_combos_by_source(uint32_t w, uint32_t a, const Pair
             uint32_t p = 0;
             float4 mb = make_float4(0.f, 0.f, 0.f, 0.f);
             if (v) {
-                if (OTHER) p = g.rev_pid[i];
+                if (OTHER) p = g.rev[i].pid;
                 const RevPt rp = g.rev[i];
                 const double2 pp = make_double2(rp.x, rp.y);
-                dec = pair_decide(pp, rp.dl, rp.il, qx, qy, g);
+                dec = pair_decide(pp, &g.rev_dl[i], rp.il, qx, qy, g);
                 mb = pair_mbr((float)pp.x, (float)pp.y, qfx, qfy);
             }
             f(v, p, dec, p, mb, SRC_REV | i);
@@ -666,16 +669,16 @@ __device__ inline void owner_combos(uint32_t w, uint32_t a, const PairGeo &g, in
             const uint32_t idx = jb + (c - jc);
             src = idx;
             if (j + 1 < nseg) { /* a member of the owner's leaf or of a cross unit's leaf; probe a */
+                if (OTHER) q = g.lp[idx]; /* read with the position, not after the decision */
                 const double2 qq = g.lxy[idx];
                 const double qx = qq.x, qy = qq.y;
-                dec = pair_decide(pa, pda, pia, qx, qy, g);
-                if (OTHER) q = g.lp[idx];
+                dec = pair_decide(pa, &g.ld[w], pia, qx, qy, g);
                 mb = pair_mbr(afx, afy, (float)qx, (float)qy);
             } else { /* a border point with a cross unit to a's leaf; probe the border point */
-                const uint32_t p = OTHER ? g.rev_pid[idx] : 0u;
                 const RevPt rp = g.rev[idx];
+                const uint32_t p = rp.pid;
                 const double2 pp = make_double2(rp.x, rp.y);
-                dec = pair_decide(pp, rp.dl, rp.il, qax, qay, g);
+                dec = pair_decide(pp, &g.rev_dl[idx], rp.il, qax, qay, g);
                 q = p;
                 probe = p;
                 src = SRC_REV | idx;
@@ -706,7 +709,7 @@ __global__ void k_rev_starts(const uint32_t *__restrict__ sorted_leaf, uint32_t 
 /* the border points' data in reverse-list order, so that owners read it coalesced */
 __global__ void k_rev_gather(const uint32_t *__restrict__ rev_pid, uint32_t nr, const double2 *__restrict__ xy,
                              const double *__restrict__ dlon, const double *__restrict__ idlon,
-                             RevPt *__restrict__ rev) {
+                             RevPt *__restrict__ rev, double *__restrict__ rev_dl) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nr) return;
     const uint32_t p = rev_pid[i];
@@ -715,8 +718,10 @@ __global__ void k_rev_gather(const uint32_t *__restrict__ rev_pid, uint32_t nr, 
     r.x = q.x;
     r.y = q.y;
     r.il = idlon[p];
-    r.dl = dlon[p];
+    r.pid = p;
+    r.pad = 0;
     rev[i] = r;
+    rev_dl[i] = dlon[p];
 }
 
 /* per cross unit (p, leaf): the leaf's kept members above p, as an index range
@@ -1624,6 +1629,7 @@ struct StcCtx {
     /* per leaf, by pid: the border points with a cross unit to it */
     DBuf<uint32_t> rev_leaf, rev_pid, rev_leaf_alt, rev_pid_alt, rev_start;
     DBuf<RevPt> rev;
+    DBuf<double> rev_dl;
     /* the step's pairs by owner segment, and the live meetings: two arrays of
      * segments (offsets [n + 1], live buckets per segment [n]) used in turn */
     DBuf<uint32_t> pcount;
@@ -1881,7 +1887,7 @@ int stc_destroy(void *vc) {
     c->amb.release();
     c->rev_leaf.release(); c->rev_pid.release(); c->rev_leaf_alt.release(); c->rev_pid_alt.release();
     c->rev_start.release();
-    c->rev.release();
+    c->rev.release(); c->rev_dl.release();
     c->pcount.release(); c->pnew.release(); c->acc.release();
     for (int i = 0; i < 2; i++) { c->meet[i].release(); c->moff[i].release(); }
     c->ptmp.release(); c->tbounds.release();
@@ -2011,7 +2017,7 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
 
     /* 7: reverse cross units: per leaf, the border points with a cross unit to it, by pid */
     if (c->rev_leaf.ensure(n_units + 1) || c->rev_pid.ensure(n_units + 1) || c->rev_leaf_alt.ensure(n_units + 1) ||
-        c->rev.ensure(n_units + 1) ||
+        c->rev.ensure(n_units + 1) || c->rev_dl.ensure(n_units + 1) ||
         c->rev_pid_alt.ensure(n_units + 1) ||
         c->rev_start.ensure(n_leaves + 1ull) ||
         c->temp.ensure(stc_radix_temp_words((uint32_t)std::max<uint64_t>(std::max<uint64_t>(n_units, n),
@@ -2024,7 +2030,8 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
         CKRC(stc_radix_sort_u32(c->rev_leaf.p, c->rev_pid.p, c->rev_leaf_alt.p, c->rev_pid_alt.p, (uint32_t)n_units,
                                 std::max(1, ilog2_ceil(n_leaves)), c->temp.p, s));
         hipLaunchKernelGGL(k_rev_gather, dim3(blocks_for(n_units, 256)), dim3(256), 0, s, (const uint32_t *)c->rev_pid.p,
-                           (uint32_t)n_units, xy, (const double *)c->dlon.p, (const double *)c->idlon.p, c->rev.p);
+                           (uint32_t)n_units, xy, (const double *)c->dlon.p, (const double *)c->idlon.p, c->rev.p,
+                           c->rev_dl.p);
         CK(hipGetLastError());
     }
     hipLaunchKernelGGL(k_rev_starts, dim3(blocks_for(n_leaves + 1ull, 256)), dim3(256), 0, s,
@@ -2049,7 +2056,7 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
                      c->leaf_count.p, c->lp.p, c->ld.p, c->lleaf.p, c->lunits.p, c->urange.p, c->prange.p,
                      c->units.p, c->uoffset.p,
                      c->ucount.p, c->rev_start.p, c->rev_pid.p,
-                     c->rev.p, n, c->buf, r, c->band, r * r * (1.0 - 1e-12),
+                     c->rev.p, c->rev_dl.p, n, c->buf, r, c->band, r * r * (1.0 - 1e-12),
                      r * r * (1.0 + 1e-12)};
     /* a multiple of 8 workgroups, for the XCD-aware owner order */
     const uint32_t pgrid = (uint32_t)std::min<uint64_t>((blocks_for((uint64_t)n * STC_WAVE, 256) + 7) & ~7ull, 256u * 64u);
