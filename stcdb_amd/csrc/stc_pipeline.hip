@@ -708,20 +708,20 @@ __global__ void k_rev_starts(const uint32_t *__restrict__ sorted_leaf, uint32_t 
 
 /* the border points' data in reverse-list order, so that owners read it coalesced */
 __global__ void k_rev_gather(const uint32_t *__restrict__ rev_pid, uint32_t nr, const double2 *__restrict__ xy,
-                             const double *__restrict__ dlon, const double *__restrict__ idlon,
-                             RevPt *__restrict__ rev, double *__restrict__ rev_dl) {
+                             const double *__restrict__ dlon, RevPt *__restrict__ rev, double *__restrict__ rev_dl) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nr) return;
     const uint32_t p = rev_pid[i];
     const double2 q = xy[p];
+    const double d = dlon[p];
     RevPt r;
     r.x = q.x;
     r.y = q.y;
-    r.il = idlon[p];
+    r.il = 1.0 / d; /* idlon[p], computed as k_prep computes it instead of a third gather */
     r.pid = p;
     r.pad = 0;
     rev[i] = r;
-    rev_dl[i] = dlon[p];
+    rev_dl[i] = d;
 }
 
 /* per cross unit (p, leaf): the leaf's kept members above p, as an index range
@@ -2030,7 +2030,7 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
         CKRC(stc_radix_sort_u32(c->rev_leaf.p, c->rev_pid.p, c->rev_leaf_alt.p, c->rev_pid_alt.p, (uint32_t)n_units,
                                 std::max(1, ilog2_ceil(n_leaves)), c->temp.p, s));
         hipLaunchKernelGGL(k_rev_gather, dim3(blocks_for(n_units, 256)), dim3(256), 0, s, (const uint32_t *)c->rev_pid.p,
-                           (uint32_t)n_units, xy, (const double *)c->dlon.p, (const double *)c->idlon.p, c->rev.p,
+                           (uint32_t)n_units, xy, (const double *)c->dlon.p, c->rev.p,
                            c->rev_dl.p);
         CK(hipGetLastError());
     }
