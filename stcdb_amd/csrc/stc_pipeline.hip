@@ -1003,11 +1003,12 @@ __global__ __launch_bounds__(256, 5) void k_seg_sort_wave(const uint32_t *__rest
 /* Pass 2: the same combinations, written to pn as each owner's segment sorted
  * by partner, its live pairs first and its dead slots (band combinations the
  * host rejected, repeats of a pair) behind them, so that pn is sorted by key.
- * A segment of at most WCAP slots is sorted in the wave's LDS as (partner,
- * source) keys, each pair's MBR recomputed from its source's position when
- * written; a longer one is written unsorted to ptmp and listed in `big` for
- * k_seg_sort. The pass repeats pass 1's arithmetic exactly; a slot beyond the
- * counted ones would be dropped and reported through table_fail. */
+ * A segment of at most WCAP slots is staged in the wave's LDS as (partner,
+ * source) keys and sorted in registers, each pair's MBR recomputed from its
+ * source's position when written; a longer one is written unsorted to ptmp
+ * and listed in `big` for the segment sorts (k_seg_sort_wave, k_seg_sort).
+ * The pass repeats pass 1's arithmetic exactly; a slot beyond the counted
+ * ones would be dropped and reported through table_fail. */
 __global__ __launch_bounds__(256, 7) void k_pair_write(PairGeo g, const uint32_t *__restrict__ poff,
                                                     const uint32_t *__restrict__ pcount,
                                                     const unsigned long long *__restrict__ acc, uint32_t n_acc,
