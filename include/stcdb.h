@@ -207,8 +207,10 @@ int stc_drop_runs(void *ctx); /* release host copies of flushed runs */
  * merge of tools/compaction.cpp:729-1206. */
 int stc_compact_runs(void *ctx, uint32_t first, uint32_t count);
 /* Wait until all work of the context is done, including the background copies
- * of flushed runs into host memory (a flush hands its run to a copy stream so
- * that the copy overlaps the following steps; stc_get_run waits for its own run). */
+ * of flushed runs into host memory (a flush hands its run to a copy stream,
+ * queued when the next step starts its pair count so that the copy overlaps
+ * it; stc_get_run, stc_dump_run and stc_sync queue a waiting copy first, and
+ * stc_get_run waits for its own run). */
 int stc_sync(void *ctx);
 
 /* ---- flushed runs on disk ----
