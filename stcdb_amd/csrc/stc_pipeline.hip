@@ -938,65 +938,24 @@ __device__ inline void write_sorted_segment(const unsigned long long *keys, uint
     }
 }
 
-/* the same for a segment staged unsorted in tmp (MBRs stored with the slots) */
-template <int K>
-__device__ inline void write_sorted_from_tmp(const PairHit *__restrict__ tmp, uint32_t slots, uint32_t a,
-                                             PairHit *__restrict__ seg, int lane) {
-    const uint64_t lt = lanemask_lt(lane);
-    unsigned long long x[K];
-#pragma unroll
-    for (int r = 0; r < K; r++) {
-        const uint32_t i = r * STC_WAVE + lane;
-        x[r] = i < slots ? (((unsigned long long)(uint32_t)tmp[i].key << 32) | i) : ~0ull;
-    }
-    reg_bitonic<K>(x, lane);
-    bool live[K];
-    uint32_t nlive = 0;
-#pragma unroll
-    for (int r = 0; r < K; r++) {
-        const unsigned long long up = __shfl_up(x[r], 1, STC_WAVE);
-        const unsigned long long last = r > 0 ? __shfl(x[r > 0 ? r - 1 : 0], STC_WAVE - 1, STC_WAVE) : ~0ull;
-        const unsigned long long prev = lane > 0 ? up : last;
-        const uint32_t i = r * STC_WAVE + lane;
-        const uint32_t b = (uint32_t)(x[r] >> 32);
-        live[r] = i < slots && b != DEAD_B && (i == 0 || (uint32_t)(prev >> 32) != b);
-        nlive += (uint32_t)__popcll(__ballot(live[r]));
-    }
-    uint32_t lr = 0, dr = 0;
-#pragma unroll
-    for (int r = 0; r < K; r++) {
-        const uint32_t i = r * STC_WAVE + lane;
-        const bool valid = i < slots;
-        const uint64_t ml = __ballot(live[r]), md = __ballot(valid && !live[r]);
-        if (valid) {
-            PairHit e = tmp[(uint32_t)x[r]];
-            e.key = ((unsigned long long)a << 32) | (live[r] ? (uint32_t)(x[r] >> 32) : DEAD_B);
-            seg[live[r] ? lr + (uint32_t)__popcll(ml & lt) : nlive + dr + (uint32_t)__popcll(md & lt)] = e;
-        }
-        lr += (uint32_t)__popcll(ml);
-        dr += (uint32_t)__popcll(md);
-    }
-}
+#define WSORT_CAP 1024u /* slots of the segments k_seg_sort_wave sorts in registers */
 
-#define WSORT_CAP 1024u /* slots a wave sorts in registers when the write pass could not */
-
-/* segments the write pass left in ptmp: one wave each, up to WSORT_CAP slots;
- * longer ones go to `spill` for the block sort */
-__global__ __launch_bounds__(256, 5) void k_seg_sort_wave(const uint32_t *__restrict__ owners, uint32_t n_owners,
-                                                       const uint32_t *__restrict__ poff,
-                                                       const PairHit *__restrict__ ptmp, PairHit *__restrict__ pn,
-                                                       uint32_t *__restrict__ spill, uint32_t *spill_count) {
+/* Segments of WCAP + 1 .. WSORT_CAP slots, which the write pass left in ptmp
+ * as (partner, source) keys only: one wave each, sorted in registers and
+ * written like the write pass's own segments (MBRs from the sources'
+ * positions). */
+__global__ __launch_bounds__(256, 5) void k_seg_sort_wave(PairGeo g, const uint32_t *__restrict__ owners,
+                                                       uint32_t n_owners, const uint32_t *__restrict__ poff,
+                                                       PairHit *__restrict__ ptmp, PairHit *__restrict__ pn) {
     const int lane = threadIdx.x & (STC_WAVE - 1);
     const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / STC_WAVE;
     const uint32_t nw = (gridDim.x * blockDim.x) / STC_WAVE;
     for (uint32_t oi = wid; oi < n_owners; oi += nw) {
         const uint32_t a = owners[oi];
         const uint32_t base = poff[a], slots = poff[a + 1] - base;
-        if (slots > WSORT_CAP) {
-            if (lane == 0) spill[atomicAdd(spill_count, 1u)] = a;
-            continue;
-        }
-        write_sorted_from_tmp<WSORT_CAP / STC_WAVE>(ptmp + base, slots, a, pn + base, lane);
+        const uint32_t w = g.leaf_start[g.pt_leaf[a]] + g.pt_rank[a]; /* a's place in leaf order */
+        write_sorted_segment<WSORT_CAP / STC_WAVE>((const unsigned long long *)(ptmp + base), slots, a, w, g,
+                                                    pn + base, lane);
     }
 }
 
@@ -1013,7 +972,8 @@ __global__ __launch_bounds__(256, 7) void k_pair_write(PairGeo g, const uint32_t
                                                     const uint32_t *__restrict__ pcount,
                                                     const unsigned long long *__restrict__ acc, uint32_t n_acc,
                                                     PairHit *__restrict__ ptmp, PairHit *__restrict__ pn,
-                                                    uint32_t *__restrict__ big, Counters *cnt) {
+                                                    uint32_t *__restrict__ big, uint32_t *__restrict__ huge,
+                                                    Counters *cnt) {
     __shared__ unsigned long long wk[256 / STC_WAVE][WCAP];
     const int lane = threadIdx.x & (STC_WAVE - 1);
     unsigned long long *keys = wk[threadIdx.x / STC_WAVE];
@@ -1037,6 +997,8 @@ __global__ __launch_bounds__(256, 7) void k_pair_write(PairGeo g, const uint32_t
                     atomicOr(&cnt->table_fail, 1u);
                 } else if (small) {
                     keys[at] = ((unsigned long long)bq << 32) | src;
+                } else if (slots <= WSORT_CAP) { /* keys only, at the start of the segment's staging room */
+                    ((unsigned long long *)(ptmp + base))[at] = ((unsigned long long)bq << 32) | src;
                 } else {
                     PairHit h;
                     h.key = ((unsigned long long)a << 32) | bq;
@@ -1047,8 +1009,11 @@ __global__ __launch_bounds__(256, 7) void k_pair_write(PairGeo g, const uint32_t
             fill += (uint32_t)__popcll(m);
         };
         owner_combos<true>((uint32_t)w, a, g, lane, f);
-        if (!small) {
-            if (lane == 0) big[atomicAdd(&cnt->big, 1u)] = a; /* the list holds every owner */
+        if (!small) { /* the lists hold every owner */
+            if (lane == 0) {
+                if (slots <= WSORT_CAP) big[atomicAdd(&cnt->big, 1u)] = a;
+                else huge[atomicAdd(&cnt->huge, 1u)] = a;
+            }
             continue;
         }
         if (slots == 0) continue;
@@ -2107,18 +2072,16 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
     CK(hipEventRecord(c->ev[6], s));
     hipLaunchKernelGGL(k_pair_write, dim3(pgrid), dim3(256), 0, s, pg, (const uint32_t *)c->moff[nx].p,
                        (const uint32_t *)c->pcount.p, c->acc.p, (uint32_t)accepted.size(), c->ptmp.p, c->pnew.p,
-                       c->spill_big.p, cnt);
+                       c->spill_big.p, c->spill_huge.p, cnt);
     CK(hipGetLastError());
     CK(hipEventRecord(c->ev[7], s));
     CKRC(sync_counters(c));
     h = *c->hcnt;
     if (h.big) { /* segments longer than the write pass sorts: by waves, by blocks, in global scratch */
-        hipLaunchKernelGGL(k_seg_sort_wave, dim3(blocks_for((uint64_t)h.big * STC_WAVE, 256)), dim3(256), 0, s,
-                           (const uint32_t *)c->spill_big.p, h.big, (const uint32_t *)c->moff[nx].p,
-                           (const PairHit *)c->ptmp.p, c->pnew.p, c->spill_huge.p, &cnt->huge);
+        hipLaunchKernelGGL(k_seg_sort_wave, dim3(blocks_for((uint64_t)h.big * STC_WAVE, 256)), dim3(256), 0, s, pg,
+                           (const uint32_t *)c->spill_big.p, h.big, (const uint32_t *)c->moff[nx].p, c->ptmp.p,
+                           c->pnew.p);
         CK(hipGetLastError());
-        CKRC(sync_counters(c));
-        h = *c->hcnt;
     }
     if (h.huge) {
         hipLaunchKernelGGL((k_seg_sort<256, SEG_CAP, false>), dim3(std::min<uint32_t>(h.huge, 4096)), dim3(256), 0, s,
