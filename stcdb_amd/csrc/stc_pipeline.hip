@@ -1354,6 +1354,9 @@ __device__ inline uint32_t lower_bound_bucket(const Bucket *s, uint32_t n, unsig
  * expires and, when it lasted long enough inside the map, becomes two records
  * (identify, geometry.cu:595-737). mnew[i] is built from pn[i]. A range
  * longer than OCAP is searched in global memory. */
+#define OPF (OCAP / STC_WAVE)   /* previous meetings per lane loaded at a tile's start */
+#define SPF (MTILE / STC_WAVE)  /* pair slots per lane loaded at a tile's start */
+
 __global__ __launch_bounds__(256) void k_meet_merge(const PairHit *__restrict__ pn, uint64_t n_p,
                                                     const Bucket *__restrict__ mold, const uint32_t *__restrict__ tb,
                                                     uint32_t n_tiles, Bucket *__restrict__ mnew, MergeArgs ma,
@@ -1366,93 +1369,117 @@ __global__ __launch_bounds__(256) void k_meet_merge(const PairHit *__restrict__ 
     Bucket *ob_l = wob[wv];
     Bucket *stage = wstage[wv];
     const uint64_t lt = lanemask_lt(lane);
-    const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / STC_WAVE;
+    const uint32_t wid = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) / STC_WAVE);
     const uint32_t nw = (gridDim.x * blockDim.x) / STC_WAVE;
+    const uint32_t n_o = tb[n_tiles];
+    const uint64_t plast = n_p ? n_p - 1 : 0;
     uint32_t staged = 0;
     unsigned long long n_live = 0, n_exp = 0, n_fresh = 0, n_cont = 0;
-    for (uint32_t t = wid; t < n_tiles; t += nw) {
+    uint32_t o0 = 0, o1 = 0, on0 = 0, on1 = 0;
+    if (wid < n_tiles) { o0 = tb[wid]; o1 = tb[wid + 1]; }
+    for (uint32_t t = wid; t < n_tiles; t += nw, o0 = on0, o1 = on1) {
+        { /* the next tile's range, read ahead */
+            const uint32_t tn = t + nw < n_tiles ? t + nw : t;
+            on0 = tb[tn];
+            on1 = tb[tn + 1];
+        }
         const uint64_t i0 = (uint64_t)t * MTILE;
         const uint32_t np = i0 < n_p ? (uint32_t)min<uint64_t>(MTILE, n_p - i0) : 0u;
-        const uint32_t o0 = tb[t], o1 = tb[t + 1];
-        if (o1 < o0) { /* the pairs were not sorted by key: report it, read nothing */
+        /* all loads of the tile are issued before the first wait; indices are clamped into the
+         * arrays (each holds at least one entry) and lanes past the tile's ends ignore theirs */
+        PairHit ps[SPF];
+        Bucket po[OPF];
+#pragma unroll
+        for (int k = 0; k < SPF; k++) ps[k] = pn[min<uint64_t>(i0 + lane + k * STC_WAVE, plast)];
+        const uint32_t olast = o1 > o0 ? o1 - 1 : (n_o ? min(o0, n_o - 1) : 0u);
+#pragma unroll
+        for (int k = 0; k < OPF; k++) po[k] = mold[min(o0 + lane + k * STC_WAVE, olast)];
+        if (o1 < o0) { /* the pairs were not sorted by key: report it, use nothing */
             if (lane == 0) atomicOr(&cnt->table_fail, 2u);
             continue;
         }
         const uint32_t no = o1 - o0;
         const bool oin = no <= OCAP;
-        const Bucket *obk = oin ? ob_l : mold + o0; /* the tile's previous meetings */
-        if (oin)
-            for (uint32_t j = lane; j < no; j += STC_WAVE) ob_l[j] = mold[o0 + j];
+        /* staged unconditionally (entries past the range are never searched), so the loads stay
+         * ahead of every branch */
+#pragma unroll
+        for (int k = 0; k < OPF; k++) ob_l[lane + k * STC_WAVE] = po[k];
         __builtin_amdgcn_wave_barrier();
-        for (uint32_t i = lane; i < np; i += STC_WAVE) {
-            const PairHit e = pn[i0 + i];
-            const unsigned long long key = e.key;
-            pk[i] = key;
-            Bucket nb;
-            nb.key = key;
-            if ((uint32_t)key == DEAD_B) {
-                nb.start = 0;
-                nb.end = 0xFFFFFFFFu;
-                nb.mbr[0] = nb.mbr[1] = nb.mbr[2] = nb.mbr[3] = 0;
-            } else {
+        /* the join, instantiated for previous meetings in LDS and, for a range longer than
+         * OCAP, in global memory (so the LDS searches wait on nothing but LDS) */
+        auto join = [&](const Bucket *obk) {
+            /* every lane joins its slots, in range or not (lanes past np store nothing), so that no
+             * load of the tile is moved behind a branch */
+#pragma unroll
+            for (int k = 0; k < SPF; k++) {
+                const uint32_t i = lane + k * STC_WAVE;
+                const bool valid = i < np;
+                const PairHit e = ps[k];
+                const unsigned long long key = e.key;
+                pk[i] = key;
+                const bool dead = (uint32_t)key == DEAD_B;
                 const uint32_t q0 = f2o(e.mbr[0]), q1 = f2o(e.mbr[1]), q2 = f2o(e.mbr[2]), q3 = f2o(e.mbr[3]);
-                nb.end = ma.now;
                 const uint32_t j = lower_bound_bucket(obk, no, key);
-                if (j < no && obk[j].key == key) {
-                    const Bucket ob = obk[j];
-                    nb.start = ob.start;
-                    nb.mbr[0] = min(ob.mbr[0], q0);
-                    nb.mbr[1] = min(ob.mbr[1], q1);
-                    nb.mbr[2] = max(ob.mbr[2], q2);
-                    nb.mbr[3] = max(ob.mbr[3], q3);
-                    n_cont++;
-                } else {
-                    nb.start = ma.now;
-                    nb.mbr[0] = q0; nb.mbr[1] = q1; nb.mbr[2] = q2; nb.mbr[3] = q3;
-                    n_fresh++;
+                const Bucket ob = obk[j < no ? j : (no ? no - 1 : 0u)];
+                const bool cont = j < no && ob.key == key;
+                Bucket nb;
+                nb.key = key;
+                nb.start = dead ? 0u : (cont ? ob.start : ma.now);
+                nb.end = dead ? 0xFFFFFFFFu : ma.now;
+                nb.mbr[0] = dead ? 0u : (cont ? min(ob.mbr[0], q0) : q0);
+                nb.mbr[1] = dead ? 0u : (cont ? min(ob.mbr[1], q1) : q1);
+                nb.mbr[2] = dead ? 0u : (cont ? max(ob.mbr[2], q2) : q2);
+                nb.mbr[3] = dead ? 0u : (cont ? max(ob.mbr[3], q3) : q3);
+                if (valid) {
+                    if (!dead) {
+                        n_live++;
+                        if (cont) n_cont++;
+                        else n_fresh++;
+                    }
+                    mnew[i0 + i] = nb;
                 }
-                n_live++;
             }
-            mnew[i0 + i] = nb;
-        }
-        __builtin_amdgcn_wave_barrier();
-        for (uint32_t j0 = 0; j0 < no; j0 += STC_WAVE) {
-            const uint32_t j = j0 + lane;
-            bool emit = false;
-            Bucket ob;
-            if (j < no) {
-                const unsigned long long key = obk[j].key;
-                if ((uint32_t)key != DEAD_B) {
-                    const uint32_t k = lower_bound_u64(pk, np, key);
-                    if (!(k < np && pk[k] == key)) {
-                        n_exp++;
-                        ob = obk[j];
-                        emit = meeting_emits(ob, ma);
+            __builtin_amdgcn_wave_barrier();
+            for (uint32_t j0 = 0; j0 < no; j0 += STC_WAVE) {
+                const uint32_t j = j0 + lane;
+                bool emit = false;
+                Bucket ob;
+                if (j < no) {
+                    const unsigned long long key = obk[j].key;
+                    if ((uint32_t)key != DEAD_B) {
+                        const uint32_t k = lower_bound_u64(pk, np, key);
+                        if (!(k < np && pk[k] == key)) {
+                            n_exp++;
+                            ob = obk[j];
+                            emit = meeting_emits(ob, ma);
+                        }
+                    }
+                }
+                const uint64_t em = __ballot(emit);
+                if (em) {
+                    const uint32_t total = (uint32_t)__popcll(em);
+                    if (staged + total > MSTAGE) {
+                        __builtin_amdgcn_wave_barrier();
+                        flush_stage(stage, staged, ma.end_time_min, mt, cnt, lane);
+                        staged = 0;
+                    }
+                    const uint32_t put = staged + (uint32_t)__popcll(em & lt);
+                    if (emit && put < MSTAGE) stage[put] = ob;
+                    /* a batch beyond the stage (more than MSTAGE at once) is written directly */
+                    if (total > MSTAGE) {
+                        unsigned long long kb = 0;
+                        if (lane == 0) kb = atomicAdd(&cnt->kv_count, 2ull * (total - MSTAGE));
+                        kb = __shfl(kb, 0, STC_WAVE);
+                        if (emit && put >= MSTAGE) write_meeting_records(ob, kb + 2ull * (put - MSTAGE), ma.end_time_min, mt);
+                        staged = MSTAGE;
+                    } else {
+                        staged += total;
                     }
                 }
             }
-            const uint64_t em = __ballot(emit);
-            if (em) {
-                const uint32_t total = (uint32_t)__popcll(em);
-                if (staged + total > MSTAGE) {
-                    __builtin_amdgcn_wave_barrier();
-                    flush_stage(stage, staged, ma.end_time_min, mt, cnt, lane);
-                    staged = 0;
-                }
-                const uint32_t put = staged + (uint32_t)__popcll(em & lt);
-                if (emit && put < MSTAGE) stage[put] = ob;
-                /* a batch beyond the stage (more than MSTAGE at once) is written directly */
-                if (total > MSTAGE) {
-                    unsigned long long kb = 0;
-                    if (lane == 0) kb = atomicAdd(&cnt->kv_count, 2ull * (total - MSTAGE));
-                    kb = __shfl(kb, 0, STC_WAVE);
-                    if (emit && put >= MSTAGE) write_meeting_records(ob, kb + 2ull * (put - MSTAGE), ma.end_time_min, mt);
-                    staged = MSTAGE;
-                } else {
-                    staged += total;
-                }
-            }
-        }
+        };
+        if (oin) join(ob_l);
+        else join(mold + o0);
         __builtin_amdgcn_wave_barrier(); /* the wave's LDS is reused by its next tile */
     }
     if (staged) {
