@@ -11,10 +11,16 @@ lon -87.9401, lat 41.6446): x = lon0 + X * dpm_lon(lat0), y = lat0 + Y * dpm_lat
       hotspots (sigma 500 m), 3,600 steps, r = 20 m, minimum meeting 10 s
   c3  10,000,000 objects, c2's walk and hotspot mix, 600 steps, memtable
       runs of 8 Mi records, 256 CTFs
+  c4  batched streams: 64 independent streams, object counts log-normal
+      (median 50k, clipped to 1k-2M, ~6M in total), each with its own
+      10-50 km box, reach uniform in 5-25 m and its own leaf capacity
+      (batched_streams below); each stream is one context of the library
   c5  5,000,000 objects with c2's mix, runs of 8 Mi records kept in HBM for
       queries: the serving workload (ingest at 1 Hz plus 2,000 queries/s)
 """
 import math
+
+import numpy as np
 
 LON0 = -87.94008647664191
 LAT0 = 41.644584027118924
@@ -86,3 +92,24 @@ def workload(name, **overrides):
         if k == "num_objects":
             gen["num_objects"] = v
     return dict(gen=gen, cfg=cfg, steps=overrides.get("steps", w["steps"]), box=map_box(gen))
+
+
+def batched_streams(n_streams=64, median=50_000, lo=1_000, hi=2_000_000, side_km=(10.0, 50.0),
+                    reach_m=(5.0, 25.0), capacities=(50, 100, 200), min_meet_time=10, steps=600, seed=4):
+    """The c4 family: independent streams, one workload each (num_objects
+    log-normal about `median` with sigma 1, clipped to [lo, hi]; a square box
+    of side uniform in `side_km`; reach uniform in `reach_m`; a leaf capacity
+    drawn from `capacities`, standing in for a per-stream tuned capacity).
+    All draws come from one seeded numpy generator, so the list depends only on
+    the arguments."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n_streams):
+        n = int(np.clip(round(median * math.exp(rng.normal(0.0, 1.0))), lo, hi))
+        cells = max(int(round(rng.uniform(*side_km) * 10.0)), 1)  # 100 m street spacing
+        reach = float(round(rng.uniform(*reach_m), 3))
+        cap = int(capacities[int(rng.integers(len(capacities)))])
+        gen = gen_params(n, cells, seed=seed * 1000 + i)
+        cfg = pipeline_config(n, reach, min_meet_time, grid_capacity=cap)
+        out.append(dict(gen=gen, cfg=cfg, steps=steps, box=map_box(gen)))
+    return out
