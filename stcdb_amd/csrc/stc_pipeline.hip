@@ -912,29 +912,39 @@ __device__ inline void write_sorted_segment(const unsigned long long *keys, uint
     }
     const float afx = (float)g.lxy[w].x, afy = (float)g.lxy[w].y;
     uint32_t lr = 0, dr = 0;
+    /* the sources' positions are loaded PB slots at a time, all of a group's loads issued before
+     * the first wait (a lane past the end reads the owner's own position) */
+    constexpr int PB = K < 4 ? K : (K > 8 ? 1 : 4);
 #pragma unroll
-    for (int r = 0; r < K; r++) {
-        const uint32_t i = r * STC_WAVE + lane;
-        const bool valid = i < slots;
-        const uint64_t ml = __ballot(live[r]), md = __ballot(valid && !live[r]);
-        if (valid) {
-            const uint32_t src = (uint32_t)x[r];
-            float4 mb;
-            if (src & SRC_REV) { /* found by the border point: its position first, as pass 1 forms it */
-                const RevPt &rp = g.rev[src & ~SRC_REV];
-                const double2 pp = make_double2(rp.x, rp.y);
-                mb = pair_mbr((float)pp.x, (float)pp.y, afx, afy);
-            } else {
-                const double2 qq = g.lxy[src];
-                mb = pair_mbr(afx, afy, (float)qq.x, (float)qq.y);
-            }
-            PairHit e;
-            e.key = ((unsigned long long)a << 32) | (live[r] ? (uint32_t)(x[r] >> 32) : DEAD_B);
-            e.mbr[0] = mb.x; e.mbr[1] = mb.y; e.mbr[2] = mb.z; e.mbr[3] = mb.w;
-            seg[live[r] ? lr + (uint32_t)__popcll(ml & lt) : nlive + dr + (uint32_t)__popcll(md & lt)] = e;
+    for (int r0 = 0; r0 < K; r0 += PB) {
+        double2 pos[PB];
+#pragma unroll
+        for (int b = 0; b < PB; b++) {
+            const uint32_t i = (r0 + b) * STC_WAVE + lane;
+            const uint32_t src = (uint32_t)x[r0 + b];
+            const double2 *pp = i >= slots ? g.lxy + w
+                                           : ((src & SRC_REV) ? (const double2 *)(g.rev + (src & ~SRC_REV)) : g.lxy + src);
+            pos[b] = *pp;
         }
-        lr += (uint32_t)__popcll(ml);
-        dr += (uint32_t)__popcll(md);
+#pragma unroll
+        for (int b = 0; b < PB; b++) {
+            const int r = r0 + b;
+            const uint32_t i = r * STC_WAVE + lane;
+            const bool valid = i < slots;
+            const uint64_t ml = __ballot(live[r]), md = __ballot(valid && !live[r]);
+            if (valid) {
+                /* found by a border point: its position first, as pass 1 forms it */
+                const bool rev = ((uint32_t)x[r] & SRC_REV) != 0;
+                const float px = (float)pos[b].x, py = (float)pos[b].y;
+                const float4 mb = rev ? pair_mbr(px, py, afx, afy) : pair_mbr(afx, afy, px, py);
+                PairHit e;
+                e.key = ((unsigned long long)a << 32) | (live[r] ? (uint32_t)(x[r] >> 32) : DEAD_B);
+                e.mbr[0] = mb.x; e.mbr[1] = mb.y; e.mbr[2] = mb.z; e.mbr[3] = mb.w;
+                seg[live[r] ? lr + (uint32_t)__popcll(ml & lt) : nlive + dr + (uint32_t)__popcll(md & lt)] = e;
+            }
+            lr += (uint32_t)__popcll(ml);
+            dr += (uint32_t)__popcll(md);
+        }
     }
 }
 
