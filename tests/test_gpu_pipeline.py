@@ -74,6 +74,14 @@ def test_c1_full_run_parity():
     assert tot["records"] > 1000 and tot["pairs"] > 10000
 
 
+def test_c2_full_size_parity():
+    # the 1M-object hotspot workload at its full size: ~11M pairs per step, every step bit-exact
+    # (the oracle takes ~3 s per step); twelve steps let meetings of 10 s expire into records
+    w = workload("c2")
+    tot = run_parity(w, 12, check_every=4, tag="c2")
+    assert tot["pairs"] > 100_000_000 and tot["records"] > 0
+
+
 def test_deep_tree_parity():
     # a small leaf capacity forces deep trees, finest-level splits and many border lookups
     w = workload("c1", grid_capacity=6, num_objects=6000)
