@@ -1496,15 +1496,24 @@ __global__ __launch_bounds__(256) void k_meet_merge(const PairHit *__restrict__ 
         __builtin_amdgcn_wave_barrier();
         flush_stage(stage, staged, ma.end_time_min, mt, cnt, lane);
     }
+    /* one update of each counter per workgroup: updates of one address serialise */
+    __shared__ unsigned long long bsum[256 / STC_WAVE][4];
     n_live = wave_sum_u64(n_live);
     n_exp = wave_sum_u64(n_exp);
     n_fresh = wave_sum_u64(n_fresh);
     n_cont = wave_sum_u64(n_cont);
     if (lane == 0) {
-        if (n_live) atomicAdd(&cnt->live, n_live);
-        if (n_exp) atomicAdd(&cnt->expired, n_exp);
-        if (n_fresh) atomicAdd(&cnt->new_keys, n_fresh);
-        if (n_cont) atomicAdd(&cnt->continuing, n_cont);
+        bsum[wv][0] = n_live;
+        bsum[wv][1] = n_exp;
+        bsum[wv][2] = n_fresh;
+        bsum[wv][3] = n_cont;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        unsigned long long v = 0;
+        for (int k = 0; k < 256 / STC_WAVE; k++) v += bsum[k][threadIdx.x];
+        unsigned long long *dst[4] = {&cnt->live, &cnt->expired, &cnt->new_keys, &cnt->continuing};
+        if (v) atomicAdd(dst[threadIdx.x], v);
     }
 }
 
@@ -2178,7 +2187,7 @@ static int step_tail(StcCtx *c, uint32_t t, StepGeom &g, stc_step_out *out) {
                        (const uint32_t *)c->moff[cu].p, (const uint32_t *)c->moff[nx].p, c->mlen, n_tiles,
                        c->tbounds.p);
     CK(hipGetLastError());
-    const uint32_t mgrid = std::max<uint32_t>(std::min<uint32_t>((n_tiles + 3) / 4, 16384), 1);
+    const uint32_t mgrid = std::max<uint32_t>(std::min<uint32_t>((n_tiles + 3) / 4, 65536), 1);
     for (int attempt = 0;; attempt++) {
         /* the merge rewrites the same buckets, so a re-run with more record room is exact */
         *c->hinit = before;
