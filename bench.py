@@ -259,7 +259,7 @@ def main():
     ap.add_argument("--workload", default="c3")
     ap.add_argument("--objects", type=int, default=0, help="override the object count")
     ap.add_argument("--cpu-steps", type=int, default=2)
-    ap.add_argument("--host-steps", type=int, default=3,
+    ap.add_argument("--host-steps", type=int, default=10,
                     help="extra steps fed from pinned host memory (PCIe-inclusive rate, reported beside value)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve", action="store_true",
@@ -326,7 +326,9 @@ def main():
     t0 = time.perf_counter()
     for t in range(args.warmup, total):
         outs.append(pipe.step_device(ctypes.c_void_p(frames[t].data_ptr()), t))
-        # flushed runs go to the writer: their pinned copies finish in the background
+        # a flushed run's copy to pinned host memory (the reference's D2H of h_keys, sids,
+        # bitmaps and oversize buffer, geometry.cu:2019-2035) is queued with the next step's
+        # pair count and always completes; dropping the run only recycles its host block
         pipe.drop_runs()
     pipe.sync_runs()
     torch.cuda.synchronize()

@@ -50,7 +50,7 @@ typedef struct stc_config {
     uint32_t kv_restriction;      /* flush once the memtable holds more records (config->kv_restriction) */
     uint32_t ctf_count;           /* config->CTF_count, a square */
     uint32_t compat;              /* STC_COMPAT_* bits */
-    uint64_t num_meeting_buckets; /* initial meeting-table size; 0 = 4 * num_objects. Grows on demand. */
+    uint64_t num_meeting_buckets; /* initial meeting-array size; 0 = 2 * num_objects (config.h:306-308). Grows on demand. */
     double reach_distance;        /* config->reach_distance (metres) */
     double area_threshold;        /* workbench::dProbFalse: object MBR area (deg^2) above which it is oversize */
     uint32_t keep_pairs;          /* bit 0: keep each step's pair list for stc_get_pairs;
@@ -79,8 +79,8 @@ typedef struct stc_step_out {
     uint64_t overflow_points;  /* points beyond the leaf buffer (not stored as targets) */
     uint64_t ambiguous_pairs;  /* pair decisions re-evaluated on the host (distance within 16 ulp of r) */
     uint64_t ambiguous_nodes;  /* box-distance decisions within 16 ulp of r (decided on the device) */
-    uint64_t table_buckets;    /* buckets of the meeting array: live meetings and tombstones */
-    uint64_t table_occupied;   /* the same (the array holds no free buckets) */
+    uint64_t table_buckets;    /* buckets of this step's meeting array: live meetings and dead slots */
+    uint64_t table_occupied;   /* num_taken_buckets: meetings held when identify ran (active + expired) */
     uint32_t flushed;          /* 1 if this step flushed a run */
     uint32_t rebuilt;          /* 0 (every step's merge writes a new meeting array) */
     float ms_partition;        /* QTree build + partition */

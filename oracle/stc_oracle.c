@@ -506,7 +506,7 @@ static int find_pairs(orc *o, const double *xy) {
     uint32_t *last_valid = (uint32_t *)malloc(sizeof(uint32_t) * (n ? n : 1));
     for (uint32_t p = 0; p < n; p++) {
         double px = xy[2 * p], py = xy[2 * p + 1];
-        uint32_t cur = 0, lv = 0;
+        uint32_t cur = 0, lv = 0, depth = 0;
         for (;;) {
             const node *nd = &o->nodes[cur];
             int loc = (py > nd->mid_y) * 2 + (px > nd->mid_x);
@@ -514,6 +514,11 @@ static int find_pairs(orc *o, const double *xy) {
             const node *c = &o->nodes[cur];
             if (px + o->x_buffer < c->hi[0] && py + o->y_buffer < c->hi[1]) lv = cur;
             if (c->type == T_LEAF) break;
+            if (++depth > 64) {
+                free(last_valid);
+                set_err(o, "QTree descent deeper than 64 levels (malformed tree)");
+                return -1;
+            }
         }
         o->pt_leaf[p] = o->nodes[cur].leaf;
         last_valid[p] = lv;

@@ -102,6 +102,7 @@ struct Counters {             /* device-side counters, copied back once per sync
     uint32_t overflow;
     uint32_t oversize_oid_count;
     uint32_t table_fail;
+    uint32_t lookup_overflow;          /* border lookups whose node stack was full */
     uint32_t big, huge, giant;         /* segments left to the wave sort, the block sort, the global-scratch sort */
 };
 
@@ -335,6 +336,16 @@ __global__ void k_partition(const double2 *__restrict__ xy, const uint32_t *__re
     if (lane == 0 && b) atomicAdd(&cnt->border, (uint32_t)__popcll(b));
 }
 
+/* positions of the two points of each band combination, for the host's decision */
+__global__ void k_amb_gather(const double2 *__restrict__ xy, const uint2 *__restrict__ amb, uint32_t n_amb,
+                             double4 *__restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_amb) return;
+    const uint2 e = amb[i];
+    const double2 p = xy[e.x], q = xy[e.y];
+    out[i] = make_double4(p.x, p.y, q.x, q.y);
+}
+
 __global__ void k_iota(uint32_t *a, uint32_t n) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) a[i] = i;
@@ -400,6 +411,7 @@ __global__ void k_lookup(const uint32_t *__restrict__ lp, const double2 *__restr
     int sp = 0;
     stk[sp++] = lv;
     uint32_t found = 0, amb = 0;
+    bool full = false;
     uint32_t out = EMIT ? offset[p] : 0;
     while (sp > 0) {
         uint32_t cur = stk[--sp];
@@ -417,6 +429,8 @@ __global__ void k_lookup(const uint32_t *__restrict__ lp, const double2 *__restr
                     }
                 } else if (sp < STACK_DEPTH) {
                     stk[sp++] = c;
+                } else {
+                    full = true;
                 }
             }
         }
@@ -424,6 +438,7 @@ __global__ void k_lookup(const uint32_t *__restrict__ lp, const double2 *__restr
     if (!EMIT) {
         count[p] = found;
         if (amb) atomicAdd(&cnt->ambiguous_nodes, (unsigned long long)amb);
+        if (full) atomicAdd(&cnt->lookup_overflow, 1u);
     }
 }
 
@@ -813,7 +828,6 @@ __device__ inline bool sorted_contains(const unsigned long long *a, uint32_t n, 
     return lo < n && a[lo] == v;
 }
 
-#define DEAD_B 0xFFFFFFFFu /* partner word of a dead slot or tombstone: sorts after the owner's live pairs */
 #define WCAP 512u          /* slots of a segment the write pass sorts in its wave's LDS */
 
 __device__ inline uint64_t lanemask_lt(int lane) { return lane == 0 ? 0ull : (~0ull >> (STC_WAVE - lane)); }
@@ -1638,6 +1652,7 @@ struct StcCtx {
     DBuf<uint32_t> ucount, uoffset;
     DBuf<uint2> units;
     DBuf<uint2> amb;                     /* band combinations (probe, other) for the host */
+    DBuf<double4> amb_xy;                /* their positions (probe x, y, other x, y) */
     /* per leaf, by pid: the border points with a cross unit to it */
     DBuf<uint32_t> rev_leaf, rev_pid, rev_leaf_alt, rev_pid_alt, rev_start;
     DBuf<RevPt> rev;
@@ -1678,6 +1693,7 @@ struct StcCtx {
     DBuf<uint32_t> uidx, uidx_alt, utemp;
     std::vector<RunHost> runs;
     PinnedPool pinned;
+    std::vector<PinnedBlock> deferred;   /* blocks of dropped runs whose host copy is not queued yet */
     QueryBufs *qb = nullptr;
     FlushWork flushw;
     hipEvent_t ev[12];
@@ -1745,10 +1761,37 @@ static int step_pairs(StcCtx *c, std::vector<PairHit> &ph) {
     return 0;
 }
 
-/* a run's pinned block back to the pool; copies into it that were never queued are dropped */
-static void release_block(StcCtx *c, PinnedBlock b) {
+/* A run's pinned block back to the pool. With finish_copy, a host copy owed to
+ * the block still happens: the block waits in `deferred` until the copy is
+ * queued (the pool then reuses it once the copy's event has fired). Without
+ * it, a copy that was never queued is dropped. */
+static void release_block(StcCtx *c, PinnedBlock b, bool finish_copy = false) {
+    if (finish_copy && c->flushw.job_set && c->flushw.job.dst == b.p) {
+        c->deferred.push_back(b);
+        return;
+    }
     stc_flush_cancel(&c->flushw, b.p);
     c->pinned.release(b);
+}
+
+/* deferred blocks whose copy has been queued go back to the pool */
+static void settle_deferred(StcCtx *c) {
+    for (size_t i = 0; i < c->deferred.size();) {
+        if (c->flushw.job_set && c->flushw.job.dst == c->deferred[i].p) {
+            i++;
+            continue;
+        }
+        c->pinned.release(c->deferred[i]);
+        c->deferred[i] = c->deferred.back();
+        c->deferred.pop_back();
+    }
+}
+
+/* queue the host copies of the last flushed run on the copy stream */
+static int queue_run_copies(StcCtx *c) {
+    if (stc_flush_release(&c->flushw, c->s)) return STC_E_HIP;
+    settle_deferred(c);
+    return 0;
 }
 
 /* Copy a just-flushed run out of the flush buffers into its own HBM block
@@ -1896,7 +1939,7 @@ int stc_destroy(void *vc) {
     c->leaf_node.release(); c->pt_leaf.release(); c->pt_lv.release(); c->pt_rank.release(); c->lp.release();
     c->lleaf.release(); c->lunits.release(); c->urange.release(); c->prange.release();
     c->lkeys.release(); c->ucount.release(); c->uoffset.release(); c->units.release();
-    c->amb.release();
+    c->amb.release(); c->amb_xy.release();
     c->rev_leaf.release(); c->rev_pid.release(); c->rev_leaf_alt.release(); c->rev_pid_alt.release();
     c->rev_start.release();
     c->rev.release(); c->rev_dl.release();
@@ -1920,6 +1963,8 @@ int stc_destroy(void *vc) {
         if (r.dev) (void)hipFree(r.dev);
     }
     c->runs.clear();
+    for (auto &b : c->deferred) release_block(c, b);
+    c->deferred.clear();
     c->pinned.clear();
     stc_query_bufs_free(c->qb);
     c->qb = nullptr;
@@ -2019,6 +2064,7 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
     CKRC(stc_exclusive_scan_u32(c->ucount.p, c->uoffset.p, n, units_total, c->temp.p, s));
     CKRC(sync_counters(c));
     h = *c->hcnt;
+    if (h.lookup_overflow) return fail(c, STC_E_CAPACITY, "border lookup: node stack depth exceeded");
     uint64_t n_units = (uint32_t)h.units;
     if (c->units.ensure(n_units + 1)) return fail(c, STC_E_CAPACITY, "unit buffer allocation failed");
     hipLaunchKernelGGL(k_lookup<true>, dim3(blocks_for(n, 128)), dim3(128), 0, s, (const uint32_t *)c->lp.p,
@@ -2073,7 +2119,7 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
     /* a multiple of 8 workgroups, for the XCD-aware owner order */
     const uint32_t pgrid = (uint32_t)std::min<uint64_t>((blocks_for((uint64_t)n * STC_WAVE, 256) + 7) & ~7ull, 256u * 64u);
     /* the last run's host copies overlap the count pass, which computes more than it writes */
-    if (stc_flush_release(&c->flushw, s)) return fail(c, STC_E_HIP, std::string("run copies: ") + g_last_fail);
+    if (queue_run_copies(c)) return fail(c, STC_E_HIP, std::string("run copies: ") + g_last_fail);
     for (int attempt = 0;; attempt++) {
         CK(hipMemsetAsync(&cnt->candidates, 0, 3 * sizeof(unsigned long long), s)); /* candidates, pairs, ambiguous */
         CK(hipEventRecord(c->ev[10], s));
@@ -2093,13 +2139,19 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
     if (n_amb) {
         /* decide the band with the C library's sin (geometry.h:40-48) */
         std::vector<uint2> ab(n_amb);
-        CK(hipMemcpy(ab.data(), c->amb.p, n_amb * sizeof(uint2), hipMemcpyDeviceToHost));
-        for (const uint2 &e : ab) {
-            double2 pp, qq;
-            CK(hipMemcpy(&pp, xy + e.x, sizeof(double2), hipMemcpyDeviceToHost));
-            CK(hipMemcpy(&qq, xy + e.y, sizeof(double2), hipMemcpyDeviceToHost));
-            double dx = (pp.x - qq.x) / host_dpm_lon(pp.y);
-            double dy = (pp.y - qq.y) / STC_DPM_LAT;
+        std::vector<double4> pos(n_amb);
+        if (c->amb_xy.ensure(n_amb)) return fail(c, STC_E_CAPACITY, "band position buffer");
+        hipLaunchKernelGGL(k_amb_gather, dim3(blocks_for(n_amb, 256)), dim3(256), 0, s, xy,
+                           (const uint2 *)c->amb.p, (uint32_t)n_amb, c->amb_xy.p);
+        CK(hipGetLastError());
+        CK(hipMemcpyAsync(ab.data(), c->amb.p, n_amb * sizeof(uint2), hipMemcpyDeviceToHost, s));
+        CK(hipMemcpyAsync(pos.data(), c->amb_xy.p, n_amb * sizeof(double4), hipMemcpyDeviceToHost, s));
+        CK(hipStreamSynchronize(s));
+        for (uint64_t i = 0; i < n_amb; i++) {
+            const uint2 e = ab[i];
+            const double4 pq = pos[i];
+            double dx = (pq.x - pq.z) / host_dpm_lon(pq.y);
+            double dy = (pq.y - pq.w) / STC_DPM_LAT;
             if (std::sqrt(dx * dx + dy * dy) <= r)
                 accepted.push_back(((unsigned long long)std::min(e.x, e.y) << 32) | std::max(e.x, e.y));
         }
@@ -2172,7 +2224,7 @@ static int step_tail(StcCtx *c, uint32_t t, StepGeom &g, stc_step_out *out) {
     Counters *cnt = c->dcnt;
     /* 9: merge the step's pairs with the previous step's meetings; meetings
      * without a pair expire and the long ones become records */
-    if (stc_flush_release(&c->flushw, s)) return fail(c, STC_E_HIP, std::string("run copies: ") + g_last_fail);
+    if (queue_run_copies(c)) return fail(c, STC_E_HIP, std::string("run copies: ") + g_last_fail);
     const int cu = c->cur, nx = cu ^ 1;
     const uint64_t kv_before = c->kv_count;
     if (ensure_records(c, kv_before + 2 * std::max<uint64_t>(c->last_expired + c->last_expired / 4, 1 << 16) + 2))
@@ -2258,6 +2310,7 @@ static int step_tail(StcCtx *c, uint32_t t, StepGeom &g, stc_step_out *out) {
         fo.acquire = pool_acquire;
         fo.acquire_ctx = &c->pinned;
         int rc = stc_flush_run(&c->flushw, &fa, &fo, s);
+        settle_deferred(c);
         if (rc) {
             release_block(c, fo.blk);
             return fail(c, rc, std::string("flush: ") + g_last_fail);
@@ -2295,7 +2348,7 @@ static int step_tail(StcCtx *c, uint32_t t, StepGeom &g, stc_step_out *out) {
         out->ambiguous_pairs = g.ambiguous;
         out->ambiguous_nodes = h.ambiguous_nodes;
         out->table_buckets = c->mlen;
-        out->table_occupied = c->mlen;
+        out->table_occupied = h.live + h.expired;
         out->flushed = flushed;
         out->rebuilt = 0;
         hipEventElapsedTime(&out->ms_partition, c->ev[0], c->ev[1]);
@@ -2435,7 +2488,7 @@ int stc_get_active(void *vc, uint64_t *keys, uint32_t *start, uint32_t *end, flo
     if (c->mlen) CK(hipMemcpy(tb.data(), c->meet[c->cur].p, c->mlen * sizeof(Bucket), hipMemcpyDeviceToHost));
     std::vector<Bucket> live; /* the array is sorted by key; tombstones trail each segment */
     for (const Bucket &b : tb)
-        if (!key_free(b.key) && !key_tomb(b.key) && b.end == c->last_t) live.push_back(b);
+        if (bucket_live(b, c->last_t)) live.push_back(b);
     *n = live.size();
     for (uint64_t i = 0; i < std::min<uint64_t>(cap, live.size()); i++) {
         if (keys) keys[i] = live[i].key;
@@ -2502,7 +2555,7 @@ int stc_get_run(void *vc, uint32_t run, stc_ctf_meta *ctfs, uint8_t *key_bytes, 
     StcCtx *c = (StcCtx *)vc;
     if (!c || run >= c->runs.size()) return STC_E_ARG;
     RunHost &R = c->runs[run];
-    if (stc_flush_release(&c->flushw, c->s)) return fail(c, STC_E_HIP, std::string("run copies: ") + g_last_fail);
+    if (queue_run_copies(c)) return fail(c, STC_E_HIP, std::string("run copies: ") + g_last_fail);
     CK(hipEventSynchronize(R.blk.ev));
     const uint8_t *src = R.blk.p;
     const RunLayout &L = R.lay;
@@ -2527,7 +2580,7 @@ int stc_get_run(void *vc, uint32_t run, stc_ctf_meta *ctfs, uint8_t *key_bytes, 
 int stc_sync(void *vc) {
     StcCtx *c = (StcCtx *)vc;
     if (!c) return STC_E_ARG;
-    if (stc_flush_release(&c->flushw, c->s)) return fail(c, STC_E_HIP, std::string("run copies: ") + g_last_fail);
+    if (queue_run_copies(c)) return fail(c, STC_E_HIP, std::string("run copies: ") + g_last_fail);
     CK(hipStreamSynchronize(c->s));
     if (c->flushw.cs) CK(hipStreamSynchronize(c->flushw.cs));
     return STC_OK;
@@ -2592,7 +2645,7 @@ int stc_dump_run(void *vc, uint32_t run, const char *dir, const char *prefix, ui
     StcCtx *c = (StcCtx *)vc;
     if (!c || !dir || !prefix || run >= c->runs.size()) return STC_E_ARG;
     RunHost &R = c->runs[run];
-    if (stc_flush_release(&c->flushw, c->s)) return fail(c, STC_E_HIP, std::string("run copies: ") + g_last_fail);
+    if (queue_run_copies(c)) return fail(c, STC_E_HIP, std::string("run copies: ") + g_last_fail);
     CK(hipEventSynchronize(R.blk.ev));
     const uint8_t *src = R.blk.p;
     const RunLayout &L = R.lay;
@@ -2651,13 +2704,14 @@ int stc_dump_run(void *vc, uint32_t run, const char *dir, const char *prefix, ui
     return STC_OK;
 }
 
-/* Releases the host copies of all flushed runs. Copies still in flight finish
- * before their memory is reused. */
+/* Releases the host copies of all flushed runs. A run's copy to host memory
+ * still completes (a copy not queued yet is queued with the next step, as it
+ * would have been) before its pinned block is reused. */
 int stc_drop_runs(void *vc) {
     StcCtx *c = (StcCtx *)vc;
     if (!c) return STC_E_ARG;
     for (auto &r : c->runs) {
-        release_block(c, r.blk);
+        release_block(c, r.blk, true);
         if (r.dev) {
             (void)hipStreamSynchronize(c->s);
             (void)hipFree(r.dev);
@@ -2757,6 +2811,7 @@ int stc_compact_runs(void *vc, uint32_t first, uint32_t count) {
     fo.acquire = pool_acquire;
     fo.acquire_ctx = &c->pinned;
     rc = stc_flush_run(&c->flushw, &fa, &fo, s);
+    settle_deferred(c);
     if (rc) {
         release_block(c, fo.blk);
         release();

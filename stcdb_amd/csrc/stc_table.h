@@ -1,14 +1,18 @@
 /*
- * stc_table.h — layout of the meeting table shared by the pipeline and the
+ * stc_table.h — layout of the live meetings, shared by the pipeline and the
  * query kernels.
  *
  * A bucket is the reference's meeting_unit (src/tracing/workbench.h:62-83):
  * 32 bytes of key, start, end and MBR. The MBR is stored as order-preserving
  * u32 images of the f32 bounds so that min/max updates are integer ops.
- * Keys are oid1 << 32 | oid2 with oid1 < oid2 < 2^26. Keys whose high word is
- * all ones mark free buckets (the low word tags the identify pass that freed
- * the bucket; a cleared table holds EMPTY_KEY); high word 0xFFFFFFFE marks a
- * tombstone, an occupied bucket that matches no key.
+ * Keys are oid1 << 32 | oid2 with oid1 < oid2 < 2^26.
+ *
+ * The live meetings after a step are one array of buckets sorted by key, one
+ * segment per owner oid1 (a pair's smaller id). Bucket i of a step's array is
+ * built from the step's pair slot i, so a slot that held a duplicate pair or
+ * a rejected band pair leaves a dead bucket: key oid1 << 32 | DEAD_B (it sorts
+ * after the owner's live pairs) and end DEAD_B. The array has no free buckets:
+ * every bucket is live or dead, and a live bucket's end is the array's step.
  */
 #ifndef STC_TABLE_H_
 #define STC_TABLE_H_
@@ -17,17 +21,18 @@
 #include <stdint.h>
 #include <string.h>
 
-#define EMPTY_KEY 0xFFFFFFFFFFFFFFFFull
-#define FREE_KEY(tag) (0xFFFFFFFF00000000ull | (unsigned long long)(uint32_t)(tag))
-#define TOMB_KEY(tag) (0xFFFFFFFE00000000ull | (unsigned long long)(uint32_t)(tag))
-__host__ __device__ inline bool key_free(unsigned long long k) { return (k >> 32) == 0xFFFFFFFFull; }
-__host__ __device__ inline bool key_tomb(unsigned long long k) { return (k >> 32) == 0xFFFFFFFEull; }
+#define DEAD_B 0xFFFFFFFFu /* partner word of a dead slot or bucket */
 
 struct __align__(32) Bucket {
     unsigned long long key;
     uint32_t start, end;
     uint32_t mbr[4];          /* ordered ints of low0, low1, high0, high1 */
 };
+
+/* a bucket of the array built at step last_t that holds a live meeting */
+__host__ __device__ inline bool bucket_live(const Bucket &b, uint32_t last_t) {
+    return (uint32_t)b.key != DEAD_B && b.end == last_t;
+}
 
 __host__ __device__ inline uint32_t f2o(float f) {
     uint32_t b;

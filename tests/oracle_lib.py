@@ -8,7 +8,8 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(ROOT, "oracle")
-ORACLE_LIB = os.path.join(ORACLE_DIR, "liboracle.so")
+# STC_ORACLE_LIB points the tests at another build of the oracle (mutation checks of the pins)
+ORACLE_LIB = os.environ.get("STC_ORACLE_LIB", os.path.join(ORACLE_DIR, "liboracle.so"))
 REF_LIB = os.path.join(ORACLE_DIR, "_ref", "libstcref.so")
 
 ORC_C = ["nodes", "leaves", "border", "units_inside", "units_cross", "candidates", "pairs", "active",
