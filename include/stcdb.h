@@ -195,7 +195,15 @@ int stc_get_run_info(void *ctx, uint32_t run, stc_run_info *info);
 int stc_get_run(void *ctx, uint32_t run, stc_ctf_meta *ctfs, uint8_t *key_bytes, uint8_t *bitmaps,
                 uint16_t *sids, uint64_t *o_keys, float *o_boxes, uint8_t *o_bitmap,
                 uint64_t *sorted_keys, float *sorted_boxes);
-int stc_drop_runs(void *ctx); /* release host copies of flushed runs */
+int stc_drop_runs(void *ctx); /* release host copies of flushed runs (owed copies still complete) */
+/* CTF::get_ctf_bits (src/tracing/step_merge.cpp:140-186), which the
+ * reference's host runs for every CTF of a flush (geometry.cu:1960-1964):
+ * fills the id, duration, end, low_x, low_y, edge and mbr bit widths, key_bit,
+ * the bitmap grid (x_grid, y_grid) and bitmap_size of `ctf` from its mbr (set
+ * by the caller), the map box, the object count and max_meet_time. flags bit 0
+ * is set where the reference would fail an assert. Host arithmetic only; needs
+ * no device. */
+int stc_ctf_widths(stc_ctf_meta *ctf, const stc_box *map_mbr, uint32_t num_objects, uint32_t max_meet_time);
 /* Compaction: merge runs [first, first + count) into one run that takes their
  * place (later runs move down by count - 1). The records of the inputs are
  * decoded as the reference's compaction reads them (CTF::parse_key and

@@ -2704,6 +2704,13 @@ int stc_dump_run(void *vc, uint32_t run, const char *dir, const char *prefix, ui
     return STC_OK;
 }
 
+int stc_ctf_widths(stc_ctf_meta *ctf, const stc_box *map_mbr, uint32_t num_objects, uint32_t max_meet_time) {
+    if (!ctf || !map_mbr || max_meet_time == 0) return STC_E_ARG;
+    const double map[4] = {map_mbr->low[0], map_mbr->low[1], map_mbr->high[0], map_mbr->high[1]};
+    stc_ctf_bits(ctf, map, num_objects, max_meet_time);
+    return STC_OK;
+}
+
 /* Releases the host copies of all flushed runs. A run's copy to host memory
  * still completes (a copy not queued yet is queued with the next step, as it
  * would have been) before its pinned block is reused. */
