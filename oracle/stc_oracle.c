@@ -65,6 +65,7 @@ typedef struct node {
     uint32_t type;
     uint32_t child[4];
     uint32_t leaf;        /* leaf id (creation order) for leaves */
+    uint32_t ovf, unf;    /* overflow_count, underflow_count of a dynamic schema */
 } node;
 
 typedef struct meeting {
@@ -102,8 +103,10 @@ typedef struct orc {
     char err[256];
     uint64_t counts[ORC_C_NUM];
 
-    /* per step */
+    /* per step (with a dynamic schema the nodes persist; free slots are listed) */
     node *nodes; uint32_t n_nodes, cap_nodes;
+    uint32_t *free_nodes; uint32_t n_free, cap_free;
+    int have_schema;
     uint32_t n_leaves;
     uint32_t *leaf_node;        /* leaf id -> node index */
     uint32_t *pt_leaf;          /* point -> leaf id */
@@ -297,7 +300,7 @@ static void free_run(run *r) {
 void orc_destroy(void *vo) {
     orc *o = (orc *)vo;
     if (!o) return;
-    free(o->nodes); free(o->leaf_node); free(o->pt_leaf); free(o->pt_border); free(o->dlon);
+    free(o->nodes); free(o->free_nodes); free(o->leaf_node); free(o->pt_leaf); free(o->pt_border); free(o->dlon);
     free(o->leaf_start); free(o->leaf_count); free(o->leaf_members);
     free(o->unit_pid); free(o->unit_node); free(o->unit_inside); free(o->pairs);
     free(o->active); free(o->recs); free(o->same_pid_count); free(o->o_boxs); free(o->sids);
@@ -441,6 +444,211 @@ static int build_tree(orc *o, const double *xy) {
     o->leaf_node = (uint32_t *)malloc(sizeof(uint32_t) * (o->n_leaves ? o->n_leaves : 1));
     for (uint32_t i = 0; i < o->n_nodes; i++)
         if (o->nodes[i].type == T_LEAF) o->leaf_node[o->nodes[i].leaf] = i;
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* dynamic schema (schema_mode 1, 2)                                    */
+/* ------------------------------------------------------------------ */
+
+/* host form of degree_per_meter_longitude (util.h:59-66) */
+static double host_dpm_lon(double lat) {
+    double absla = fabs(lat);
+    if (absla == 90) absla = 89;
+    return 360.0 / (sin((90 - absla) * PI_REF / 180) * 40076.0 * 1000.0);
+}
+
+static int take_node(orc *o, uint32_t *id) {
+    if (o->n_free) { *id = o->free_nodes[--o->n_free]; return 0; }
+    if (o->n_nodes == o->cap_nodes) {
+        uint32_t nc = o->cap_nodes ? o->cap_nodes * 2 : 4096;
+        node *nn = (node *)realloc(o->nodes, sizeof(node) * nc);
+        if (!nn) return -1;
+        o->nodes = nn;
+        o->cap_nodes = nc;
+    }
+    *id = o->n_nodes++;
+    return 0;
+}
+
+/* QTNode::split of the initial tree (QTree.h:157-185): a node holding at
+ * least grid_capacity points whose half width in metres (box::width(true),
+ * geometry.h:166-172, at its low latitude) exceeds reach_distance splits into
+ * the exact halves of its box, and so on below. Points reach a node by the
+ * descent rule (y > mid_y) * 2 + (x > mid_x), so the tree does not depend on
+ * the insertion order of partitioner::build_schema's threads; it is built
+ * here recursively over lists of point ids. */
+static int schema_build_rec(orc *o, uint32_t nd, const double *xy, uint32_t *ids, uint32_t cnt, uint32_t *tmp,
+                            int depth) {
+    node b = o->nodes[nd];
+    double width_m = (b.hi[0] - b.lo[0]) / host_dpm_lon(b.lo[1]);
+    if (!(cnt >= o->cfg.grid_capacity && width_m / 2 > o->cfg.reach_distance)) return 0;
+    if (depth > 60) { set_err(o, "initial schema deeper than 60 levels"); return -1; }
+    const double lo0[4] = {b.lo[0], b.mid_x, b.lo[0], b.mid_x}, lo1[4] = {b.lo[1], b.lo[1], b.mid_y, b.mid_y};
+    const double hi0[4] = {b.mid_x, b.hi[0], b.mid_x, b.hi[0]}, hi1[4] = {b.mid_y, b.mid_y, b.hi[1], b.hi[1]};
+    uint32_t kids[4];
+    for (int c = 0; c < 4; c++) {
+        if (take_node(o, &kids[c])) { set_err(o, "out of memory"); return -1; }
+        node *k = &o->nodes[kids[c]];
+        memset(k, 0, sizeof(node));
+        k->lo[0] = lo0[c]; k->lo[1] = lo1[c]; k->hi[0] = hi0[c]; k->hi[1] = hi1[c];
+        k->mid_x = (k->lo[0] + k->hi[0]) / 2;
+        k->mid_y = (k->lo[1] + k->hi[1]) / 2;
+        k->type = T_LEAF;
+    }
+    o->nodes[nd].type = T_BRANCH;
+    for (int c = 0; c < 4; c++) o->nodes[nd].child[c] = kids[c];
+    /* distribute the ids by quadrant (stable) */
+    uint32_t n4[4] = {0, 0, 0, 0};
+    for (uint32_t i = 0; i < cnt; i++) {
+        uint32_t p = ids[i];
+        n4[(xy[2 * p + 1] > b.mid_y) * 2 + (xy[2 * p] > b.mid_x)]++;
+    }
+    uint32_t at[4] = {0, n4[0], n4[0] + n4[1], n4[0] + n4[1] + n4[2]};
+    uint32_t st[4] = {at[0], at[1], at[2], at[3]};
+    for (uint32_t i = 0; i < cnt; i++) {
+        uint32_t p = ids[i];
+        tmp[at[(xy[2 * p + 1] > b.mid_y) * 2 + (xy[2 * p] > b.mid_x)]++] = p;
+    }
+    memcpy(ids, tmp, sizeof(uint32_t) * cnt);
+    for (int c = 0; c < 4; c++)
+        if (schema_build_rec(o, kids[c], xy, ids + st[c], n4[c], tmp + st[c], depth + 1)) return -1;
+    return 0;
+}
+
+/* partitioner::build_schema (filter.cpp:27-93): the tree over the map box
+ * from one timestep's points */
+static int schema_build(orc *o, const double *xy) {
+    uint32_t n = o->cfg.num_objects;
+    o->n_nodes = 0;
+    o->n_free = 0;
+    new_node(o);
+    node *r = &o->nodes[0];
+    r->lo[0] = o->cfg.map_lo[0]; r->lo[1] = o->cfg.map_lo[1];
+    r->hi[0] = o->cfg.map_hi[0]; r->hi[1] = o->cfg.map_hi[1];
+    r->mid_x = (r->lo[0] + r->hi[0]) / 2;
+    r->mid_y = (r->lo[1] + r->hi[1]) / 2;
+    r->type = T_LEAF;
+    uint32_t *ids = (uint32_t *)malloc(sizeof(uint32_t) * (n ? n : 1));
+    uint32_t *tmp = (uint32_t *)malloc(sizeof(uint32_t) * (n ? n : 1));
+    if (!ids || !tmp) { free(ids); free(tmp); set_err(o, "out of memory"); return -1; }
+    for (uint32_t i = 0; i < n; i++) ids[i] = i;
+    int rc = schema_build_rec(o, 0, xy, ids, n, tmp, 0);
+    free(ids);
+    free(tmp);
+    if (rc) return -1;
+    if (o->nodes[0].type != T_BRANCH) { set_err(o, "root does not split: too few objects for grid_capacity"); return -1; }
+    o->have_schema = 1;
+    return 0;
+}
+
+int orc_build_schema(void *vo, const double *xy) {
+    orc *o = (orc *)vo;
+    if (o->cfg.schema_mode == 0) { set_err(o, "the schema is rebuilt every step (schema_mode 0)"); return -1; }
+    return schema_build(o, xy);
+}
+
+/* leaf ids of the current schema, in node order */
+static int schema_number_leaves(orc *o) {
+    uint32_t nl = 0;
+    for (uint32_t i = 0; i < o->n_nodes; i++)
+        if (o->nodes[i].type == T_LEAF) nl++;
+    free(o->leaf_node);
+    o->leaf_node = (uint32_t *)malloc(sizeof(uint32_t) * (nl ? nl : 1));
+    if (!o->leaf_node) { set_err(o, "out of memory"); return -1; }
+    o->n_leaves = 0;
+    for (uint32_t i = 0; i < o->n_nodes; i++)
+        if (o->nodes[i].type == T_LEAF) {
+            o->nodes[i].leaf = o->n_leaves;
+            o->leaf_node[o->n_leaves++] = i;
+        }
+    return 0;
+}
+
+static uint32_t schema_valid_nodes(const orc *o) {
+    uint32_t v = 0;
+    for (uint32_t i = 0; i < o->n_nodes; i++) v += o->nodes[i].type != 0;
+    return v;
+}
+
+/* The update after a step, from the step's leaf counts (all points of a
+ * leaf, before the buffer cap). A leaf splits when it holds more than
+ * grid_capacity points and is higher than 2 * reach_distance in metres
+ * (GPU, cuda_update_schema_collect geometry.cu:1193-1212) or wider than that
+ * at its low latitude (CPU, update_schema_unit filter.cpp:386-397), for
+ * schema_update_delay consecutive steps; its children are the halving boxes
+ * of cuda_update_schema_split (geometry.cu:1114-1158, filter.cpp:338-375).
+ * A branch whose four children are leaves holding fewer than grid_capacity
+ * points together merges back into a leaf (geometry.cu:1213-1232,
+ * :1160-1191; filter.cpp:398-419). All decisions are taken on the state
+ * before the update, then splits, then merges, as the three kernels do. */
+static int schema_update(orc *o) {
+    const uint32_t cap = o->cfg.grid_capacity, delay = o->cfg.schema_update_delay ? o->cfg.schema_update_delay : 1;
+    const double r2 = o->cfg.reach_distance * 2;
+    const uint32_t n = o->n_nodes;
+    uint8_t *act = (uint8_t *)calloc(n ? n : 1, 1);
+    if (!act) { set_err(o, "out of memory"); return -1; }
+    for (uint32_t i = 0; i < n; i++) {
+        node *nd = &o->nodes[i];
+        if (nd->type == T_LEAF) {
+            double extent = o->cfg.schema_mode == 1 ? (nd->hi[1] - nd->lo[1]) / DPM_LAT
+                                                    : (nd->hi[0] - nd->lo[0]) / host_dpm_lon(nd->lo[1]);
+            if (extent > r2 && o->leaf_count[nd->leaf] > cap) {
+                if (++nd->ovf >= delay) { act[i] = 1; nd->ovf = 0; }
+            } else {
+                nd->ovf = 0;
+            }
+        } else if (nd->type == T_BRANCH) {
+            uint32_t leafchild = 0, sum = 0;
+            for (int c = 0; c < 4; c++) {
+                const node *ch = &o->nodes[nd->child[c]];
+                if (ch->type == T_LEAF) { leafchild++; sum += o->leaf_count[ch->leaf]; }
+            }
+            if (leafchild == 4 && sum < cap) {
+                if (++nd->unf >= delay) { act[i] = 2; nd->unf = 0; }
+            } else {
+                nd->unf = 0;
+            }
+        }
+    }
+    for (uint32_t i = 0; i < n; i++) {
+        if (act[i] != 1) continue;
+        uint32_t kids[4];
+        for (int c = 0; c < 4; c++)
+            if (take_node(o, &kids[c])) { free(act); set_err(o, "out of memory"); return -1; }
+        node *p = &o->nodes[i];
+        p->type = T_BRANCH;
+        double xhalf = p->mid_x - p->lo[0], yhalf = p->mid_y - p->lo[1];
+        for (int c = 0; c < 4; c++) {
+            node *k = &o->nodes[kids[c]];
+            memset(k, 0, sizeof(node));
+            k->type = T_LEAF;
+            k->lo[0] = p->lo[0] + (double)(c % 2 == 1) * xhalf;
+            k->lo[1] = p->lo[1] + (double)(c / 2 == 1) * yhalf;
+            k->hi[0] = p->mid_x + (double)(c % 2 == 1) * xhalf;
+            k->hi[1] = p->mid_y + (double)(c / 2 == 1) * yhalf;
+            k->mid_x = (k->lo[0] + k->hi[0]) / 2;
+            k->mid_y = (k->lo[1] + k->hi[1]) / 2;
+            p->child[c] = kids[c];
+        }
+    }
+    for (uint32_t i = 0; i < n; i++) {
+        if (act[i] != 2) continue;
+        node *p = &o->nodes[i];
+        for (int c = 0; c < 4; c++) {
+            o->nodes[p->child[c]].type = 0;
+            if (o->n_free == o->cap_free) {
+                uint32_t nc = o->cap_free ? o->cap_free * 2 : 1024;
+                uint32_t *nf = (uint32_t *)realloc(o->free_nodes, sizeof(uint32_t) * nc);
+                if (!nf) { free(act); set_err(o, "out of memory"); return -1; }
+                o->free_nodes = nf;
+                o->cap_free = nc;
+            }
+            o->free_nodes[o->n_free++] = p->child[c];
+        }
+        p->type = T_LEAF;
+    }
+    free(act);
     return 0;
 }
 
@@ -985,8 +1193,15 @@ oom:
 int orc_step(void *vo, const double *xy, uint32_t t) {
     orc *o = (orc *)vo;
     memset(o->counts, 0, sizeof(o->counts));
-    if (build_tree(o, xy)) return -1;
-    o->counts[ORC_C_NODES] = o->n_nodes;
+    if (o->cfg.schema_mode == 0) {
+        if (build_tree(o, xy)) return -1;
+        o->counts[ORC_C_NODES] = o->n_nodes;
+    } else {
+        /* the first step's points build the schema (process.cpp:401-402) */
+        if (!o->have_schema && schema_build(o, xy)) return -1;
+        if (schema_number_leaves(o)) return -1;
+        o->counts[ORC_C_NODES] = schema_valid_nodes(o);
+    }
     o->counts[ORC_C_LEAVES] = o->n_leaves;
     if (find_pairs(o, xy)) return -1;
     o->last_t = t;
@@ -996,6 +1211,8 @@ int orc_step(void *vo, const double *xy, uint32_t t) {
         if (flush(o, t)) return -1;
         o->counts[ORC_C_FLUSHED] = 1;
     }
+    /* the schema update for the next step (geometry.cu:2159-2182; process.cpp:608-612) */
+    if (o->cfg.schema_mode != 0 && schema_update(o)) return -1;
     return 0;
 }
 

@@ -38,11 +38,20 @@ typedef struct orc_config {
     uint32_t ctf_count;       /* must be a square */
     uint32_t start_time;
     uint32_t compat;          /* bit 0: reproduce the half-dropped CTF reductions */
-    uint32_t pad;
+    uint32_t schema_mode;     /* 0: QTree rebuilt every step (--disable_dynamic_schema,
+                                    src/cuda/geometry.cu:1267-1393);
+                                 1: dynamic schema: a tree built once from the first step's
+                                    points (partitioner::build_schema, src/tracing/filter.cpp:27-93)
+                                    and split/merged after every step by the GPU rules
+                                    (cuda_update_schema_*, geometry.cu:1114-1234);
+                                 2: as 1 with the CPU path's rules (update_schema_unit,
+                                    filter.cpp:377-424) */
     double reach_distance;    /* metres */
     double area_threshold;    /* dProbFalse: object MBR area (deg^2) above which it is oversize */
     double map_lo[2];
     double map_hi[2];
+    uint32_t schema_update_delay; /* config->schema_update_delay (config.h:38); 0 counts as 1 */
+    uint32_t pad;
 } orc_config;
 
 /* CTF metadata, same fields and widths as the reference CTF minus its pointers
@@ -79,6 +88,9 @@ enum {
 };
 
 void *orc_create(const orc_config *cfg);
+/* dynamic schema: build the initial tree from xy now (otherwise the first
+ * step builds it from its own points) */
+int orc_build_schema(void *o, const double *xy);
 void orc_destroy(void *o);
 /* process one timestep; xy is N AoS (x, y) doubles in degrees. 0 ok, <0 error */
 int orc_step(void *o, const double *xy, uint32_t t);
@@ -95,7 +107,9 @@ uint64_t orc_num_leaves(void *o);
 void orc_get_leaves(void *o, double *boxes, uint32_t *counts);
 /* per point: canonical index of its leaf, and whether it is a border point */
 void orc_get_point_leaf(void *o, uint32_t *leaf, uint8_t *border);
-/* all nodes: boxes, type (1 branch, 2 leaf), children as node indices; node 0 is the root */
+/* all node slots: boxes, type (0 free, 1 branch, 2 leaf), children as node
+ * indices; node 0 is the root. With a dynamic schema these are the nodes
+ * after the last step's update (the leaves above are the ones the step used). */
 uint64_t orc_num_nodes(void *o);
 void orc_get_nodes(void *o, double *boxes, double *mids, uint32_t *type, uint32_t *children);
 /* check units of the last step: (pid, node index, inside) */

@@ -20,9 +20,11 @@ class OrcConfig(C.Structure):
     _fields_ = [
         ("num_objects", C.c_uint32), ("grid_capacity", C.c_uint32), ("leaf_buffer", C.c_uint32),
         ("min_meet_time", C.c_uint32), ("max_meet_time", C.c_uint32), ("kv_restriction", C.c_uint32),
-        ("ctf_count", C.c_uint32), ("start_time", C.c_uint32), ("compat", C.c_uint32), ("pad", C.c_uint32),
+        ("ctf_count", C.c_uint32), ("start_time", C.c_uint32), ("compat", C.c_uint32),
+        ("schema_mode", C.c_uint32),
         ("reach_distance", C.c_double), ("area_threshold", C.c_double),
         ("map_lo", C.c_double * 2), ("map_hi", C.c_double * 2),
+        ("schema_update_delay", C.c_uint32), ("pad", C.c_uint32),
     ]
 
 
@@ -70,6 +72,8 @@ def load():
         lib.orc_create.restype = V
         lib.orc_create.argtypes = [C.POINTER(OrcConfig)]
         lib.orc_destroy.argtypes = [V]
+        lib.orc_build_schema.restype = C.c_int
+        lib.orc_build_schema.argtypes = [V, V]
         lib.orc_step.restype = C.c_int
         lib.orc_step.argtypes = [V, V, C.c_uint32]
         lib.orc_last_error.restype = C.c_char_p
@@ -158,6 +162,9 @@ def orc_config(cfg, box):
     c.ctf_count = cfg["ctf_count"]
     c.start_time = cfg["start_time"]
     c.compat = cfg["compat"]
+    # 0 rebuild per step; 1 dynamic schema with the GPU's update rules; 2 with the CPU path's rules
+    c.schema_mode = cfg.get("schema_mode", 1 if cfg.get("dynamic_schema", 0) else 0)
+    c.schema_update_delay = cfg.get("schema_update_delay", 1)
     c.reach_distance = cfg["reach_distance"]
     c.area_threshold = cfg["area_threshold"]
     c.map_lo[0], c.map_lo[1] = box[0]
@@ -180,6 +187,12 @@ class Oracle:
         if rc != 0:
             raise RuntimeError("oracle: " + self.lib.orc_last_error(self.h).decode())
         return self.counts()
+
+    def build_schema(self, xy):
+        """dynamic schema: the initial tree from one timestep's points"""
+        xy = np.ascontiguousarray(xy, dtype=np.float64)
+        if self.lib.orc_build_schema(self.h, _p(xy)) != 0:
+            raise RuntimeError("oracle: " + self.lib.orc_last_error(self.h).decode())
 
     def step_meetings(self, units, t):
         """a step from persisted meeting units instead of positions"""
