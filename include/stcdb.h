@@ -56,7 +56,29 @@ typedef struct stc_config {
     uint32_t keep_pairs;          /* bit 0: keep each step's pair list for stc_get_pairs;
                                      bit 1: keep each run's sorted records for stc_get_run */
     uint32_t device_runs;         /* nonzero: keep every flushed run in HBM for stc_query until stc_drop_runs */
+    uint32_t dynamic_schema;      /* config->dynamic_schema (config.h:35, the reference's default).
+                                     Nonzero: one QTree kept across steps, built from the first
+                                     step's points as partitioner::build_schema builds it
+                                     (filter.cpp:27-93) or installed with stc_set_schema, and
+                                     split/merged after every step (geometry.cu:1114-1234, 2159-2182).
+                                     Zero: the QTree is rebuilt from every step's points
+                                     (--disable_dynamic_schema, geometry.cu:1629-1651). */
+    uint32_t schema_update_delay; /* config->schema_update_delay (config.h:38); 0 counts as 1 */
 } stc_config;
+
+/* A QTree node: the reference's QTSchema (src/index/QTree.h:49-62), 80 bytes,
+ * as create_schema flattens a QTNode tree (QTree.h:286-311): node 0 is the
+ * root, a branch's children are node indices in the order bottom-left,
+ * bottom-right, top-left, top-right (QTree.h:24-29). */
+typedef struct stc_qtnode {
+    uint32_t grid_id;          /* a leaf's grid (ignored by stc_set_schema: leaves are renumbered) */
+    int16_t level;
+    int16_t type;              /* 0 invalid (a free slot), 1 BRANCH, 2 LEAF */
+    int16_t overflow_count, underflow_count;
+    double mid_x, mid_y;
+    double mbr[4];             /* low0, low1, high0, high1 */
+    uint32_t children[4];
+} stc_qtnode;
 
 typedef struct stc_box {
     double low[2];
@@ -94,7 +116,7 @@ typedef struct stc_step_out {
     uint64_t continuing;       /* pairs that continued a meeting of the previous step */
     uint64_t new_keys;         /* pairs that opened a meeting */
     float ms_count_kernel;     /* the pair count pass (last launch) */
-    float ms_reserved;
+    float ms_schema;           /* dynamic schema update after the step (pro.index_update_time) */
 } stc_step_out;
 
 /* CTF metadata of a flushed run: the fields of the reference's CTF
@@ -142,6 +164,26 @@ int stc_device_info_get(int32_t device, stc_device_info *info);
  * Allocates all device state for `cfg` on the current HIP device. map_mbr is
  * the map box (tracer::mbr). */
 int stc_create(const stc_config *cfg, const stc_box *map_mbr, void **ctx);
+
+/* ---- dynamic schema (cfg.dynamic_schema) ----
+ * stc_build_schema builds the initial QTree from one timestep's num_objects
+ * AoS points (host memory, or device memory when xy_on_device) as
+ * partitioner::build_schema does (filter.cpp:27-93 with QTNode::insert/split,
+ * QTree.h:157-214): a node splits when at least grid_capacity points reach it
+ * and half its width in metres at its low latitude exceeds reach_distance;
+ * children are the exact halves of its box. The build runs on the device.
+ * Without it, the first stc_step builds the tree from its own points (the
+ * reference builds it from the first timestep, process.cpp:401-402).
+ * stc_set_schema installs a tree the host built instead (bench->schema, which
+ * cuda_create_device_bench copies to the device, geometry.cu:1395-1565); it is
+ * checked to be a tree over the map box with leaves and branches only below
+ * node 0, and STC_E_ARG is returned otherwise. stc_get_schema copies the
+ * current tree out (after a step: the tree the next step will use); grid_id
+ * of a leaf is its leaf number in stc_get_leaves order of slots. All three
+ * return STC_E_STATE when the context rebuilds its tree every step. */
+int stc_build_schema(void *ctx, const double *xy, int32_t xy_on_device);
+int stc_set_schema(void *ctx, const stc_qtnode *nodes, uint32_t n_nodes);
+int stc_get_schema(void *ctx, stc_qtnode *nodes, uint64_t cap, uint64_t *n);
 int stc_destroy(void *ctx);
 const char *stc_last_error(void *ctx);
 

@@ -35,6 +35,8 @@ class StcConfig(C.Structure):
         ("area_threshold", C.c_double),
         ("keep_pairs", C.c_uint32),
         ("device_runs", C.c_uint32),
+        ("dynamic_schema", C.c_uint32),
+        ("schema_update_delay", C.c_uint32),
     ]
 
 
@@ -72,7 +74,7 @@ class StcStepOut(C.Structure):
         ("continuing", C.c_uint64),
         ("new_keys", C.c_uint64),
         ("ms_count_kernel", C.c_float),
-        ("ms_reserved", C.c_float),
+        ("ms_schema", C.c_float),
     ]
 
     def as_dict(self):
@@ -99,6 +101,21 @@ class StcCtfMeta(C.Structure):
         ("x_grid", C.c_uint16),
         ("y_grid", C.c_uint16),
         ("flags", C.c_uint32),
+    ]
+
+
+class StcQtNode(C.Structure):
+    """QTSchema (src/index/QTree.h:49-62), 80 bytes"""
+    _fields_ = [
+        ("grid_id", C.c_uint32),
+        ("level", C.c_int16),
+        ("type", C.c_int16),
+        ("overflow_count", C.c_int16),
+        ("underflow_count", C.c_int16),
+        ("mid_x", C.c_double),
+        ("mid_y", C.c_double),
+        ("mbr", C.c_double * 4),
+        ("children", C.c_uint32 * 4),
     ]
 
 
@@ -169,6 +186,9 @@ SIGNATURES = {
     "stc_device_info_get": (C.c_int, [C.c_int32, C.POINTER(StcDeviceInfo)]),
     "stc_create": (C.c_int, [C.POINTER(StcConfig), C.POINTER(StcBox), C.POINTER(C.c_void_p)]),
     "stc_destroy": (C.c_int, [P]),
+    "stc_build_schema": (C.c_int, [P, P, C.c_int32]),
+    "stc_set_schema": (C.c_int, [P, P, C.c_uint32]),
+    "stc_get_schema": (C.c_int, [P, P, C.c_uint64, U64P]),
     "stc_last_error": (C.c_char_p, [P]),
     "stc_step": (C.c_int, [P, P, C.c_uint32, C.POINTER(StcStepOut)]),
     "stc_step_device": (C.c_int, [P, P, C.c_uint32, C.POINTER(StcStepOut)]),

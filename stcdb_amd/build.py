@@ -11,7 +11,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libstcdb_hip.so")
-SOURCES = ["stc_radix.hip", "stc_pipeline.hip", "stc_flush.hip", "stc_query.hip", "stc_trace.hip"]
+SOURCES = ["stc_radix.hip", "stc_pipeline.hip", "stc_schema.hip", "stc_flush.hip", "stc_query.hip", "stc_trace.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("STC_OFFLOAD_ARCH", "gfx950")
 

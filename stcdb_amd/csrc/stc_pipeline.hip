@@ -41,6 +41,7 @@
 #include "stc_gen.h"
 #include "stc_flush.h"
 #include "stc_table.h"
+#include "stc_schema.h"
 #include "stc_query.h"
 #include "../../include/stcdb.h"
 
@@ -53,8 +54,6 @@
 
 #define ONE_DIM 16384u
 #define NLEVELS 15
-#define T_BRANCH 1u
-#define T_LEAF 2u
 #define STACK_DEPTH 96
 
 static thread_local std::string g_last_fail;
@@ -69,14 +68,6 @@ int stc_hip_fail(hipError_t e, const char *expr, const char *file, int line) {
 /* ------------------------------------------------------------------ */
 /* device structures                                                   */
 /* ------------------------------------------------------------------ */
-
-struct Node {                 /* QTSchema (src/index/QTree.h:49-62) without counters */
-    double lo0, lo1, hi0, hi1;
-    double mid_x, mid_y;
-    uint32_t child[4];
-    uint32_t type, leaf;
-    uint32_t code, level;     /* Morton code and level of the block (branches) */
-};
 
 struct FrontierItem { uint32_t node, code; };
 
@@ -127,8 +118,6 @@ __device__ inline uint32_t compact1by1(uint32_t x) {
 /* ------------------------------------------------------------------ */
 /* kernels: prep, tree, partition                                      */
 /* ------------------------------------------------------------------ */
-
-struct MapBox { double lo0, lo1, hi0, hi1; };
 
 __global__ void k_prep(const double2 *__restrict__ xy, uint32_t n, MapBox m, uint32_t *__restrict__ codes,
                        uint32_t *__restrict__ idx, double *__restrict__ dlon, double *__restrict__ idlon,
@@ -1696,8 +1685,13 @@ struct StcCtx {
     std::vector<PinnedBlock> deferred;   /* blocks of dropped runs whose host copy is not queued yet */
     QueryBufs *qb = nullptr;
     FlushWork flushw;
-    hipEvent_t ev[12];
+    hipEvent_t ev[14];
+    bool dyn = false;                    /* dynamic schema: the QTree persists across steps */
+    SchemaState sch;
 };
+
+/* the QTree the step's kernels descend */
+static inline Node *tree_nodes(StcCtx *c) { return c->dyn ? c->sch.nodes : c->nodes.p; }
 
 static int fail(StcCtx *c, int code, const std::string &msg) {
     c->err = msg;
@@ -1891,9 +1885,18 @@ int stc_create(const stc_config *cfg, const stc_box *mbr, void **out) {
         return STC_E_HIP;
     };
     if (hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking) != hipSuccess) return bad("stream");
-    for (int i = 0; i < 12; i++) hipEventCreate(&c->ev[i]);
+    for (int i = 0; i < 14; i++) hipEventCreate(&c->ev[i]);
     uint32_t n = c->n;
     size_t node_cap = 1 + 4ull * NLEVELS * (n / (c->cap + 1) + 1);
+    c->dyn = cfg->dynamic_schema != 0;
+    if (c->dyn) {
+        /* schema_stack_capacity = 1.6 * grids_stack_capacity, grids_stack_capacity =
+         * 4 * max(100000, N / grid_capacity) (workbench.cpp:36-41); grows on demand */
+        const uint64_t grids = 4ull * std::max<uint64_t>(100000, n / c->cap);
+        const uint32_t slots = (uint32_t)std::min<uint64_t>(grids * 8 / 5, 0x7FFFFFFFull);
+        if (stc_schema_init(&c->sch, n, slots)) return bad("schema buffers");
+        node_cap = std::max<size_t>(node_cap, c->sch.cap);
+    }
     size_t front_cap = n / (c->cap + 1) + 2;
     if (c->xy_in.ensure(n) || c->codes.ensure(n) || c->idx.ensure(n) || c->codes_alt.ensure(n) ||
         c->idx_alt.ensure(n) || c->dlon.ensure(n) || c->idlon.ensure(n) || c->lxy.ensure(n) || c->ld.ensure(n) ||
@@ -1968,7 +1971,8 @@ int stc_destroy(void *vc) {
     c->pinned.clear();
     stc_query_bufs_free(c->qb);
     c->qb = nullptr;
-    for (int i = 0; i < 12; i++) if (c->ev[i]) hipEventDestroy(c->ev[i]);
+    for (int i = 0; i < 14; i++) if (c->ev[i]) hipEventDestroy(c->ev[i]);
+    stc_schema_free(&c->sch);
     if (c->s) hipStreamDestroy(c->s);
     delete c;
     return STC_OK;
@@ -1993,23 +1997,12 @@ struct StepGeom {
 };
 static int step_tail(StcCtx *c, uint32_t t, StepGeom &g, stc_step_out *out);
 
-static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out) {
+/* the QTree rebuilt from the step's sorted cell codes (--disable_dynamic_schema,
+ * geometry.cu:1629-1651) */
+static int rebuild_tree(StcCtx *c, uint32_t *n_nodes, uint32_t *n_leaves) {
     const uint32_t n = c->n;
     hipStream_t s = c->s;
     Counters *cnt = c->dcnt;
-    memset(c->hinit, 0, sizeof(Counters));
-    c->hinit->kv_count = c->kv_count;
-    c->hinit->oversize_oid_count = c->oversize_oid_count;
-    CK(hipMemcpyAsync(cnt, c->hinit, sizeof(Counters), hipMemcpyHostToDevice, s));
-    CK(hipEventRecord(c->ev[0], s));
-
-    /* 1-2: cells, Morton codes, dpm_lon; sort codes */
-    hipLaunchKernelGGL(k_prep, dim3(blocks_for(n, 256)), dim3(256), 0, s, xy, n, c->map, c->codes.p, c->idx.p,
-                       c->dlon.p, c->idlon.p, cnt);
-    CK(hipGetLastError());
-    CKRC(stc_radix_sort_u32(c->codes.p, c->idx.p, c->codes_alt.p, c->idx_alt.p, n, 28, c->temp.p, s));
-
-    /* 3: tree */
     uint32_t node_cap = (uint32_t)c->nodes.cap;
     hipLaunchKernelGGL(k_tree_root, dim3(1), dim3(64), 0, s, c->nodes.p, c->map, c->frontier_a.p, cnt, n, c->cap);
     FrontierItem *fin = c->frontier_a.p, *fout = c->frontier_b.p;
@@ -2026,20 +2019,62 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
     }
     CK(hipGetLastError());
     CKRC(sync_counters(c));
-    Counters h = *c->hcnt;
+    const Counters &h = *c->hcnt;
     if (h.bad_points) return fail(c, STC_E_INPUT, "point outside the map box");
     if (h.table_fail) return fail(c, STC_E_CAPACITY, "QTree node capacity exceeded");
     if (n <= c->cap) return fail(c, STC_E_ARG, "root does not split: too few objects for grid_capacity");
-    uint32_t n_nodes = h.nodes, n_leaves = h.leaves;
+    *n_nodes = h.nodes;
+    *n_leaves = h.leaves;
+    return 0;
+}
+
+static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out) {
+    const uint32_t n = c->n;
+    hipStream_t s = c->s;
+    Counters *cnt = c->dcnt;
+    memset(c->hinit, 0, sizeof(Counters));
+    c->hinit->kv_count = c->kv_count;
+    c->hinit->oversize_oid_count = c->oversize_oid_count;
+    CK(hipMemcpyAsync(cnt, c->hinit, sizeof(Counters), hipMemcpyHostToDevice, s));
+    CK(hipEventRecord(c->ev[0], s));
+
+    /* 1-2: cells, Morton codes, dpm_lon; sort codes */
+    hipLaunchKernelGGL(k_prep, dim3(blocks_for(n, 256)), dim3(256), 0, s, xy, n, c->map, c->codes.p, c->idx.p,
+                       c->dlon.p, c->idlon.p, cnt);
+    CK(hipGetLastError());
+    CKRC(stc_radix_sort_u32(c->codes.p, c->idx.p, c->codes_alt.p, c->idx_alt.p, n, 28, c->temp.p, s));
+
+    /* 3: tree: rebuilt from the step's points, or the persistent dynamic schema */
+    uint32_t n_nodes, n_leaves;
+    if (c->dyn) {
+        if (!c->sch.present) { /* the first step's points build it (process.cpp:401-402) */
+            int rc = stc_schema_build(&c->sch, xy, n, c->map, c->cap, c->cfg.reach_distance, c->temp.p, c->temp.cap, s);
+            if (rc) return fail(c, rc, rc == STC_E_ARG ? "root does not split: too few objects for grid_capacity"
+                                                       : std::string("schema build: ") + g_last_fail);
+        }
+        const uint32_t nl = c->sch.n_leaves;
+        uint32_t *before = c->leaf_node.p;
+        if (c->leaf_count.ensure(nl + 1) || c->leaf_start.ensure(nl + 1) || c->leaf_node.ensure(nl + 1) ||
+            c->rev_start.ensure(nl + 1ull))
+            return fail(c, STC_E_CAPACITY, "leaf buffers");
+        if (c->leaf_node.p != before) c->sch.dirty = true;
+        if (c->sch.dirty) CKRC(stc_schema_number_leaves(&c->sch, c->leaf_node.p, c->temp.p, c->temp.cap, s));
+        n_nodes = c->sch.n_valid;
+        n_leaves = nl;
+    } else {
+        CKRC(rebuild_tree(c, &n_nodes, &n_leaves));
+    }
+    Counters h;
     c->last_nodes = n_nodes;
     c->last_leaves = n_leaves;
 
     /* 4: partition */
     CK(hipMemsetAsync(c->leaf_count.p, 0, 4ull * n_leaves, s));
-    hipLaunchKernelGGL(k_leaf_nodes, dim3(blocks_for(n_nodes, 256)), dim3(256), 0, s, c->nodes.p, n_nodes,
-                       c->leaf_node.p);
+    if (!c->dyn)
+        hipLaunchKernelGGL(k_leaf_nodes, dim3(blocks_for(n_nodes, 256)), dim3(256), 0, s, c->nodes.p, n_nodes,
+                           c->leaf_node.p);
     hipLaunchKernelGGL(k_partition, dim3(blocks_for(n, 256)), dim3(256), 0, s, xy, (const uint32_t *)c->idx.p, n,
-                       c->nodes.p, c->x_buffer, c->y_buffer, c->pt_leaf.p, c->pt_lv.p, c->leaf_count.p, cnt);
+                       (const Node *)tree_nodes(c), c->x_buffer, c->y_buffer, c->pt_leaf.p, c->pt_lv.p, c->leaf_count.p, cnt);
     CK(hipGetLastError());
     /* 5: leaves in id order */
     CKRC(stc_exclusive_scan_u32(c->leaf_count.p, c->leaf_start.p, n_leaves, nullptr, c->temp.p, s));
@@ -2057,18 +2092,19 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
     /* 6: border lookup (count, scan, emit) */
     double r = c->cfg.reach_distance;
     hipLaunchKernelGGL(k_lookup<false>, dim3(blocks_for(n, 128)), dim3(128), 0, s, (const uint32_t *)c->lp.p,
-                       (const double2 *)c->lxy.p, (const double *)c->ld.p, n, c->nodes.p,
+                       (const double2 *)c->lxy.p, (const double *)c->ld.p, n, (const Node *)tree_nodes(c),
                        c->pt_lv.p, r, c->band, c->ucount.p, (const uint32_t *)nullptr, (uint2 *)nullptr, cnt);
     CK(hipGetLastError());
     uint32_t *units_total = (uint32_t *)((char *)cnt + offsetof(Counters, units));
     CKRC(stc_exclusive_scan_u32(c->ucount.p, c->uoffset.p, n, units_total, c->temp.p, s));
     CKRC(sync_counters(c));
     h = *c->hcnt;
+    if (h.bad_points) return fail(c, STC_E_INPUT, "point outside the map box");
     if (h.lookup_overflow) return fail(c, STC_E_CAPACITY, "border lookup: node stack depth exceeded");
     uint64_t n_units = (uint32_t)h.units;
     if (c->units.ensure(n_units + 1)) return fail(c, STC_E_CAPACITY, "unit buffer allocation failed");
     hipLaunchKernelGGL(k_lookup<true>, dim3(blocks_for(n, 128)), dim3(128), 0, s, (const uint32_t *)c->lp.p,
-                       (const double2 *)c->lxy.p, (const double *)c->ld.p, n, c->nodes.p,
+                       (const double2 *)c->lxy.p, (const double *)c->ld.p, n, (const Node *)tree_nodes(c),
                        c->pt_lv.p, r, c->band, c->ucount.p, (const uint32_t *)c->uoffset.p, c->units.p, cnt);
     CK(hipGetLastError());
     CK(hipEventRecord(c->ev[2], s));
@@ -2079,7 +2115,7 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
         c->rev_pid_alt.ensure(n_units + 1) ||
         c->rev_start.ensure(n_leaves + 1ull) ||
         c->temp.ensure(stc_radix_temp_words((uint32_t)std::max<uint64_t>(std::max<uint64_t>(n_units, n),
-                                                                          c->nodes.cap)) + 64))
+                                                                          std::max<size_t>(c->nodes.cap, c->sch.cap))) + 64))
         return fail(c, STC_E_CAPACITY, "reverse unit buffers");
     if (n_units) {
         hipLaunchKernelGGL(k_units_split, dim3(blocks_for(n_units, 256)), dim3(256), 0, s, c->units.p,
@@ -2328,6 +2364,15 @@ static int step_tail(StcCtx *c, uint32_t t, StepGeom &g, stc_step_out *out) {
         c->end_time_min = t; /* process.cpp:271 with end_time_max = cur_time */
         flushed = 1;
     }
+    /* 11: the dynamic schema's update for the next step (geometry.cu:2159-2182) */
+    const bool schema_step = c->dyn && g.nodes;
+    if (schema_step) {
+        CK(hipEventRecord(c->ev[12], s));
+        int rc = stc_schema_update(&c->sch, c->leaf_count.p, c->cap, c->cfg.reach_distance,
+                                   c->cfg.schema_update_delay, c->temp.p, c->temp.cap, s);
+        if (rc) return fail(c, rc, std::string("schema update: ") + g_last_fail);
+        CK(hipEventRecord(c->ev[13], s));
+    }
     CK(hipEventRecord(c->ev[5], s));
     CK(hipEventSynchronize(c->ev[5]));
 
@@ -2362,6 +2407,7 @@ static int step_tail(StcCtx *c, uint32_t t, StepGeom &g, stc_step_out *out) {
         out->continuing = h.continuing;
         out->new_keys = h.new_keys;
         if (g.units || g.nodes) hipEventElapsedTime(&out->ms_count_kernel, c->ev[10], c->ev[11]);
+        if (schema_step) hipEventElapsedTime(&out->ms_schema, c->ev[12], c->ev[13]);
     }
     return STC_OK;
 }
@@ -2513,14 +2559,22 @@ int stc_get_leaves(void *vc, double *boxes, uint32_t *counts, uint64_t cap, uint
     StcCtx *c = (StcCtx *)vc;
     if (!c || !n) return STC_E_ARG;
     if (!c->stepped) return fail(c, STC_E_STATE, "no step processed");
-    std::vector<Node> nd(c->last_nodes);
-    std::vector<uint32_t> lc(c->last_leaves);
-    CK(hipMemcpy(nd.data(), c->nodes.p, nd.size() * sizeof(Node), hipMemcpyDeviceToHost));
-    if (!lc.empty()) CK(hipMemcpy(lc.data(), c->leaf_count.p, lc.size() * 4, hipMemcpyDeviceToHost));
+    /* the leaves the last step used (a dynamic schema has been updated since,
+     * but a leaf's slot keeps its box until a later step reuses it) */
+    const uint32_t nl = c->last_leaves;
+    std::vector<uint32_t> ln(nl), lc(nl);
+    if (nl) {
+        CK(hipMemcpy(ln.data(), c->leaf_node.p, nl * 4ull, hipMemcpyDeviceToHost));
+        CK(hipMemcpy(lc.data(), c->leaf_count.p, nl * 4ull, hipMemcpyDeviceToHost));
+    }
     struct L { double lo1, lo0, hi0, hi1; uint32_t cnt; };
-    std::vector<L> ls;
-    for (const Node &x : nd)
-        if (x.type == T_LEAF) ls.push_back(L{x.lo1, x.lo0, x.hi0, x.hi1, lc[x.leaf]});
+    std::vector<L> ls(nl);
+    const Node *tn = tree_nodes(c);
+    for (uint32_t l = 0; l < nl; l++) {
+        Node x;
+        CK(hipMemcpy(&x, tn + ln[l], sizeof(Node), hipMemcpyDeviceToHost));
+        ls[l] = L{x.lo1, x.lo0, x.hi0, x.hi1, lc[l]};
+    }
     std::sort(ls.begin(), ls.end(), [](const L &a, const L &b) {
         if (a.lo1 != b.lo1) return a.lo1 < b.lo1;
         return a.lo0 < b.lo0;
@@ -2533,6 +2587,44 @@ int stc_get_leaves(void *vc, double *boxes, uint32_t *counts, uint64_t cap, uint
         }
         if (counts) counts[i] = ls[i].cnt;
     }
+    return STC_OK;
+}
+
+int stc_build_schema(void *vc, const double *xy, int32_t xy_on_device) {
+    StcCtx *c = (StcCtx *)vc;
+    if (!c || !xy) return STC_E_ARG;
+    if (!c->dyn) return fail(c, STC_E_STATE, "the context rebuilds its QTree every step (dynamic_schema = 0)");
+    const double2 *src = (const double2 *)xy;
+    if (!xy_on_device) {
+        CK(hipMemcpyAsync(c->xy_in.p, xy, (size_t)c->n * 16, hipMemcpyHostToDevice, c->s));
+        src = c->xy_in.p;
+    }
+    int rc = stc_schema_build(&c->sch, src, c->n, c->map, c->cap, c->cfg.reach_distance, c->temp.p, c->temp.cap, c->s);
+    if (rc) return fail(c, rc, rc == STC_E_ARG ? "root does not split: too few objects for grid_capacity"
+                                               : std::string("schema build: ") + g_last_fail);
+    CK(hipStreamSynchronize(c->s));
+    return STC_OK;
+}
+
+int stc_set_schema(void *vc, const stc_qtnode *nodes, uint32_t n_nodes) {
+    StcCtx *c = (StcCtx *)vc;
+    if (!c) return STC_E_ARG;
+    if (!c->dyn) return fail(c, STC_E_STATE, "the context rebuilds its QTree every step (dynamic_schema = 0)");
+    int rc = stc_schema_set(&c->sch, nodes, n_nodes, c->s);
+    if (rc == STC_E_ARG) return fail(c, rc, "not a QTree below node 0");
+    if (rc) return fail(c, rc, std::string("schema upload: ") + g_last_fail);
+    if (c->temp.ensure(stc_radix_temp_words(std::max<uint32_t>(c->n, c->sch.cap)) + 64))
+        return fail(c, STC_E_CAPACITY, "scan scratch");
+    return STC_OK;
+}
+
+int stc_get_schema(void *vc, stc_qtnode *nodes, uint64_t cap, uint64_t *n) {
+    StcCtx *c = (StcCtx *)vc;
+    if (!c || !n) return STC_E_ARG;
+    if (!c->dyn) return fail(c, STC_E_STATE, "the context rebuilds its QTree every step (dynamic_schema = 0)");
+    int rc = stc_schema_get(&c->sch, nodes, cap, n, c->s);
+    if (rc == STC_E_STATE) return fail(c, rc, "no schema yet");
+    if (rc) return fail(c, rc, std::string("schema download: ") + g_last_fail);
     return STC_OK;
 }
 

@@ -40,6 +40,11 @@ Q_OID, Q_RANGE, Q_PAIR = 1, 2, 3
 # stc_meeting_unit: the reference's 32-byte meeting_unit (src/tracing/workbench.h:62-83)
 MEETING_UNIT_DTYPE = np.dtype([("key", "<u8"), ("start", "<u4"), ("end", "<u4"), ("mbr", "<f4", (4,))])
 assert MEETING_UNIT_DTYPE.itemsize == 32
+# stc_qtnode: the reference's 80-byte QTSchema (src/index/QTree.h:49-62)
+QTNODE_DTYPE = np.dtype([("grid_id", "<u4"), ("level", "<i2"), ("type", "<i2"), ("overflow_count", "<i2"),
+                         ("underflow_count", "<i2"), ("pad", "<u4"), ("mid_x", "<f8"), ("mid_y", "<f8"),
+                         ("mbr", "<f8", (4,)), ("children", "<u4", (4,))])
+assert QTNODE_DTYPE.itemsize == 80
 
 
 class HostTrace:
@@ -177,6 +182,29 @@ class Pipeline:
         m = C.c_uint64()
         _check(self.lib.stc_get_records(self.ctx, _ptr(keys), _ptr(boxes), n, C.byref(m)), self.ctx)
         return keys, boxes
+
+    def build_schema(self, xy, on_device=False):
+        """dynamic schema: the initial QTree from one timestep's points (host
+        array (N, 2), or a device pointer with on_device)"""
+        if on_device:
+            _check(self.lib.stc_build_schema(self.ctx, xy, 1), self.ctx)
+        else:
+            xy = np.ascontiguousarray(xy, dtype=np.float64)
+            assert xy.shape == (self.n, 2)
+            _check(self.lib.stc_build_schema(self.ctx, _ptr(xy), 0), self.ctx)
+
+    def set_schema(self, nodes):
+        """dynamic schema: install a QTree (QTNODE_DTYPE array, node 0 the root)"""
+        nodes = np.ascontiguousarray(nodes, dtype=QTNODE_DTYPE)
+        _check(self.lib.stc_set_schema(self.ctx, _ptr(nodes), len(nodes)), self.ctx)
+
+    def schema(self):
+        """the current QTree as a QTNODE_DTYPE array (slots; type 0 = free)"""
+        n = self._count(self.lib.stc_get_schema, None)
+        out = np.zeros(n, dtype=QTNODE_DTYPE)
+        m = C.c_uint64()
+        _check(self.lib.stc_get_schema(self.ctx, _ptr(out), n, C.byref(m)), self.ctx)
+        return out
 
     def leaves(self):
         n = self._count(self.lib.stc_get_leaves, None, None)

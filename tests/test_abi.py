@@ -33,6 +33,7 @@ def test_binding_covers_every_declared_symbol():
 def test_struct_layouts_match_the_c_compiler(tmp_path):
     """every ctypes mirror has the size and field offsets gcc gives the C structs"""
     structs = {"stc_config": L.StcConfig, "stc_box": L.StcBox, "stc_step_out": L.StcStepOut,
+               "stc_qtnode": L.StcQtNode,
                "stc_ctf_meta": L.StcCtfMeta, "stc_run_info": L.StcRunInfo, "stc_device_info": L.StcDeviceInfo,
                "stc_gen_params": L.StcGenParams, "stc_walker": L.StcWalker}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "stcdb_amd/csrc/stc_gen.h"',

@@ -21,16 +21,37 @@ def sorted_records(keys, boxes):
     return keys[order], boxes[order]
 
 
-def run_parity(w, steps, check_every=1, tag=""):
+def schema_set(nodes_boxes, types, children):
+    """a QTree as {box: (type, child boxes)} over its valid slots"""
+    out = {}
+    for i in np.nonzero(types != 0)[0]:
+        b = tuple(nodes_boxes[i].tolist())
+        out[b] = (int(types[i]), tuple(tuple(nodes_boxes[c].tolist()) for c in children[i]) if types[i] == 1 else None)
+    return out
+
+
+def gpu_schema(gpu):
+    q = gpu.schema()
+    return schema_set(q["mbr"], q["type"].astype(np.int64), q["children"])
+
+
+def oracle_schema(orc):
+    boxes, mids, typ, ch = orc.nodes()
+    return schema_set(boxes, typ.astype(np.int64), ch)
+
+
+def run_parity(w, steps, check_every=1, tag="", frames=None):
+    """the library and the oracle on the same steps (frames: an iterable of
+    (t, xy); by default the workload's host trace)"""
     cfg, box = dict(w["cfg"]), w["box"]
     cfg["keep_pairs"] = 1
     print(f"[parity {tag}] n={cfg['num_objects']} cap={cfg['grid_capacity']} r={cfg['reach_distance']} "
-          f"steps={steps}", flush=True)
+          f"steps={steps} dynamic={cfg.get('dynamic_schema', 0)}", flush=True)
     gpu = Pipeline(cfg, box)
     orc = O.Oracle(cfg, box)
-    tr = HostTrace(w["gen"])
-    totals = dict(pairs=0, records=0, ambiguous=0)
-    for t, xy in tr.steps(steps):
+    dyn = cfg.get("dynamic_schema", 0)
+    totals = dict(pairs=0, records=0, ambiguous=0, leaves=set())
+    for t, xy in (frames if frames is not None else HostTrace(w["gen"]).steps(steps)):
         g = gpu.step(xy, t)
         o = orc.step(xy, t)
         assert g["leaves"] == o["leaves"], t
@@ -56,6 +77,9 @@ def run_parity(w, steps, check_every=1, tag=""):
             assert np.array_equal(gk, ok), t
             assert np.array_equal(gs, os_) and np.array_equal(ge, oe), t
             assert np.array_equal(gm.view(np.uint32), om.view(np.uint32)), t
+        if dyn:
+            assert gpu_schema(gpu) == oracle_schema(orc), t
+        totals["leaves"].add(o["leaves"])
         totals["pairs"] += o["pairs"]
     gk, gb = sorted_records(*gpu.records())
     ok, ob = sorted_records(*orc.records())
