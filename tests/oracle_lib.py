@@ -340,3 +340,90 @@ def point_distance(x1, y1, x2, y2):
 def box_distance(box4, px, py):
     b = np.ascontiguousarray(box4, dtype=np.float64)
     return load().orc_box_distance(_p(b), px, py)
+
+
+CPATH_LIB = os.path.join(ORACLE_DIR, "libcpupath.so")
+_cpath = None
+
+
+def load_cpath():
+    global _cpath
+    if _cpath is None:
+        if not os.path.exists(CPATH_LIB):
+            subprocess.run(["make", "-C", ORACLE_DIR, "libcpupath.so"], check=True, capture_output=True)
+        lib = C.CDLL(CPATH_LIB)
+        V = C.c_void_p
+        lib.cpath_create.restype = V
+        lib.cpath_create.argtypes = [C.POINTER(OrcConfig), C.c_uint32, C.c_uint64, C.c_uint32]
+        lib.cpath_destroy.argtypes = [V]
+        lib.cpath_step.restype = C.c_int
+        lib.cpath_step.argtypes = [V, V, C.c_uint32]
+        lib.cpath_last_error.restype = C.c_char_p
+        lib.cpath_last_error.argtypes = [V]
+        lib.cpath_stage_ms.argtypes = [V, V]
+        lib.cpath_counts.argtypes = [V, V]
+        for name in ["cpath_num_pairs", "cpath_num_meetings", "cpath_num_leaves"]:
+            getattr(lib, name).restype = C.c_uint64
+            getattr(lib, name).argtypes = [V]
+        lib.cpath_get_pairs.argtypes = [V, V]
+        lib.cpath_get_meetings.argtypes = [V, V]
+        lib.cpath_get_leaves.argtypes = [V, V, V]
+        _cpath = lib
+    return _cpath
+
+
+class CpuPath:
+    """the reference's CPU path (filter, reachability, update_meetings,
+    update_schema on `threads` pthreads; oracle/stc_cpu_path.c)"""
+
+    STAGES = ["filter", "reachability", "update_meetings", "update_schema"]
+
+    def __init__(self, cfg, box, threads, buckets=0, keep_pairs=True):
+        self.lib = load_cpath()
+        c = dict(cfg)
+        c.setdefault("schema_mode", 2 if c.get("dynamic_schema", 1) else 0)
+        self._c = orc_config(c, box)
+        self.h = self.lib.cpath_create(C.byref(self._c), threads, buckets, 1 if keep_pairs else 0)
+        if not self.h:
+            raise MemoryError("cpath_create")
+
+    def step(self, xy, t):
+        xy = np.ascontiguousarray(xy, dtype=np.float64)
+        if self.lib.cpath_step(self.h, _p(xy), t) != 0:
+            raise RuntimeError("cpu path: " + self.lib.cpath_last_error(self.h).decode())
+        ms = np.zeros(4)
+        self.lib.cpath_stage_ms(self.h, _p(ms))
+        cnt = np.zeros(6, dtype=np.uint64)
+        self.lib.cpath_counts(self.h, _p(cnt))
+        out = dict(zip(["units", "candidates", "pairs", "taken", "active", "identified"], map(int, cnt)))
+        out["ms"] = dict(zip(self.STAGES, ms.tolist()))
+        return out
+
+    def pairs(self):
+        k = np.empty(self.lib.cpath_num_pairs(self.h), dtype=np.uint64)
+        self.lib.cpath_get_pairs(self.h, _p(k))
+        return k
+
+    def meetings(self):
+        m = np.empty((self.lib.cpath_num_meetings(self.h), 4), dtype=np.uint32)
+        self.lib.cpath_get_meetings(self.h, _p(m))
+        return m
+
+    def leaves(self):
+        n = self.lib.cpath_num_leaves(self.h)
+        b = np.empty((n, 4), dtype=np.float64)
+        c = np.empty(n, dtype=np.uint32)
+        self.lib.cpath_get_leaves(self.h, _p(b), _p(c))
+        order = np.lexsort((b[:, 0], b[:, 1]))
+        return b[order], c[order]
+
+    def close(self):
+        if self.h:
+            self.lib.cpath_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
