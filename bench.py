@@ -13,9 +13,11 @@ Gaussian hotspots (r = 20 m, minimum meeting 10 s, runs of 8 Mi records in
 trace is generated on the device before timing, so the timed region holds
 only pipeline steps on points already resident in HBM.
 
-Output: one JSON line (metric, value, roofline of the dominant kernel from
-HIP events on the library's stream, CPU baseline from the oracle on a bounded
-sample of the same workload).
+Output: one JSON line: metric and value; the roofline of the dominant
+kernel from HIP events on the library's stream, with every stage's
+algorithmic GB/s; the reference's CPU path timed on the host's cores on a
+bounded sample of the same workload; and a serving phase (5M objects at 1 Hz
+plus 2,000 queries/s with every hit materialised) reporting p50/p99.
 """
 import argparse
 import ctypes
@@ -69,25 +71,57 @@ def cpu_sample_ids(gen):
     return np.nonzero((hot & (hid == 0)) | (~hot & (oid % np.uint64(20) == 0)))[0]
 
 
-def cpu_baseline(w, frames_host, steps):
-    """the CPU oracle (single thread) on the sample over `steps` timesteps"""
+def cpu_threads():
+    """the host threads a CPU baseline may use: the process's CPUs, capped by
+    OMP_NUM_THREADS (the GPU box's share of its host)"""
+    n = len(os.sched_getaffinity(0))
+    cap = os.environ.get("OMP_NUM_THREADS")
+    return max(1, min(n, int(cap))) if cap and cap.isdigit() else n
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
+def cpu_baseline(w, frames_host, steps, schema):
+    """The reference's CPU path (tracer::process without -g: filter,
+    reachability, update_meetings, update_schema on pthreads; restated in
+    oracle/stc_cpu_path.c) on a bounded sample of the same workload. The
+    first step builds the tree (once, as build_schema does before the loop)
+    and is not timed; the rate is sample objects x timed steps / the sum of
+    the four stages' wall times."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
     ids = cpu_sample_ids(w["gen"])
     cfg = dict(w["cfg"])
     cfg["num_objects"] = int(len(ids))
-    orc = O.Oracle(cfg, w["box"])
-    t0 = time.perf_counter()
+    cfg["schema_mode"] = 2 if schema == "dynamic" else 0
+    threads = cpu_threads()
+    buckets = 1 << 27  # ~0.37 of the sample's ~49M pairs per step: no probe chain near a full table
+    cp = O.CpuPath(cfg, w["box"], threads=threads, buckets=buckets, keep_pairs=False)
+    cp.step(np.ascontiguousarray(frames_host[0][ids]), 0)
+    stage = {k: 0.0 for k in O.CpuPath.STAGES}
     pairs = 0
-    for t in range(steps):
-        c = orc.step(np.ascontiguousarray(frames_host[t][ids]), t)
-        pairs += c["pairs"]
-    dt = time.perf_counter() - t0
-    orc.close()
-    return dict(value=len(ids) * steps / dt, unit="point-updates/s", cores=1, kind="port",
-                sample=f"CPU oracle, 1 thread, {steps} timesteps of {len(ids)} objects (all of hotspot 0 and every "
-                       f"20th uniformly started object of the same workload, density preserved); "
-                       f"{pairs} pairs found; {dt:.1f} s")
+    for t in range(1, steps + 1):
+        r = cp.step(np.ascontiguousarray(frames_host[t][ids]), t)
+        pairs += r["pairs"]
+        for k, v in r["ms"].items():
+            stage[k] += v
+    cp.close()
+    total_s = sum(stage.values()) / 1e3
+    return dict(value=round(len(ids) * steps / total_s, 1), unit="point-updates/s", cores=threads, kind="port",
+                cpu_model=cpu_model(), nproc=os.cpu_count(),
+                ms_per_step={k: round(v / steps, 2) for k, v in stage.items()},
+                sample=f"reference CPU path (filter, reachability, update_meetings, update_schema; "
+                       f"{threads} threads, {buckets} meeting buckets) on {len(ids)} objects of the same workload "
+                       f"(all of hotspot 0 and every 20th uniformly started object, density preserved), "
+                       f"{steps} timesteps after the first; {pairs} pairs found; {total_s:.1f} s of stages")
 
 
 def measured_traffic(kernel, workload, n):
@@ -114,11 +148,14 @@ def make_query_stream(rng, n_objects, box, t_now, count):
     """C5's query mix: 50 % object history, 40 % square range, 10 % pair
     lookup; time windows log-uniform in [10 s, 1 h] ending now; range sides
     log-uniform in [100 m, 5 km] centred uniformly on the map"""
-    from stcdb_amd.engine import QUERY_DTYPE, Q_OID, Q_PAIR, Q_RANGE
+    from stcdb_amd.engine import QF_COUNT, QUERY_DTYPE, Q_OID, Q_PAIR, Q_RANGE
     from stcdb_amd.workloads import DPM_LAT, dpm_lon
     q = np.zeros(count, dtype=QUERY_DTYPE)
     u = rng.random(count)
     q["kind"] = np.where(u < 0.5, Q_OID, np.where(u < 0.9, Q_RANGE, Q_PAIR))
+    # range co-occurrence queries report exact counts, as the reference's box search does
+    # (search_count, workbench.cpp:478-572); object and pair queries return every hit
+    q["flags"] = np.where(q["kind"] == Q_RANGE, QF_COUNT, 0)
     q["oid"] = rng.integers(0, n_objects, count)
     q["oid2"] = rng.integers(0, n_objects, count)
     window = np.exp(rng.uniform(np.log(10.0), np.log(3600.0), count)).astype(np.int64)
@@ -132,13 +169,26 @@ def make_query_stream(rng, n_objects, box, t_now, count):
     return q
 
 
-def serve(args):
+def answer(pipe, q, buf):
+    """a query batch with every hit of its object and pair queries in host
+    memory (buffer grown when short); range queries count"""
+    from stcdb_amd.engine import HIT_DTYPE
+    hits, counts = pipe.query(q, cap=len(buf[0]), into=buf[0])
+    stored = int(counts[q["flags"] == 0].sum())
+    if stored > len(buf[0]):
+        buf[0] = np.zeros(stored + stored // 2, dtype=HIT_DTYPE)
+        hits, counts = pipe.query(q, cap=len(buf[0]), into=buf[0])
+    return stored, int(counts.sum())
+
+
+def serve(args, objects=0, schema="dynamic"):
     """ingest at 1 Hz plus a Poisson query stream, answered in batches between
-    steps; latency = answer time - arrival time on the wall clock; each query
-    returns its exact hit count"""
+    steps; latency = answer time - arrival time on the wall clock; every hit
+    of every query is materialised in host memory"""
     from stcdb_amd import DeviceTrace, Pipeline, workload
-    over = {"num_objects": args.objects} if args.objects else {}
-    w = workload(args.workload if args.workload != "c3" else "c5", **over)
+    from stcdb_amd.engine import HIT_DTYPE
+    over = {"num_objects": objects} if objects else {}
+    w = workload("c5", dynamic_schema=1 if schema == "dynamic" else 0, **over)
     cfg, gen = w["cfg"], w["gen"]
     n = cfg["num_objects"]
     pipe = Pipeline(cfg, w["box"])
@@ -149,7 +199,8 @@ def serve(args):
         r = pipe.step_device(trace.next(t), t)
         print(f"[serve] warmup step {t}: {r['ms_total']:.1f} ms, {r['pairs']} pairs, runs {pipe.num_runs()}",
               file=sys.stderr, flush=True)
-    lat, batches, step_ms, hits_total = [], [], [], 0
+    lat, batches, step_ms, hits_total, counted_total = [], [], [], 0, 0
+    buf = [np.zeros(1 << 21, dtype=HIT_DTYPE)]
     for s in range(args.serve_seconds):
         t = args.warmup + s
         t_sec = time.perf_counter()
@@ -166,12 +217,12 @@ def serve(args):
             if ready <= done:
                 time.sleep(max(arrivals[done] - now, 0.0))
                 continue
-            # exact counts per query, as the reference's searches report them (search_count)
-            _, counts = pipe.query(queries[done:ready], cap=0)
+            stored, counted = answer(pipe, queries[done:ready], buf)
             fin = time.perf_counter() - t_sec
             lat.extend((fin - arrivals[done:ready]) * 1000.0)
             batches.append(ready - done)
-            hits_total += int(counts.sum())
+            hits_total += stored
+            counted_total += counted
             done = ready
         rest = 1.0 - (time.perf_counter() - t_sec)
         if rest > 0:
@@ -179,27 +230,25 @@ def serve(args):
         print(f"[serve] second {s}: step {r['ms_total']:.1f} ms, {k} queries in {len(batches)} batches so far, "
               f"runs {pipe.num_runs()}", file=sys.stderr, flush=True)
     lat = np.array(lat)
-    line = {
-        "metric": "p50/p99 query ms at 2k QPS with ingest at 1 Hz (5M objects)",
-        "value": round(float(np.percentile(lat, 99)), 3),
-        "unit": "ms (p99)",
+    out = {
         "p50_ms": round(float(np.percentile(lat, 50)), 3),
         "p99_ms": round(float(np.percentile(lat, 99)), 3),
         "queries": int(len(lat)),
         "qps": args.qps,
-        "hits": hits_total,
+        "hits_materialised": hits_total,
+        "hits_counted": counted_total,
         "batches": len(batches),
         "mean_batch": round(float(np.mean(batches)), 1),
         "ingest_ms_per_step": round(float(np.mean(step_ms)), 3),
-        "n_gpus": 1,
-        "higher_is_better": False,
-        "data": "synthetic (seeded street-grid walk on the device; seeded Poisson query stream)",
-        "config": {"workload": "c5", "objects": n, "seconds": args.serve_seconds, "warmup_steps": args.warmup,
-                   "runs_in_hbm": pipe.num_runs()},
+        "objects": n,
+        "seconds": args.serve_seconds,
+        "warmup_steps": args.warmup,
+        "runs_in_hbm": pipe.num_runs(),
+        "schema": schema,
     }
-    print(json.dumps(line), flush=True)
     pipe.close()
     trace.close()
+    return out
 
 
 def compact(args):
@@ -251,6 +300,39 @@ def compact(args):
     pipe.close()
 
 
+def stage_roofline(outs, n):
+    """per stage of a step: mean ms (HIP events on the library's stream) and
+    the stage's algorithmic HBM bytes (points in, pairs and meetings out; the
+    flush over the steps that flushed), as GB/s and a fraction of the peak"""
+    def mean(k, rows=None):
+        rows = outs if rows is None else rows
+        return float(np.mean([r[k] for r in rows])) if rows else 0.0
+    pairs = mean("pairs")
+    slots = mean("table_buckets")
+    flushed = [r for r in outs if r["flushed"]]
+    recs = mean("new_records")
+    stages = {
+        # QTree descent and leaf placement: 16 B point in, 4 B leaf-ordered id out
+        "partition": (mean("ms_partition"), 20.0 * n),
+        # border lookup: the border points' 16 B positions
+        "filter": (mean("ms_filter"), 16.0 * mean("border_points")),
+        # pair refinement (reverse lists, count, write, segment sorts): points in, 8 B pair keys out
+        "refine": (mean("ms_refine"), 20.0 * n + 8.0 * pairs),
+        # meetings: a 32 B bucket read and written per pair, the previous array scanned,
+        # 2 x (16 B key + 16 B box) per recorded meeting
+        "meetings": (mean("ms_meetings"), 64.0 * pairs + 32.0 * slots + 32.0 * recs),
+    }
+    if flushed:
+        R = mean("kv_count", flushed)
+        # STR 2 x 12 B per object, main sort 2 x 32 B, bitmap 16 B and encode 32 B + ~13 B packed per record
+        stages["flush"] = (mean("ms_flush", flushed), 24.0 * n + (64.0 + 16.0 + 32.0 + 13.0) * R)
+    out = {}
+    for k, (ms, b) in stages.items():
+        gbs = b / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+        out[k] = dict(ms=round(ms, 3), alg_bytes=b, gbs=round(gbs, 1), frac=round(gbs / HBM_PEAK_GBS, 4))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -258,14 +340,19 @@ def main():
     ap.add_argument("--warmup", type=int, default=12)
     ap.add_argument("--workload", default="c3")
     ap.add_argument("--objects", type=int, default=0, help="override the object count")
+    ap.add_argument("--schema", choices=["dynamic", "rebuild"], default="dynamic",
+                    help="QTree mode: kept and updated across steps (the reference's default) or rebuilt every step "
+                         "(--disable_dynamic_schema)")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--host-steps", type=int, default=10,
                     help="extra steps fed from pinned host memory (PCIe-inclusive rate, reported beside value)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving phase of the default run")
     ap.add_argument("--serve", action="store_true",
-                    help="serving mode: 5M objects ingested at 1 Hz plus 2,000 queries/s; prints p50/p99 latency")
+                    help="serving only: 5M objects ingested at 1 Hz plus 2,000 queries/s; prints p50/p99 latency")
     ap.add_argument("--qps", type=float, default=2000.0)
-    ap.add_argument("--serve-seconds", type=int, default=10)
+    ap.add_argument("--serve-seconds", type=int, default=8)
+    ap.add_argument("--serve-warmup", type=int, default=12)
     ap.add_argument("--compact", type=int, default=0,
                     help="compaction mode: flush this many runs into HBM, then time merging them into one")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -276,7 +363,11 @@ def main():
               f"see the sharding design for several GPUs)", file=sys.stderr)
         sys.exit(2)
     if args.serve:
-        serve(args)
+        sv = serve(argparse.Namespace(**dict(vars(args), warmup=args.serve_warmup)), args.objects, args.schema)
+        print(json.dumps(dict({"metric": "p50/p99 query ms at 2k QPS with ingest at 1 Hz (5M objects)",
+                               "value": sv["p99_ms"], "unit": "ms (p99)", "n_gpus": 1, "higher_is_better": False,
+                               "data": "synthetic (seeded street-grid walk on the device; seeded Poisson queries)"},
+                              **sv)), flush=True)
         return
     if args.compact:
         compact(args)
@@ -285,7 +376,7 @@ def main():
     import torch
     from stcdb_amd import DeviceTrace, Pipeline, workload
 
-    over = {}
+    over = {"dynamic_schema": 1 if args.schema == "dynamic" else 0}
     if args.objects:
         over["num_objects"] = args.objects
     w = workload(args.workload, **over)
@@ -303,7 +394,8 @@ def main():
         assert h.hipMemcpy(frames[t].data_ptr(), p.value, n * 16, 3) == 0
     trace.close()
     torch.cuda.synchronize()
-    print(f"[bench] workload {args.workload}: {n} objects, {nframes} frames resident", file=sys.stderr, flush=True)
+    print(f"[bench] workload {args.workload}: {n} objects, {nframes} frames resident, schema {args.schema}",
+          file=sys.stderr, flush=True)
 
     if args.copy_probe_mib:
         src = torch.ones(args.copy_probe_mib << 18, dtype=torch.float32, device="cuda")
@@ -319,7 +411,8 @@ def main():
     for t in range(args.warmup):
         r = pipe.step_device(ctypes.c_void_p(frames[t].data_ptr()), t)
         pipe.drop_runs()
-        print(f"[bench] warmup step {t}: {r['ms_total']:.1f} ms, {r['pairs']} pairs", file=sys.stderr, flush=True)
+        print(f"[bench] warmup step {t}: {r['ms_total']:.1f} ms, {r['pairs']} pairs, {r['leaves']} leaves",
+              file=sys.stderr, flush=True)
 
     outs = []
     torch.cuda.synchronize()
@@ -335,8 +428,9 @@ def main():
     elapsed = time.perf_counter() - t0
     for i, r in enumerate(outs):
         print(f"[bench] step {args.warmup + i}: {r['ms_total']:.1f} ms (count {r['ms_count_kernel']:.1f}, "
-              f"write {r['ms_refine_kernel']:.1f}, merge {r['ms_identify_kernel']:.1f}, flush {r['ms_flush']:.1f}) "
-              f"pairs {r['pairs']} records {r['new_records']} flushed {r['flushed']}", file=sys.stderr, flush=True)
+              f"write {r['ms_refine_kernel']:.1f}, merge {r['ms_identify_kernel']:.1f}, flush {r['ms_flush']:.1f}, "
+              f"schema {r['ms_schema']:.2f}) pairs {r['pairs']} records {r['new_records']} leaves {r['leaves']} "
+              f"flushed {r['flushed']}", file=sys.stderr, flush=True)
 
     value = n * args.steps / elapsed
 
@@ -345,7 +439,6 @@ def main():
     if args.host_steps:
         pinned = torch.empty((args.host_steps, n, 2), dtype=torch.float64, pin_memory=True)
         pinned.copy_(frames[total:total + args.host_steps])
-        del frames
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for i in range(args.host_steps):
@@ -354,6 +447,11 @@ def main():
         pipe.sync_runs()
         torch.cuda.synchronize()
         host_value = n * args.host_steps / (time.perf_counter() - t0)
+        del pinned
+    del frames
+    pipe.close()
+    torch.cuda.empty_cache()
+
     # roofline of the dominant kernel (algorithmic bytes per launch / its HIP-event time)
     def mean(k):
         return float(np.mean([r[k] for r in outs]))
@@ -379,20 +477,28 @@ def main():
                 alg_bytes_per_launch=alg, avg_ms=round(ms, 3),
                 kernels={k: dict(ms=round(v[1], 3), alg_bytes=v[2], gbs=round(v[2] / (v[1] * 1e-3) / 1e9, 1),
                                  traffic=measured_traffic(k, args.workload, n))
-                         for k, v in kernels.items()})
+                         for k, v in kernels.items()},
+                stages=stage_roofline(outs, n))
 
     cpu = None
     if not args.no_cpu:
         trace = DeviceTrace(gen)
         frames_host = []
-        for t in range(args.cpu_steps):
+        for t in range(args.cpu_steps + 1):
             p = trace.next(t)
             buf = np.empty((n, 2), dtype=np.float64)
             assert h.hipMemcpy(buf.ctypes.data, p.value, n * 16, 2) == 0
             frames_host.append(buf)
         trace.close()
-        cpu = cpu_baseline(w, frames_host, args.cpu_steps)
-        cpu["cpu_model"] = platform.processor() or platform.machine()
+        cpu = cpu_baseline(w, frames_host, args.cpu_steps, args.schema)
+        del frames_host
+        print(f"[bench] cpu baseline: {cpu['value']:.0f} point-updates/s on {cpu['cores']} threads", file=sys.stderr,
+              flush=True)
+
+    serving = None
+    if not args.no_serve:
+        sargs = argparse.Namespace(**dict(vars(args), warmup=args.serve_warmup))
+        serving = serve(sargs, 0, args.schema)
 
     line = {
         "metric": "point-updates/s ingested at 10M objects",
@@ -407,7 +513,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded street-grid walk generated on the device)",
-        "config": {"workload": args.workload, "objects": n, "reach_m": cfg["reach_distance"],
+        "config": {"workload": args.workload, "objects": n, "schema": args.schema, "reach_m": cfg["reach_distance"],
                    "min_meet_s": cfg["min_meet_time"], "leaf_capacity": cfg["grid_capacity"],
                    "kv_restriction": cfg["kv_restriction"], "ctf_count": cfg["ctf_count"],
                    "pairs_per_step": int(pairs),
@@ -415,9 +521,9 @@ def main():
         "host_input_value": round(host_value, 1) if host_value else None,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "serving": serving,
     }
     print(json.dumps(line), flush=True)
-    pipe.close()
 
 
 if __name__ == "__main__":

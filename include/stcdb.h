@@ -291,16 +291,22 @@ int stc_dump_run(void *ctx, uint32_t run, const char *dir, const char *prefix, u
  * in HBM (device_runs), whose keys are decoded as CTF::ctf_get_key_oid,
  * parse_key and new_parse_mbr decode them (step_merge.cpp:325-379). counts[q]
  * is the number of hits of query q; at most `cap` hits are stored, sorted by
- * (query, source, pid, target, start, end); *n_hits is the total. */
+ * (query, source, pid, target, start, end), none of a query flagged
+ * STC_QF_COUNT (the reference's box search counts, search_count at
+ * workbench.cpp:478-572); *n_hits is the number of hits of the queries not
+ * flagged STC_QF_COUNT (all of which fit when cap >= *n_hits). */
 #define STC_Q_OID 1u
 #define STC_Q_RANGE 2u
 #define STC_Q_PAIR 3u
+
+#define STC_QF_COUNT 1u     /* count the query's hits and store none (the reference's
+                              * search_count; a range query can meet millions of records) */
 
 typedef struct stc_query {
     uint32_t kind;
     uint32_t oid, oid2;
     uint32_t t0, t1;           /* inclusive time window */
-    uint32_t pad;
+    uint32_t flags;            /* STC_QF_* bits */
     double box[4];             /* STC_Q_RANGE: low0, low1, high0, high1 (degrees) */
 } stc_query;
 
@@ -322,6 +328,7 @@ int stc_query_batch(void *ctx, const stc_query *q, uint32_t nq, stc_hit *hits, u
  * stc_hit records seen from the searched object (source 0). A second call
  * with the targets found is the reference's second hop. */
 int stc_live_search(void *ctx, const uint32_t *pids, uint32_t n, stc_hit *hits, uint64_t cap, uint64_t *n_hits);
+
 
 /* ---- seeded synthetic traces (the reference's generator is unseeded) ----
  * Parameter and state layouts are those of stcdb_amd/csrc/stc_gen.h. */

@@ -1413,11 +1413,13 @@ typedef struct qhit_ctx {
     orc_hit *hits;
     uint64_t cap, n;
     uint64_t *counts;
+    const orc_query *qs;
 } qhit_ctx;
 
 static void q_emit(qhit_ctx *h, uint32_t qi, uint32_t pid, uint32_t target, uint32_t start, uint32_t end,
                    uint32_t source, const double *mbr) {
     h->counts[qi]++;
+    if (h->qs[qi].flags & 1u) return; /* counted only */
     if (h->n < h->cap) {
         orc_hit *x = &h->hits[h->n];
         x->query = qi; x->pid = pid; x->target = target; x->start = start; x->end = end; x->source = source;
@@ -1616,7 +1618,7 @@ done:
 
 uint64_t orc_query_batch(void *vo, const orc_query *qs, uint32_t nq, orc_hit *hits, uint64_t cap, uint64_t *counts) {
     orc *o = (orc *)vo;
-    qhit_ctx h = {hits, cap, 0, counts};
+    qhit_ctx h = {hits, cap, 0, counts, qs};
     memset(counts, 0, sizeof(uint64_t) * nq);
     double m[4];
     /* live meetings: both directions */

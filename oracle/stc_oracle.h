@@ -154,7 +154,8 @@ void orc_ctf_bits(orc_ctf *ctf, const double *map4, uint32_t num_objects, uint32
  * src/tracing/step_merge.cpp:325-379; oversize records unpacked). source: 0
  * live, 1 memtable, 2 + r run r. Hits beyond `cap` are counted, not stored. */
 typedef struct orc_query {
-    uint32_t kind, oid, oid2, t0, t1, pad;
+    uint32_t kind, oid, oid2, t0, t1;
+    uint32_t flags;   /* bit 0: count the query's hits, store none */
     double box[4];
 } orc_query;
 typedef struct orc_hit {

@@ -1854,6 +1854,7 @@ int stc_create(const stc_config *cfg, const stc_box *mbr, void **out) {
     if (!cfg || !mbr || !out) return STC_E_ARG;
     *out = nullptr;
     StcCtx *c = new StcCtx();
+    c->err.reserve(1024); /* a failure message then needs no allocation */
     c->cfg = *cfg;
     c->map = MapBox{mbr->low[0], mbr->low[1], mbr->high[0], mbr->high[1]};
     c->n = cfg->num_objects;

@@ -130,7 +130,7 @@ __device__ inline void q_emit(const QDev &Q, unsigned int *lc, uint32_t qi, uint
                               uint32_t start, uint32_t end, uint32_t source, const double *m) {
     if (Q.lds) atomicAdd(&lc[qi], 1u);
     else atomicAdd(&Q.counts[qi], 1ull);
-    if (!Q.cap) return;
+    if (!Q.cap || (Q.q[qi].flags & STC_QF_COUNT)) return;
     /* one reservation per call for all lanes that emit together: a counter
      * updated by every hit serialises on its address */
     const uint64_t act = __ballot(1);
@@ -748,7 +748,8 @@ int stc_query_run(QueryBufs *B, const QueryState &st, const stc_query *q, uint32
                                   x.box[3] >= qq.box[1]) || x.start_min > qq.t1 || x.end_max < qq.t0)
                                 continue;
                             /* every forward record matches: count without decoding */
-                            if (!Q.cap && x.box[0] >= qq.box[0] && x.box[2] <= qq.box[2] && x.box[1] >= qq.box[1] &&
+                            if ((!Q.cap || (qq.flags & STC_QF_COUNT)) && x.box[0] >= qq.box[0] &&
+                                x.box[2] <= qq.box[2] && x.box[1] >= qq.box[1] &&
                                 x.box[3] <= qq.box[3] && x.start_min >= qq.t0 && x.end_max <= qq.t1) {
                                 planned[qi] += x.n_fwd;
                                 continue;
@@ -775,10 +776,10 @@ int stc_query_run(QueryBufs *B, const QueryState &st, const stc_query *q, uint32
     std::vector<unsigned long long> cnt((uint64_t)nq + 1);
     QCK(hipMemcpyAsync(cnt.data(), B->counters, cnt.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     QCK(hipStreamSynchronize(s));
-    uint64_t total = 0;
+    uint64_t total = 0; /* the hits of the queries that store theirs */
     for (uint32_t i = 0; i < nq; i++) {
         const uint64_t ci = cnt[i + 1] + planned[i];
-        total += ci;
+        if (!(q[i].flags & STC_QF_COUNT)) total += ci;
         if (counts) counts[i] = ci;
     }
     *n_hits = total;

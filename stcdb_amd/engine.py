@@ -33,10 +33,11 @@ def _gen_struct(gp):
 
 # stc_query / stc_hit of include/stcdb.h
 QUERY_DTYPE = np.dtype([("kind", "<u4"), ("oid", "<u4"), ("oid2", "<u4"), ("t0", "<u4"), ("t1", "<u4"),
-                        ("pad", "<u4"), ("box", "<f8", (4,))])
+                        ("flags", "<u4"), ("box", "<f8", (4,))])
 HIT_DTYPE = np.dtype([("query", "<u4"), ("pid", "<u4"), ("target", "<u4"), ("start", "<u4"), ("end", "<u4"),
                       ("source", "<u4"), ("mbr", "<f8", (4,))])
 Q_OID, Q_RANGE, Q_PAIR = 1, 2, 3
+QF_COUNT = 1  # STC_QF_COUNT: count the query's hits, store none
 # stc_meeting_unit: the reference's 32-byte meeting_unit (src/tracing/workbench.h:62-83)
 MEETING_UNIT_DTYPE = np.dtype([("key", "<u8"), ("start", "<u4"), ("end", "<u4"), ("mbr", "<f4", (4,))])
 assert MEETING_UNIT_DTYPE.itemsize == 32
@@ -244,17 +245,18 @@ class Pipeline:
                     key_bytes=key_bytes, bitmaps=bitmaps, sids=sids, o_keys=o_keys, o_boxes=o_boxes,
                     o_bitmap=o_bitmap, sorted_keys=sk, sorted_boxes=sb)
 
-    def query(self, q, cap=None):
+    def query(self, q, cap=None, into=None):
         """answer a batch of queries (QUERY_DTYPE) over the state after the last
         step; returns (hits as HIT_DTYPE sorted by query, source, pid, target,
-        start, end; counts per query)"""
+        start, end; counts per query). Queries flagged QF_COUNT store no hits.
+        `into`: a HIT_DTYPE array of at least cap entries to store them in."""
         q = np.ascontiguousarray(q, dtype=QUERY_DTYPE)
         counts = np.zeros(len(q), dtype=np.uint64)
         n = C.c_uint64()
         if cap is None:
             _check(self.lib.stc_query_batch(self.ctx, _ptr(q), len(q), None, 0, C.byref(n), _ptr(counts)), self.ctx)
             cap = n.value
-        hits = np.zeros(cap, dtype=HIT_DTYPE)
+        hits = into if into is not None and len(into) >= cap else np.zeros(cap, dtype=HIT_DTYPE)
         _check(self.lib.stc_query_batch(self.ctx, _ptr(q), len(q), _ptr(hits), cap, C.byref(n), _ptr(counts)), self.ctx)
         return hits[:min(n.value, cap)], counts
 

@@ -36,7 +36,7 @@ class OrcRunInfo(C.Structure):
 
 
 QUERY_DTYPE = np.dtype([("kind", "<u4"), ("oid", "<u4"), ("oid2", "<u4"), ("t0", "<u4"), ("t1", "<u4"),
-                        ("pad", "<u4"), ("box", "<f8", (4,))])
+                        ("flags", "<u4"), ("box", "<f8", (4,))])
 HIT_DTYPE = np.dtype([("query", "<u4"), ("pid", "<u4"), ("target", "<u4"), ("start", "<u4"), ("end", "<u4"),
                       ("source", "<u4"), ("mbr", "<f8", (4,))])
 

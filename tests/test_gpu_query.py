@@ -88,3 +88,26 @@ def test_count_only_queries_match_oracle():
     assert np.array_equal(gc, oc)
     gpu.close()
     orc.close()
+
+
+def test_count_flag_mixed_batch_matches_oracle():
+    """a batch whose range queries are flagged STC_QF_COUNT (as the serving
+    loop sends them) and whose object and pair queries store every hit"""
+    w = dense()
+    gpu = Pipeline(w["cfg"], w["box"])
+    orc = O.Oracle(w["cfg"], w["box"])
+    for t, xy in HostTrace(w["gen"]).steps(44):
+        gpu.step(xy, t)
+        orc.step(xy, t)
+    q = make_queries(orc, w, np.random.default_rng(9), 150)
+    (lo0, lo1), (hi0, hi1) = w["box"]
+    big = q[:10].copy()
+    big["kind"], big["t0"], big["t1"] = 2, 0, 1000
+    for i in range(10):
+        big[i]["box"] = (lo0, lo1, lo0 + (0.3 + 0.07 * i) * (hi0 - lo0), lo1 + (0.3 + 0.07 * i) * (hi1 - lo1))
+    q = np.concatenate([q, big])
+    q["flags"] = np.where(q["kind"] == 2, 1, 0)
+    n = compare(gpu, orc, q, "count flag")
+    assert n > 0
+    gpu.close()
+    orc.close()

@@ -130,3 +130,19 @@ def test_oracle_queries_match_brute_force():
     assert counts.sum() == len(got)
     # every source contributes
     assert set(np.unique(got["source"])) >= {0, 1, 2}
+
+
+def test_oracle_count_flag_stores_no_hits():
+    """STC_QF_COUNT: the query's count is unchanged and none of its hits is stored"""
+    w = workload("c1", num_objects=3000, kv_restriction=300, min_meet_time=3)
+    w["gen"].update(n_hotspots=3, hotspot_frac=0.7, hotspot_sigma=150.0)
+    orc = O.Oracle(w["cfg"], w["box"])
+    for t, xy in HostTrace(w["gen"]).steps(45):
+        orc.step(xy, t)
+    q = make_queries(orc, w, np.random.default_rng(12), 90)
+    all_hits, all_counts = orc.query(q)
+    q["flags"][::2] = 1
+    hits, counts = orc.query(q)
+    assert np.array_equal(counts, all_counts)
+    keep = all_hits[q["flags"][all_hits["query"]] == 0]
+    assert np.array_equal(hits, keep) and 0 < len(hits) < len(all_hits)
