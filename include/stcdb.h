@@ -329,6 +329,18 @@ int stc_query_batch(void *ctx, const stc_query *q, uint32_t nq, stc_hit *hits, u
  * with the targets found is the reference's second hop. */
 int stc_live_search(void *ctx, const uint32_t *pids, uint32_t n, stc_hit *hits, uint64_t cap, uint64_t *n_hits);
 
+/* stc_live_search without allocation. This entry point has no counterpart in
+ * the reference; it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). Its results
+ * are those of stc_live_search. Its scratch (the sorted search ids, a hit
+ * counter and the staged hits) is one static device array of 14.8 MB per
+ * device, set up by stc_create; on success the call makes no device or host
+ * allocation. Limits: at most 16384 objects per call (STC_E_CAPACITY beyond)
+ * and at most 262144 hits (STC_E_CAPACITY, *n_hits still set). Every call on
+ * one GPU shares that GPU's array: calls that overlap on one GPU, from any
+ * context or thread, race. */
+int stc_live_search_noalloc(void *ctx, const uint32_t *pids, uint32_t n, stc_hit *hits, uint64_t cap,
+                            uint64_t *n_hits);
 
 /* ---- seeded synthetic traces (the reference's generator is unseeded) ----
  * Parameter and state layouts are those of stcdb_amd/csrc/stc_gen.h. */

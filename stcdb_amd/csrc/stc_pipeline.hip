@@ -47,6 +47,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -1531,6 +1532,98 @@ __global__ void k_fill_obox(uint4 *a, uint32_t n) {
 /* host context                                                        */
 /* ------------------------------------------------------------------ */
 
+/* ------------------------------------------------------------------ */
+/* the allocation-free live search: one scratch array per device         */
+/* ------------------------------------------------------------------ */
+/* Layout: [0, 256) hit counter; then NA_MAX_PIDS search keys
+ * (pid << 32 | index in the call's list); then NA_MAX_HITS staged hits. */
+#define NA_MAX_PIDS 16384u
+#define NA_MAX_HITS 262144u
+#define NA_KEYS_AT 256u
+#define NA_HITS_AT (NA_KEYS_AT + NA_MAX_PIDS * 8u)
+#define NA_SCRATCH_BYTES (NA_HITS_AT + NA_MAX_HITS * (uint32_t)sizeof(stc_hit))
+static __device__ __align__(256) uint8_t scratch[NA_SCRATCH_BYTES];
+
+/* scratch's device address on each device, resolved once per device */
+static std::mutex g_na_mu;
+static uint8_t *g_na_ptr[64];
+
+static int noalloc_setup(int device) {
+    if (device < 0 || device >= 64) return STC_E_ARG;
+    std::lock_guard<std::mutex> lk(g_na_mu);
+    if (g_na_ptr[device]) return STC_OK;
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return stc_hip_fail(e, "hipGetSymbolAddress(scratch)", __FILE__, __LINE__);
+    g_na_ptr[device] = (uint8_t *)p;
+    return STC_OK;
+}
+
+/* the search ids, sorted as (pid, index) in LDS by one workgroup; zeroes the
+ * hit counter (the array keeps the previous call's contents) */
+__global__ __launch_bounds__(1024) void k_na_prep(uint8_t *sc, uint32_t n) {
+    __shared__ unsigned long long k[NA_MAX_PIDS];
+    const uint32_t *pids = (const uint32_t *)(sc + NA_HITS_AT); /* staged by the host copy */
+    uint32_t P = 1;
+    while (P < n) P <<= 1;
+    for (uint32_t i = threadIdx.x; i < P; i += blockDim.x)
+        k[i] = i < n ? (((unsigned long long)pids[i] << 32) | i) : ~0ull;
+    if (threadIdx.x == 0) *(unsigned long long *)sc = 0ull;
+    __syncthreads();
+    for (uint32_t size = 2; size <= P; size <<= 1)
+        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+            for (uint32_t i = threadIdx.x; i < P; i += blockDim.x) {
+                const uint32_t j = i ^ stride;
+                if (j > i) {
+                    const bool up = (i & size) == 0;
+                    const unsigned long long a = k[i], b = k[j];
+                    if ((a > b) == up) { k[i] = b; k[j] = a; }
+                }
+            }
+            __syncthreads();
+        }
+    unsigned long long *out = (unsigned long long *)(sc + NA_KEYS_AT);
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) out[i] = k[i];
+}
+
+__device__ inline void na_emit(uint8_t *sc, uint32_t qi, uint32_t pid, uint32_t target, const Bucket &b) {
+    const unsigned long long at = atomicAdd((unsigned long long *)sc, 1ull);
+    if (at >= NA_MAX_HITS) return;
+    stc_hit h;
+    h.query = qi;
+    h.pid = pid;
+    h.target = target;
+    h.start = b.start;
+    h.end = b.end;
+    h.source = 0;
+    for (int i = 0; i < 4; i++) h.mbr[i] = o2f(b.mbr[i]);
+    ((stc_hit *)(sc + NA_HITS_AT))[at] = h;
+}
+
+/* every search id equal to `pid`: the hits seen from pid */
+__device__ inline void na_match(uint8_t *sc, uint32_t n, uint32_t pid, uint32_t target, const Bucket &b) {
+    const unsigned long long *keys = (const unsigned long long *)(sc + NA_KEYS_AT);
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if ((uint32_t)(keys[mid] >> 32) < pid) lo = mid + 1; else hi = mid;
+    }
+    for (uint32_t i = lo; i < n && (uint32_t)(keys[i] >> 32) == pid; i++) na_emit(sc, (uint32_t)keys[i], pid, target, b);
+}
+
+/* live meetings of the last step established by t_est (start <= t_est) */
+__global__ void k_na_live(const Bucket *__restrict__ t, uint64_t nb, uint32_t last_t, uint32_t t_est, uint8_t *sc,
+                          uint32_t n) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nb; i += stride) {
+        const Bucket b = t[i];
+        if (!bucket_live(b, last_t) || b.start > t_est) continue;
+        const uint32_t a = (uint32_t)(b.key >> 32), c = (uint32_t)b.key;
+        na_match(sc, n, a, c, b);
+        na_match(sc, n, c, a, b);
+    }
+}
+
 template <typename T>
 struct DBuf {
     T *p = nullptr;
@@ -1872,6 +1965,11 @@ int stc_create(const stc_config *cfg, const stc_box *mbr, void **out) {
         return STC_E_HIP;
     }
     hipGetDevice(&c->device);
+    if (noalloc_setup(c->device)) {
+        c->err = g_last_fail;
+        stc_destroy(c);
+        return STC_E_HIP;
+    }
     /* border buffers from the map's lowest latitude, host formula (filter.cpp:30-31) */
     double absla = std::fabs(mbr->low[1]);
     if (absla == 90) absla = 89;
@@ -2699,6 +2797,50 @@ int stc_live_search(void *vc, const uint32_t *pids, uint32_t n, stc_hit *hits, u
         q[i].t1 = t - m;
     }
     return stc_query_batch(vc, q.data(), n, hits, cap, n_hits, nullptr);
+}
+
+int stc_live_search_noalloc(void *vc, const uint32_t *pids, uint32_t n, stc_hit *hits, uint64_t cap,
+                            uint64_t *n_hits) {
+    StcCtx *c = (StcCtx *)vc;
+    if (!c || (!pids && n) || !n_hits || (!hits && cap)) return STC_E_ARG;
+    if (n > NA_MAX_PIDS) return fail(c, STC_E_CAPACITY, "stc_live_search_noalloc: more than 16384 objects per call");
+    if (!c->stepped) return fail(c, STC_E_STATE, "no step processed");
+    uint8_t *sc = c->device >= 0 && c->device < 64 ? g_na_ptr[c->device] : nullptr;
+    if (!sc) return fail(c, STC_E_STATE, "scratch not set up on this device");
+    const uint32_t t = c->last_t, m = c->cfg.min_meet_time + 1;
+    *n_hits = 0;
+    if (t < m || n == 0) return STC_OK;
+    hipStream_t s = c->s;
+    CK(hipMemcpyAsync(sc + NA_HITS_AT, pids, (size_t)n * 4, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_na_prep, dim3(1), dim3(1024), 0, s, sc, n);
+    CK(hipGetLastError());
+    if (c->mlen) {
+        const uint32_t g = (uint32_t)std::min<uint64_t>(blocks_for(c->mlen, 256), 8192);
+        hipLaunchKernelGGL(k_na_live, dim3(g), dim3(256), 0, s, (const Bucket *)c->meet[c->cur].p, c->mlen, t, t - m,
+                           sc, n);
+        CK(hipGetLastError());
+    }
+    unsigned long long total = 0;
+    CK(hipMemcpyAsync(&total, sc, 8, hipMemcpyDeviceToHost, s));
+    CK(hipStreamSynchronize(s));
+    if (total > NA_MAX_HITS) {
+        *n_hits = total;
+        return fail(c, STC_E_CAPACITY, "stc_live_search_noalloc: more than 262144 hits");
+    }
+    const uint64_t stored = std::min<uint64_t>(total, cap);
+    if (stored) {
+        CK(hipMemcpyAsync(hits, sc + NA_HITS_AT, stored * sizeof(stc_hit), hipMemcpyDeviceToHost, s));
+        CK(hipStreamSynchronize(s));
+        std::sort(hits, hits + stored, [](const stc_hit &a, const stc_hit &b) {
+            if (a.query != b.query) return a.query < b.query;
+            if (a.pid != b.pid) return a.pid < b.pid;
+            if (a.target != b.target) return a.target < b.target;
+            if (a.start != b.start) return a.start < b.start;
+            return a.end < b.end;
+        });
+    }
+    *n_hits = total;
+    return STC_OK;
 }
 
 /* x86-64 images of the reference's CTF and CTB classes (step_merge.h:64-131) */
