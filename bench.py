@@ -146,7 +146,8 @@ def measured_traffic(kernel, workload, n):
 
 def make_query_stream(rng, n_objects, box, t_now, count):
     """C5's query mix: 50 % object history, 40 % square range, 10 % pair
-    lookup; time windows log-uniform in [10 s, 1 h] ending now; range sides
+    lookup; time windows log-uniform in [10 s, 1 h] ending at t_now (a scalar or
+    one step per query); range sides
     log-uniform in [100 m, 5 km] centred uniformly on the map"""
     from stcdb_amd.engine import QF_COUNT, QUERY_DTYPE, Q_OID, Q_PAIR, Q_RANGE
     from stcdb_amd.workloads import DPM_LAT, dpm_lon
@@ -170,23 +171,39 @@ def make_query_stream(rng, n_objects, box, t_now, count):
 
 
 def answer(pipe, q, buf):
-    """a query batch with every hit of its object and pair queries in host
-    memory (buffer grown when short); range queries count"""
-    from stcdb_amd.engine import HIT_DTYPE
-    hits, counts = pipe.query(q, cap=len(buf[0]), into=buf[0])
+    """a query batch over the last published state: every hit of its object and
+    pair queries in host memory (the buffer grows when short), range queries
+    counted; a batch that admission by free device memory refuses is split"""
+    import stcdb_amd._lib as L
+    try:
+        hits, counts, st = pipe.query_at(q, buf[0])
+    except L.StcError as e:
+        if e.code != L.STC_E_CAPACITY or len(q) == 1:
+            raise
+        a = answer(pipe, q[:len(q) // 2], buf)
+        b = answer(pipe, q[len(q) // 2:], buf)
+        return a[0] + b[0], a[1] + b[1], min(a[2], b[2])
     stored = int(counts[q["flags"] == 0].sum())
     if stored > len(buf[0]):
-        buf[0] = np.zeros(stored + stored // 2, dtype=HIT_DTYPE)
-        hits, counts = pipe.query(q, cap=len(buf[0]), into=buf[0])
-    return stored, int(counts.sum())
+        buf[0] = np.zeros(stored + stored // 2, dtype=HIT_DTYPE_BENCH())
+        hits, counts, st = pipe.query_at(q, buf[0])
+    return stored, int(counts.sum()), st
+
+
+def HIT_DTYPE_BENCH():
+    from stcdb_amd.engine import HIT_DTYPE
+    return HIT_DTYPE
 
 
 def serve(args, objects=0, schema="dynamic"):
-    """ingest at 1 Hz plus a Poisson query stream, answered in batches between
-    steps; latency = answer time - arrival time on the wall clock; every hit
-    of every query is materialised in host memory"""
+    """Continuous batching: an ingest thread steps the 5M-object walk at 1 Hz
+    while this thread answers a Poisson stream of queries; every query that
+    has arrived when the previous batch returns joins the next batch, which
+    runs on the library's query stream over the state of the last completed
+    step, overlapping a step in flight. Latency = answer time - arrival time on
+    the wall clock."""
+    import threading
     from stcdb_amd import DeviceTrace, Pipeline, workload
-    from stcdb_amd.engine import HIT_DTYPE
     over = {"num_objects": objects} if objects else {}
     w = workload("c5", dynamic_schema=1 if schema == "dynamic" else 0, **over)
     cfg, gen = w["cfg"], w["gen"]
@@ -194,41 +211,57 @@ def serve(args, objects=0, schema="dynamic"):
     pipe = Pipeline(cfg, w["box"])
     trace = DeviceTrace(gen)
     rng = np.random.default_rng(5)
-    t = 0
     for t in range(args.warmup):
         r = pipe.step_device(trace.next(t), t)
         print(f"[serve] warmup step {t}: {r['ms_total']:.1f} ms, {r['pairs']} pairs, runs {pipe.num_runs()}",
               file=sys.stderr, flush=True)
-    lat, batches, step_ms, hits_total, counted_total = [], [], [], 0, 0
-    buf = [np.zeros(1 << 21, dtype=HIT_DTYPE)]
-    for s in range(args.serve_seconds):
-        t = args.warmup + s
-        t_sec = time.perf_counter()
-        # arrivals of this second (Poisson process), as offsets from the start of the second
-        k = rng.poisson(args.qps)
-        arrivals = np.sort(rng.uniform(0.0, 1.0, k))
-        queries = make_query_stream(rng, n, w["box"], t, k)
-        r = pipe.step_device(trace.next(t), t)
-        step_ms.append(r["ms_total"])
-        done = 0
-        while done < k:
-            now = time.perf_counter() - t_sec
-            ready = int(np.searchsorted(arrivals, now, side="right"))
-            if ready <= done:
-                time.sleep(max(arrivals[done] - now, 0.0))
-                continue
-            stored, counted = answer(pipe, queries[done:ready], buf)
-            fin = time.perf_counter() - t_sec
-            lat.extend((fin - arrivals[done:ready]) * 1000.0)
-            batches.append(ready - done)
-            hits_total += stored
-            counted_total += counted
-            done = ready
-        rest = 1.0 - (time.perf_counter() - t_sec)
-        if rest > 0:
-            time.sleep(rest)
-        print(f"[serve] second {s}: step {r['ms_total']:.1f} ms, {k} queries in {len(batches)} batches so far, "
-              f"runs {pipe.num_runs()}", file=sys.stderr, flush=True)
+    T = args.serve_seconds
+    k = int(rng.poisson(args.qps * T))
+    arrivals = np.sort(rng.uniform(0.0, T, k))
+    # each query asks about the window ending at the step current when it arrives
+    queries = make_query_stream(rng, n, w["box"], args.warmup + np.floor(arrivals).astype(np.int64), k)
+    step_spans, step_ms, errors = [], [], []
+    t_start = time.perf_counter()
+
+    def ingest():
+        try:
+            for s in range(T):
+                t = args.warmup + s
+                wait = s - (time.perf_counter() - t_start)
+                if wait > 0:
+                    time.sleep(wait)
+                a = time.perf_counter() - t_start
+                r = pipe.step_device(trace.next(t), t)
+                step_spans.append((a, time.perf_counter() - t_start))
+                step_ms.append(r["ms_total"])
+                print(f"[serve] second {s}: step {r['ms_total']:.1f} ms, runs {pipe.num_runs()}", file=sys.stderr,
+                      flush=True)
+        except Exception as e:
+            errors.append(e)
+
+    th = threading.Thread(target=ingest)
+    th.start()
+    lat, batches, spans, hits_total, counted_total, done = [], [], [], 0, 0, 0
+    buf = [np.zeros(1 << 21, dtype=HIT_DTYPE_BENCH())]
+    while done < k:
+        now = time.perf_counter() - t_start
+        ready = int(np.searchsorted(arrivals, now, side="right"))
+        if ready <= done:
+            time.sleep(min(max(arrivals[done] - now, 0.0), 0.002))
+            continue
+        a = time.perf_counter() - t_start
+        stored, counted, _ = answer(pipe, queries[done:ready], buf)
+        fin = time.perf_counter() - t_start
+        spans.append((a, fin))
+        lat.extend((fin - arrivals[done:ready]) * 1000.0)
+        batches.append(ready - done)
+        hits_total += stored
+        counted_total += counted
+        done = ready
+    th.join()
+    if errors:
+        raise errors[0]
+    overlapping = sum(any(x < d and c < y for c, d in step_spans) for x, y in spans)
     lat = np.array(lat)
     out = {
         "p50_ms": round(float(np.percentile(lat, 50)), 3),
@@ -238,10 +271,11 @@ def serve(args, objects=0, schema="dynamic"):
         "hits_materialised": hits_total,
         "hits_counted": counted_total,
         "batches": len(batches),
+        "batches_overlapping_a_step": overlapping,
         "mean_batch": round(float(np.mean(batches)), 1),
         "ingest_ms_per_step": round(float(np.mean(step_ms)), 3),
         "objects": n,
-        "seconds": args.serve_seconds,
+        "seconds": T,
         "warmup_steps": args.warmup,
         "runs_in_hbm": pipe.num_runs(),
         "schema": schema,

@@ -320,6 +320,21 @@ typedef struct stc_hit {
 
 int stc_query_batch(void *ctx, const stc_query *q, uint32_t nq, stc_hit *hits, uint64_t cap, uint64_t *n_hits,
               uint64_t *counts);
+/* Serving. A query batch answers over the state the last completed step
+ * published (its meetings, memtable and runs) and runs on the context's own
+ * query stream, so batches issued from another host thread overlap the next
+ * step on the device; that step waits only where it overwrites what an
+ * earlier batch still reads (the other meeting array; the memtable from its
+ * start after a flush). Batches are admitted by free device memory: one whose
+ * query list and hit buffer would leave less than 1 GiB free returns
+ * STC_E_CAPACITY (split it and retry). stc_query_batch_at also reports the
+ * timestep of the state it answered. Only stc_query_batch, stc_query_batch_at
+ * and stc_live_search may run concurrently with stc_step on one context;
+ * stc_drop_runs and stc_compact_runs wait for batches in flight. The
+ * reference answers its live search from the command thread between steps
+ * (process.cpp:350-371, 443-495). */
+int stc_query_batch_at(void *ctx, const stc_query *q, uint32_t nq, stc_hit *hits, uint64_t cap, uint64_t *n_hits,
+                       uint64_t *counts, uint32_t *state_step);
 
 /* Live contact search (the reference's search_single / search_multi inside
  * cuda_identify_meetings, src/cuda/geometry.cu:605-662, driven from
@@ -335,7 +350,8 @@ int stc_live_search(void *ctx, const uint32_t *pids, uint32_t n, stc_hit *hits, 
  * are those of stc_live_search. Its scratch (the sorted search ids, a hit
  * counter and the staged hits) is one static device array of 14.8 MB per
  * device, set up by stc_create; on success the call makes no device or host
- * allocation. Limits: at most 16384 objects per call (STC_E_CAPACITY beyond)
+ * allocation. It reads the context's state directly: call it from the thread
+ * that steps, not while a step runs. Limits: at most 16384 objects per call (STC_E_CAPACITY beyond)
  * and at most 262144 hits (STC_E_CAPACITY, *n_hits still set). Every call on
  * one GPU shares that GPU's array: calls that overlap on one GPU, from any
  * context or thread, race. */

@@ -209,6 +209,7 @@ SIGNATURES = {
     "stc_live_search_noalloc": (C.c_int, [P, P, C.c_uint32, P, C.c_uint64, U64P]),
     "stc_dump_run": (C.c_int, [P, C.c_uint32, C.c_char_p, C.c_char_p, C.c_uint32]),
     "stc_query_batch": (C.c_int, [P, P, C.c_uint32, P, C.c_uint64, U64P, P]),
+    "stc_query_batch_at": (C.c_int, [P, P, C.c_uint32, P, C.c_uint64, U64P, P, C.POINTER(C.c_uint32)]),
     "stc_gen_init_host": (C.c_int, [C.POINTER(StcGenParams), P, C.c_uint32, C.c_uint32]),
     "stc_gen_step_host": (C.c_int, [C.POINTER(StcGenParams), P, C.c_uint32, C.c_uint32]),
     "stc_gen_pos_host": (C.c_int, [C.POINTER(StcGenParams), P, C.c_uint32, P]),

@@ -47,6 +47,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <condition_variable>
 #include <mutex>
 #include <cstdio>
 #include <cstring>
@@ -56,6 +57,7 @@
 #define ONE_DIM 16384u
 #define NLEVELS 15
 #define STACK_DEPTH 96
+#define STC_QUERY_RESERVE (1ull << 30) /* device memory a query batch leaves free */
 
 static thread_local std::string g_last_fail;
 
@@ -1781,7 +1783,69 @@ struct StcCtx {
     hipEvent_t ev[14];
     bool dyn = false;                    /* dynamic schema: the QTree persists across steps */
     SchemaState sch;
+    /* Serving: query batches run on their own stream against the state the
+     * last completed step published, while the next step runs. A batch holds
+     * a lease on the meeting array and the memtable epoch it reads; a step
+     * waits for the leases on what it overwrites (the other meeting array,
+     * and the memtable from index 0 once a flush emptied it). */
+    hipStream_t qs = nullptr;
+    std::mutex q_mu;                     /* one query batch at a time: its buffers and stream */
+    std::mutex pub_mu;                   /* pub_*, readers_* */
+    std::condition_variable pub_cv;
+    QueryState pub;                      /* the state queries read */
+    bool pub_valid = false;
+    int pub_meet = 0;                    /* meeting array of pub */
+    uint32_t epoch = 0;                  /* memtable epoch: +1 at every flush */
+    uint32_t pub_epoch = 0;
+    uint32_t readers_meet[2] = {0, 0};
+    uint32_t readers_epoch[2] = {0, 0};
+    uint32_t readers_total = 0;
 };
+
+/* publish the state after a step (or a change of the runs) to queries */
+static void publish(StcCtx *c) {
+    QueryState st;
+    st.table = c->meet[c->cur].p;
+    st.nb = c->mlen;
+    st.last_t = c->last_t;
+    st.stepped = c->stepped ? 1 : 0;
+    st.rec_keys = c->rec_keys;
+    st.rec_boxes = c->rec_boxes;
+    st.n_rec = c->kv_count;
+    st.end_time_min = c->end_time_min;
+    st.num_objects = c->n;
+    st.map[0] = c->map.lo0;
+    st.map[1] = c->map.lo1;
+    st.map[2] = c->map.hi0;
+    st.map[3] = c->map.hi1;
+    for (uint32_t r = 0; r < c->runs.size(); r++) {
+        if (!c->runs[r].dev) continue; /* kept on the host only */
+        QueryRunDev d = c->runs[r].qdev;
+        d.source = 2 + r;
+        st.runs.push_back(d);
+        st.run_ctfs.push_back(c->runs[r].ctfs.data());
+        st.run_stats.push_back(c->runs[r].stats.data());
+    }
+    std::lock_guard<std::mutex> lk(c->pub_mu);
+    c->pub = std::move(st);
+    c->pub_meet = c->cur;
+    c->pub_epoch = c->epoch;
+    c->pub_valid = c->stepped;
+}
+
+/* wait until no query batch reads meeting array mi (and, with old_memtable,
+ * the memtable of an earlier epoch) */
+static void wait_readers(StcCtx *c, int mi, bool old_memtable) {
+    std::unique_lock<std::mutex> lk(c->pub_mu);
+    c->pub_cv.wait(lk, [&] {
+        return c->readers_meet[mi] == 0 && (!old_memtable || c->readers_epoch[(c->epoch + 1) & 1] == 0);
+    });
+}
+/* wait until no query batch is in flight */
+static void wait_all_readers(StcCtx *c) {
+    std::unique_lock<std::mutex> lk(c->pub_mu);
+    c->pub_cv.wait(lk, [&] { return c->readers_total == 0; });
+}
 
 /* the QTree the step's kernels descend */
 static inline Node *tree_nodes(StcCtx *c) { return c->dyn ? c->sch.nodes : c->nodes.p; }
@@ -1812,8 +1876,10 @@ static int sync_counters(StcCtx *c) {
     return 0;
 }
 
+static void wait_all_readers(StcCtx *c);
 static int ensure_records(StcCtx *c, uint64_t need) {
     if (need <= c->rec_cap) return 0;
+    wait_all_readers(c); /* the old buffers are freed below */
     uint64_t nc = std::max<uint64_t>(need + need / 2, 1 << 16);
     uint64_t *k = nullptr;
     float4 *b = nullptr;
@@ -1984,6 +2050,7 @@ int stc_create(const stc_config *cfg, const stc_box *mbr, void **out) {
         return STC_E_HIP;
     };
     if (hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking) != hipSuccess) return bad("stream");
+    if (hipStreamCreateWithFlags(&c->qs, hipStreamNonBlocking) != hipSuccess) return bad("query stream");
     for (int i = 0; i < 14; i++) hipEventCreate(&c->ev[i]);
     uint32_t n = c->n;
     size_t node_cap = 1 + 4ull * NLEVELS * (n / (c->cap + 1) + 1);
@@ -2034,6 +2101,7 @@ int stc_create(const stc_config *cfg, const stc_box *mbr, void **out) {
 int stc_destroy(void *vc) {
     StcCtx *c = (StcCtx *)vc;
     if (!c) return STC_OK;
+    wait_all_readers(c);
     if (c->s) hipStreamSynchronize(c->s);
     c->xy_in.release(); c->codes.release(); c->idx.release(); c->codes_alt.release(); c->idx_alt.release();
     c->temp.release(); c->dlon.release(); c->idlon.release(); c->lxy.release(); c->ld.release(); c->nodes.release();
@@ -2073,6 +2141,7 @@ int stc_destroy(void *vc) {
     for (int i = 0; i < 14; i++) if (c->ev[i]) hipEventDestroy(c->ev[i]);
     stc_schema_free(&c->sch);
     if (c->s) hipStreamDestroy(c->s);
+    if (c->qs) hipStreamDestroy(c->qs);
     delete c;
     return STC_OK;
 }
@@ -2298,6 +2367,7 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
         CK(hipMemcpy(c->acc.p, accepted.data(), accepted.size() * sizeof(unsigned long long), hipMemcpyHostToDevice));
     const int nx = c->cur ^ 1;
     CKRC(stc_exclusive_scan_u32(c->pcount.p, c->moff[nx].p, n, c->moff[nx].p + n, c->temp.p, s));
+    wait_readers(c, nx, false); /* queries of an earlier state may still read meet[nx] */
     if (c->ptmp.ensure(total + 1) || c->pnew.ensure(total + 1) || c->meet[nx].ensure(total + 1))
         return fail(c, STC_E_CAPACITY, "pair and meeting arrays");
     CK(hipMemsetAsync(&cnt->table_fail, 0, sizeof(uint32_t), s));
@@ -2378,6 +2448,8 @@ static int step_tail(StcCtx *c, uint32_t t, StepGeom &g, stc_step_out *out) {
                        c->tbounds.p);
     CK(hipGetLastError());
     const uint32_t mgrid = std::max<uint32_t>(std::min<uint32_t>((n_tiles + 3) / 4, 65536), 1);
+    /* the merge writes meet[nx] and, after a flush emptied the memtable, records from index 0 */
+    wait_readers(c, nx, kv_before == 0);
     for (int attempt = 0;; attempt++) {
         /* the merge rewrites the same buckets, so a re-run with more record room is exact */
         *c->hinit = before;
@@ -2461,6 +2533,7 @@ static int step_tail(StcCtx *c, uint32_t t, StepGeom &g, stc_step_out *out) {
         c->kv_count = 0;
         c->oversize_oid_count = 0;
         c->end_time_min = t; /* process.cpp:271 with end_time_max = cur_time */
+        c->epoch++;
         flushed = 1;
     }
     /* 11: the dynamic schema's update for the next step (geometry.cu:2159-2182) */
@@ -2477,6 +2550,7 @@ static int step_tail(StcCtx *c, uint32_t t, StepGeom &g, stc_step_out *out) {
 
     c->stepped = true;
     c->last_t = t;
+    publish(c);
     if (out) {
         memset(out, 0, sizeof(*out));
         out->nodes = g.nodes;
@@ -2552,7 +2626,7 @@ int stc_step_meetings(void *vc, const stc_meeting_unit *units, uint64_t nu64, ui
     if (c->units_in.ensure(nu + 1ull) || c->ukeys.ensure(nu + 1ull) || c->ukeys_alt.ensure(nu + 1ull) ||
         c->uidx.ensure(nu + 1ull) || c->uidx_alt.ensure(nu + 1ull) ||
         c->utemp.ensure(stc_radix_temp_words(std::max<uint32_t>(nu + 1, c->n)) + 64) || c->pnew.ensure(nu + 1ull) ||
-        c->meet[c->cur ^ 1].ensure(nu + 1ull))
+        (wait_readers(c, c->cur ^ 1, false), c->meet[c->cur ^ 1].ensure(nu + 1ull)))
         return fail(c, STC_E_CAPACITY, "meeting unit buffers");
     if (nu) {
         CK(hipMemcpyAsync(c->units_in.p, units, (size_t)nu * sizeof(stc_meeting_unit), hipMemcpyHostToDevice, s));
@@ -2952,6 +3026,7 @@ int stc_ctf_widths(stc_ctf_meta *ctf, const stc_box *map_mbr, uint32_t num_objec
 int stc_drop_runs(void *vc) {
     StcCtx *c = (StcCtx *)vc;
     if (!c) return STC_E_ARG;
+    wait_all_readers(c); /* the runs' device blocks are freed below */
     for (auto &r : c->runs) {
         release_block(c, r.blk, true);
         if (r.dev) {
@@ -2961,6 +3036,7 @@ int stc_drop_runs(void *vc) {
         }
     }
     c->runs.clear();
+    publish(c);
     return STC_OK;
 }
 
@@ -2973,6 +3049,7 @@ int stc_drop_runs(void *vc) {
 int stc_compact_runs(void *vc, uint32_t first, uint32_t count) {
     StcCtx *c = (StcCtx *)vc;
     if (!c || count == 0) return STC_E_ARG;
+    wait_all_readers(c); /* the merged runs' blocks are freed and the memtable buffers reused */
     if ((uint64_t)first + count > c->runs.size()) return fail(c, STC_E_ARG, "run range out of bounds");
     if (count == 1) return STC_OK;
     const uint32_t n = c->n;
@@ -3079,42 +3156,54 @@ int stc_compact_runs(void *vc, uint32_t first, uint32_t count) {
     }
     c->runs.erase(c->runs.begin() + first, c->runs.begin() + first + count);
     c->runs.insert(c->runs.begin() + first, std::move(run));
+    publish(c);
+    return STC_OK;
+}
+
+int stc_query_batch_at(void *vc, const stc_query *q, uint32_t nq, stc_hit *hits, uint64_t cap, uint64_t *n_hits,
+                       uint64_t *counts, uint32_t *state_step) {
+    StcCtx *c = (StcCtx *)vc;
+    if (!c || (!q && nq) || !n_hits) return STC_E_ARG;
+    for (uint32_t i = 0; i < nq; i++)
+        if (q[i].kind < STC_Q_OID || q[i].kind > STC_Q_PAIR) return fail(c, STC_E_ARG, "unknown query kind");
+    std::lock_guard<std::mutex> ql(c->q_mu);
+    /* admission by free device memory: what the batch's query list, counters and
+     * hit buffer add to the query buffers */
+    const uint64_t need = stc_query_bufs_growth(c->qb, nq, hits ? cap : 0);
+    size_t free_b = 0, total_b = 0;
+    if (need && hipMemGetInfo(&free_b, &total_b) == hipSuccess && need + STC_QUERY_RESERVE > free_b)
+        return fail(c, STC_E_CAPACITY, "admission: the batch needs " + std::to_string(need) + " bytes of device memory, " +
+                                           std::to_string(free_b) + " free");
+    QueryState st;
+    int mi;
+    uint32_t ep;
+    {
+        std::lock_guard<std::mutex> lk(c->pub_mu);
+        if (!c->pub_valid) return fail(c, STC_E_STATE, "no step processed");
+        st = c->pub;
+        mi = c->pub_meet;
+        ep = c->pub_epoch;
+        c->readers_meet[mi]++;
+        c->readers_epoch[ep & 1]++;
+        c->readers_total++;
+    }
+    if (!c->qb) c->qb = stc_query_bufs_new();
+    int rc = stc_query_run(c->qb, st, q, nq, hits, cap, n_hits, counts, c->qs);
+    {
+        std::lock_guard<std::mutex> lk(c->pub_mu);
+        c->readers_meet[mi]--;
+        c->readers_epoch[ep & 1]--;
+        c->readers_total--;
+    }
+    c->pub_cv.notify_all();
+    if (rc) return fail(c, rc, std::string("query: ") + g_last_fail);
+    if (state_step) *state_step = st.last_t;
     return STC_OK;
 }
 
 int stc_query_batch(void *vc, const stc_query *q, uint32_t nq, stc_hit *hits, uint64_t cap, uint64_t *n_hits,
-              uint64_t *counts) {
-    StcCtx *c = (StcCtx *)vc;
-    if (!c || (!q && nq) || !n_hits) return STC_E_ARG;
-    if (!c->stepped) return fail(c, STC_E_STATE, "no step processed");
-    for (uint32_t i = 0; i < nq; i++)
-        if (q[i].kind < STC_Q_OID || q[i].kind > STC_Q_PAIR) return fail(c, STC_E_ARG, "unknown query kind");
-    if (!c->qb) c->qb = stc_query_bufs_new();
-    QueryState st;
-    st.table = c->meet[c->cur].p;
-    st.nb = c->mlen;
-    st.last_t = c->last_t;
-    st.stepped = 1;
-    st.rec_keys = c->rec_keys;
-    st.rec_boxes = c->rec_boxes;
-    st.n_rec = c->kv_count;
-    st.end_time_min = c->end_time_min;
-    st.num_objects = c->n;
-    st.map[0] = c->map.lo0;
-    st.map[1] = c->map.lo1;
-    st.map[2] = c->map.hi0;
-    st.map[3] = c->map.hi1;
-    for (uint32_t r = 0; r < c->runs.size(); r++) {
-        if (!c->runs[r].dev) continue; /* kept on the host only */
-        QueryRunDev d = c->runs[r].qdev;
-        d.source = 2 + r;
-        st.runs.push_back(d);
-        st.run_ctfs.push_back(c->runs[r].ctfs.data());
-        st.run_stats.push_back(c->runs[r].stats.data());
-    }
-    int rc = stc_query_run(c->qb, st, q, nq, hits, cap, n_hits, counts, c->s);
-    if (rc) return fail(c, rc, std::string("query: ") + g_last_fail);
-    return STC_OK;
+                    uint64_t *counts) {
+    return stc_query_batch_at(vc, q, nq, hits, cap, n_hits, counts, nullptr);
 }
 
 } /* extern "C" */

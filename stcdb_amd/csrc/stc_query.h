@@ -53,6 +53,8 @@ struct QueryState {
 struct QueryBufs;                    /* device scratch of the query path, grown on demand */
 QueryBufs *stc_query_bufs_new();
 void stc_query_bufs_free(QueryBufs *b);
+/* device bytes a batch of nq queries storing up to cap hits would add to b */
+uint64_t stc_query_bufs_growth(const QueryBufs *b, uint32_t nq, uint64_t cap);
 int stc_query_run(QueryBufs *b, const QueryState &st, const stc_query *q, uint32_t nq, stc_hit *hits, uint64_t cap,
                   uint64_t *n_hits, uint64_t *counts, hipStream_t s);
 /* Compaction input: every record of run R (the kv_count records of its CTFs,

@@ -69,6 +69,15 @@ struct QueryBufs {
 
 QueryBufs *stc_query_bufs_new() { return new QueryBufs(); }
 
+uint64_t stc_query_bufs_growth(const QueryBufs *b, uint32_t nq, uint64_t cap) {
+    uint64_t need = 0;
+    const uint64_t q_cap = b ? b->q_cap : 0, c_cap = b ? b->counters_cap : 0, h_cap = b ? b->hits_cap : 0;
+    if (nq > q_cap) need += (uint64_t)nq * sizeof(stc_query) * 3 / 2;
+    if (nq + 1ull > c_cap) need += (nq + 1ull) * 8 * 3 / 2;
+    if (cap > h_cap) need += cap * sizeof(stc_hit) * 3 / 2;
+    return need;
+}
+
 void stc_query_bufs_free(QueryBufs *b) {
     if (!b) return;
     void *ptrs[] = {b->q, b->oh_key, b->oh_head, b->q_next, b->oq, b->g_start, b->g_list, b->runs, b->cand,

@@ -260,6 +260,18 @@ class Pipeline:
         _check(self.lib.stc_query_batch(self.ctx, _ptr(q), len(q), _ptr(hits), cap, C.byref(n), _ptr(counts)), self.ctx)
         return hits[:min(n.value, cap)], counts
 
+    def query_at(self, q, into):
+        """a query batch over the last published state (safe while another
+        thread steps); returns (hits, counts, the state's timestep). `into` is
+        a HIT_DTYPE buffer; hits beyond it are counted, not stored"""
+        q = np.ascontiguousarray(q, dtype=QUERY_DTYPE)
+        counts = np.zeros(len(q), dtype=np.uint64)
+        n = C.c_uint64()
+        st = C.c_uint32()
+        _check(self.lib.stc_query_batch_at(self.ctx, _ptr(q), len(q), _ptr(into), len(into), C.byref(n),
+                                           _ptr(counts), C.byref(st)), self.ctx)
+        return into[:min(n.value, len(into))], counts, st.value
+
     def live_search(self, pids):
         """established live contacts of the given objects (stc_live_search);
         hits as HIT_DTYPE with `query` the index into pids"""

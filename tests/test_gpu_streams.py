@@ -60,3 +60,68 @@ def test_batched_streams_parity():
     print(f"[streams] ok: pairs {pairs} records {records}", flush=True)
     assert sum(p > 0 for p in pairs) >= len(runs) // 2
     assert sum(records) > 0
+
+
+def run_stream(w, steps, trace=None):
+    """one stream alone: its context, its steps; returns per-step pairs and the final state"""
+    cfg = dict(w["cfg"], keep_pairs=1)
+    gpu = Pipeline(cfg, w["box"])
+    out = []
+    for t, xy in (trace or HostTrace(w["gen"]).steps(steps)):
+        gpu.step(xy, t)
+        out.append(gpu.pairs())
+    k, s, e, m = gpu.active()
+    rk, rb = sorted_records(*gpu.records())
+    gpu.close()
+    return out, (k, s, e, m.view(np.uint32)), (rk, rb.view(np.uint32))
+
+
+def test_c4_full_size_streams():
+    """c4 at its size: 64 streams of ~6M objects in total (log-normal, median
+    50k, 10-50 km boxes, reach 5-25 m), all contexts alive and stepped in turn
+    for a few steps; each stream equals its own run alone, and four sampled
+    streams equal their oracle runs"""
+    steps = 6
+    streams = batched_streams()  # the c4 parameters
+    total = sum(w["cfg"]["num_objects"] for w in streams)
+    print(f"[c4] {len(streams)} streams, {total} objects", flush=True)
+    assert len(streams) == 64 and 3_000_000 < total < 12_000_000
+    frames = [[xy.copy() for _, xy in HostTrace(w["gen"]).steps(steps)] for w in streams]
+    ctxs = [Pipeline(dict(w["cfg"], keep_pairs=1), w["box"]) for w in streams]
+    mixed_pairs = [[] for _ in streams]
+    for t in range(steps):
+        for i, gpu in enumerate(ctxs):
+            gpu.step(frames[i][t], t)
+            mixed_pairs[i].append(gpu.pairs())
+    mixed_state = []
+    for gpu in ctxs:
+        k, s, e, m = gpu.active()
+        rk, rb = sorted_records(*gpu.records())
+        mixed_state.append(((k, s, e, m.view(np.uint32)), (rk, rb.view(np.uint32))))
+        gpu.close()
+    n_pairs = 0
+    for i, w in enumerate(streams):
+        alone, act, rec = run_stream(w, steps, trace=enumerate(frames[i]))
+        for t in range(steps):
+            assert np.array_equal(alone[t], mixed_pairs[i][t]), (i, t)
+            n_pairs += len(alone[t])
+        for a, b in zip(act, mixed_state[i][0]):
+            assert np.array_equal(a, b), i
+        for a, b in zip(rec, mixed_state[i][1]):
+            assert np.array_equal(a, b), i
+    print(f"[c4] every stream alone equals the mix: {n_pairs} pairs", flush=True)
+    assert n_pairs > 0
+    # oracle parity on a sample: the smallest, the largest and two in between
+    order = np.argsort([w["cfg"]["num_objects"] for w in streams])
+    for i in [int(order[0]), int(order[21]), int(order[42]), int(order[-1])]:
+        w = streams[i]
+        orc = O.Oracle(dict(w["cfg"], keep_pairs=1), w["box"])
+        for t in range(steps):
+            orc.step(frames[i][t], t)
+            assert np.array_equal(orc.pairs(), mixed_pairs[i][t]), (i, t)
+        ok, os_, oe, om = orc.active()
+        k, s, e, m = mixed_state[i][0]
+        assert np.array_equal(ok, k) and np.array_equal(os_, s) and np.array_equal(oe, e), i
+        assert np.array_equal(om.view(np.uint32), m), i
+        print(f"[c4] stream {i} ({w['cfg']['num_objects']} objects) equals the oracle", flush=True)
+        orc.close()
