@@ -128,7 +128,7 @@ def measured_traffic(kernel, workload, n):
     """HBM bytes per launch of a kernel, from the committed rocprofv3 counter
     summary for this workload (FETCH_SIZE and WRITE_SIZE, calibrated with a copy
     of known size); None when no summary covers it."""
-    path = os.path.join(PROFILES, f"pair_merge_{workload}_traffic.md")
+    path = os.path.join(PROFILES, f"pipeline_{workload}_traffic.md")
     if not os.path.exists(path):
         return None
     vals = {}
