@@ -53,6 +53,7 @@
 #include <cstring>
 #include <string>
 #include <vector>
+#include <type_traits>
 
 #define ONE_DIM 16384u
 #define NLEVELS 15
@@ -1791,6 +1792,7 @@ struct StcCtx {
     hipStream_t qs = nullptr;
     std::mutex q_mu;                     /* one query batch at a time: its buffers and stream */
     std::mutex pub_mu;                   /* pub_*, readers_* */
+    std::mutex err_mu;                   /* err: a query thread and the stepping thread can both fail */
     std::condition_variable pub_cv;
     QueryState pub;                      /* the state queries read */
     bool pub_valid = false;
@@ -1851,6 +1853,7 @@ static void wait_all_readers(StcCtx *c) {
 static inline Node *tree_nodes(StcCtx *c) { return c->dyn ? c->sch.nodes : c->nodes.p; }
 
 static int fail(StcCtx *c, int code, const std::string &msg) {
+    std::lock_guard<std::mutex> lk(c->err_mu);
     c->err = msg;
     return code;
 }
@@ -2149,7 +2152,11 @@ int stc_destroy(void *vc) {
 const char *stc_last_error(void *vc) {
     StcCtx *c = (StcCtx *)vc;
     if (!c) return g_last_fail.c_str();
-    return c->err.c_str();
+    /* a copy per calling thread: the context's message can change under another thread's failure */
+    static thread_local std::string last;
+    std::lock_guard<std::mutex> lk(c->err_mu);
+    last = c->err;
+    return last.c_str();
 }
 
 /* host re-evaluation of ambiguous pair decisions with the C library's sin,
@@ -2216,7 +2223,8 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
     uint32_t n_nodes, n_leaves;
     if (c->dyn) {
         if (!c->sch.present) { /* the first step's points build it (process.cpp:401-402) */
-            int rc = stc_schema_build(&c->sch, xy, n, c->map, c->cap, c->cfg.reach_distance, c->temp.p, c->temp.cap, s);
+            int rc = stc_schema_build(&c->sch, xy, c->idx.p, n, c->map, c->cap, c->cfg.reach_distance, c->temp.p,
+                                      c->temp.cap, s);
             if (rc) return fail(c, rc, rc == STC_E_ARG ? "root does not split: too few objects for grid_capacity"
                                                        : std::string("schema build: ") + g_last_fail);
         }
@@ -2772,7 +2780,13 @@ int stc_build_schema(void *vc, const double *xy, int32_t xy_on_device) {
         CK(hipMemcpyAsync(c->xy_in.p, xy, (size_t)c->n * 16, hipMemcpyHostToDevice, c->s));
         src = c->xy_in.p;
     }
-    int rc = stc_schema_build(&c->sch, src, c->n, c->map, c->cap, c->cfg.reach_distance, c->temp.p, c->temp.cap, c->s);
+    /* the points' Morton order keeps each wave of the build's counting on few nodes */
+    hipLaunchKernelGGL(k_prep, dim3(blocks_for(c->n, 256)), dim3(256), 0, c->s, src, c->n, c->map, c->codes.p,
+                       c->idx.p, c->dlon.p, c->idlon.p, c->dcnt);
+    CK(hipGetLastError());
+    CKRC(stc_radix_sort_u32(c->codes.p, c->idx.p, c->codes_alt.p, c->idx_alt.p, c->n, 28, c->temp.p, c->s));
+    int rc = stc_schema_build(&c->sch, src, c->idx.p, c->n, c->map, c->cap, c->cfg.reach_distance, c->temp.p,
+                              c->temp.cap, c->s);
     if (rc) return fail(c, rc, rc == STC_E_ARG ? "root does not split: too few objects for grid_capacity"
                                                : std::string("schema build: ") + g_last_fail);
     CK(hipStreamSynchronize(c->s));
@@ -2851,26 +2865,80 @@ int stc_sync(void *vc) {
     return STC_OK;
 }
 
+/* One query batch over the state the last completed step published: admission by free device
+ * memory, a lease on that state's meeting array and memtable epoch for the batch's duration, the
+ * batch on the query stream. `make` (when given) builds the batch from the published step instead
+ * of `q`; returning false answers with no hits. */
+extern "C++" {
+template <class Make>
+static int query_published(StcCtx *c, const stc_query *q, uint32_t nq, stc_hit *hits, uint64_t cap, uint64_t *n_hits,
+                           uint64_t *counts, uint32_t *state_step, Make make) {
+    std::lock_guard<std::mutex> ql(c->q_mu);
+    /* admission: what the batch's query list, counters and hit buffer add to the query buffers */
+    const uint64_t need = stc_query_bufs_growth(c->qb, nq, hits ? cap : 0);
+    size_t free_b = 0, total_b = 0;
+    if (need && hipMemGetInfo(&free_b, &total_b) == hipSuccess && need + STC_QUERY_RESERVE > free_b)
+        return fail(c, STC_E_CAPACITY, "admission: the batch needs " + std::to_string(need) + " bytes of device memory, " +
+                                           std::to_string(free_b) + " free");
+    QueryState st;
+    int mi;
+    uint32_t ep;
+    {
+        std::lock_guard<std::mutex> lk(c->pub_mu);
+        if (!c->pub_valid) return fail(c, STC_E_STATE, "no step processed");
+        st = c->pub;
+        mi = c->pub_meet;
+        ep = c->pub_epoch;
+        c->readers_meet[mi]++;
+        c->readers_epoch[ep & 1]++;
+        c->readers_total++;
+    }
+    std::vector<stc_query> built;
+    bool run = true;
+    if constexpr (!std::is_same<Make, std::nullptr_t>::value) {
+        run = make(st.last_t, built);
+        q = built.data();
+        nq = (uint32_t)built.size();
+    }
+    int rc = 0;
+    if (run) {
+        if (!c->qb) c->qb = stc_query_bufs_new();
+        rc = stc_query_run(c->qb, st, q, nq, hits, cap, n_hits, counts, c->qs);
+    } else {
+        *n_hits = 0;
+    }
+    {
+        std::lock_guard<std::mutex> lk(c->pub_mu);
+        c->readers_meet[mi]--;
+        c->readers_epoch[ep & 1]--;
+        c->readers_total--;
+    }
+    c->pub_cv.notify_all();
+    if (rc) return fail(c, rc, std::string("query: ") + g_last_fail);
+    if (state_step) *state_step = st.last_t;
+    return STC_OK;
+}
+} /* extern "C++" */
+
 int stc_live_search(void *vc, const uint32_t *pids, uint32_t n, stc_hit *hits, uint64_t cap, uint64_t *n_hits) {
     StcCtx *c = (StcCtx *)vc;
     if (!c || (!pids && n) || !n_hits) return STC_E_ARG;
-    if (!c->stepped) return fail(c, STC_E_STATE, "no step processed");
     /* live at t and established (t - start >= min_meet + 1): start <= t - min_meet - 1 and end >= t,
-     * the overlap predicate of an object query with t0 = t, t1 = t - min_meet - 1 */
-    const uint32_t t = c->last_t, m = c->cfg.min_meet_time + 1;
-    if (t < m) {
-        *n_hits = 0;
-        return STC_OK;
-    }
-    std::vector<stc_query> q(n);
-    for (uint32_t i = 0; i < n; i++) {
-        memset(&q[i], 0, sizeof(stc_query));
-        q[i].kind = STC_Q_OID;
-        q[i].oid = pids[i];
-        q[i].t0 = t;
-        q[i].t1 = t - m;
-    }
-    return stc_query_batch(vc, q.data(), n, hits, cap, n_hits, nullptr);
+     * the overlap predicate of an object query with t0 = t, t1 = t - min_meet - 1, where t is the
+     * step of the published state the batch answers */
+    const uint32_t m = c->cfg.min_meet_time + 1;
+    return query_published(c, nullptr, n, hits, cap, n_hits, nullptr, nullptr, [&](uint32_t t, std::vector<stc_query> &q) {
+        if (t < m) return false;
+        q.resize(n);
+        for (uint32_t i = 0; i < n; i++) {
+            memset(&q[i], 0, sizeof(stc_query));
+            q[i].kind = STC_Q_OID;
+            q[i].oid = pids[i];
+            q[i].t0 = t;
+            q[i].t1 = t - m;
+        }
+        return true;
+    });
 }
 
 int stc_live_search_noalloc(void *vc, const uint32_t *pids, uint32_t n, stc_hit *hits, uint64_t cap,
@@ -3166,39 +3234,7 @@ int stc_query_batch_at(void *vc, const stc_query *q, uint32_t nq, stc_hit *hits,
     if (!c || (!q && nq) || !n_hits) return STC_E_ARG;
     for (uint32_t i = 0; i < nq; i++)
         if (q[i].kind < STC_Q_OID || q[i].kind > STC_Q_PAIR) return fail(c, STC_E_ARG, "unknown query kind");
-    std::lock_guard<std::mutex> ql(c->q_mu);
-    /* admission by free device memory: what the batch's query list, counters and
-     * hit buffer add to the query buffers */
-    const uint64_t need = stc_query_bufs_growth(c->qb, nq, hits ? cap : 0);
-    size_t free_b = 0, total_b = 0;
-    if (need && hipMemGetInfo(&free_b, &total_b) == hipSuccess && need + STC_QUERY_RESERVE > free_b)
-        return fail(c, STC_E_CAPACITY, "admission: the batch needs " + std::to_string(need) + " bytes of device memory, " +
-                                           std::to_string(free_b) + " free");
-    QueryState st;
-    int mi;
-    uint32_t ep;
-    {
-        std::lock_guard<std::mutex> lk(c->pub_mu);
-        if (!c->pub_valid) return fail(c, STC_E_STATE, "no step processed");
-        st = c->pub;
-        mi = c->pub_meet;
-        ep = c->pub_epoch;
-        c->readers_meet[mi]++;
-        c->readers_epoch[ep & 1]++;
-        c->readers_total++;
-    }
-    if (!c->qb) c->qb = stc_query_bufs_new();
-    int rc = stc_query_run(c->qb, st, q, nq, hits, cap, n_hits, counts, c->qs);
-    {
-        std::lock_guard<std::mutex> lk(c->pub_mu);
-        c->readers_meet[mi]--;
-        c->readers_epoch[ep & 1]--;
-        c->readers_total--;
-    }
-    c->pub_cv.notify_all();
-    if (rc) return fail(c, rc, std::string("query: ") + g_last_fail);
-    if (state_step) *state_step = st.last_t;
-    return STC_OK;
+    return query_published(c, q, nq, hits, cap, n_hits, counts, state_step, nullptr);
 }
 
 int stc_query_batch(void *vc, const stc_query *q, uint32_t nq, stc_hit *hits, uint64_t cap, uint64_t *n_hits,

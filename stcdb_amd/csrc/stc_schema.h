@@ -51,9 +51,10 @@ struct SchemaState {
 
 int stc_schema_init(SchemaState *S, uint32_t n_points, uint32_t slots);
 void stc_schema_free(SchemaState *S);
-/* partitioner::build_schema: the initial tree from one step's points (device) */
-int stc_schema_build(SchemaState *S, const double2 *xy, uint32_t n, MapBox m, uint32_t cap, double r,
-                     uint32_t *temp, size_t temp_words, hipStream_t s);
+/* partitioner::build_schema: the initial tree from one step's points (device);
+ * order: the points in Morton order (a permutation of 0..n-1), or null */
+int stc_schema_build(SchemaState *S, const double2 *xy, const uint32_t *order, uint32_t n, MapBox m, uint32_t cap,
+                     double r, uint32_t *temp, size_t temp_words, hipStream_t s);
 /* install a host-built tree (the schema cuda_create_device_bench copies in) */
 int stc_schema_set(SchemaState *S, const stc_qtnode *nodes, uint32_t n, hipStream_t s);
 int stc_schema_get(SchemaState *S, stc_qtnode *nodes, uint64_t cap, uint64_t *n, hipStream_t s);

@@ -73,22 +73,21 @@ __global__ void k_sb_root(Node *nodes, uint32_t *nstate, MapBox m, uint32_t *cnt
     cnt[0] = 0;
 }
 
-/* every unsettled point adds itself to its node; a wave whose active points
- * share one node (the upper levels) adds once */
-__global__ void k_sb_count(const uint32_t *__restrict__ pnode, uint32_t n, uint32_t *__restrict__ cnt) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+/* every unsettled point adds itself to its node; points taken in Morton
+ * order (`order`, or id order when null) share nodes along a wave, and each
+ * run of lanes on one node adds its length once */
+__global__ void k_sb_count(const uint32_t *__restrict__ pnode, const uint32_t *__restrict__ order, uint32_t n,
+                           uint32_t *__restrict__ cnt) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & (STC_WAVE - 1);
-    const uint32_t pn = i < n ? pnode[i] : SETTLED;
-    const bool act = !(pn & SETTLED);
-    const uint64_t am = __ballot(act);
-    if (!am) return;
-    const int first = __ffsll((long long)am) - 1;
-    const uint32_t p0 = __shfl(pn, first, STC_WAVE);
-    const uint64_t same = __ballot(act && pn == p0);
-    if (same == am) {
-        if (lane == first) atomicAdd(&cnt[p0], (uint32_t)__popcll(am));
-    } else if (act) {
-        atomicAdd(&cnt[pn], 1u);
+    const uint32_t pn = j < n ? pnode[order ? order[j] : j] : SETTLED;
+    const uint32_t prev = __shfl_up(pn, 1, STC_WAVE);
+    const bool bnd = lane == 0 || prev != pn;
+    const uint64_t bmask = __ballot(bnd);
+    if (bnd && !(pn & SETTLED)) {
+        const uint64_t later = lane == STC_WAVE - 1 ? 0ull : (bmask >> (lane + 1));
+        const uint32_t end = later ? (uint32_t)(lane + __ffsll((long long)later)) : (uint32_t)STC_WAVE;
+        atomicAdd(&cnt[pn], end - (uint32_t)lane);
     }
 }
 
@@ -310,8 +309,8 @@ static int read_counts(SchemaState *S, hipStream_t s) {
     return 0;
 }
 
-int stc_schema_build(SchemaState *S, const double2 *xy, uint32_t n, MapBox m, uint32_t cap, double r,
-                     uint32_t *temp, size_t temp_words, hipStream_t s) {
+int stc_schema_build(SchemaState *S, const double2 *xy, const uint32_t *order, uint32_t n, MapBox m, uint32_t cap,
+                     double r, uint32_t *temp, size_t temp_words, hipStream_t s) {
     if (n != S->n_points) return STC_E_ARG;
     const double band = 16.0 * (std::nextafter(r, INFINITY) - r);
     hipLaunchKernelGGL(k_sb_root, dim3(1), dim3(64), 0, s, S->nodes, S->nstate, m, S->cnt);
@@ -322,7 +321,7 @@ int stc_schema_build(SchemaState *S, const double2 *xy, uint32_t n, MapBox m, ui
         if (depth > SB_MAX_DEPTH) return STC_E_CAPACITY;
         const uint32_t nf = fe - fb;
         STC_CHECK(hipMemsetAsync(S->dcount, 0, DC_WORDS * 4, s));
-        hipLaunchKernelGGL(k_sb_count, dim3(nblk(n, 256)), dim3(256), 0, s, (const uint32_t *)S->pnode, n, S->cnt);
+        hipLaunchKernelGGL(k_sb_count, dim3(nblk(n, 256)), dim3(256), 0, s, (const uint32_t *)S->pnode, order, n, S->cnt);
         hipLaunchKernelGGL(k_sb_decide, dim3(nblk(nf, 256)), dim3(256), 0, s, (const Node *)S->nodes,
                            (const uint32_t *)S->cnt, fb, fe, cap, r, band, S->fs, S->amb, S->dcount);
         STC_CHECK(hipGetLastError());
