@@ -98,18 +98,18 @@ def cpu_baseline(w, frames_host, steps, schema):
     the four stages' wall times."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
-    ids = cpu_sample_ids(w["gen"])
+    ids = cpu_sample_ids(w["gen"])  # frames_host holds these rows, in this order
     cfg = dict(w["cfg"])
     cfg["num_objects"] = int(len(ids))
     cfg["schema_mode"] = 2 if schema == "dynamic" else 0
     threads = cpu_threads()
     buckets = 1 << 27  # ~0.37 of the sample's ~49M pairs per step: no probe chain near a full table
     cp = O.CpuPath(cfg, w["box"], threads=threads, buckets=buckets, keep_pairs=False)
-    cp.step(np.ascontiguousarray(frames_host[0][ids]), 0)
+    cp.step(frames_host[0], 0)
     stage = {k: 0.0 for k in O.CpuPath.STAGES}
     pairs = 0
     for t in range(1, steps + 1):
-        r = cp.step(np.ascontiguousarray(frames_host[t][ids]), t)
+        r = cp.step(frames_host[t], t)
         pairs += r["pairs"]
         for k, v in r["ms"].items():
             stage[k] += v
@@ -377,7 +377,8 @@ def main():
     ap.add_argument("--schema", choices=["dynamic", "rebuild"], default="dynamic",
                     help="QTree mode: kept and updated across steps (the reference's default) or rebuilt every step "
                          "(--disable_dynamic_schema)")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=40,
+                    help="timed steps of the CPU baseline (about 0.25 s each on 16 threads: ~10 s of CPU work)")
     ap.add_argument("--host-steps", type=int, default=10,
                     help="extra steps fed from pinned host memory (PCIe-inclusive rate, reported beside value)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -461,8 +462,8 @@ def main():
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     for i, r in enumerate(outs):
-        print(f"[bench] step {args.warmup + i}: {r['ms_total']:.1f} ms (count {r['ms_count_kernel']:.1f}, "
-              f"write {r['ms_refine_kernel']:.1f}, merge {r['ms_identify_kernel']:.1f}, flush {r['ms_flush']:.1f}, "
+        print(f"[bench] step {args.warmup + i}: {r['ms_total']:.1f} ms (collect {r['ms_count_kernel']:.1f}, "
+              f"place {r['ms_refine_kernel']:.1f}, merge {r['ms_identify_kernel']:.1f}, flush {r['ms_flush']:.1f}, "
               f"schema {r['ms_schema']:.2f}) pairs {r['pairs']} records {r['new_records']} leaves {r['leaves']} "
               f"flushed {r['flushed']}", file=sys.stderr, flush=True)
 
@@ -493,11 +494,14 @@ def main():
     pairs = mean("pairs")
     slots = mean("table_buckets")  # pair slots of a step = buckets of its meeting array
     kernels = {
-        # count pass: the N points in (16 B position + 4 B id), a 4 B count per owner out
-        "k_pair_count": ("pair count pass, owner by owner", mean("ms_count_kernel"), 24.0 * n),
-        # write pass: the N points in, one 24 B slot (key + float MBR) per pair out
-        "k_pair_write": ("pair write pass: owner segments sorted by partner", mean("ms_refine_kernel"),
-                         20.0 * n + 24.0 * slots),
+        # collect pass: the N points in (16 B position + 4 B id), a 4 B count per owner out, one
+        # 8 B key (partner, source) per slot out
+        "k_pair_collect": ("pair collect pass: each combination decided once, keys per owner", mean("ms_count_kernel"),
+                           24.0 * n + 8.0 * slots),
+        # place pass: each owner's 8 B run descriptor and 8 B keys in, one 24 B slot (key + float
+        # MBR) per pair out
+        "k_pair_place": ("pair place pass: owner segments sorted by partner", mean("ms_refine_kernel"),
+                         8.0 * n + 32.0 * slots),
         # merge: the step's slots (24 B) and the previous meetings (32 B) in, the new
         # meetings (32 B) and 2 records (32 B each) per recorded meeting out
         "k_meet_merge": ("merge with the previous step's meetings, expiry, records", mean("ms_identify_kernel"),
@@ -518,11 +522,12 @@ def main():
     if not args.no_cpu:
         trace = DeviceTrace(gen)
         frames_host = []
+        ids = cpu_sample_ids(gen)
+        buf = np.empty((n, 2), dtype=np.float64)
         for t in range(args.cpu_steps + 1):
             p = trace.next(t)
-            buf = np.empty((n, 2), dtype=np.float64)
             assert h.hipMemcpy(buf.ctypes.data, p.value, n * 16, 2) == 0
-            frames_host.append(buf)
+            frames_host.append(np.ascontiguousarray(buf[ids]))  # the sample's rows only
         trace.close()
         cpu = cpu_baseline(w, frames_host, args.cpu_steps, args.schema)
         del frames_host

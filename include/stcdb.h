@@ -107,15 +107,15 @@ typedef struct stc_step_out {
     uint32_t rebuilt;          /* 0 (every step's merge writes a new meeting array) */
     float ms_partition;        /* QTree build + partition */
     float ms_filter;           /* border lookup */
-    float ms_refine;           /* pair refinement: reverse lists, count, write, segment sorts */
+    float ms_refine;           /* pair refinement: reverse lists, collect, place, segment sorts */
     float ms_meetings;         /* merge with the previous meetings + records */
     float ms_flush;            /* flush (0 when no flush) */
     float ms_total;
-    float ms_refine_kernel;    /* the pair write pass (owner segments, sorted) */
+    float ms_refine_kernel;    /* the pair place pass (owner segments, sorted) */
     float ms_identify_kernel;  /* the merge with the previous meetings (last launch) */
     uint64_t continuing;       /* pairs that continued a meeting of the previous step */
     uint64_t new_keys;         /* pairs that opened a meeting */
-    float ms_count_kernel;     /* the pair count pass (last launch) */
+    float ms_count_kernel;     /* the pair collect pass (last launch) */
     float ms_schema;           /* dynamic schema update after the step (pro.index_update_time) */
 } stc_step_out;
 

@@ -55,6 +55,8 @@
 #include <vector>
 #include <type_traits>
 
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4))); /* a native vector for 16-byte streaming accesses */
+
 #define ONE_DIM 16384u
 #define NLEVELS 15
 #define STACK_DEPTH 96
@@ -99,6 +101,8 @@ struct Counters {             /* device-side counters, copied back once per sync
     uint32_t table_fail;
     uint32_t lookup_overflow;          /* border lookups whose node stack was full */
     uint32_t big, huge, giant;         /* segments left to the wave sort, the block sort, the global-scratch sort */
+    uint32_t pk_over;                  /* a collecting wave's reservation exceeded psrc */
+    unsigned long long pk_used;        /* psrc slots reserved by the collecting waves */
 };
 
 
@@ -443,7 +447,8 @@ __global__ void k_lookup(const uint32_t *__restrict__ lp, const double2 *__restr
  * (oid1 << 32 | oid2): one segment per owner oid1 (the pair's smaller id), in
  * owner order, each segment's live buckets followed by tombstones for the
  * step's duplicate or rejected pairs. A step finds its pairs owner by owner,
- * twice (count, then write into the owner's segment), and one merge per owner
+ * once (collect each owner's pair sources, then place them sorted into the
+ * owner's segment), and one merge per owner
  * joins the new segment with the previous step's: a pair continues a meeting
  * of the previous step or opens one (insert/extend, geometry.cu:469-494), a
  * meeting of the previous step without a pair now expires and, when it lasted
@@ -619,6 +624,18 @@ __device__ inline void owner_combos_by_source(uint32_t w, uint32_t a, const Pair
     }
 }
 
+/* the number of combinations owner_combos visits for the owner at leaf-order
+ * place w: its later leaf members, the members above it of its cross units'
+ * leaves, the border points above it with a unit to its leaf. Wave-uniform. */
+__device__ inline uint32_t owner_candidates(uint32_t w, const PairGeo &g, int lane) {
+    const uint4 pr = g.prange[w];
+    const uint2 un = g.lunits[w];
+    unsigned long long t = 0;
+    for (uint32_t k = (uint32_t)lane; k < un.y; k += STC_WAVE) t += g.urange[un.x + k].y;
+    t = wave_sum_u64(t);
+    return pr.y + pr.w + (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)t);
+}
+
 /* The same combinations as owner_combos_by_source, with the owner's sources
  * laid end to end so that every call of f but the last fills all 64 lanes
  * (OTHER false: the other point's id and the probe are left out of the call,
@@ -626,12 +643,14 @@ __device__ inline void owner_combos_by_source(uint32_t w, uint32_t a, const Pair
  * lane j describes source j (j = 0: (i); 1 .. nu: the cross units; nu + 1:
  * (iii)) by its first index and length, and each lane finds the source of its
  * combination from the prefix of the lengths. Owners with more than 62 cross
- * units take the per-source loops. Wave-uniform. */
-template <bool OTHER, class F>
-__device__ inline void owner_combos(uint32_t w, uint32_t a, const PairGeo &g, int lane, F &f) {
+ * units take the per-source loops. on_total receives the number of
+ * combinations before the first call of f. Wave-uniform. */
+template <bool OTHER, class F, class T>
+__device__ inline void owner_combos(uint32_t w, uint32_t a, const PairGeo &g, int lane, F &f, T &&on_total) {
     const uint2 un = g.lunits[w];
     const uint32_t u0 = un.x, nu = un.y;
     if (nu + 2 > STC_WAVE) {
+        on_total(owner_candidates(w, g, lane));
         owner_combos_by_source<OTHER>(w, a, g, lane, f);
         return;
     }
@@ -660,6 +679,7 @@ __device__ inline void owner_combos(uint32_t w, uint32_t a, const PairGeo &g, in
         if (lane >= o) scum += y;
     }
     const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)scum, STC_WAVE - 1);
+    on_total(total);
     scum -= slen;
     const uint32_t nseg = nu + 2;
     const double qax = pa.x, qay = pa.y; /* a's position as a member of its leaf */
@@ -773,41 +793,94 @@ __global__ void k_point_ranges(const uint32_t *__restrict__ lp, const uint32_t *
     prange[w] = r;
 }
 
-/* Pass 1: per owner, the combinations that are pairs or lie in the guard band
- * (each reserves a slot of the owner's segment); band combinations are listed
- * as (probe, other) for the host. One wave per owner, owners in leaf order. */
-__global__ __launch_bounds__(256, 8) void k_pair_count(PairGeo g, uint32_t *__restrict__ pcount,
-                                                    uint2 *__restrict__ amb, uint32_t amb_cap, Counters *cnt) {
+#define WCAP 512u            /* slots of a segment the place pass sorts in registers */
+#define WSORT_CAP 1024u      /* slots of the segments k_seg_sort_wave sorts in registers */
+
+__device__ inline uint64_t lanemask_lt(int lane) { return lane == 0 ? 0ull : (~0ull >> (STC_WAVE - lane)); }
+
+#define SRC_AMB 0x40000000u  /* key flag: a band combination, a pair only if the host accepted it */
+#define PK_REGION 4096u      /* key slots a collecting wave reserves at a time */
+
+/* The pair pass: per owner (one wave, owners in leaf order), every
+ * combination is decided once; those that are pairs or lie in the guard band
+ * take a slot of the owner's segment and their sources (the other point's
+ * index in the leaf-ordered arrays or SRC_REV | its reverse-list index, with
+ * SRC_AMB on band combinations) are written, unsorted, to a run of psrc that
+ * the wave reserved (PK_REGION slots at a time, fewer for a wave with few
+ * owners left, or the owner's candidate count when larger, so an owner's
+ * sources stay contiguous). Per owner: the slot count (pcount, by id) and the
+ * run's start and length (pkrun, leaf order); band combinations are listed as
+ * (probe, other) for the host; segments longer than the place pass sorts go
+ * to `big` or `huge`. A reservation beyond pk_cap writes nothing and sets
+ * pk_over (the host grows psrc and repeats the pass; pk_used tells by how
+ * much). */
+__global__ __launch_bounds__(256, 8) void k_pair_collect(PairGeo g, uint32_t *__restrict__ pcount,
+                                                      uint2 *__restrict__ pkrun, uint32_t *__restrict__ psrc,
+                                                      uint64_t pk_cap, uint2 *__restrict__ amb, uint32_t amb_cap,
+                                                      uint32_t *__restrict__ big, uint32_t *__restrict__ huge,
+                                                      Counters *cnt) {
     const int lane = threadIdx.x & (STC_WAVE - 1);
     /* wave-uniform values go through readfirstlane: scalar registers, scalar loads, scalar loop control */
     const uint64_t wid = (uint32_t)__builtin_amdgcn_readfirstlane((int)((xcd_block_id() * blockDim.x + threadIdx.x) / STC_WAVE));
     const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) / STC_WAVE;
+    const uint64_t lt = lanemask_lt(lane);
     unsigned long long cand = 0, total = 0;
+    uint64_t rcur = 0, rend = 0; /* the wave's reserved run of psrc */
+    bool over = false;
     for (uint64_t w = wid; w < g.n; w += nw) {
         const uint32_t a = (uint32_t)__builtin_amdgcn_readfirstlane((int)g.lp[w]);
-        unsigned long long c = 0;
-        auto f = [&](bool v, uint32_t, int dec, uint32_t, float4, uint32_t src) {
-            if (!v) return;
-            cand++;
-            if (dec) c++;
-            if (dec == 2) { /* (probe, other): the border point probes in a reverse-list combination */
-                const bool rev = (src & SRC_REV) != 0;
-                const uint32_t o = rev ? g.rev_pid[src & ~SRC_REV] : g.lp[src];
-                const unsigned long long k = atomicAdd(&cnt->ambiguous, 1ull);
-                if (k < amb_cap) amb[k] = rev ? make_uint2(o, a) : make_uint2(a, o);
+        bool room = true;
+        uint32_t *out = psrc;
+        auto reserve = [&](uint32_t tot) {
+            if (rend - rcur < tot) {
+                /* PK_REGION slots, or fewer when the wave's remaining owners need less at this
+                 * owner's count each (a wave of a small step has one owner) */
+                const uint64_t left = (g.n - 1 - w) / nw + 1;
+                const uint64_t want = left * tot < PK_REGION ? left * tot : PK_REGION;
+                const uint64_t sz = tot > want ? tot : want;
+                unsigned long long b0 = 0;
+                if (lane == 0) b0 = atomicAdd(&cnt->pk_used, (unsigned long long)sz);
+                b0 = __shfl(b0, 0, STC_WAVE);
+                rcur = b0;
+                rend = b0 + sz;
             }
+            room = rend <= pk_cap;
+            over |= !room;
+            out = psrc + rcur;
         };
-        owner_combos<false>((uint32_t)w, a, g, lane, f);
-        c = wave_sum_u64(c);
+        uint32_t fill = 0;
+        auto f = [&](bool v, uint32_t, int dec, uint32_t, float4, uint32_t src) {
+            cand += v;
+            const bool take = v && dec != 0;
+            const uint64_t m = __ballot(take);
+            if (take) {
+                if (room) __builtin_nontemporal_store(dec == 2 ? (src | SRC_AMB) : src, out + fill + (uint32_t)__popcll(m & lt));
+                if (dec == 2) { /* (probe, other): the border point probes in a reverse-list combination */
+                    const bool rev = (src & SRC_REV) != 0;
+                    const uint32_t o = rev ? g.rev_pid[src & ~SRC_REV] : g.lp[src];
+                    const unsigned long long k = atomicAdd(&cnt->ambiguous, 1ull);
+                    if (k < amb_cap) amb[k] = rev ? make_uint2(o, a) : make_uint2(a, o);
+                }
+            }
+            fill += (uint32_t)__popcll(m);
+        };
+        owner_combos<false>((uint32_t)w, a, g, lane, f, reserve);
         if (lane == 0) {
-            pcount[a] = (uint32_t)c;
-            total += c;
+            pcount[a] = fill;
+            pkrun[w] = make_uint2((uint32_t)rcur, fill);
+            total += fill;
+            if (fill > WCAP) {
+                if (fill <= WSORT_CAP) big[atomicAdd(&cnt->big, 1u)] = a;
+                else huge[atomicAdd(&cnt->huge, 1u)] = a;
+            }
         }
+        rcur += fill;
     }
     cand = wave_sum_u64(cand);
     if (lane == 0) {
         if (cand) atomicAdd(&cnt->candidates, cand);
         if (total) atomicAdd(&cnt->pairs, total);
+        if (over) atomicOr(&cnt->pk_over, 1u);
     }
 }
 
@@ -821,9 +894,6 @@ __device__ inline bool sorted_contains(const unsigned long long *a, uint32_t n, 
     return lo < n && a[lo] == v;
 }
 
-#define WCAP 512u          /* slots of a segment the write pass sorts in its wave's LDS */
-
-__device__ inline uint64_t lanemask_lt(int lane) { return lane == 0 ? 0ull : (~0ull >> (STC_WAVE - lane)); }
 
 
 /* the value that lane (lane ^ J) holds, J a power of two below 64: quad
@@ -892,18 +962,36 @@ __device__ inline void reg_bitonic(unsigned long long (&x)[K], int lane) {
     bitonic_stages<K, 2>(x, lane);
 }
 
-/* Sort an owner's slots (keys[0, slots), slots <= 64 * K) in registers and
- * write its segment: live pairs in partner order, then the dead slots. */
+/* A collected source as the sort takes it, (partner << 32 | source): the
+ * partner is the source's id; a band combination the host did not accept
+ * becomes a dead slot (partner DEAD_B), its band flag cleared. */
+__device__ inline unsigned long long settle_key(uint32_t src, uint32_t a, const PairGeo &g,
+                                                const unsigned long long *acc, uint32_t n_acc) {
+    const uint32_t idx = src & ~(SRC_REV | SRC_AMB);
+    const uint32_t q = (src & SRC_REV) ? g.rev[idx].pid : g.lp[idx];
+    uint32_t b = q;
+    if ((src & SRC_AMB) && !sorted_contains(acc, n_acc, ((unsigned long long)a << 32) | q)) b = DEAD_B;
+    return ((unsigned long long)b << 32) | (src & ~SRC_AMB);
+}
+
+/* Sort an owner's collected sources (srcs[0, slots), slots <= 64 * K) by
+ * partner in registers and write its segment: live pairs in partner order,
+ * then the dead slots (band combinations the host rejected, repeats of a
+ * pair). */
 template <int K>
-__device__ inline void write_sorted_segment(const unsigned long long *keys, uint32_t slots, uint32_t a, uint32_t w,
-                                            const PairGeo &g, PairHit *__restrict__ seg, int lane) {
+__device__ inline void write_sorted_segment(const uint32_t *srcs, uint32_t slots, uint32_t a, uint32_t w,
+                                            const PairGeo &g, const unsigned long long *acc, uint32_t n_acc,
+                                            PairHit *__restrict__ seg, int lane) {
     const uint64_t lt = lanemask_lt(lane);
     unsigned long long x[K];
+    uint32_t sv[K];
 #pragma unroll
     for (int r = 0; r < K; r++) {
         const uint32_t i = r * STC_WAVE + lane;
-        x[r] = i < slots ? keys[i] : ~0ull;
+        sv[r] = i < slots ? __builtin_nontemporal_load(srcs + i) : 0u;
     }
+#pragma unroll
+    for (int r = 0; r < K; r++) x[r] = r * STC_WAVE + lane < slots ? settle_key(sv[r], a, g, acc, n_acc) : ~0ull;
     reg_bitonic<K>(x, lane);
     bool live[K];
     uint32_t nlive = 0;
@@ -947,7 +1035,14 @@ __device__ inline void write_sorted_segment(const unsigned long long *keys, uint
                 PairHit e;
                 e.key = ((unsigned long long)a << 32) | (live[r] ? (uint32_t)(x[r] >> 32) : DEAD_B);
                 e.mbr[0] = mb.x; e.mbr[1] = mb.y; e.mbr[2] = mb.z; e.mbr[3] = mb.w;
-                seg[live[r] ? lr + (uint32_t)__popcll(ml & lt) : nlive + dr + (uint32_t)__popcll(md & lt)] = e;
+                /* streamed out past the caches: the merge reads the slots once, a step's worth of
+                 * traffic later */
+                PairHit *d = seg + (live[r] ? lr + (uint32_t)__popcll(ml & lt) : nlive + dr + (uint32_t)__popcll(md & lt));
+                __builtin_nontemporal_store(e.key, &d->key);
+                __builtin_nontemporal_store(mb.x, &d->mbr[0]);
+                __builtin_nontemporal_store(mb.y, &d->mbr[1]);
+                __builtin_nontemporal_store(mb.z, &d->mbr[2]);
+                __builtin_nontemporal_store(mb.w, &d->mbr[3]);
             }
             lr += (uint32_t)__popcll(ml);
             dr += (uint32_t)__popcll(md);
@@ -955,15 +1050,15 @@ __device__ inline void write_sorted_segment(const unsigned long long *keys, uint
     }
 }
 
-#define WSORT_CAP 1024u /* slots of the segments k_seg_sort_wave sorts in registers */
 
-/* Segments of WCAP + 1 .. WSORT_CAP slots, which the write pass left in ptmp
- * as (partner, source) keys only: one wave each, sorted in registers and
- * written like the write pass's own segments (MBRs from the sources'
- * positions). */
+/* Segments of WCAP + 1 .. WSORT_CAP slots: one wave each, sorted in
+ * registers and written like the place pass's own segments. */
 __global__ __launch_bounds__(256, 5) void k_seg_sort_wave(PairGeo g, const uint32_t *__restrict__ owners,
                                                        uint32_t n_owners, const uint32_t *__restrict__ poff,
-                                                       PairHit *__restrict__ ptmp, PairHit *__restrict__ pn) {
+                                                       const uint2 *__restrict__ pkrun,
+                                                       const uint32_t *__restrict__ psrc,
+                                                       const unsigned long long *__restrict__ acc, uint32_t n_acc,
+                                                       PairHit *__restrict__ pn) {
     const int lane = threadIdx.x & (STC_WAVE - 1);
     const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / STC_WAVE;
     const uint32_t nw = (gridDim.x * blockDim.x) / STC_WAVE;
@@ -971,75 +1066,65 @@ __global__ __launch_bounds__(256, 5) void k_seg_sort_wave(PairGeo g, const uint3
         const uint32_t a = owners[oi];
         const uint32_t base = poff[a], slots = poff[a + 1] - base;
         const uint32_t w = g.leaf_start[g.pt_leaf[a]] + g.pt_rank[a]; /* a's place in leaf order */
-        write_sorted_segment<WSORT_CAP / STC_WAVE>((const unsigned long long *)(ptmp + base), slots, a, w, g,
-                                                    pn + base, lane);
+        write_sorted_segment<WSORT_CAP / STC_WAVE>(psrc + pkrun[w].x, slots, a, w, g, acc, n_acc, pn + base, lane);
     }
 }
 
-/* Pass 2: the same combinations, written to pn as each owner's segment sorted
- * by partner, its live pairs first and its dead slots (band combinations the
- * host rejected, repeats of a pair) behind them, so that pn is sorted by key.
- * A segment of at most WCAP slots is staged in the wave's LDS as (partner,
- * source) keys and sorted in registers, each pair's MBR recomputed from its
- * source's position when written; a longer one is written unsorted to ptmp
- * and listed in `big` for the segment sorts (k_seg_sort_wave, k_seg_sort).
- * The pass repeats pass 1's arithmetic exactly; a slot beyond the counted
- * ones would be dropped and reported through table_fail. */
-__global__ __launch_bounds__(256, 7) void k_pair_write(PairGeo g, const uint32_t *__restrict__ poff,
-                                                    const uint32_t *__restrict__ pcount,
-                                                    const unsigned long long *__restrict__ acc, uint32_t n_acc,
-                                                    PairHit *__restrict__ ptmp, PairHit *__restrict__ pn,
-                                                    uint32_t *__restrict__ big, uint32_t *__restrict__ huge,
-                                                    Counters *cnt) {
-    __shared__ unsigned long long wk[256 / STC_WAVE][WCAP];
+/* Segments longer than WSORT_CAP slots: their collected keys as pair slots
+ * (key, MBR from the source's position), unsorted, at the segment's place in
+ * ptmp for the block sorts (k_seg_sort). One wave per owner. */
+__global__ void k_huge_expand(PairGeo g, const uint32_t *__restrict__ owners, uint32_t n_owners,
+                              const uint32_t *__restrict__ poff, const uint2 *__restrict__ pkrun,
+                              const uint32_t *__restrict__ psrc, const unsigned long long *__restrict__ acc,
+                              uint32_t n_acc, PairHit *__restrict__ ptmp) {
     const int lane = threadIdx.x & (STC_WAVE - 1);
-    unsigned long long *keys = wk[threadIdx.x / STC_WAVE];
-    /* wave-uniform values go through readfirstlane: scalar registers, scalar loads, scalar loop control */
+    const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / STC_WAVE;
+    const uint32_t nw = (gridDim.x * blockDim.x) / STC_WAVE;
+    for (uint32_t oi = wid; oi < n_owners; oi += nw) {
+        const uint32_t a = owners[oi];
+        const uint32_t base = poff[a], slots = poff[a + 1] - base;
+        const uint32_t w = g.leaf_start[g.pt_leaf[a]] + g.pt_rank[a];
+        const uint32_t *srcs = psrc + pkrun[w].x;
+        const float afx = (float)g.lxy[w].x, afy = (float)g.lxy[w].y;
+        for (uint32_t i = lane; i < slots; i += STC_WAVE) {
+            const unsigned long long k = settle_key(srcs[i], a, g, acc, n_acc);
+            const uint32_t src = (uint32_t)k;
+            const bool rev = (src & SRC_REV) != 0;
+            const double2 pos = rev ? *(const double2 *)(g.rev + (src & ~SRC_REV)) : g.lxy[src];
+            const float px = (float)pos.x, py = (float)pos.y;
+            const float4 mb = rev ? pair_mbr(px, py, afx, afy) : pair_mbr(afx, afy, px, py);
+            PairHit h;
+            h.key = ((unsigned long long)a << 32) | (uint32_t)(k >> 32);
+            h.mbr[0] = mb.x; h.mbr[1] = mb.y; h.mbr[2] = mb.z; h.mbr[3] = mb.w;
+            ptmp[base + i] = h;
+        }
+    }
+}
+
+/* The place pass: per owner (one wave, owners in leaf order) with at most
+ * WCAP slots, its collected keys sorted in registers and written to pn as the
+ * owner's segment sorted by partner, its live pairs first and its dead slots
+ * behind them, so that pn is sorted by key; each pair's MBR is computed from
+ * its source's position when written. */
+__global__ __launch_bounds__(256, 7) void k_pair_place(PairGeo g, const uint32_t *__restrict__ poff,
+                                                    const uint2 *__restrict__ pkrun,
+                                                    const uint32_t *__restrict__ psrc,
+                                                    const unsigned long long *__restrict__ acc, uint32_t n_acc,
+                                                    PairHit *__restrict__ pn) {
+    const int lane = threadIdx.x & (STC_WAVE - 1);
     const uint64_t wid = (uint32_t)__builtin_amdgcn_readfirstlane((int)((xcd_block_id() * blockDim.x + threadIdx.x) / STC_WAVE));
     const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) / STC_WAVE;
-    const uint64_t lt = lanemask_lt(lane);
     for (uint64_t w = wid; w < g.n; w += nw) {
+        const uint2 run = pkrun[w];
+        const uint32_t slots = (uint32_t)__builtin_amdgcn_readfirstlane((int)run.y);
+        if (slots == 0 || slots > WCAP) continue;
         const uint32_t a = (uint32_t)__builtin_amdgcn_readfirstlane((int)g.lp[w]);
-        const uint32_t base = poff[a], slots = pcount[a];
-        const bool small = slots <= WCAP;
-        uint32_t fill = 0;
-        auto f = [&](bool v, uint32_t q, int dec, uint32_t, float4 mb, uint32_t src) {
-            const bool take = v && dec != 0;
-            const uint64_t m = __ballot(take);
-            if (take) {
-                uint32_t bq = q;
-                if (dec == 2 && !sorted_contains(acc, n_acc, ((unsigned long long)a << 32) | q)) bq = DEAD_B;
-                const uint32_t at = fill + (uint32_t)__popcll(m & lt);
-                if (at >= slots) {
-                    atomicOr(&cnt->table_fail, 1u);
-                } else if (small) {
-                    keys[at] = ((unsigned long long)bq << 32) | src;
-                } else if (slots <= WSORT_CAP) { /* keys only, at the start of the segment's staging room */
-                    ((unsigned long long *)(ptmp + base))[at] = ((unsigned long long)bq << 32) | src;
-                } else {
-                    PairHit h;
-                    h.key = ((unsigned long long)a << 32) | bq;
-                    h.mbr[0] = mb.x; h.mbr[1] = mb.y; h.mbr[2] = mb.z; h.mbr[3] = mb.w;
-                    ptmp[base + at] = h;
-                }
-            }
-            fill += (uint32_t)__popcll(m);
-        };
-        owner_combos<true>((uint32_t)w, a, g, lane, f);
-        if (!small) { /* the lists hold every owner */
-            if (lane == 0) {
-                if (slots <= WSORT_CAP) big[atomicAdd(&cnt->big, 1u)] = a;
-                else huge[atomicAdd(&cnt->huge, 1u)] = a;
-            }
-            continue;
-        }
-        if (slots == 0) continue;
-        __builtin_amdgcn_wave_barrier();
-        if (slots <= STC_WAVE) write_sorted_segment<1>(keys, slots, a, (uint32_t)w, g, pn + base, lane);
-        else if (slots <= 2 * STC_WAVE) write_sorted_segment<2>(keys, slots, a, (uint32_t)w, g, pn + base, lane);
-        else if (slots <= 4 * STC_WAVE) write_sorted_segment<4>(keys, slots, a, (uint32_t)w, g, pn + base, lane);
-        else write_sorted_segment<8>(keys, slots, a, (uint32_t)w, g, pn + base, lane);
-        __builtin_amdgcn_wave_barrier(); /* the keys are reused by the wave's next owner */
+        const uint32_t *keys = psrc + (uint32_t)__builtin_amdgcn_readfirstlane((int)run.x);
+        PairHit *seg = pn + poff[a];
+        if (slots <= STC_WAVE) write_sorted_segment<1>(keys, slots, a, (uint32_t)w, g, acc, n_acc, seg, lane);
+        else if (slots <= 2 * STC_WAVE) write_sorted_segment<2>(keys, slots, a, (uint32_t)w, g, acc, n_acc, seg, lane);
+        else if (slots <= 4 * STC_WAVE) write_sorted_segment<4>(keys, slots, a, (uint32_t)w, g, acc, n_acc, seg, lane);
+        else write_sorted_segment<8>(keys, slots, a, (uint32_t)w, g, acc, n_acc, seg, lane);
     }
 }
 
@@ -1197,7 +1282,7 @@ __device__ inline uint32_t block_rank(bool f, uint32_t *wsum, uint32_t *total) {
     return off + r;
 }
 
-/* The segments the write pass left unsorted (longer than WCAP): the same
+/* Segments longer than WSORT_CAP, expanded unsorted into ptmp: the same
  * ordering by a block, with the sort keys in LDS (LCAP entries) or, with
  * GLOBAL, in a per-block slice of gcap entries of global scratch. Longer
  * segments go to `spill` (or set table_fail when spill is null). */
@@ -1453,7 +1538,12 @@ __global__ __launch_bounds__(256) void k_meet_merge(const PairHit *__restrict__ 
                         if (cont) n_cont++;
                         else n_fresh++;
                     }
-                    mnew[i0 + i] = nb;
+                    /* streamed out past the caches: the array is read again only by the next step */
+                    u32x4 *d = (u32x4 *)(mnew + i0 + i);
+                    const u32x4 lo = {(uint32_t)nb.key, (uint32_t)(nb.key >> 32), nb.start, nb.end};
+                    const u32x4 hi = {nb.mbr[0], nb.mbr[1], nb.mbr[2], nb.mbr[3]};
+                    __builtin_nontemporal_store(lo, d);
+                    __builtin_nontemporal_store(hi, d + 1);
                 }
             }
             __builtin_amdgcn_wave_barrier();
@@ -1745,7 +1835,9 @@ struct StcCtx {
     /* the step's pairs by owner segment, and the live meetings: two arrays of
      * segments (offsets [n + 1], live buckets per segment [n]) used in turn */
     DBuf<uint32_t> pcount;
-    DBuf<PairHit> ptmp, pnew;            /* the pair slots as found, and sorted by key */
+    DBuf<PairHit> ptmp, pnew;            /* the long segments' pair slots as found, and all slots sorted by key */
+    DBuf<uint32_t> psrc;                 /* collected pair sources, a run per owner */
+    DBuf<uint2> pkrun;                   /* per owner in leaf order: its run of psrc (start, length) */
     DBuf<Bucket> meet[2];
     DBuf<uint32_t> moff[2];
     DBuf<uint32_t> tbounds;              /* merge tiles' ranges in the previous meetings */
@@ -2015,6 +2107,11 @@ int stc_device_info_get(int32_t device, stc_device_info *info) {
 int stc_create(const stc_config *cfg, const stc_box *mbr, void **out) {
     if (!cfg || !mbr || !out) return STC_E_ARG;
     *out = nullptr;
+    /* pair sources carry two tag bits above a 30-bit index */
+    if (cfg->num_objects >= (1u << 30)) {
+        g_last_fail = "num_objects must be below 2^30";
+        return STC_E_ARG;
+    }
     StcCtx *c = new StcCtx();
     c->err.reserve(1024); /* a failure message then needs no allocation */
     c->cfg = *cfg;
@@ -2085,7 +2182,7 @@ int stc_create(const stc_config *cfg, const stc_box *mbr, void **out) {
         return bad("pinned");
     /* no meetings yet: every segment of the current array is empty */
     uint64_t nb = cfg->num_meeting_buckets ? cfg->num_meeting_buckets : 2ull * n;
-    if (c->ptmp.ensure(nb) || c->pnew.ensure(nb) || c->meet[0].ensure(nb) || c->meet[1].ensure(nb))
+    if (c->pnew.ensure(nb) || c->meet[0].ensure(nb) || c->meet[1].ensure(nb))
         return bad("meeting arrays");
     if (hipMemsetAsync(c->moff[0].p, 0, 4ull * (n + 1), c->s) != hipSuccess) return bad("meeting offsets");
     if (hipMalloc((void **)&c->same_pid_count, 4ull * n) != hipSuccess ||
@@ -2118,7 +2215,7 @@ int stc_destroy(void *vc) {
     c->rev.release(); c->rev_dl.release();
     c->pcount.release(); c->pnew.release(); c->acc.release();
     for (int i = 0; i < 2; i++) { c->meet[i].release(); c->moff[i].release(); }
-    c->ptmp.release(); c->tbounds.release();
+    c->ptmp.release(); c->tbounds.release(); c->psrc.release(); c->pkrun.release();
     c->spill_big.release(); c->spill_huge.release(); c->spill_giant.release(); c->gsrt.release();
     c->units_in.release(); c->ukeys.release(); c->ukeys_alt.release();
     c->uidx.release(); c->uidx_alt.release(); c->utemp.release();
@@ -2278,6 +2375,7 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
     if (h.bad_points) return fail(c, STC_E_INPUT, "point outside the map box");
     if (h.lookup_overflow) return fail(c, STC_E_CAPACITY, "border lookup: node stack depth exceeded");
     uint64_t n_units = (uint32_t)h.units;
+    if (n_units >= (1u << 30)) return fail(c, STC_E_CAPACITY, "2^30 or more cross units in one step");
     if (c->units.ensure(n_units + 1)) return fail(c, STC_E_CAPACITY, "unit buffer allocation failed");
     hipLaunchKernelGGL(k_lookup<true>, dim3(blocks_for(n, 128)), dim3(128), 0, s, (const uint32_t *)c->lp.p,
                        (const double2 *)c->lxy.p, (const double *)c->ld.p, n, (const Node *)tree_nodes(c),
@@ -2330,20 +2428,36 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
                      r * r * (1.0 + 1e-12)};
     /* a multiple of 8 workgroups, for the XCD-aware owner order */
     const uint32_t pgrid = (uint32_t)std::min<uint64_t>((blocks_for((uint64_t)n * STC_WAVE, 256) + 7) & ~7ull, 256u * 64u);
-    /* the last run's host copies overlap the count pass, which computes more than it writes */
+    /* the last run's host copies overlap the collect pass, which computes more than it writes */
     if (queue_run_copies(c)) return fail(c, STC_E_HIP, std::string("run copies: ") + g_last_fail);
+    if (c->pkrun.ensure(n)) return fail(c, STC_E_CAPACITY, "pair key runs");
+    if (!c->psrc.p && c->psrc.ensure(std::max<size_t>((size_t)n * 16, 1u << 20)))
+        return fail(c, STC_E_CAPACITY, "pair sources");
     for (int attempt = 0;; attempt++) {
         CK(hipMemsetAsync(&cnt->candidates, 0, 3 * sizeof(unsigned long long), s)); /* candidates, pairs, ambiguous */
+        CK(hipMemsetAsync(&cnt->big, 0, 3 * sizeof(uint32_t), s));                 /* big, huge, giant */
+        CK(hipMemsetAsync(&cnt->pk_over, 0, sizeof(uint32_t), s));
+        CK(hipMemsetAsync(&cnt->pk_used, 0, sizeof(unsigned long long), s));
         CK(hipEventRecord(c->ev[10], s));
-        hipLaunchKernelGGL(k_pair_count, dim3(pgrid), dim3(256), 0, s, pg, c->pcount.p, c->amb.p,
-                           (uint32_t)std::min<size_t>(c->amb.cap, 0xFFFFFFFFu), cnt);
+        hipLaunchKernelGGL(k_pair_collect, dim3(pgrid), dim3(256), 0, s, pg, c->pcount.p, c->pkrun.p, c->psrc.p,
+                           (uint64_t)std::min<size_t>(c->psrc.cap, 0xFFFFFFFFu), c->amb.p,
+                           (uint32_t)std::min<size_t>(c->amb.cap, 0xFFFFFFFFu), c->spill_big.p, c->spill_huge.p, cnt);
         CK(hipGetLastError());
         CK(hipEventRecord(c->ev[11], s));
         CKRC(sync_counters(c));
         h = *c->hcnt;
-        if (h.ambiguous <= c->amb.cap) break;
-        if (attempt >= 2 || c->amb.ensure(h.ambiguous + 1024))
-            return fail(c, STC_E_CAPACITY, "ambiguous list allocation failed");
+        bool again = false;
+        if (h.ambiguous > c->amb.cap) {
+            if (c->amb.ensure(h.ambiguous + 1024)) return fail(c, STC_E_CAPACITY, "ambiguous list allocation failed");
+            again = true;
+        }
+        if (h.pk_over) { /* the keys did not fit: grow to what the waves reserved, with room */
+            if (h.pk_used >= 0xFFFFFFFFull) return fail(c, STC_E_CAPACITY, "more than 2^32 - 1 pair sources in one step");
+            if (c->psrc.ensure(h.pk_used)) return fail(c, STC_E_CAPACITY, "pair sources");
+            again = true;
+        }
+        if (!again) break;
+        if (attempt >= 2) return fail(c, STC_E_CAPACITY, "pair sources or ambiguous list still short after growing");
     }
     const uint64_t total = h.pairs, n_amb = h.ambiguous;
     if (total >= 0xFFFFFFFFull) return fail(c, STC_E_CAPACITY, "more than 2^32 - 2 pairs in one step");
@@ -2376,28 +2490,31 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
     const int nx = c->cur ^ 1;
     CKRC(stc_exclusive_scan_u32(c->pcount.p, c->moff[nx].p, n, c->moff[nx].p + n, c->temp.p, s));
     wait_readers(c, nx, false); /* queries of an earlier state may still read meet[nx] */
-    if (c->ptmp.ensure(total + 1) || c->pnew.ensure(total + 1) || c->meet[nx].ensure(total + 1))
+    if (c->pnew.ensure(total + 1) || c->meet[nx].ensure(total + 1))
         return fail(c, STC_E_CAPACITY, "pair and meeting arrays");
-    CK(hipMemsetAsync(&cnt->table_fail, 0, sizeof(uint32_t), s));
-    CK(hipMemsetAsync(&cnt->big, 0, 3 * sizeof(uint32_t), s)); /* big, huge, giant */
     CK(hipEventRecord(c->ev[6], s));
-    /* the write pass runs faster with up to four times the count pass's workgroups (measured at 10M
-     * objects; the count pass does not); both counts are multiples of 8 for the XCD order */
+    /* the place pass runs faster with up to four times the collect pass's workgroups (measured at 10M
+     * objects); both counts are multiples of 8 for the XCD order */
     const uint32_t wgrid = (uint32_t)std::min<uint64_t>((blocks_for((uint64_t)n * STC_WAVE, 256) + 7) & ~7ull, 65536u);
-    hipLaunchKernelGGL(k_pair_write, dim3(wgrid), dim3(256), 0, s, pg, (const uint32_t *)c->moff[nx].p,
-                       (const uint32_t *)c->pcount.p, c->acc.p, (uint32_t)accepted.size(), c->ptmp.p, c->pnew.p,
-                       c->spill_big.p, c->spill_huge.p, cnt);
+    hipLaunchKernelGGL(k_pair_place, dim3(wgrid), dim3(256), 0, s, pg, (const uint32_t *)c->moff[nx].p,
+                       (const uint2 *)c->pkrun.p, (const uint32_t *)c->psrc.p, c->acc.p,
+                       (uint32_t)accepted.size(), c->pnew.p);
     CK(hipGetLastError());
     CK(hipEventRecord(c->ev[7], s));
-    CKRC(sync_counters(c));
-    h = *c->hcnt;
-    if (h.big) { /* segments longer than the write pass sorts: by waves, by blocks, in global scratch */
+    if (h.big) { /* segments longer than the place pass sorts: by waves, by blocks, in global scratch */
         hipLaunchKernelGGL(k_seg_sort_wave, dim3(blocks_for((uint64_t)h.big * STC_WAVE, 256)), dim3(256), 0, s, pg,
-                           (const uint32_t *)c->spill_big.p, h.big, (const uint32_t *)c->moff[nx].p, c->ptmp.p,
-                           c->pnew.p);
+                           (const uint32_t *)c->spill_big.p, h.big, (const uint32_t *)c->moff[nx].p,
+                           (const uint2 *)c->pkrun.p, (const uint32_t *)c->psrc.p, c->acc.p,
+                           (uint32_t)accepted.size(), c->pnew.p);
         CK(hipGetLastError());
     }
-    if (h.huge) {
+    if (h.huge) { /* staged at their segments' places: room for all slots, allocated only when needed */
+        if (c->ptmp.ensure(total + 1)) return fail(c, STC_E_CAPACITY, "long pair segments");
+        hipLaunchKernelGGL(k_huge_expand, dim3(blocks_for((uint64_t)h.huge * STC_WAVE, 256)), dim3(256), 0, s, pg,
+                           (const uint32_t *)c->spill_huge.p, h.huge, (const uint32_t *)c->moff[nx].p,
+                           (const uint2 *)c->pkrun.p, (const uint32_t *)c->psrc.p, c->acc.p,
+                           (uint32_t)accepted.size(), c->ptmp.p);
+        CK(hipGetLastError());
         hipLaunchKernelGGL((k_seg_sort<256, SEG_CAP, false>), dim3(std::min<uint32_t>(h.huge, 4096)), dim3(256), 0, s,
                            (const uint32_t *)c->spill_huge.p, h.huge, (const uint32_t *)c->moff[nx].p,
                            (const PairHit *)c->ptmp.p, c->pnew.p, c->spill_giant.p, &cnt->giant,
@@ -2445,7 +2562,7 @@ static int step_tail(StcCtx *c, uint32_t t, StepGeom &g, stc_step_out *out) {
     const MergeArgs ma{t, c->cfg.min_meet_time, c->cfg.max_meet_time, c->end_time_min, c->map};
     CKRC(sync_counters(c));
     Counters h = *c->hcnt;
-    if (h.table_fail) return fail(c, STC_E_HIP, "the pair write pass found more pairs than its count");
+    if (h.table_fail) return fail(c, STC_E_HIP, "a pair segment was longer than its sort's scratch");
     const Counters before = h;
     /* tiles of the step's pairs and their ranges in the previous meetings */
     const uint32_t n_tiles = (uint32_t)std::max<uint64_t>((c->plen + MTILE - 1) / MTILE, 1);
