@@ -77,10 +77,27 @@ int stc_hip_fail(hipError_t e, const char *expr, const char *file, int line) {
 
 struct FrontierItem { uint32_t node, code; };
 
-struct PairHit {              /* 24 B: key and the float MBR of the two positions */
+struct PairHit {              /* 24 B: key and the float MBR of the two positions (replayed meeting units) */
     unsigned long long key;
     float mbr[4];
 };
+
+/* 16 B: key and the partner's position as floats. The owner's is posf[owner],
+ * and the pair's MBR is the box of the two (the box of their float
+ * conversions, as the reference's f_box of the two points). */
+struct __align__(16) PairSlot {
+    unsigned long long key;
+    float px, py;
+};
+
+/* the ordered-int MBR of a pair slot (f2o of low0, low1, high0, high1) */
+__device__ inline uint4 slot_mbr(const PairHit &e, const float2 *) {
+    return make_uint4(f2o(e.mbr[0]), f2o(e.mbr[1]), f2o(e.mbr[2]), f2o(e.mbr[3]));
+}
+__device__ inline uint4 slot_mbr(const PairSlot &e, const float2 *posf) {
+    const float2 o = posf[(uint32_t)(e.key >> 32)];
+    return make_uint4(f2o(fminf(o.x, e.px)), f2o(fminf(o.y, e.py)), f2o(fmaxf(o.x, e.px)), f2o(fmaxf(o.y, e.py)));
+}
 
 struct Counters {             /* device-side counters, copied back once per sync point */
     unsigned long long units;
@@ -128,7 +145,7 @@ __device__ inline uint32_t compact1by1(uint32_t x) {
 /* ------------------------------------------------------------------ */
 
 __global__ void k_prep(const double2 *__restrict__ xy, uint32_t n, MapBox m, uint32_t *__restrict__ codes,
-                       uint32_t *__restrict__ idx, double *__restrict__ dlon, double *__restrict__ idlon,
+                       uint32_t *__restrict__ idx, double *__restrict__ dlon, float2 *__restrict__ posf,
                        Counters *cnt) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -143,9 +160,8 @@ __global__ void k_prep(const double2 *__restrict__ xy, uint32_t n, MapBox m, uin
     }
     codes[i] = code;
     idx[i] = i;
-    double d = stc_dpm_lon(p.y);
-    dlon[i] = d;
-    idlon[i] = 1.0 / d;
+    dlon[i] = stc_dpm_lon(p.y);
+    if (posf) posf[i] = make_float2((float)p.x, (float)p.y);
 }
 
 /* box of block (x, y) with xdim blocks per side (geometry.cu:1306-1311) */
@@ -472,7 +488,7 @@ struct __align__(32) RevPt {
 
 struct PairGeo {
     const double2 *xy;
-    const double *dlon, *idlon;
+    const double *dlon;
     const double2 *lxy;                  /* positions in leaf order (one 16-byte read per candidate) */
     const uint32_t *pt_leaf, *pt_rank, *leaf_start, *leaf_count, *lp;
     const double *ld;                    /* dpm_lon in leaf order */
@@ -559,7 +575,7 @@ __device__ inline void owner_combos_by_source(uint32_t w, uint32_t a, const Pair
     if (s > g.buf) s = g.buf;
     const uint32_t st = g.leaf_start[L], rank = w - st;
     const double2 pa = g.lxy[w];
-    const double pda = g.ld[w], pia = 1.0 / pda; /* the values of dlon[a] and idlon[a] (k_prep) */
+    const double pda = g.ld[w], pia = 1.0 / pda; /* dlon[a] (k_prep) and its reciprocal */
     const float afx = (float)pa.x, afy = (float)pa.y;
     /* (i) */
     for (uint32_t m0 = rank + 1; m0 < s; m0 += STC_WAVE) {
@@ -656,7 +672,7 @@ __device__ inline void owner_combos(uint32_t w, uint32_t a, const PairGeo &g, in
     }
     const uint4 pr = g.prange[w];
     const double2 pa = g.lxy[w];
-    const double pda = g.ld[w], pia = 1.0 / pda; /* the values of dlon[a] and idlon[a] (k_prep) */
+    const double pda = g.ld[w], pia = 1.0 / pda; /* dlon[a] (k_prep) and its reciprocal */
     const float afx = (float)pa.x, afy = (float)pa.y;
     /* source table: first index (into the leaf-ordered arrays, or rev_pid for
      * the last source) and length, held by lane j for source j */
@@ -745,7 +761,7 @@ __global__ void k_rev_gather(const uint32_t *__restrict__ rev_pid, uint32_t nr, 
     RevPt r;
     r.x = q.x;
     r.y = q.y;
-    r.il = 1.0 / d; /* idlon[p], computed as k_prep computes it instead of a third gather */
+    r.il = 1.0 / d; /* the reciprocal of dlon[p], as the owners' loops compute theirs */
     r.pid = p;
     r.pad = 0;
     rev[i] = r;
@@ -981,7 +997,7 @@ __device__ inline unsigned long long settle_key(uint32_t src, uint32_t a, const 
 template <int K>
 __device__ inline void write_sorted_segment(const uint32_t *srcs, uint32_t slots, uint32_t a, uint32_t w,
                                             const PairGeo &g, const unsigned long long *acc, uint32_t n_acc,
-                                            PairHit *__restrict__ seg, int lane) {
+                                            PairSlot *__restrict__ seg, int lane) {
     const uint64_t lt = lanemask_lt(lane);
     unsigned long long x[K];
     uint32_t sv[K];
@@ -1005,7 +1021,6 @@ __device__ inline void write_sorted_segment(const uint32_t *srcs, uint32_t slots
         live[r] = i < slots && b != DEAD_B && (i == 0 || (uint32_t)(prev >> 32) != b);
         nlive += (uint32_t)__popcll(__ballot(live[r]));
     }
-    const float afx = (float)g.lxy[w].x, afy = (float)g.lxy[w].y;
     uint32_t lr = 0, dr = 0;
     /* the sources' positions are loaded PB slots at a time, all of a group's loads issued before
      * the first wait (a lane past the end reads the owner's own position) */
@@ -1028,21 +1043,13 @@ __device__ inline void write_sorted_segment(const uint32_t *srcs, uint32_t slots
             const bool valid = i < slots;
             const uint64_t ml = __ballot(live[r]), md = __ballot(valid && !live[r]);
             if (valid) {
-                /* found by a border point: its position first, as pass 1 forms it */
-                const bool rev = ((uint32_t)x[r] & SRC_REV) != 0;
+                /* the key and the partner's position (the source's), streamed out past the caches:
+                 * the merge reads the slots once, a step's worth of traffic later */
+                const uint32_t partner = live[r] ? (uint32_t)(x[r] >> 32) : DEAD_B;
                 const float px = (float)pos[b].x, py = (float)pos[b].y;
-                const float4 mb = rev ? pair_mbr(px, py, afx, afy) : pair_mbr(afx, afy, px, py);
-                PairHit e;
-                e.key = ((unsigned long long)a << 32) | (live[r] ? (uint32_t)(x[r] >> 32) : DEAD_B);
-                e.mbr[0] = mb.x; e.mbr[1] = mb.y; e.mbr[2] = mb.z; e.mbr[3] = mb.w;
-                /* streamed out past the caches: the merge reads the slots once, a step's worth of
-                 * traffic later */
-                PairHit *d = seg + (live[r] ? lr + (uint32_t)__popcll(ml & lt) : nlive + dr + (uint32_t)__popcll(md & lt));
-                __builtin_nontemporal_store(e.key, &d->key);
-                __builtin_nontemporal_store(mb.x, &d->mbr[0]);
-                __builtin_nontemporal_store(mb.y, &d->mbr[1]);
-                __builtin_nontemporal_store(mb.z, &d->mbr[2]);
-                __builtin_nontemporal_store(mb.w, &d->mbr[3]);
+                const u32x4 v = {partner, a, __float_as_uint(px), __float_as_uint(py)};
+                __builtin_nontemporal_store(v, (u32x4 *)(seg + (live[r] ? lr + (uint32_t)__popcll(ml & lt)
+                                                                        : nlive + dr + (uint32_t)__popcll(md & lt))));
             }
             lr += (uint32_t)__popcll(ml);
             dr += (uint32_t)__popcll(md);
@@ -1058,7 +1065,7 @@ __global__ __launch_bounds__(256, 5) void k_seg_sort_wave(PairGeo g, const uint3
                                                        const uint2 *__restrict__ pkrun,
                                                        const uint32_t *__restrict__ psrc,
                                                        const unsigned long long *__restrict__ acc, uint32_t n_acc,
-                                                       PairHit *__restrict__ pn) {
+                                                       PairSlot *__restrict__ pn) {
     const int lane = threadIdx.x & (STC_WAVE - 1);
     const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / STC_WAVE;
     const uint32_t nw = (gridDim.x * blockDim.x) / STC_WAVE;
@@ -1070,13 +1077,13 @@ __global__ __launch_bounds__(256, 5) void k_seg_sort_wave(PairGeo g, const uint3
     }
 }
 
-/* Segments longer than WSORT_CAP slots: their collected keys as pair slots
- * (key, MBR from the source's position), unsorted, at the segment's place in
+/* Segments longer than WSORT_CAP slots: their collected sources as pair
+ * slots (key, the partner's position), unsorted, at the segment's place in
  * ptmp for the block sorts (k_seg_sort). One wave per owner. */
 __global__ void k_huge_expand(PairGeo g, const uint32_t *__restrict__ owners, uint32_t n_owners,
                               const uint32_t *__restrict__ poff, const uint2 *__restrict__ pkrun,
                               const uint32_t *__restrict__ psrc, const unsigned long long *__restrict__ acc,
-                              uint32_t n_acc, PairHit *__restrict__ ptmp) {
+                              uint32_t n_acc, PairSlot *__restrict__ ptmp) {
     const int lane = threadIdx.x & (STC_WAVE - 1);
     const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / STC_WAVE;
     const uint32_t nw = (gridDim.x * blockDim.x) / STC_WAVE;
@@ -1085,17 +1092,14 @@ __global__ void k_huge_expand(PairGeo g, const uint32_t *__restrict__ owners, ui
         const uint32_t base = poff[a], slots = poff[a + 1] - base;
         const uint32_t w = g.leaf_start[g.pt_leaf[a]] + g.pt_rank[a];
         const uint32_t *srcs = psrc + pkrun[w].x;
-        const float afx = (float)g.lxy[w].x, afy = (float)g.lxy[w].y;
         for (uint32_t i = lane; i < slots; i += STC_WAVE) {
             const unsigned long long k = settle_key(srcs[i], a, g, acc, n_acc);
             const uint32_t src = (uint32_t)k;
-            const bool rev = (src & SRC_REV) != 0;
-            const double2 pos = rev ? *(const double2 *)(g.rev + (src & ~SRC_REV)) : g.lxy[src];
-            const float px = (float)pos.x, py = (float)pos.y;
-            const float4 mb = rev ? pair_mbr(px, py, afx, afy) : pair_mbr(afx, afy, px, py);
-            PairHit h;
+            const double2 pos = (src & SRC_REV) ? *(const double2 *)(g.rev + (src & ~SRC_REV)) : g.lxy[src];
+            PairSlot h;
             h.key = ((unsigned long long)a << 32) | (uint32_t)(k >> 32);
-            h.mbr[0] = mb.x; h.mbr[1] = mb.y; h.mbr[2] = mb.z; h.mbr[3] = mb.w;
+            h.px = (float)pos.x;
+            h.py = (float)pos.y;
             ptmp[base + i] = h;
         }
     }
@@ -1110,7 +1114,7 @@ __global__ __launch_bounds__(256, 7) void k_pair_place(PairGeo g, const uint32_t
                                                     const uint2 *__restrict__ pkrun,
                                                     const uint32_t *__restrict__ psrc,
                                                     const unsigned long long *__restrict__ acc, uint32_t n_acc,
-                                                    PairHit *__restrict__ pn) {
+                                                    PairSlot *__restrict__ pn) {
     const int lane = threadIdx.x & (STC_WAVE - 1);
     const uint64_t wid = (uint32_t)__builtin_amdgcn_readfirstlane((int)((xcd_block_id() * blockDim.x + threadIdx.x) / STC_WAVE));
     const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) / STC_WAVE;
@@ -1120,7 +1124,7 @@ __global__ __launch_bounds__(256, 7) void k_pair_place(PairGeo g, const uint32_t
         if (slots == 0 || slots > WCAP) continue;
         const uint32_t a = (uint32_t)__builtin_amdgcn_readfirstlane((int)g.lp[w]);
         const uint32_t *keys = psrc + (uint32_t)__builtin_amdgcn_readfirstlane((int)run.x);
-        PairHit *seg = pn + poff[a];
+        PairSlot *seg = pn + poff[a];
         if (slots <= STC_WAVE) write_sorted_segment<1>(keys, slots, a, (uint32_t)w, g, acc, n_acc, seg, lane);
         else if (slots <= 2 * STC_WAVE) write_sorted_segment<2>(keys, slots, a, (uint32_t)w, g, acc, n_acc, seg, lane);
         else if (slots <= 4 * STC_WAVE) write_sorted_segment<4>(keys, slots, a, (uint32_t)w, g, acc, n_acc, seg, lane);
@@ -1288,8 +1292,8 @@ __device__ inline uint32_t block_rank(bool f, uint32_t *wsum, uint32_t *total) {
  * segments go to `spill` (or set table_fail when spill is null). */
 template <int THREADS, int LCAP, bool GLOBAL>
 __global__ __launch_bounds__(THREADS) void k_seg_sort(const uint32_t *__restrict__ owners, uint32_t n_owners,
-                                                      const uint32_t *__restrict__ poff, const PairHit *__restrict__ ptmp,
-                                                      PairHit *__restrict__ pn, uint32_t *__restrict__ spill,
+                                                      const uint32_t *__restrict__ poff, const PairSlot *__restrict__ ptmp,
+                                                      PairSlot *__restrict__ pn, uint32_t *__restrict__ spill,
                                                       uint32_t *spill_count, unsigned long long *gsrt, uint32_t gcap,
                                                       Counters *cnt) {
     __shared__ unsigned long long lsrt[GLOBAL ? 1 : LCAP];
@@ -1337,7 +1341,7 @@ __global__ __launch_bounds__(THREADS) void k_seg_sort(const uint32_t *__restrict
             const uint32_t rl = block_rank<THREADS>(live, wsum, &tl);
             const uint32_t rd = block_rank<THREADS>(k < slots && !live, wsum, &td);
             if (k < slots) {
-                PairHit e = ptmp[base + (uint32_t)keys[k]];
+                PairSlot e = ptmp[base + (uint32_t)keys[k]];
                 uint32_t dest;
                 if (live) {
                     dest = lr + rl;
@@ -1374,7 +1378,8 @@ __device__ inline void flush_stage(const Bucket *stage, uint32_t staged, uint32_
  * the owner's new slots, scaled to the owner's previous segment, puts it (the
  * two segments mostly hold the same partners) and widens by doubling steps
  * until it brackets the key, so a few dependent reads usually suffice. */
-__global__ void k_tile_bounds(const PairHit *__restrict__ pn, uint64_t n_p, const Bucket *__restrict__ mold,
+template <class S>
+__global__ void k_tile_bounds(const S *__restrict__ pn, uint64_t n_p, const Bucket *__restrict__ mold,
                               const uint32_t *__restrict__ ooff, const uint32_t *__restrict__ poff, uint64_t n_o,
                               uint32_t n_tiles, uint32_t *__restrict__ tb) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1459,10 +1464,11 @@ __device__ inline uint32_t lower_bound_bucket(const Bucket *s, uint32_t n, unsig
 #define OPF (OCAP / STC_WAVE)   /* previous meetings per lane loaded at a tile's start */
 #define SPF (MTILE / STC_WAVE)  /* pair slots per lane loaded at a tile's start */
 
-__global__ __launch_bounds__(256) void k_meet_merge(const PairHit *__restrict__ pn, uint64_t n_p,
+template <class S>
+__global__ __launch_bounds__(256) void k_meet_merge(const S *__restrict__ pn, uint64_t n_p,
                                                     const Bucket *__restrict__ mold, const uint32_t *__restrict__ tb,
                                                     uint32_t n_tiles, Bucket *__restrict__ mnew, MergeArgs ma,
-                                                    MemtableDev mt, Counters *cnt) {
+                                                    MemtableDev mt, const float2 *__restrict__ posf, Counters *cnt) {
     __shared__ unsigned long long wpk[256 / STC_WAVE][MTILE];
     __shared__ Bucket wob[256 / STC_WAVE][OCAP];
     __shared__ Bucket wstage[256 / STC_WAVE][MSTAGE];
@@ -1489,7 +1495,7 @@ __global__ __launch_bounds__(256) void k_meet_merge(const PairHit *__restrict__ 
         const uint32_t np = i0 < n_p ? (uint32_t)min<uint64_t>(MTILE, n_p - i0) : 0u;
         /* all loads of the tile are issued before the first wait; indices are clamped into the
          * arrays (each holds at least one entry) and lanes past the tile's ends ignore theirs */
-        PairHit ps[SPF];
+        S ps[SPF];
         Bucket po[OPF];
 #pragma unroll
         for (int k = 0; k < SPF; k++) ps[k] = pn[min<uint64_t>(i0 + lane + k * STC_WAVE, plast)];
@@ -1516,12 +1522,13 @@ __global__ __launch_bounds__(256) void k_meet_merge(const PairHit *__restrict__ 
             for (int k = 0; k < SPF; k++) {
                 const uint32_t i = lane + k * STC_WAVE;
                 const bool valid = i < np;
-                const PairHit e = ps[k];
+                const S e = ps[k];
                 const unsigned long long key = e.key;
                 pk[i] = key;
                 const bool dead = (uint32_t)key == DEAD_B;
-                const uint32_t q0 = f2o(e.mbr[0]), q1 = f2o(e.mbr[1]), q2 = f2o(e.mbr[2]), q3 = f2o(e.mbr[3]);
                 const uint32_t j = lower_bound_bucket(obk, no, key);
+                const uint4 q = slot_mbr(e, posf);
+                const uint32_t q0 = q.x, q1 = q.y, q2 = q.z, q3 = q.w;
                 const Bucket ob = obk[j < no ? j : (no ? no - 1 : 0u)];
                 const bool cont = j < no && ob.key == key;
                 Bucket nb;
@@ -1814,7 +1821,8 @@ struct StcCtx {
 
     DBuf<double2> xy_in;
     DBuf<uint32_t> codes, idx, codes_alt, idx_alt, temp;
-    DBuf<double> dlon, idlon, ld;
+    DBuf<double> dlon, ld;
+    DBuf<float2> posf;                   /* positions by id as floats: the owners' side of the pair slots */
     DBuf<double2> lxy;
     DBuf<Node> nodes;
     DBuf<FrontierItem> frontier_a, frontier_b;
@@ -1835,7 +1843,9 @@ struct StcCtx {
     /* the step's pairs by owner segment, and the live meetings: two arrays of
      * segments (offsets [n + 1], live buckets per segment [n]) used in turn */
     DBuf<uint32_t> pcount;
-    DBuf<PairHit> ptmp, pnew;            /* the long segments' pair slots as found, and all slots sorted by key */
+    DBuf<PairSlot> ptmp, pnew;           /* the long segments' pair slots as found, and all slots sorted by key */
+    DBuf<PairHit> pbox;                  /* a replayed step's pairs (key and MBR), sorted by key */
+    bool pairs_boxed = false;            /* the last step's pairs are in pbox (replayed units), not pnew */
     DBuf<uint32_t> psrc;                 /* collected pair sources, a run per owner */
     DBuf<uint2> pkrun;                   /* per owner in leaf order: its run of psrc (start, length) */
     DBuf<Bucket> meet[2];
@@ -2002,7 +2012,22 @@ static int ilog2_ceil(uint64_t v) {
 /* the last step's pairs (its pair slots without the dead ones), sorted by key */
 static int step_pairs(StcCtx *c, std::vector<PairHit> &ph) {
     ph.resize(c->plen);
-    if (c->plen) CK(hipMemcpy(ph.data(), c->pnew.p, c->plen * sizeof(PairHit), hipMemcpyDeviceToHost));
+    if (c->pairs_boxed) {
+        if (c->plen) CK(hipMemcpy(ph.data(), c->pbox.p, c->plen * sizeof(PairHit), hipMemcpyDeviceToHost));
+    } else if (c->plen) { /* slots: the MBR is the box of the owner's and the partner's float positions */
+        std::vector<PairSlot> sl(c->plen);
+        std::vector<float2> pos(c->n);
+        CK(hipMemcpy(sl.data(), c->pnew.p, c->plen * sizeof(PairSlot), hipMemcpyDeviceToHost));
+        CK(hipMemcpy(pos.data(), c->posf.p, (size_t)c->n * sizeof(float2), hipMemcpyDeviceToHost));
+        for (uint64_t i = 0; i < c->plen; i++) {
+            const float2 o = pos[(uint32_t)(sl[i].key >> 32)];
+            ph[i].key = sl[i].key;
+            ph[i].mbr[0] = std::fmin(o.x, sl[i].px);
+            ph[i].mbr[1] = std::fmin(o.y, sl[i].py);
+            ph[i].mbr[2] = std::fmax(o.x, sl[i].px);
+            ph[i].mbr[3] = std::fmax(o.y, sl[i].py);
+        }
+    }
     ph.erase(std::remove_if(ph.begin(), ph.end(), [](const PairHit &e) { return (uint32_t)e.key == DEAD_B; }),
              ph.end());
     std::sort(ph.begin(), ph.end(), [](const PairHit &a, const PairHit &b) { return a.key < b.key; });
@@ -2165,7 +2190,7 @@ int stc_create(const stc_config *cfg, const stc_box *mbr, void **out) {
     }
     size_t front_cap = n / (c->cap + 1) + 2;
     if (c->xy_in.ensure(n) || c->codes.ensure(n) || c->idx.ensure(n) || c->codes_alt.ensure(n) ||
-        c->idx_alt.ensure(n) || c->dlon.ensure(n) || c->idlon.ensure(n) || c->lxy.ensure(n) || c->ld.ensure(n) ||
+        c->idx_alt.ensure(n) || c->dlon.ensure(n) || c->posf.ensure(n) || c->lxy.ensure(n) || c->ld.ensure(n) ||
         c->nodes.ensure(node_cap) || c->frontier_a.ensure(front_cap) || c->frontier_b.ensure(front_cap) ||
         c->pt_leaf.ensure(n) || c->pt_lv.ensure(n) || c->pt_rank.ensure(n) || c->lp.ensure(n) ||
         c->lleaf.ensure(n) || c->lunits.ensure(n) || c->prange.ensure(n) ||
@@ -2204,7 +2229,7 @@ int stc_destroy(void *vc) {
     wait_all_readers(c);
     if (c->s) hipStreamSynchronize(c->s);
     c->xy_in.release(); c->codes.release(); c->idx.release(); c->codes_alt.release(); c->idx_alt.release();
-    c->temp.release(); c->dlon.release(); c->idlon.release(); c->lxy.release(); c->ld.release(); c->nodes.release();
+    c->temp.release(); c->dlon.release(); c->posf.release(); c->lxy.release(); c->ld.release(); c->nodes.release();
     c->frontier_a.release(); c->frontier_b.release(); c->leaf_count.release(); c->leaf_start.release();
     c->leaf_node.release(); c->pt_leaf.release(); c->pt_lv.release(); c->pt_rank.release(); c->lp.release();
     c->lleaf.release(); c->lunits.release(); c->urange.release(); c->prange.release();
@@ -2312,7 +2337,7 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
 
     /* 1-2: cells, Morton codes, dpm_lon; sort codes */
     hipLaunchKernelGGL(k_prep, dim3(blocks_for(n, 256)), dim3(256), 0, s, xy, n, c->map, c->codes.p, c->idx.p,
-                       c->dlon.p, c->idlon.p, cnt);
+                       c->dlon.p, c->posf.p, cnt);
     CK(hipGetLastError());
     CKRC(stc_radix_sort_u32(c->codes.p, c->idx.p, c->codes_alt.p, c->idx_alt.p, n, 28, c->temp.p, s));
 
@@ -2420,7 +2445,7 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
                        (const uint32_t *)c->leaf_count.p, (const uint32_t *)c->rev_start.p,
                        (const uint32_t *)c->rev_pid.p, c->buf, c->prange.p);
     CK(hipGetLastError());
-    const PairGeo pg{xy, c->dlon.p, c->idlon.p, c->lxy.p, c->pt_leaf.p, c->pt_rank.p, c->leaf_start.p,
+    const PairGeo pg{xy, c->dlon.p, c->lxy.p, c->pt_leaf.p, c->pt_rank.p, c->leaf_start.p,
                      c->leaf_count.p, c->lp.p, c->ld.p, c->lleaf.p, c->lunits.p, c->urange.p, c->prange.p,
                      c->units.p, c->uoffset.p,
                      c->ucount.p, c->rev_start.p, c->rev_pid.p,
@@ -2517,7 +2542,7 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
         CK(hipGetLastError());
         hipLaunchKernelGGL((k_seg_sort<256, SEG_CAP, false>), dim3(std::min<uint32_t>(h.huge, 4096)), dim3(256), 0, s,
                            (const uint32_t *)c->spill_huge.p, h.huge, (const uint32_t *)c->moff[nx].p,
-                           (const PairHit *)c->ptmp.p, c->pnew.p, c->spill_giant.p, &cnt->giant,
+                           (const PairSlot *)c->ptmp.p, c->pnew.p, c->spill_giant.p, &cnt->giant,
                            (unsigned long long *)nullptr, 0u, cnt);
         CK(hipGetLastError());
         CKRC(sync_counters(c));
@@ -2536,11 +2561,12 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
         const uint32_t hgrid = std::min<uint32_t>(h.giant, 64);
         if (c->gsrt.ensure((size_t)gcap * hgrid)) return fail(c, STC_E_CAPACITY, "sort scratch of a large segment");
         hipLaunchKernelGGL((k_seg_sort<256, 1, true>), dim3(hgrid), dim3(256), 0, s, (const uint32_t *)c->spill_giant.p,
-                           h.giant, (const uint32_t *)c->moff[nx].p, (const PairHit *)c->ptmp.p, c->pnew.p,
+                           h.giant, (const uint32_t *)c->moff[nx].p, (const PairSlot *)c->ptmp.p, c->pnew.p,
                            (uint32_t *)nullptr, (uint32_t *)nullptr, c->gsrt.p, gcap, cnt);
         CK(hipGetLastError());
     }
     c->plen = total;
+    c->pairs_boxed = false;
     CK(hipEventRecord(c->ev[3], s));
     StepGeom g{n_nodes, n_leaves, n_units, 0, n_amb, 0};
     return step_tail(c, t, g, out);
@@ -2567,10 +2593,16 @@ static int step_tail(StcCtx *c, uint32_t t, StepGeom &g, stc_step_out *out) {
     /* tiles of the step's pairs and their ranges in the previous meetings */
     const uint32_t n_tiles = (uint32_t)std::max<uint64_t>((c->plen + MTILE - 1) / MTILE, 1);
     if (c->tbounds.ensure(n_tiles + 1ull)) return fail(c, STC_E_CAPACITY, "merge tile bounds");
-    hipLaunchKernelGGL(k_tile_bounds, dim3(blocks_for(n_tiles + 1ull, 256)), dim3(256), 0, s,
-                       (const PairHit *)c->pnew.p, c->plen, (const Bucket *)c->meet[cu].p,
-                       (const uint32_t *)c->moff[cu].p, (const uint32_t *)c->moff[nx].p, c->mlen, n_tiles,
-                       c->tbounds.p);
+    if (c->pairs_boxed)
+        hipLaunchKernelGGL(k_tile_bounds<PairHit>, dim3(blocks_for(n_tiles + 1ull, 256)), dim3(256), 0, s,
+                           (const PairHit *)c->pbox.p, c->plen, (const Bucket *)c->meet[cu].p,
+                           (const uint32_t *)c->moff[cu].p, (const uint32_t *)c->moff[nx].p, c->mlen, n_tiles,
+                           c->tbounds.p);
+    else
+        hipLaunchKernelGGL(k_tile_bounds<PairSlot>, dim3(blocks_for(n_tiles + 1ull, 256)), dim3(256), 0, s,
+                           (const PairSlot *)c->pnew.p, c->plen, (const Bucket *)c->meet[cu].p,
+                           (const uint32_t *)c->moff[cu].p, (const uint32_t *)c->moff[nx].p, c->mlen, n_tiles,
+                           c->tbounds.p);
     CK(hipGetLastError());
     const uint32_t mgrid = std::max<uint32_t>(std::min<uint32_t>((n_tiles + 3) / 4, 65536), 1);
     /* the merge writes meet[nx] and, after a flush emptied the memtable, records from index 0 */
@@ -2584,9 +2616,14 @@ static int step_tail(StcCtx *c, uint32_t t, StepGeom &g, stc_step_out *out) {
         CK(hipMemcpyAsync(cnt, c->hinit, sizeof(Counters), hipMemcpyHostToDevice, s));
         const MemtableDev mt{c->rec_keys, c->rec_boxes, c->rec_cap, c->same_pid_count, c->o_boxs, c->sids};
         CK(hipEventRecord(c->ev[8], s));
-        hipLaunchKernelGGL(k_meet_merge, dim3(mgrid), dim3(256), 0, s, (const PairHit *)c->pnew.p, c->plen,
-                           (const Bucket *)c->meet[cu].p, (const uint32_t *)c->tbounds.p, n_tiles, c->meet[nx].p, ma,
-                           mt, cnt);
+        if (c->pairs_boxed)
+            hipLaunchKernelGGL(k_meet_merge<PairHit>, dim3(mgrid), dim3(256), 0, s, (const PairHit *)c->pbox.p,
+                               c->plen, (const Bucket *)c->meet[cu].p, (const uint32_t *)c->tbounds.p, n_tiles,
+                               c->meet[nx].p, ma, mt, (const float2 *)nullptr, cnt);
+        else
+            hipLaunchKernelGGL(k_meet_merge<PairSlot>, dim3(mgrid), dim3(256), 0, s, (const PairSlot *)c->pnew.p,
+                               c->plen, (const Bucket *)c->meet[cu].p, (const uint32_t *)c->tbounds.p, n_tiles,
+                               c->meet[nx].p, ma, mt, (const float2 *)c->posf.p, cnt);
         CK(hipGetLastError());
         CK(hipEventRecord(c->ev[9], s));
         CKRC(sync_counters(c));
@@ -2750,7 +2787,7 @@ int stc_step_meetings(void *vc, const stc_meeting_unit *units, uint64_t nu64, ui
     CK(hipEventRecord(c->ev[2], s));
     if (c->units_in.ensure(nu + 1ull) || c->ukeys.ensure(nu + 1ull) || c->ukeys_alt.ensure(nu + 1ull) ||
         c->uidx.ensure(nu + 1ull) || c->uidx_alt.ensure(nu + 1ull) ||
-        c->utemp.ensure(stc_radix_temp_words(std::max<uint32_t>(nu + 1, c->n)) + 64) || c->pnew.ensure(nu + 1ull) ||
+        c->utemp.ensure(stc_radix_temp_words(std::max<uint32_t>(nu + 1, c->n)) + 64) || c->pbox.ensure(nu + 1ull) ||
         (wait_readers(c, c->cur ^ 1, false), c->meet[c->cur ^ 1].ensure(nu + 1ull)))
         return fail(c, STC_E_CAPACITY, "meeting unit buffers");
     if (nu) {
@@ -2761,7 +2798,7 @@ int stc_step_meetings(void *vc, const stc_meeting_unit *units, uint64_t nu64, ui
         CKRC(stc_radix_sort_u64(c->ukeys.p, c->uidx.p, c->ukeys_alt.p, c->uidx_alt.p, nu, 0,
                                 32 + std::max(1, ilog2_ceil(c->n)), c->utemp.p, s));
         hipLaunchKernelGGL(k_units_gather, dim3(blocks_for(nu, 256)), dim3(256), 0, s, c->ukeys.p, c->uidx.p,
-                           c->units_in.p, nu, c->pnew.p, cnt);
+                           c->units_in.p, nu, c->pbox.p, cnt);
         CK(hipGetLastError());
     }
     CKRC(sync_counters(c));
@@ -2773,13 +2810,14 @@ int stc_step_meetings(void *vc, const stc_meeting_unit *units, uint64_t nu64, ui
     CK(hipEventRecord(c->ev[6], s));
     CK(hipMemsetAsync(c->pcount.p, 0, 4ull * c->n, s));
     if (nu) {
-        hipLaunchKernelGGL(k_owner_hist, dim3(blocks_for(nu, 256)), dim3(256), 0, s, (const PairHit *)c->pnew.p, nu,
+        hipLaunchKernelGGL(k_owner_hist, dim3(blocks_for(nu, 256)), dim3(256), 0, s, (const PairHit *)c->pbox.p, nu,
                            c->pcount.p);
         CK(hipGetLastError());
     }
     CKRC(stc_exclusive_scan_u32(c->pcount.p, c->moff[nx].p, c->n, c->moff[nx].p + c->n, c->utemp.p, s));
     CK(hipEventRecord(c->ev[7], s));
     c->plen = nu;
+    c->pairs_boxed = true;
     CK(hipEventRecord(c->ev[3], s));
     StepGeom g{0, 0, 0, 0, 0, 0};
     int rc = step_tail(c, t, g, out);
@@ -2899,7 +2937,7 @@ int stc_build_schema(void *vc, const double *xy, int32_t xy_on_device) {
     }
     /* the points' Morton order keeps each wave of the build's counting on few nodes */
     hipLaunchKernelGGL(k_prep, dim3(blocks_for(c->n, 256)), dim3(256), 0, c->s, src, c->n, c->map, c->codes.p,
-                       c->idx.p, c->dlon.p, c->idlon.p, c->dcnt);
+                       c->idx.p, c->dlon.p, (float2 *)nullptr, c->dcnt); /* posf keeps the last step's positions */
     CK(hipGetLastError());
     CKRC(stc_radix_sort_u32(c->codes.p, c->idx.p, c->codes_alt.p, c->idx_alt.p, c->n, 28, c->temp.p, c->s));
     int rc = stc_schema_build(&c->sch, src, c->idx.p, c->n, c->map, c->cap, c->cfg.reach_distance, c->temp.p,
