@@ -20,6 +20,7 @@ bounded sample of the same workload; and a serving phase (5M objects at 1 Hz
 plus 2,000 queries/s with every hit materialised) reporting p50/p99.
 """
 import argparse
+import collections
 import ctypes
 import json
 import os
@@ -285,6 +286,82 @@ def serve(args, objects=0, schema="dynamic"):
     return out
 
 
+def batched(args):
+    """The batched-streams workload (c4): 64 independent streams (log-normal
+    sizes, median 50k objects, ~6M in total; each its own box, reach and leaf
+    capacity), one library context each. Each stream's leaf capacity is first
+    chosen by the tuner on the stream's own first frames; then the streams
+    step together through stc_step_batch (host threads, each context on its
+    own streams) and, for comparison, one after another as 64 plain sequential
+    runs would. Frames are generated on the device before timing."""
+    import torch
+    from stcdb_amd import DeviceTrace, Pipeline, batched_streams, step_batch
+    from stcdb_amd.tuner import CANDIDATES, tune_leaf_capacity
+    streams = batched_streams()
+    total = sum(w["cfg"]["num_objects"] for w in streams)
+    W, K = args.warmup, args.steps
+    nframes = W + 2 * K
+    h = hip()
+    frames = []
+    for w in streams:
+        n = w["cfg"]["num_objects"]
+        f = torch.empty((nframes, n, 2), dtype=torch.float64, device="cuda")
+        tr = DeviceTrace(w["gen"])
+        for t in range(nframes):
+            assert h.hipMemcpy(f[t].data_ptr(), tr.next(t).value, n * 16, 3) == 0
+        tr.close()
+        frames.append(f)
+    torch.cuda.synchronize()
+    print(f"[c4] {len(streams)} streams, {total} objects, {nframes} frames each resident", file=sys.stderr, flush=True)
+    ptr = lambda i, t: ctypes.c_void_p(frames[i][t].data_ptr())
+    t0 = time.perf_counter()
+    tuned = []
+    for i, w in enumerate(streams):
+        best, _ = tune_leaf_capacity(w["cfg"], w["box"], [ptr(i, t) for t in range(args.tune_steps)],
+                                     on_device=True)
+        tuned.append(best)
+    tune_s = time.perf_counter() - t0
+    print(f"[c4] tuned in {tune_s:.1f} s: capacities {dict(collections.Counter(tuned))}", file=sys.stderr, flush=True)
+    pipes = [Pipeline(dict(w["cfg"], grid_capacity=c), w["box"]) for w, c in zip(streams, tuned)]
+    for t in range(W):
+        step_batch(pipes, [ptr(i, t) for i in range(len(pipes))], t, threads=args.threads, on_device=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for t in range(W, W + K):  # one stream after another
+        for i, p in enumerate(pipes):
+            p.step_device(ptr(i, t), t)
+    torch.cuda.synchronize()
+    seq_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for t in range(W + K, W + 2 * K):  # all streams at once
+        step_batch(pipes, [ptr(i, t) for i in range(len(pipes))], t, threads=args.threads, on_device=True)
+    torch.cuda.synchronize()
+    batch_s = time.perf_counter() - t0
+    for p in pipes:
+        p.close()
+    line = {
+        "metric": "point-updates/s over 64 batched streams (c4)",
+        "value": round(total * K / batch_s, 1),
+        "unit": "point-updates/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": round(batch_s * 1000 / K, 3),
+        "higher_is_better": True,
+        "dtype": "f64",
+        "data": "synthetic (64 seeded street-grid walks generated on the device)",
+        "sequential_value": round(total * K / seq_s, 1),
+        "sequential_ms_per_step": round(seq_s * 1000 / K, 3),
+        "threads": args.threads,
+        "tuner": {"candidates": list(CANDIDATES), "steps": args.tune_steps, "seconds": round(tune_s, 1),
+                  "chosen": dict(collections.Counter(tuned))},
+        "config": {"workload": "c4", "streams": len(streams), "objects": total,
+                   "objects_min": min(w["cfg"]["num_objects"] for w in streams),
+                   "objects_max": max(w["cfg"]["num_objects"] for w in streams)},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def compact(args):
     """compaction of runs kept in HBM: the workload's steps run until --compact
     runs exist, then one stc_compact_runs merges them; reports records merged
@@ -388,6 +465,8 @@ def main():
     ap.add_argument("--qps", type=float, default=2000.0)
     ap.add_argument("--serve-seconds", type=int, default=8)
     ap.add_argument("--serve-warmup", type=int, default=12)
+    ap.add_argument("--threads", type=int, default=16, help="host threads of stc_step_batch (workload c4)")
+    ap.add_argument("--tune-steps", type=int, default=4, help="frames each tuner candidate steps (workload c4)")
     ap.add_argument("--compact", type=int, default=0,
                     help="compaction mode: flush this many runs into HBM, then time merging them into one")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -406,6 +485,9 @@ def main():
         return
     if args.compact:
         compact(args)
+        return
+    if args.workload == "c4":
+        batched(args)
         return
 
     import torch

@@ -197,6 +197,21 @@ const char *stc_last_error(void *ctx);
 int stc_step(void *ctx, const double *xy_host, uint32_t t, stc_step_out *out);
 int stc_step_device(void *ctx, const double *xy_dev, uint32_t t, stc_step_out *out);
 
+/* ---- batched streams ----
+ * One timestep of n independent contexts (the reference's batched streams,
+ * one context each): every context steps as stc_step (xy[i] in host memory)
+ * or stc_step_device (on_device[i] nonzero; on_device may be null) would, on
+ * its own streams, driven from up to `threads` host threads (largest contexts
+ * first) so that their kernels overlap on the device. outs[i] and rcs[i]
+ * (either may be null) receive context i's counters and status. Returns
+ * STC_OK when every step succeeded, else the first failing context's code,
+ * its message in that context's stc_last_error. Contexts must be distinct and
+ * must not be in use by other threads during the call. Each context's result
+ * equals its own sequential run. No counterpart in the reference, which runs
+ * one stream per process invocation. */
+int stc_step_batch(void *const *ctxs, uint32_t n, const double *const *xy, const int32_t *on_device, uint32_t t,
+                   uint32_t threads, stc_step_out *outs, int32_t *rcs);
+
 /* ---- per-second meeting persistence ----
  * The reference can save the pairs each step finds and replay them later in
  * place of the positions (save_meetings_pers / load_meetings_pers:

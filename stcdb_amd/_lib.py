@@ -192,6 +192,7 @@ SIGNATURES = {
     "stc_last_error": (C.c_char_p, [P]),
     "stc_step": (C.c_int, [P, P, C.c_uint32, C.POINTER(StcStepOut)]),
     "stc_step_device": (C.c_int, [P, P, C.c_uint32, C.POINTER(StcStepOut)]),
+    "stc_step_batch": (C.c_int, [P, C.c_uint32, P, P, C.c_uint32, C.c_uint32, C.POINTER(StcStepOut), P]),
     "stc_get_pairs": (C.c_int, [P, P, C.c_uint64, U64P]),
     "stc_get_step_meetings": (C.c_int, [P, P, C.c_uint64, U64P]),
     "stc_step_meetings": (C.c_int, [P, P, C.c_uint64, C.c_uint32, C.POINTER(StcStepOut)]),

@@ -49,6 +49,8 @@
 #include <cmath>
 #include <condition_variable>
 #include <mutex>
+#include <thread>
+#include <atomic>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -2763,6 +2765,57 @@ int stc_step(void *vc, const double *xy_host, uint32_t t, stc_step_out *out) {
     if (!c || !xy_host) return STC_E_ARG;
     CK(hipMemcpyAsync(c->xy_in.p, xy_host, 16ull * c->n, hipMemcpyHostToDevice, c->s));
     return stc_step_device(c, (const double *)c->xy_in.p, t, out);
+}
+
+/* Batched streams: one timestep of many independent contexts. Each context
+ * steps on its own streams; up to `threads` host threads drive them, taking
+ * the contexts largest first, so that the contexts' kernels and host round
+ * trips overlap on the device instead of queueing behind one another. */
+int stc_step_batch(void *const *ctxs, uint32_t n, const double *const *xy, const int32_t *on_device, uint32_t t,
+                   uint32_t threads, stc_step_out *outs, int32_t *rcs) {
+    if ((!ctxs || !xy) && n) return STC_E_ARG;
+    for (uint32_t i = 0; i < n; i++) {
+        if (!ctxs[i] || !xy[i]) return STC_E_ARG;
+        for (uint32_t j = 0; j < i; j++)
+            if (ctxs[j] == ctxs[i]) {
+                g_last_fail = "stc_step_batch: a context appears twice";
+                return STC_E_ARG;
+            }
+    }
+    std::vector<uint32_t> order(n);
+    for (uint32_t i = 0; i < n; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](uint32_t a, uint32_t b) { return ((StcCtx *)ctxs[a])->n > ((StcCtx *)ctxs[b])->n; });
+    std::vector<int> rc(n, STC_OK);
+    std::atomic<uint32_t> next{0};
+    auto work = [&]() {
+        for (uint32_t k; (k = next.fetch_add(1)) < n;) {
+            const uint32_t i = order[k];
+            StcCtx *c = (StcCtx *)ctxs[i];
+            if (hipSetDevice(c->device) != hipSuccess) {
+                rc[i] = fail(c, STC_E_HIP, "stc_step_batch: hipSetDevice failed");
+                continue;
+            }
+            stc_step_out tmp;
+            stc_step_out *o = outs ? outs + i : &tmp;
+            rc[i] = on_device && on_device[i] ? stc_step_device(c, xy[i], t, o) : stc_step(c, xy[i], t, o);
+        }
+    };
+    const uint32_t nt = std::max<uint32_t>(1, std::min<uint32_t>(threads, n));
+    if (nt == 1) {
+        work();
+    } else {
+        std::vector<std::thread> pool;
+        pool.reserve(nt);
+        for (uint32_t k = 0; k < nt; k++) pool.emplace_back(work);
+        for (auto &th : pool) th.join();
+    }
+    int first = STC_OK;
+    for (uint32_t i = 0; i < n; i++) {
+        if (rcs) rcs[i] = rc[i];
+        if (rc[i] != STC_OK && first == STC_OK) first = rc[i];
+    }
+    return first;
 }
 
 /* A step from persisted meeting units (the reference's load_meetings_pers

@@ -319,6 +319,34 @@ CTF_DTYPE = np.dtype([
 ])
 
 
+def step_batch(pipes, frames, t, threads=16, on_device=False):
+    """One timestep of independent contexts (batched streams) through
+    stc_step_batch: frames[i] is pipes[i]'s points, host arrays (N_i, 2)
+    float64 or, with on_device, device pointers. Returns each context's
+    counters; raises on the first failing context."""
+    n = len(pipes)
+    lib = L.load()
+    ctxs = (C.c_void_p * n)(*[p.ctx.value for p in pipes])
+    keep = []
+    ptrs = (C.c_void_p * n)()
+    for i, f in enumerate(frames):
+        if on_device:
+            ptrs[i] = f.value if isinstance(f, C.c_void_p) else int(f)
+        else:
+            a = np.ascontiguousarray(f, dtype=np.float64)
+            assert a.shape == (pipes[i].n, 2)
+            keep.append(a)
+            ptrs[i] = a.ctypes.data
+    dev = (C.c_int32 * n)(*([1 if on_device else 0] * n))
+    outs = (L.StcStepOut * n)()
+    rcs = (C.c_int32 * n)()
+    rc = lib.stc_step_batch(ctxs, n, ptrs, dev, t, threads, outs, rcs)
+    if rc != L.STC_OK:
+        bad = next(i for i in range(n) if rcs[i] != L.STC_OK)
+        _check(rcs[bad], pipes[bad].ctx)
+    return [outs[i].as_dict() for i in range(n)]
+
+
 def ctfs_to_numpy(arr):
     raw = bytes(memoryview(arr))
     return np.frombuffer(raw, dtype=CTF_DTYPE).copy()

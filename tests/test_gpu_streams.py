@@ -1,7 +1,9 @@
 """Batched streams (the c4 family): independent streams, each one context of
-the library with its own box, reach and leaf capacity, stepped in turn on one
-GPU. Every stream must match its own oracle run bit for bit, as a plain
-sequential run of that stream would: contexts share no state.
+the library with its own box, reach and leaf capacity, stepped together
+through stc_step_batch (host threads, each context on its own streams, their
+kernels overlapping on the GPU). Every stream must match its own oracle run
+bit for bit, as a plain sequential run of that stream would: contexts share
+no state.
 
 The streams are scaled down from c4's (median 3,000 objects instead of 50,000,
 boxes of 2-6 km instead of 10-50 km, about the same density) so that the
@@ -10,7 +12,8 @@ import numpy as np
 import pytest
 
 import oracle_lib as O
-from stcdb_amd import HostTrace, Pipeline, batched_streams
+from stcdb_amd import HostTrace, Pipeline, batched_streams, step_batch
+from stcdb_amd.tuner import tune_leaf_capacity
 
 pytestmark = pytest.mark.gpu
 
@@ -31,11 +34,15 @@ def test_batched_streams_parity():
     pairs = [0] * len(runs)
     records = [0] * len(runs)
     for t in range(streams[0]["steps"]):
-        for i, (gpu, orc, trace) in enumerate(runs):  # one step of every stream in turn
+        xys = []
+        for gpu, orc, trace in runs:
             tt, xy = next(trace)
             assert tt == t
-            g = gpu.step(xy, t)
-            o = orc.step(xy, t)
+            xys.append(xy.copy())
+        outs = step_batch([r[0] for r in runs], xys, t, threads=8)  # every stream's step at once
+        for i, (gpu, orc, trace) in enumerate(runs):
+            g = outs[i]
+            o = orc.step(xys[i], t)
             assert g["leaves"] == o["leaves"], (i, t)
             assert g["cross_units"] == o["units_cross"], (i, t)
             assert g["candidates"] == o["candidates"], (i, t)
@@ -78,9 +85,10 @@ def run_stream(w, steps, trace=None):
 
 def test_c4_full_size_streams():
     """c4 at its size: 64 streams of ~6M objects in total (log-normal, median
-    50k, 10-50 km boxes, reach 5-25 m), all contexts alive and stepped in turn
-    for a few steps; each stream equals its own run alone, and four sampled
-    streams equal their oracle runs"""
+    50k, 10-50 km boxes, reach 5-25 m), all contexts alive and stepped together
+    through stc_step_batch on 16 host threads for a few steps; each stream
+    equals its own sequential run alone, and four sampled streams equal their
+    oracle runs"""
     steps = 6
     streams = batched_streams()  # the c4 parameters
     total = sum(w["cfg"]["num_objects"] for w in streams)
@@ -90,8 +98,8 @@ def test_c4_full_size_streams():
     ctxs = [Pipeline(dict(w["cfg"], keep_pairs=1), w["box"]) for w in streams]
     mixed_pairs = [[] for _ in streams]
     for t in range(steps):
+        step_batch(ctxs, [frames[i][t] for i in range(len(streams))], t, threads=16)
         for i, gpu in enumerate(ctxs):
-            gpu.step(frames[i][t], t)
             mixed_pairs[i].append(gpu.pairs())
     mixed_state = []
     for gpu in ctxs:
@@ -125,3 +133,24 @@ def test_c4_full_size_streams():
         assert np.array_equal(om.view(np.uint32), m), i
         print(f"[c4] stream {i} ({w['cfg']['num_objects']} objects) equals the oracle", flush=True)
         orc.close()
+
+
+def test_tuned_capacity_runs_like_the_oracle():
+    """the tuner picks one of its candidate capacities for a stream; the stream
+    then runs at that capacity exactly as the oracle does"""
+    w = batched_streams(4, median=3000, lo=2000, hi=8000, side_km=(2.0, 4.0), steps=12, seed=9)[2]
+    frames = [xy.copy() for _, xy in HostTrace(w["gen"]).steps(w["steps"])]
+    best, timings = tune_leaf_capacity(w["cfg"], w["box"], frames[:4], candidates=(25, 50, 100, 200))
+    print(f"[tuner] n={w['cfg']['num_objects']} timings {timings} -> {best}", flush=True)
+    assert best in (25, 50, 100, 200) and set(timings) == {25, 50, 100, 200}
+    cfg = dict(w["cfg"], grid_capacity=best, keep_pairs=1)
+    gpu, orc = Pipeline(cfg, w["box"]), O.Oracle(cfg, w["box"])
+    for t, xy in enumerate(frames):
+        g, o = gpu.step(xy, t), orc.step(xy, t)
+        assert g["leaves"] == o["leaves"] and g["kv_count"] == o["kv_count"], t
+        assert np.array_equal(gpu.pairs(), orc.pairs()), t
+    gk, gb = sorted_records(*gpu.records())
+    ok, ob = sorted_records(*orc.records())
+    assert np.array_equal(gk, ok) and np.array_equal(gb.view(np.uint32), ob.view(np.uint32))
+    gpu.close()
+    orc.close()
