@@ -120,8 +120,9 @@ struct Counters {             /* device-side counters, copied back once per sync
     uint32_t table_fail;
     uint32_t lookup_overflow;          /* border lookups whose node stack was full */
     uint32_t big, huge, giant;         /* segments left to the wave sort, the block sort, the global-scratch sort */
-    uint32_t pk_over;                  /* a collecting wave's reservation exceeded psrc */
-    unsigned long long pk_used;        /* psrc slots reserved by the collecting waves */
+    uint32_t pk_over;                  /* a collecting wave's reservation exceeded its slice of psrc */
+    unsigned long long pk_used[64];    /* psrc slots reserved in each slice by the collecting waves */
+    unsigned long long cand_part[64], pairs_part[64]; /* the collect pass's totals, by workgroup mod 64 */
 };
 
 
@@ -817,7 +818,9 @@ __global__ void k_point_ranges(const uint32_t *__restrict__ lp, const uint32_t *
 __device__ inline uint64_t lanemask_lt(int lane) { return lane == 0 ? 0ull : (~0ull >> (STC_WAVE - lane)); }
 
 #define SRC_AMB 0x40000000u  /* key flag: a band combination, a pair only if the host accepted it */
-#define PK_REGION 4096u      /* key slots a collecting wave reserves at a time */
+#define PK_REGION 4096u      /* source slots a collecting wave reserves at a time */
+#define PK_SLICES 64u        /* psrc slices, each with its own reservation counter */
+static_assert(sizeof(((Counters *)nullptr)->pk_used) == PK_SLICES * sizeof(unsigned long long), "one counter per slice");
 
 /* The pair pass: per owner (one wave, owners in leaf order), every
  * combination is decided once; those that are pairs or lie in the guard band
@@ -826,12 +829,15 @@ __device__ inline uint64_t lanemask_lt(int lane) { return lane == 0 ? 0ull : (~0
  * SRC_AMB on band combinations) are written, unsorted, to a run of psrc that
  * the wave reserved (PK_REGION slots at a time, fewer for a wave with few
  * owners left, or the owner's candidate count when larger, so an owner's
- * sources stay contiguous). Per owner: the slot count (pcount, by id) and the
- * run's start and length (pkrun, leaf order); band combinations are listed as
- * (probe, other) for the host; segments longer than the place pass sorts go
- * to `big` or `huge`. A reservation beyond pk_cap writes nothing and sets
- * pk_over (the host grows psrc and repeats the pass; pk_used tells by how
- * much). */
+ * sources stay contiguous). psrc is split into PK_SLICES equal slices, wave w
+ * reserving in slice w % PK_SLICES from that slice's counter, so that no one
+ * address takes every reservation (with one owner per wave, as in a small
+ * step, a single counter serialises the whole pass). Per owner: the slot count
+ * (pcount, by id) and the run's start and length (pkrun, leaf order); band
+ * combinations are listed as (probe, other) for the host; segments longer
+ * than the place pass sorts go to `big` or `huge`. A reservation beyond its
+ * slice writes nothing and sets pk_over (the host grows psrc and repeats the
+ * pass; pk_used tells by how much). */
 __global__ __launch_bounds__(256, 8) void k_pair_collect(PairGeo g, uint32_t *__restrict__ pcount,
                                                       uint2 *__restrict__ pkrun, uint32_t *__restrict__ psrc,
                                                       uint64_t pk_cap, uint2 *__restrict__ amb, uint32_t amb_cap,
@@ -845,6 +851,8 @@ __global__ __launch_bounds__(256, 8) void k_pair_collect(PairGeo g, uint32_t *__
     unsigned long long cand = 0, total = 0;
     uint64_t rcur = 0, rend = 0; /* the wave's reserved run of psrc */
     bool over = false;
+    const uint32_t slice_id = (uint32_t)(wid % PK_SLICES);
+    const uint64_t slice = pk_cap / PK_SLICES, slice_base = slice_id * slice;
     for (uint64_t w = wid; w < g.n; w += nw) {
         const uint32_t a = (uint32_t)__builtin_amdgcn_readfirstlane((int)g.lp[w]);
         bool room = true;
@@ -857,12 +865,12 @@ __global__ __launch_bounds__(256, 8) void k_pair_collect(PairGeo g, uint32_t *__
                 const uint64_t want = left * tot < PK_REGION ? left * tot : PK_REGION;
                 const uint64_t sz = tot > want ? tot : want;
                 unsigned long long b0 = 0;
-                if (lane == 0) b0 = atomicAdd(&cnt->pk_used, (unsigned long long)sz);
+                if (lane == 0) b0 = atomicAdd(&cnt->pk_used[slice_id], (unsigned long long)sz);
                 b0 = __shfl(b0, 0, STC_WAVE);
-                rcur = b0;
-                rend = b0 + sz;
+                rcur = slice_base + b0;
+                rend = rcur + sz;
             }
-            room = rend <= pk_cap;
+            room = rend <= slice_base + slice;
             over |= !room;
             out = psrc + rcur;
         };
@@ -894,11 +902,25 @@ __global__ __launch_bounds__(256, 8) void k_pair_collect(PairGeo g, uint32_t *__
         }
         rcur += fill;
     }
+    /* totals: summed per workgroup, then one update of a counter of 64 (the host adds them), so
+     * that no address takes an update per wave */
+    __shared__ unsigned long long bsum[256 / STC_WAVE][2];
     cand = wave_sum_u64(cand);
     if (lane == 0) {
-        if (cand) atomicAdd(&cnt->candidates, cand);
-        if (total) atomicAdd(&cnt->pairs, total);
+        bsum[threadIdx.x / STC_WAVE][0] = cand;
+        bsum[threadIdx.x / STC_WAVE][1] = total;
         if (over) atomicOr(&cnt->pk_over, 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long c0 = 0, t0 = 0;
+        for (uint32_t k = 0; k < blockDim.x / STC_WAVE; k++) {
+            c0 += bsum[k][0];
+            t0 += bsum[k][1];
+        }
+        const uint32_t part = blockIdx.x % PK_SLICES;
+        if (c0) atomicAdd(&cnt->cand_part[part], c0);
+        if (t0) atomicAdd(&cnt->pairs_part[part], t0);
     }
 }
 
@@ -2293,6 +2315,7 @@ struct StepGeom {
     uint32_t nodes, leaves;
     uint64_t units, pairs, ambiguous;
     uint32_t rebuilt;
+    uint64_t candidates = 0;
 };
 static int step_tail(StcCtx *c, uint32_t t, StepGeom &g, stc_step_out *out);
 
@@ -2464,7 +2487,8 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
         CK(hipMemsetAsync(&cnt->candidates, 0, 3 * sizeof(unsigned long long), s)); /* candidates, pairs, ambiguous */
         CK(hipMemsetAsync(&cnt->big, 0, 3 * sizeof(uint32_t), s));                 /* big, huge, giant */
         CK(hipMemsetAsync(&cnt->pk_over, 0, sizeof(uint32_t), s));
-        CK(hipMemsetAsync(&cnt->pk_used, 0, sizeof(unsigned long long), s));
+        CK(hipMemsetAsync(cnt->pk_used, 0, sizeof(cnt->pk_used), s));
+        CK(hipMemsetAsync(cnt->cand_part, 0, sizeof(cnt->cand_part) + sizeof(cnt->pairs_part), s));
         CK(hipEventRecord(c->ev[10], s));
         hipLaunchKernelGGL(k_pair_collect, dim3(pgrid), dim3(256), 0, s, pg, c->pcount.p, c->pkrun.p, c->psrc.p,
                            (uint64_t)std::min<size_t>(c->psrc.cap, 0xFFFFFFFFu), c->amb.p,
@@ -2473,20 +2497,27 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
         CK(hipEventRecord(c->ev[11], s));
         CKRC(sync_counters(c));
         h = *c->hcnt;
+        for (uint32_t k = 0; k < PK_SLICES; k++) {
+            h.candidates += h.cand_part[k];
+            h.pairs += h.pairs_part[k];
+        }
         bool again = false;
         if (h.ambiguous > c->amb.cap) {
             if (c->amb.ensure(h.ambiguous + 1024)) return fail(c, STC_E_CAPACITY, "ambiguous list allocation failed");
             again = true;
         }
-        if (h.pk_over) { /* the keys did not fit: grow to what the waves reserved, with room */
-            if (h.pk_used >= 0xFFFFFFFFull) return fail(c, STC_E_CAPACITY, "more than 2^32 - 1 pair sources in one step");
-            if (c->psrc.ensure(h.pk_used)) return fail(c, STC_E_CAPACITY, "pair sources");
+        if (h.pk_over) { /* a slice was short: grow every slice to the fullest one's reservations, with room */
+            unsigned long long most = 0;
+            for (uint32_t k = 0; k < PK_SLICES; k++) most = std::max(most, h.pk_used[k]);
+            const unsigned long long want = most * PK_SLICES + most * PK_SLICES / 8;
+            if (want >= 0xFFFFFFFFull) return fail(c, STC_E_CAPACITY, "more than 2^32 - 1 pair sources in one step");
+            if (c->psrc.ensure(want)) return fail(c, STC_E_CAPACITY, "pair sources");
             again = true;
         }
         if (!again) break;
         if (attempt >= 2) return fail(c, STC_E_CAPACITY, "pair sources or ambiguous list still short after growing");
     }
-    const uint64_t total = h.pairs, n_amb = h.ambiguous;
+    const uint64_t total = h.pairs, n_amb = h.ambiguous, n_cand = h.candidates;
     if (total >= 0xFFFFFFFFull) return fail(c, STC_E_CAPACITY, "more than 2^32 - 2 pairs in one step");
     std::vector<unsigned long long> accepted;
     if (n_amb) {
@@ -2570,7 +2601,7 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
     c->plen = total;
     c->pairs_boxed = false;
     CK(hipEventRecord(c->ev[3], s));
-    StepGeom g{n_nodes, n_leaves, n_units, 0, n_amb, 0};
+    StepGeom g{n_nodes, n_leaves, n_units, 0, n_amb, 0, n_cand};
     return step_tail(c, t, g, out);
 }
 
@@ -2721,7 +2752,7 @@ static int step_tail(StcCtx *c, uint32_t t, StepGeom &g, stc_step_out *out) {
         out->leaves = g.leaves;
         out->border_points = h.border;
         out->cross_units = g.units;
-        out->candidates = h.candidates;
+        out->candidates = g.candidates;
         out->pairs = g.pairs;
         out->active_meetings = h.live;
         out->expired = h.expired;
