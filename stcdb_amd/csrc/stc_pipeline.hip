@@ -114,7 +114,6 @@ struct Counters {             /* device-side counters, copied back once per sync
     uint32_t nodes, leaves;
     uint32_t frontier[NLEVELS + 1];
     uint32_t bad_points;
-    uint32_t border;
     uint32_t overflow;
     uint32_t oversize_oid_count;
     uint32_t table_fail;
@@ -123,6 +122,7 @@ struct Counters {             /* device-side counters, copied back once per sync
     uint32_t pk_over;                  /* a collecting wave's reservation exceeded its slice of psrc */
     unsigned long long pk_used[64];    /* psrc slots reserved in each slice by the collecting waves */
     unsigned long long cand_part[64], pairs_part[64]; /* the collect pass's totals, by workgroup mod 64 */
+    uint32_t border_part[64];          /* border points, by partition workgroup mod 64 */
 };
 
 
@@ -348,8 +348,16 @@ __global__ void k_partition(const double2 *__restrict__ xy, const uint32_t *__re
         const uint32_t end = later ? (uint32_t)(lane + __ffsll((long long)later)) : (uint32_t)__popcll(valid);
         atomicAdd(&leaf_count[leaf], end - (uint32_t)lane);
     }
-    uint64_t b = __ballot(border);
-    if (lane == 0 && b) atomicAdd(&cnt->border, (uint32_t)__popcll(b));
+    /* border points: summed per workgroup, then one update of a counter of 64 (the host adds them) */
+    __shared__ uint32_t bsum[1024 / STC_WAVE];
+    const uint64_t b = __ballot(border);
+    if (lane == 0) bsum[threadIdx.x / STC_WAVE] = (uint32_t)__popcll(b);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tb = 0;
+        for (uint32_t k = 0; k < blockDim.x / STC_WAVE; k++) tb += bsum[k];
+        if (tb) atomicAdd(&cnt->border_part[blockIdx.x % 64], tb);
+    }
 }
 
 /* positions of the two points of each band combination, for the host's decision */
@@ -2750,7 +2758,9 @@ static int step_tail(StcCtx *c, uint32_t t, StepGeom &g, stc_step_out *out) {
         memset(out, 0, sizeof(*out));
         out->nodes = g.nodes;
         out->leaves = g.leaves;
-        out->border_points = h.border;
+        uint32_t border = 0;
+        for (uint32_t k = 0; k < 64; k++) border += h.border_part[k];
+        out->border_points = border;
         out->cross_units = g.units;
         out->candidates = g.candidates;
         out->pairs = g.pairs;

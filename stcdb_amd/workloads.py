@@ -14,7 +14,8 @@ lon -87.9401, lat 41.6446): x = lon0 + X * dpm_lon(lat0), y = lat0 + Y * dpm_lat
   c4  batched streams: 64 independent streams, object counts log-normal
       (median 50k, clipped to 1k-2M, ~6M in total), each with its own
       10-50 km box, reach uniform in 5-25 m and its own leaf capacity
-      (batched_streams below); each stream is one context of the library
+      (batched_streams below), the reference's default dynamic schema; each
+      stream is one context of the library
   c5  5,000,000 objects with c2's mix, runs of 8 Mi records kept in HBM for
       queries: the serving workload (ingest at 1 Hz plus 2,000 queries/s)
 """
@@ -96,12 +97,14 @@ def workload(name, **overrides):
 
 
 def batched_streams(n_streams=64, median=50_000, lo=1_000, hi=2_000_000, side_km=(10.0, 50.0),
-                    reach_m=(5.0, 25.0), capacities=(50, 100, 200), min_meet_time=10, steps=600, seed=4):
+                    reach_m=(5.0, 25.0), capacities=(50, 100, 200), min_meet_time=10, steps=600, seed=4,
+                    dynamic_schema=1):
     """The c4 family: independent streams, one workload each (num_objects
     log-normal about `median` with sigma 1, clipped to [lo, hi]; a square box
     of side uniform in `side_km`; reach uniform in `reach_m`; a leaf capacity
-    drawn from `capacities`, standing in for a per-stream tuned capacity).
-    All draws come from one seeded numpy generator, so the list depends only on
+    drawn from `capacities`, the starting point of the leaf-capacity tuner;
+    the reference's default dynamic schema unless dynamic_schema is 0). All
+    draws come from one seeded numpy generator, so the list depends only on
     the arguments."""
     rng = np.random.default_rng(seed)
     out = []
@@ -111,6 +114,6 @@ def batched_streams(n_streams=64, median=50_000, lo=1_000, hi=2_000_000, side_km
         reach = float(round(rng.uniform(*reach_m), 3))
         cap = int(capacities[int(rng.integers(len(capacities)))])
         gen = gen_params(n, cells, seed=seed * 1000 + i)
-        cfg = pipeline_config(n, reach, min_meet_time, grid_capacity=cap)
+        cfg = pipeline_config(n, reach, min_meet_time, grid_capacity=cap, dynamic_schema=dynamic_schema)
         out.append(dict(gen=gen, cfg=cfg, steps=steps, box=map_box(gen)))
     return out
