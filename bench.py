@@ -427,7 +427,7 @@ def stage_roofline(outs, n):
         "partition": (mean("ms_partition"), 20.0 * n),
         # border lookup: the border points' 16 B positions
         "filter": (mean("ms_filter"), 16.0 * mean("border_points")),
-        # pair refinement (reverse lists, count, write, segment sorts): points in, 8 B pair keys out
+        # pair refinement (reverse lists, collect, place, segment sorts): points in, 8 B pair keys out
         "refine": (mean("ms_refine"), 20.0 * n + 8.0 * pairs),
         # meetings: a 32 B bucket read and written per pair, the previous array scanned,
         # 2 x (16 B key + 16 B box) per recorded meeting
@@ -580,14 +580,15 @@ def main():
         # 8 B key (partner, source) per slot out
         "k_pair_collect": ("pair collect pass: each combination decided once, keys per owner", mean("ms_count_kernel"),
                            24.0 * n + 8.0 * slots),
-        # place pass: each owner's 8 B run descriptor and 8 B keys in, one 24 B slot (key + float
-        # MBR) per pair out
+        # place pass: each owner's 8 B run descriptor and 4 B sources in, one 16 B slot (key +
+        # the partner's float position) per pair out
         "k_pair_place": ("pair place pass: owner segments sorted by partner", mean("ms_refine_kernel"),
-                         8.0 * n + 32.0 * slots),
-        # merge: the step's slots (24 B) and the previous meetings (32 B) in, the new
-        # meetings (32 B) and 2 records (32 B each) per recorded meeting out
+                         8.0 * n + 20.0 * slots),
+        # merge: the step's slots (16 B) and the previous meetings (32 B) in, the owners' float
+        # positions (8 B each) once, the new meetings (32 B) and 2 records (32 B each) per
+        # recorded meeting out
         "k_meet_merge": ("merge with the previous step's meetings, expiry, records", mean("ms_identify_kernel"),
-                         24.0 * slots + 32.0 * slots + 32.0 * slots + 32.0 * mean("new_records")),
+                         16.0 * slots + 32.0 * slots + 32.0 * slots + 8.0 * n + 32.0 * mean("new_records")),
     }
     kname = max(kernels, key=lambda k: kernels[k][1])
     desc, ms, alg = kernels[kname]

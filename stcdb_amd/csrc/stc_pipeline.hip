@@ -51,6 +51,7 @@
 #include <mutex>
 #include <thread>
 #include <atomic>
+#include <cstddef>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -103,9 +104,6 @@ __device__ inline uint4 slot_mbr(const PairSlot &e, const float2 *posf) {
 
 struct Counters {             /* device-side counters, copied back once per sync point */
     unsigned long long units;
-    unsigned long long candidates;
-    unsigned long long pairs;          /* slots reserved by the pair count (pairs and band combinations) */
-    unsigned long long ambiguous;
     unsigned long long new_keys;
     unsigned long long continuing;
     unsigned long long ambiguous_nodes;
@@ -118,11 +116,15 @@ struct Counters {             /* device-side counters, copied back once per sync
     uint32_t oversize_oid_count;
     uint32_t table_fail;
     uint32_t lookup_overflow;          /* border lookups whose node stack was full */
-    uint32_t big, huge, giant;         /* segments left to the wave sort, the block sort, the global-scratch sort */
-    uint32_t pk_over;                  /* a collecting wave's reservation exceeded its slice of psrc */
+    uint32_t border_part[64];          /* border points, by partition workgroup mod 64 */
+    /* the collect pass's counters, from candidates to pk_over: cleared together before each pass */
+    unsigned long long candidates;
+    unsigned long long pairs;          /* slots reserved by the collect pass (pairs and band combinations) */
+    unsigned long long ambiguous;
     unsigned long long pk_used[64];    /* psrc slots reserved in each slice by the collecting waves */
     unsigned long long cand_part[64], pairs_part[64]; /* the collect pass's totals, by workgroup mod 64 */
-    uint32_t border_part[64];          /* border points, by partition workgroup mod 64 */
+    uint32_t big, huge, giant;         /* segments left to the wave sort, the block sort, the global-scratch sort */
+    uint32_t pk_over;                  /* a collecting wave's reservation exceeded its slice of psrc */
 };
 
 
@@ -2492,11 +2494,9 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
     if (!c->psrc.p && c->psrc.ensure(std::max<size_t>((size_t)n * 16, 1u << 20)))
         return fail(c, STC_E_CAPACITY, "pair sources");
     for (int attempt = 0;; attempt++) {
-        CK(hipMemsetAsync(&cnt->candidates, 0, 3 * sizeof(unsigned long long), s)); /* candidates, pairs, ambiguous */
-        CK(hipMemsetAsync(&cnt->big, 0, 3 * sizeof(uint32_t), s));                 /* big, huge, giant */
-        CK(hipMemsetAsync(&cnt->pk_over, 0, sizeof(uint32_t), s));
-        CK(hipMemsetAsync(cnt->pk_used, 0, sizeof(cnt->pk_used), s));
-        CK(hipMemsetAsync(cnt->cand_part, 0, sizeof(cnt->cand_part) + sizeof(cnt->pairs_part), s));
+        /* one clear for candidates .. pk_over (a fill launch can wait behind the run copies) */
+        CK(hipMemsetAsync(&cnt->candidates, 0,
+                          offsetof(Counters, pk_over) + sizeof(uint32_t) - offsetof(Counters, candidates), s));
         CK(hipEventRecord(c->ev[10], s));
         hipLaunchKernelGGL(k_pair_collect, dim3(pgrid), dim3(256), 0, s, pg, c->pcount.p, c->pkrun.p, c->psrc.p,
                            (uint64_t)std::min<size_t>(c->psrc.cap, 0xFFFFFFFFu), c->amb.p,
