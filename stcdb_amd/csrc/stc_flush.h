@@ -10,6 +10,7 @@
 
 struct FlushWork {
     uint32_t n = 0, ctf_count = 0;
+    bool str_full = false;                         /* STR sorts over the full 64-bit keys (a compact key overflowed) */
     /* device scratch, grown on demand */
     uint64_t *mid = nullptr, *mid_alt = nullptr;   /* per-object STR keys */
     uint32_t *oids = nullptr, *oids_alt = nullptr;

@@ -34,6 +34,7 @@
  *                    slot, copied out and cleared, as the reference does.
  */
 #include "stc_flush.h"
+#include "stc_table.h"
 
 #include <algorithm>
 #include <cmath>
@@ -113,6 +114,20 @@ struct FlushCounters {
     uint32_t o_start_min, o_start_max;
     uint32_t bad;                /* inconsistent capacities (reference would misbehave) */
     uint32_t raster_capped;
+    uint32_t str_wide;           /* an STR key fell outside the compact key's range (the flush is redone at full width) */
+};
+
+/* The STR keys in fewer bits: (mid x - base0 + 1) << by | (mid y - base1) for
+ * an object with a CTF, 0 otherwise, where base0/base1 lie below every mid
+ * coordinate a point inside the map can give and by covers the y range. The
+ * map is monotone and one-to-one on that domain, so the stable sorts order the
+ * objects exactly as over the full 64-bit keys, in fewer 8-bit passes. on = 0
+ * keeps the full keys. */
+struct StrCompact {
+    uint32_t on, by;
+    uint32_t base0, base1;
+    uint64_t lim0, lim1;         /* x - base0 + 1 < lim0 and y - base1 < lim1 = 2^by */
+    uint32_t str_bits, strip_bits; /* key bits of the two sorts (host side) */
 };
 
 /* float/double -> unsigned, saturating as the reference GPU's conversions do */
@@ -134,7 +149,7 @@ __device__ inline float q2f(uint32_t f) { return (float)f / 100000 - 180; } /* u
 
 __global__ void k_set_oid(uint32_t n, const uint32_t *__restrict__ same_pid_count, const uint4 *__restrict__ o_boxs,
                           uint16_t *__restrict__ sids, uint64_t *__restrict__ mid, uint32_t *__restrict__ oids,
-                          uint8_t *__restrict__ part_tail, double area_threshold, uint32_t sticky,
+                          uint8_t *__restrict__ part_tail, double area_threshold, uint32_t sticky, StrCompact sc,
                           FlushCounters *fc) {
     uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= n) return;
@@ -150,7 +165,13 @@ __global__ void k_set_oid(uint32_t n, const uint32_t *__restrict__ same_pid_coun
         } else {
             uint32_t am0 = ob.x / 2 + ob.z / 2;
             uint32_t am1 = ob.w / 2 + ob.w / 2; /* high[1] twice, as the reference */
-            m = ((uint64_t)am0 << 32) + (uint64_t)am1;
+            if (!sc.on) {
+                m = ((uint64_t)am0 << 32) + (uint64_t)am1;
+            } else {
+                const uint64_t x = (uint64_t)am0 - sc.base0 + 1, y = (uint64_t)am1 - sc.base1;
+                if (am0 < sc.base0 || am1 < sc.base1 || x >= sc.lim0 || y >= sc.lim1) atomicOr(&fc->str_wide, 1u);
+                m = (x << sc.by) | (y & (sc.lim1 - 1));
+            }
         }
     }
     mid[id] = m;
@@ -178,15 +199,17 @@ __global__ void k_str_prepare(FlushCounters *fc, uint32_t n, uint32_t split) {
     fc->split = split;
 }
 
-/* narrow_xy_to_y + strip ranges: key = (strip + 1) << 32 | y for positions >= z */
-__global__ void k_strip_keys(uint64_t *mid, uint32_t n, const FlushCounters *fc) {
+/* narrow_xy_to_y + strip ranges: key = (strip + 1) << 32 | y for positions >= z
+ * (with compact keys, (strip + 1) << by | y - base1) */
+__global__ void k_strip_keys(uint64_t *mid, uint32_t n, StrCompact sc, const FlushCounters *fc) {
     uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= n) return;
     uint32_t z = fc->z;
     if (id < z) { mid[id] = 0; return; }
     uint32_t strip = (id - z) / fc->xpc;
     if (strip > fc->split - 1) strip = fc->split - 1;
-    mid[id] = ((uint64_t)(strip + 1) << 32) | (mid[id] & 0xFFFFFFFFull);
+    if (sc.on) mid[id] = ((uint64_t)(strip + 1) << sc.by) | (mid[id] & (sc.lim1 - 1));
+    else mid[id] = ((uint64_t)(strip + 1) << 32) | (mid[id] & 0xFFFFFFFFull);
 }
 
 __global__ void k_make_sid(const uint32_t *oids, uint32_t n, uint16_t *sids, const FlushCounters *fc) {
@@ -649,15 +672,35 @@ int stc_flush_run(FlushWork *w, const FlushArgs *a, FlushOut *out, hipStream_t s
     FCK(hipMemsetAsync(w->bitmaps, 0, (size_t)C * 8192, s));
     const uint32_t B = 256;
 
+    /* compact STR keys from the map box: every record box lies inside it, so every object's quantised
+     * mid point does (f2q is monotone), and x and y each need only the bits of the map's extent */
+    StrCompact sc{};
+    {
+        const uint32_t qlo0 = f2q((float)a->map[0]), qlo1 = f2q((float)a->map[1]);
+        const uint32_t qhi0 = f2q((float)a->map[2]), qhi1 = f2q((float)a->map[3]);
+        sc.base0 = qlo0 ? qlo0 - 1 : 0; /* a mid point (x/2 + z/2) can fall one below a bound */
+        sc.base1 = qlo1 ? qlo1 - 1 : 0;
+        auto bits_for = [](uint64_t v) { uint32_t b = 0; while (b < 63 && (1ull << b) <= v) b++; return b; };
+        const uint32_t bx = qhi0 >= sc.base0 ? bits_for((uint64_t)qhi0 - sc.base0 + 1) : 64;
+        const uint32_t by = qhi1 >= sc.base1 ? bits_for((uint64_t)qhi1 - sc.base1) : 64;
+        sc.on = !w->str_full && bx + by <= 56 && by + bits_for(split) <= 56;
+        sc.by = by;
+        sc.lim0 = sc.on ? 1ull << bx : 0;
+        sc.lim1 = sc.on ? 1ull << by : 0;
+        if (sc.on) {
+            sc.str_bits = bx + by;
+            sc.strip_bits = by + bits_for(split);
+        }
+    }
     /* F1 */
     hipLaunchKernelGGL(k_set_oid, dim3(blocks_for(n, B)), dim3(B), 0, s, n, a->same_pid_count, a->o_boxs, a->sids,
-                       w->mid, w->oids, w->part_tail, a->area_threshold, a->sticky_oversize, fc);
+                       w->mid, w->oids, w->part_tail, a->area_threshold, a->sticky_oversize, sc, fc);
     hipLaunchKernelGGL(k_str_prepare, dim3(1), dim3(1), 0, s, fc, n, split);
     FCK(hipGetLastError());
     /* F2: STR over the full 64-bit key (x high, y low), stable from oid order */
-    FRC(stc_radix_sort_u64(w->mid, w->oids, w->mid_alt, w->oids_alt, n, 0, 64, w->scan_temp, s));
-    hipLaunchKernelGGL(k_strip_keys, dim3(blocks_for(n, B)), dim3(B), 0, s, w->mid, n, fc);
-    FRC(stc_radix_sort_u64(w->mid, w->oids, w->mid_alt, w->oids_alt, n, 0, 40, w->scan_temp, s));
+    FRC(stc_radix_sort_u64(w->mid, w->oids, w->mid_alt, w->oids_alt, n, 0, sc.on ? (int)sc.str_bits : 64, w->scan_temp, s));
+    hipLaunchKernelGGL(k_strip_keys, dim3(blocks_for(n, B)), dim3(B), 0, s, w->mid, n, sc, fc);
+    FRC(stc_radix_sort_u64(w->mid, w->oids, w->mid_alt, w->oids_alt, n, 0, sc.on ? (int)sc.strip_bits : 40, w->scan_temp, s));
     hipLaunchKernelGGL(k_make_sid, dim3(blocks_for(n, B)), dim3(B), 0, s, w->oids, n, a->sids, fc);
     /* F3 */
     hipLaunchKernelGGL(k_ctf_capacity, dim3(blocks_for(n, B)), dim3(B), 0, s, w->oids, n, a->sids, a->same_pid_count,
@@ -694,6 +737,10 @@ int stc_flush_run(FlushWork *w, const FlushArgs *a, FlushOut *out, hipStream_t s
     FCK(hipMemcpyAsync(cap.data(), w->capacity, C * 4, hipMemcpyDeviceToHost, s));
     FCK(hipMemcpyAsync(base.data(), w->ctf_base, (C + 1) * 4, hipMemcpyDeviceToHost, s));
     FCK(hipStreamSynchronize(s));
+    if (hfc.str_wide && sc.on) { /* cannot happen for records inside the map; redone over the full keys */
+        w->str_full = true;
+        return stc_flush_run(w, a, out, s);
+    }
     std::vector<uint64_t> byte_base(C + 1, 0);
     for (uint32_t c = 0; c < C; c++) {
         stc_ctf_bits(&ctfs[c], a->map, n, a->max_meet_time);
