@@ -1027,11 +1027,16 @@ __device__ inline unsigned long long settle_key(uint32_t src, uint32_t a, const 
 /* Sort an owner's collected sources (srcs[0, slots), slots <= 64 * K) by
  * partner in registers and write its segment: live pairs in partner order,
  * then the dead slots (band combinations the host rejected, repeats of a
- * pair). */
+ * pair). The sources arrive as ascending runs (each source leaf's members and
+ * the reverse list are in id order), so with a wave's LDS buffer (64 * K
+ * keys) a segment that is one run is not sorted and one of two runs is merged
+ * (the second run reversed behind the padding makes the keys bitonic, and the
+ * last stage of the network merges them); more runs take the full sort. */
 template <int K>
 __device__ inline void write_sorted_segment(const uint32_t *srcs, uint32_t slots, uint32_t a, uint32_t w,
                                             const PairGeo &g, const unsigned long long *acc, uint32_t n_acc,
-                                            PairSlot *__restrict__ seg, int lane) {
+                                            PairSlot *__restrict__ seg, int lane,
+                                            unsigned long long *runbuf = nullptr) {
     const uint64_t lt = lanemask_lt(lane);
     unsigned long long x[K];
     uint32_t sv[K];
@@ -1042,7 +1047,43 @@ __device__ inline void write_sorted_segment(const uint32_t *srcs, uint32_t slots
     }
 #pragma unroll
     for (int r = 0; r < K; r++) x[r] = r * STC_WAVE + lane < slots ? settle_key(sv[r], a, g, acc, n_acc) : ~0ull;
-    reg_bitonic<K>(x, lane);
+    if (runbuf) {
+        /* descents: places i (0 < i < slots) whose key is below its predecessor's */
+        uint32_t d = 0, p = 0;
+#pragma unroll
+        for (int r = 0; r < K; r++) {
+            const unsigned long long up = __shfl_up(x[r], 1, STC_WAVE);
+            const unsigned long long last = r > 0 ? __shfl(x[r > 0 ? r - 1 : 0], STC_WAVE - 1, STC_WAVE) : 0ull;
+            const unsigned long long prev = lane > 0 ? up : last;
+            const uint32_t i = r * STC_WAVE + lane;
+            const uint64_t m = __ballot(i < slots && x[r] < prev);
+            d += (uint32_t)__popcll(m);
+            if (m) p = r * STC_WAVE + 63 - (uint32_t)__clzll((long long)m);
+        }
+        if (d == 1) {
+            /* runs [0, p) and [p, slots): the second reversed at the end, the padding between */
+            constexpr uint32_t P = K * STC_WAVE;
+#pragma unroll
+            for (int r = 0; r < K; r++) {
+                const uint32_t i = r * STC_WAVE + lane;
+                const uint32_t t = i < p ? i : (i < slots ? P - 1 - (i - p) : p + (i - slots));
+                runbuf[t] = x[r];
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+            for (int r = 0; r < K; r++) x[r] = runbuf[r * STC_WAVE + lane];
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier(); /* the buffer is reused by the wave's next owner */
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            bitonic_stage_steps<K, P, P / 2>(x, lane);
+        } else if (d > 1) {
+            reg_bitonic<K>(x, lane);
+        }
+    } else {
+        reg_bitonic<K>(x, lane);
+    }
     bool live[K];
     uint32_t nlive = 0;
 #pragma unroll
@@ -1144,12 +1185,14 @@ __global__ void k_huge_expand(PairGeo g, const uint32_t *__restrict__ owners, ui
  * owner's segment sorted by partner, its live pairs first and its dead slots
  * behind them, so that pn is sorted by key; each pair's MBR is computed from
  * its source's position when written. */
-__global__ __launch_bounds__(256, 7) void k_pair_place(PairGeo g, const uint32_t *__restrict__ poff,
+__global__ __launch_bounds__(256, 6) void k_pair_place(PairGeo g, const uint32_t *__restrict__ poff,
                                                     const uint2 *__restrict__ pkrun,
                                                     const uint32_t *__restrict__ psrc,
                                                     const unsigned long long *__restrict__ acc, uint32_t n_acc,
                                                     PairSlot *__restrict__ pn) {
+    __shared__ unsigned long long wrun[256 / STC_WAVE][WCAP];
     const int lane = threadIdx.x & (STC_WAVE - 1);
+    unsigned long long *runbuf = wrun[threadIdx.x / STC_WAVE];
     const uint64_t wid = (uint32_t)__builtin_amdgcn_readfirstlane((int)((xcd_block_id() * blockDim.x + threadIdx.x) / STC_WAVE));
     const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) / STC_WAVE;
     for (uint64_t w = wid; w < g.n; w += nw) {
@@ -1159,10 +1202,10 @@ __global__ __launch_bounds__(256, 7) void k_pair_place(PairGeo g, const uint32_t
         const uint32_t a = (uint32_t)__builtin_amdgcn_readfirstlane((int)g.lp[w]);
         const uint32_t *keys = psrc + (uint32_t)__builtin_amdgcn_readfirstlane((int)run.x);
         PairSlot *seg = pn + poff[a];
-        if (slots <= STC_WAVE) write_sorted_segment<1>(keys, slots, a, (uint32_t)w, g, acc, n_acc, seg, lane);
-        else if (slots <= 2 * STC_WAVE) write_sorted_segment<2>(keys, slots, a, (uint32_t)w, g, acc, n_acc, seg, lane);
-        else if (slots <= 4 * STC_WAVE) write_sorted_segment<4>(keys, slots, a, (uint32_t)w, g, acc, n_acc, seg, lane);
-        else write_sorted_segment<8>(keys, slots, a, (uint32_t)w, g, acc, n_acc, seg, lane);
+        if (slots <= STC_WAVE) write_sorted_segment<1>(keys, slots, a, (uint32_t)w, g, acc, n_acc, seg, lane, runbuf);
+        else if (slots <= 2 * STC_WAVE) write_sorted_segment<2>(keys, slots, a, (uint32_t)w, g, acc, n_acc, seg, lane, runbuf);
+        else if (slots <= 4 * STC_WAVE) write_sorted_segment<4>(keys, slots, a, (uint32_t)w, g, acc, n_acc, seg, lane, runbuf);
+        else write_sorted_segment<8>(keys, slots, a, (uint32_t)w, g, acc, n_acc, seg, lane, runbuf);
     }
 }
 
