@@ -154,3 +154,32 @@ def test_tuned_capacity_runs_like_the_oracle():
     assert np.array_equal(gk, ok) and np.array_equal(gb.view(np.uint32), ob.view(np.uint32))
     gpu.close()
     orc.close()
+
+
+def test_step_batch_reports_each_context():
+    """a context whose step fails (a point outside its map box) reports its own
+    code through stc_step_batch while the others step normally; the batch
+    returns the failure and the failing context's message names it"""
+    import ctypes as C
+    from stcdb_amd import _lib as L
+    streams = batched_streams(3, median=2000, lo=1000, hi=4000, side_km=(2.0, 3.0), steps=3, seed=11)
+    pipes = [Pipeline(w["cfg"], w["box"]) for w in streams]
+    frames = [next(iter(HostTrace(w["gen"]).steps(1)))[1].copy() for w in streams]
+    outs = step_batch(pipes, frames, 0, threads=3)
+    assert all(o["leaves"] > 0 for o in outs)
+    bad = [f.copy() for f in frames]
+    bad[1][0] = (streams[1]["box"][1][0] + 1.0, streams[1]["box"][1][1] + 1.0)  # outside stream 1's box
+    lib = L.load()
+    n = len(pipes)
+    ctxs = (C.c_void_p * n)(*[p.ctx.value for p in pipes])
+    ptrs = (C.c_void_p * n)(*[f.ctypes.data for f in bad])
+    res = (L.StcStepOut * n)()
+    rcs = (C.c_int32 * n)()
+    rc = lib.stc_step_batch(ctxs, n, ptrs, None, 1, 3, res, rcs)
+    assert rc == L.STC_E_INPUT and list(rcs) == [L.STC_OK, L.STC_E_INPUT, L.STC_OK]
+    assert b"outside the map box" in lib.stc_last_error(pipes[1].ctx)
+    assert res[0].leaves > 0 and res[2].leaves > 0
+    with pytest.raises(L.StcError):
+        step_batch(pipes, bad, 2, threads=3)
+    for p in pipes:
+        p.close()
