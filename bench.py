@@ -125,6 +125,35 @@ def cpu_baseline(w, frames_host, steps, schema):
                        f"{steps} timesteps after the first; {pairs} pairs found; {total_s:.1f} s of stages")
 
 
+def cpu_flush_baseline(w, frames_host, steps):
+    """A CPU flush baseline, timed apart from the CPU path (which has no
+    flush): the oracle's single-thread flush (STR sort, record placement, CTF
+    widths, bitmaps, key packing) on every 10th object of the CPU sample, with
+    a flush at every step (kv_restriction 1) and records from the third step
+    on (minimum meeting 1 s). Reports records flushed per second of flush."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    sub = [f[::10].copy() for f in frames_host[:steps + 1]]
+    cfg = dict(w["cfg"], num_objects=int(len(sub[0])), kv_restriction=1, min_meet_time=1,
+               schema_mode=1 if w["cfg"].get("dynamic_schema") else 0)
+    orc = O.Oracle(cfg, w["box"])
+    ms, recs, flushes = 0.0, 0, 0
+    for t, xy in enumerate(sub):
+        r = orc.step(xy, t)
+        if r["flushed"]:
+            m, k = orc.last_flush()
+            ms += m
+            recs += k
+            flushes += 1
+    orc.close()
+    if not flushes or ms <= 0:
+        return None
+    return dict(value=round(recs / (ms * 1e-3), 1), unit="records/s", cores=1, kind="port",
+                ms_per_flush=round(ms / flushes, 2), records_per_flush=int(recs / flushes),
+                sample=f"the oracle's flush on {len(sub[0])} objects of the same workload (every 10th of the CPU "
+                       f"path's sample), {flushes} flushes of {steps + 1} steps; {ms / 1e3:.1f} s of flush")
+
+
 def measured_traffic(kernel, workload, n):
     """HBM bytes per launch of a kernel, from the committed rocprofv3 counter
     summary for this workload (FETCH_SIZE and WRITE_SIZE, calibrated with a copy
@@ -601,7 +630,13 @@ def main():
                          for k, v in kernels.items()},
                 stages=stage_roofline(outs, n))
 
-    cpu = None
+    cpu = cpu_flush = None
+    flushed = [r for r in outs if r["flushed"]]
+    gpu_flush = None
+    if flushed:
+        fr = float(np.mean([r["kv_count"] for r in flushed]))
+        fm = float(np.mean([r["ms_flush"] for r in flushed]))
+        gpu_flush = dict(records_per_s=round(fr / (fm * 1e-3), 1), ms_per_flush=round(fm, 3), records_per_flush=int(fr))
     if not args.no_cpu:
         trace = DeviceTrace(gen)
         frames_host = []
@@ -613,6 +648,7 @@ def main():
             frames_host.append(np.ascontiguousarray(buf[ids]))  # the sample's rows only
         trace.close()
         cpu = cpu_baseline(w, frames_host, args.cpu_steps, args.schema)
+        cpu_flush = cpu_flush_baseline(w, frames_host, 12)
         del frames_host
         print(f"[bench] cpu baseline: {cpu['value']:.0f} point-updates/s on {cpu['cores']} threads", file=sys.stderr,
               flush=True)
@@ -643,6 +679,7 @@ def main():
         "host_input_value": round(host_value, 1) if host_value else None,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "flush": {"gpu": gpu_flush, "cpu_baseline": cpu_flush},
         "serving": serving,
     }
     print(json.dumps(line), flush=True)

@@ -40,6 +40,7 @@
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
+#include <time.h>
 #include <string.h>
 
 typedef unsigned __int128 u128;
@@ -101,6 +102,8 @@ typedef struct orc {
     orc_config cfg;
     double x_buffer, y_buffer;
     char err[256];
+    double last_flush_ms;           /* wall time of the last flush inside a step (a CPU flush baseline) */
+    uint64_t last_flush_records;
     uint64_t counts[ORC_C_NUM];
 
     /* per step (with a dynamic schema the nodes persist; free slots are listed) */
@@ -1186,6 +1189,24 @@ oom:
     return -1;
 }
 
+/* a step's flush, timed on the wall clock (the CPU flush baseline's unit) */
+static int timed_flush(orc *o, uint32_t t) {
+    struct timespec t0, t1;
+    const uint64_t recs = o->n_recs;
+    clock_gettime(CLOCK_MONOTONIC, &t0);
+    int rc = flush(o, t);
+    clock_gettime(CLOCK_MONOTONIC, &t1);
+    o->last_flush_ms = (double)(t1.tv_sec - t0.tv_sec) * 1e3 + (double)(t1.tv_nsec - t0.tv_nsec) / 1e6;
+    o->last_flush_records = recs;
+    return rc;
+}
+
+double orc_last_flush_ms(void *vo, uint64_t *records) {
+    orc *o = (orc *)vo;
+    if (records) *records = o->last_flush_records;
+    return o->last_flush_ms;
+}
+
 /* ------------------------------------------------------------------ */
 /* step                                                                */
 /* ------------------------------------------------------------------ */
@@ -1208,7 +1229,7 @@ int orc_step(void *vo, const double *xy, uint32_t t) {
     if (update_meetings(o, t)) return -1;
     o->counts[ORC_C_KV_COUNT] = o->n_recs;
     if (o->n_recs > o->cfg.kv_restriction) { /* geometry.cu:1752 */
-        if (flush(o, t)) return -1;
+        if (timed_flush(o, t)) return -1;
         o->counts[ORC_C_FLUSHED] = 1;
     }
     /* the schema update for the next step (geometry.cu:2159-2182; process.cpp:608-612) */
@@ -1277,7 +1298,7 @@ int orc_step_meetings(void *vo, const orc_meeting_unit *u, uint64_t n, uint32_t 
     if (update_meetings(o, t)) return -1;
     o->counts[ORC_C_KV_COUNT] = o->n_recs;
     if (o->n_recs > o->cfg.kv_restriction) {
-        if (flush(o, t)) return -1;
+        if (timed_flush(o, t)) return -1;
         o->counts[ORC_C_FLUSHED] = 1;
     }
     return 0;

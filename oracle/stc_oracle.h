@@ -176,6 +176,10 @@ void orc_get_step_meetings(void *o, orc_meeting_unit *u);
 int orc_step_meetings(void *o, const orc_meeting_unit *u, uint64_t n, uint32_t t);
 /* merge runs [first, first + count) into one run in their place (tools/compaction.cpp) */
 int orc_compact_runs(void *o, uint32_t first, uint32_t count);
+/* wall-clock ms of the last flush a step ran, and its record count (the
+ * single-thread CPU flush baseline: STR, record placement, widths, bitmaps,
+ * key packing) */
+double orc_last_flush_ms(void *o, uint64_t *records);
 
 uint64_t orc_brute_pairs(const double *xy, uint32_t n, double r, int probe_low,
                          uint64_t *keys, uint64_t cap);

@@ -79,6 +79,8 @@ def load():
         lib.orc_last_error.restype = C.c_char_p
         lib.orc_last_error.argtypes = [V]
         lib.orc_counts.argtypes = [V, V]
+        lib.orc_last_flush_ms.restype = C.c_double
+        lib.orc_last_flush_ms.argtypes = [V, C.POINTER(C.c_uint64)]
         for name in ["orc_num_pairs", "orc_num_leaves", "orc_num_nodes", "orc_num_units", "orc_num_active",
                      "orc_num_records"]:
             getattr(lib, name).restype = C.c_uint64
@@ -187,6 +189,12 @@ class Oracle:
         if rc != 0:
             raise RuntimeError("oracle: " + self.lib.orc_last_error(self.h).decode())
         return self.counts()
+
+    def last_flush(self):
+        """(wall ms, records) of the last flush a step ran"""
+        r = C.c_uint64()
+        ms = self.lib.orc_last_flush_ms(self.h, C.byref(r))
+        return ms, r.value
 
     def build_schema(self, xy):
         """dynamic schema: the initial tree from one timestep's points"""
