@@ -465,7 +465,9 @@ def stage_roofline(outs, n):
     if flushed:
         R = mean("kv_count", flushed)
         # STR 2 x 12 B per object, main sort 2 x 32 B, bitmap 16 B and encode 32 B + ~13 B packed per record
-        stages["flush"] = (mean("ms_flush", flushed), 24.0 * n + (64.0 + 16.0 + 32.0 + 13.0) * R)
+        # the flush's own device time: with async_flush a step reports the previous step's flush
+        done = [r for r in outs if r["ms_flush_done"] > 0]
+        stages["flush"] = (mean("ms_flush_done", done), 24.0 * n + (64.0 + 16.0 + 32.0 + 13.0) * R)
     out = {}
     for k, (ms, b) in stages.items():
         gbs = b / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
@@ -483,6 +485,9 @@ def main():
     ap.add_argument("--schema", choices=["dynamic", "rebuild"], default="dynamic",
                     help="QTree mode: kept and updated across steps (the reference's default) or rebuilt every step "
                          "(--disable_dynamic_schema)")
+    ap.add_argument("--flush", choices=["async", "sync"], default="sync",
+                    help="a step's flush overlaps the next step up to its merge (stc_config.async_flush), or "
+                         "completes inside the step (process_with_gpu's order)")
     ap.add_argument("--cpu-steps", type=int, default=40,
                     help="timed steps of the CPU baseline (about 0.25 s each on 16 threads: ~10 s of CPU work)")
     ap.add_argument("--host-steps", type=int, default=10,
@@ -522,7 +527,7 @@ def main():
     import torch
     from stcdb_amd import DeviceTrace, Pipeline, workload
 
-    over = {"dynamic_schema": 1 if args.schema == "dynamic" else 0}
+    over = {"dynamic_schema": 1 if args.schema == "dynamic" else 0, "async_flush": 1 if args.flush == "async" else 0}
     if args.objects:
         over["num_objects"] = args.objects
     w = workload(args.workload, **over)
@@ -574,7 +579,7 @@ def main():
     elapsed = time.perf_counter() - t0
     for i, r in enumerate(outs):
         print(f"[bench] step {args.warmup + i}: {r['ms_total']:.1f} ms (collect {r['ms_count_kernel']:.1f}, "
-              f"place {r['ms_refine_kernel']:.1f}, merge {r['ms_identify_kernel']:.1f}, flush {r['ms_flush']:.1f}, "
+              f"place {r['ms_refine_kernel']:.1f}, merge {r['ms_identify_kernel']:.1f}, flush {r['ms_flush_done']:.1f}, "
               f"schema {r['ms_schema']:.2f}) pairs {r['pairs']} records {r['new_records']} leaves {r['leaves']} "
               f"flushed {r['flushed']}", file=sys.stderr, flush=True)
 
@@ -635,7 +640,7 @@ def main():
     gpu_flush = None
     if flushed:
         fr = float(np.mean([r["kv_count"] for r in flushed]))
-        fm = float(np.mean([r["ms_flush"] for r in flushed]))
+        fm = float(np.mean([r["ms_flush_done"] for r in outs if r["ms_flush_done"] > 0]))
         gpu_flush = dict(records_per_s=round(fr / (fm * 1e-3), 1), ms_per_flush=round(fm, 3), records_per_flush=int(fr))
     if not args.no_cpu:
         trace = DeviceTrace(gen)
@@ -671,7 +676,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded street-grid walk generated on the device)",
-        "config": {"workload": args.workload, "objects": n, "schema": args.schema, "reach_m": cfg["reach_distance"],
+        "config": {"workload": args.workload, "objects": n, "schema": args.schema, "flush": args.flush,
+                   "reach_m": cfg["reach_distance"],
                    "min_meet_s": cfg["min_meet_time"], "leaf_capacity": cfg["grid_capacity"],
                    "kv_restriction": cfg["kv_restriction"], "ctf_count": cfg["ctf_count"],
                    "pairs_per_step": int(pairs),

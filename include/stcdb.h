@@ -64,6 +64,14 @@ typedef struct stc_config {
                                      Zero: the QTree is rebuilt from every step's points
                                      (--disable_dynamic_schema, geometry.cu:1629-1651). */
     uint32_t schema_update_delay; /* config->schema_update_delay (config.h:38); 0 counts as 1 */
+    uint32_t async_flush;         /* nonzero: a step's flush runs on its own stream from a host thread
+                                     and overlaps the next step up to that step's merge, which reuses
+                                     the memtable. The run is the same; every call that reads runs
+                                     (stc_num_runs, stc_get_run_info, stc_get_run, stc_dump_run,
+                                     stc_drop_runs, stc_compact_runs, stc_sync, stc_get_records)
+                                     waits for it, and query batches see the state before the flush
+                                     until then. Zero: the flush completes inside the step
+                                     (process_with_gpu's order). */
 } stc_config;
 
 /* A QTree node: the reference's QTSchema (src/index/QTree.h:49-62), 80 bytes,
@@ -117,6 +125,8 @@ typedef struct stc_step_out {
     uint64_t new_keys;         /* pairs that opened a meeting */
     float ms_count_kernel;     /* the pair collect pass (last launch) */
     float ms_schema;           /* dynamic schema update after the step (pro.index_update_time) */
+    float ms_flush_done;       /* device time of the flush that completed during this call: this
+                                  step's (= ms_flush), or with async_flush the previous step's */
 } stc_step_out;
 
 /* CTF metadata of a flushed run: the fields of the reference's CTF
@@ -252,7 +262,9 @@ int stc_get_run_info(void *ctx, uint32_t run, stc_run_info *info);
 int stc_get_run(void *ctx, uint32_t run, stc_ctf_meta *ctfs, uint8_t *key_bytes, uint8_t *bitmaps,
                 uint16_t *sids, uint64_t *o_keys, float *o_boxes, uint8_t *o_bitmap,
                 uint64_t *sorted_keys, float *sorted_boxes);
-int stc_drop_runs(void *ctx); /* release host copies of flushed runs (owed copies still complete) */
+int stc_drop_runs(void *ctx); /* release host copies of flushed runs (owed copies still complete); with an
+                                 asynchronous flush in progress the release happens when it joins, its run
+                                 included */
 /* CTF::get_ctf_bits (src/tracing/step_merge.cpp:140-186), which the
  * reference's host runs for every CTF of a flush (geometry.cu:1960-1964):
  * fills the id, duration, end, low_x, low_y, edge and mbr bit widths, key_bit,

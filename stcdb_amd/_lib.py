@@ -37,6 +37,7 @@ class StcConfig(C.Structure):
         ("device_runs", C.c_uint32),
         ("dynamic_schema", C.c_uint32),
         ("schema_update_delay", C.c_uint32),
+        ("async_flush", C.c_uint32),
     ]
 
 
@@ -75,6 +76,7 @@ class StcStepOut(C.Structure):
         ("new_keys", C.c_uint64),
         ("ms_count_kernel", C.c_float),
         ("ms_schema", C.c_float),
+        ("ms_flush_done", C.c_float),
     ]
 
     def as_dict(self):

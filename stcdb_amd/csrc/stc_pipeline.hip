@@ -1833,6 +1833,7 @@ struct RunHost {
 /* Pinned host blocks for run payloads, recycled once their copies are done. */
 struct PinnedPool {
     std::vector<PinnedBlock> idle, retiring;
+    std::mutex mu; /* an asynchronous flush acquires from its own thread */
     void reclaim() {
         for (size_t i = 0; i < retiring.size();) {
             if (hipEventQuery(retiring[i].ev) == hipSuccess) {
@@ -1845,6 +1846,7 @@ struct PinnedPool {
         }
     }
     PinnedBlock acquire(size_t bytes) {
+        std::lock_guard<std::mutex> lk(mu);
         reclaim();
         int best = -1;
         for (size_t i = 0; i < idle.size(); i++)
@@ -1871,9 +1873,11 @@ struct PinnedPool {
         return b;
     }
     void release(PinnedBlock b) {
+        std::lock_guard<std::mutex> lk(mu);
         if (b.p) retiring.push_back(b);
     }
     void clear() {
+        std::lock_guard<std::mutex> lk(mu);
         for (auto *v : {&idle, &retiring})
             for (auto &b : *v) {
                 (void)hipEventSynchronize(b.ev);
@@ -1956,6 +1960,19 @@ struct StcCtx {
     DBuf<uint64_t> ukeys, ukeys_alt;
     DBuf<uint32_t> uidx, uidx_alt, utemp;
     std::vector<RunHost> runs;
+    /* asynchronous flush (cfg.async_flush): a step's flush runs on fs from a host thread until the
+     * next step's merge, or any call that reads runs, joins it */
+    hipStream_t fs = nullptr;
+    hipEvent_t fev[3] = {};              /* go (recorded on s), start and end (on fs) */
+    std::thread fthr;
+    bool frun = false;                   /* a flush thread started and not yet joined */
+    int frc = 0;                         /* its status and message */
+    std::string ferr;
+    FlushOut fout;
+    bool flush_fresh = false;            /* the joined run's copies wait for the next collect pass */
+    bool fms_pending = false;            /* a joined flush whose device time is not reported yet */
+    float flush_done_ms = 0;
+    bool drop_pending = false;           /* stc_drop_runs came while a flush thread ran */
     PinnedPool pinned;
     std::vector<PinnedBlock> deferred;   /* blocks of dropped runs whose host copy is not queued yet */
     QueryBufs *qb = nullptr;
@@ -2052,6 +2069,13 @@ static int fail(StcCtx *c, int code, const std::string &msg) {
 
 static inline uint32_t blocks_for(uint64_t n, uint32_t t) { return (uint32_t)((n + t - 1) / t); }
 
+/* return the status of joining a pending asynchronous flush (its message is already set) */
+#define JOIN_FLUSH(c)                  \
+    do {                               \
+        const int jrc_ = join_flush(c); \
+        if (jrc_) return jrc_;         \
+    } while (0)
+
 static int sync_counters(StcCtx *c) {
     CK(hipMemcpyAsync(c->hcnt, c->dcnt, sizeof(Counters), hipMemcpyDeviceToHost, c->s));
     CK(hipStreamSynchronize(c->s));
@@ -2137,8 +2161,14 @@ static void settle_deferred(StcCtx *c) {
     }
 }
 
-/* queue the host copies of the last flushed run on the copy stream */
-static int queue_run_copies(StcCtx *c) {
+/* queue the host copies of the last flushed run on the copy stream. None
+ * while a flush thread owns the flush buffers; a run just taken from one waits
+ * for the next collect pass (copies to the host slow concurrent kernels that
+ * write memory, and the collect pass mostly computes) unless `now`. */
+static int queue_run_copies(StcCtx *c, bool now = true) {
+    if (c->frun) return 0;
+    if (c->flush_fresh && !now) return 0;
+    c->flush_fresh = false;
     if (stc_flush_release(&c->flushw, c->s)) return STC_E_HIP;
     settle_deferred(c);
     return 0;
@@ -2182,6 +2212,58 @@ static int keep_run_on_device(StcCtx *c, RunHost &run) {
     q.source = 0; /* set per query from the run's position */
     CK(hipStreamSynchronize(s)); /* the host metadata vector is read by the copy above */
     return stc_query_ctf_stats(q, run.ctfs.data(), run.stats, s) ? fail(c, STC_E_HIP, std::string("run statistics: ") + g_last_fail) : 0;
+}
+
+/* release every run's host block (a queued copy still completes) and HBM block */
+static void drop_all_runs(StcCtx *c) {
+    wait_all_readers(c); /* the runs' device blocks are freed below */
+    for (auto &r : c->runs) {
+        release_block(c, r.blk, true);
+        if (r.dev) {
+            (void)hipStreamSynchronize(c->s);
+            (void)hipFree(r.dev);
+            r.dev = nullptr;
+        }
+    }
+    c->runs.clear();
+    publish(c);
+}
+
+/* Wait for an asynchronous flush and take its run: the main stream waits for
+ * the flush's device work (the next merge reuses the memtable that the flush
+ * reads and resets), the run joins the list (and HBM, with device_runs), and
+ * the state with it is published. */
+static int join_flush(StcCtx *c) {
+    if (!c->frun) return STC_OK;
+    c->fthr.join();
+    c->frun = false;
+    settle_deferred(c);
+    if (c->frc) {
+        release_block(c, c->fout.blk);
+        c->fout = FlushOut();
+        return fail(c, c->frc, "flush: " + c->ferr);
+    }
+    CK(hipStreamWaitEvent(c->s, c->fev[2], 0));
+    CK(hipEventSynchronize(c->fev[2]));
+    CK(hipEventElapsedTime(&c->flush_done_ms, c->fev[1], c->fev[2]));
+    c->fms_pending = true;
+    RunHost run;
+    run.info = c->fout.info;
+    run.ctfs.swap(c->fout.ctfs);
+    run.lay = c->fout.lay;
+    run.has_sorted = c->fout.has_sorted;
+    run.blk = c->fout.blk;
+    c->fout = FlushOut();
+    if (c->cfg.device_runs) CKRC(keep_run_on_device(c, run));
+    c->runs.push_back(std::move(run));
+    c->flush_fresh = true;
+    if (c->drop_pending) { /* stc_drop_runs came while the flush ran */
+        c->drop_pending = false;
+        drop_all_runs(c);
+    } else {
+        publish(c);
+    }
+    return STC_OK;
 }
 
 extern "C" {
@@ -2253,6 +2335,9 @@ int stc_create(const stc_config *cfg, const stc_box *mbr, void **out) {
     };
     if (hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking) != hipSuccess) return bad("stream");
     if (hipStreamCreateWithFlags(&c->qs, hipStreamNonBlocking) != hipSuccess) return bad("query stream");
+    if (hipStreamCreateWithFlags(&c->fs, hipStreamNonBlocking) != hipSuccess) return bad("flush stream");
+    for (int i = 0; i < 3; i++)
+        if (hipEventCreate(&c->fev[i]) != hipSuccess) return bad("flush events");
     for (int i = 0; i < 14; i++) hipEventCreate(&c->ev[i]);
     uint32_t n = c->n;
     size_t node_cap = 1 + 4ull * NLEVELS * (n / (c->cap + 1) + 1);
@@ -2303,6 +2388,7 @@ int stc_create(const stc_config *cfg, const stc_box *mbr, void **out) {
 int stc_destroy(void *vc) {
     StcCtx *c = (StcCtx *)vc;
     if (!c) return STC_OK;
+    (void)join_flush(c);
     wait_all_readers(c);
     if (c->s) hipStreamSynchronize(c->s);
     c->xy_in.release(); c->codes.release(); c->idx.release(); c->codes_alt.release(); c->idx_alt.release();
@@ -2344,6 +2430,8 @@ int stc_destroy(void *vc) {
     stc_schema_free(&c->sch);
     if (c->s) hipStreamDestroy(c->s);
     if (c->qs) hipStreamDestroy(c->qs);
+    if (c->fs) hipStreamDestroy(c->fs);
+    for (int i = 0; i < 3; i++) if (c->fev[i]) hipEventDestroy(c->fev[i]);
     delete c;
     return STC_OK;
 }
@@ -2663,8 +2751,14 @@ static int step_tail(StcCtx *c, uint32_t t, StepGeom &g, stc_step_out *out) {
     hipStream_t s = c->s;
     Counters *cnt = c->dcnt;
     /* 9: merge the step's pairs with the previous step's meetings; meetings
-     * without a pair expire and the long ones become records */
-    if (queue_run_copies(c)) return fail(c, STC_E_HIP, std::string("run copies: ") + g_last_fail);
+     * without a pair expire and the long ones become records. An asynchronous
+     * flush of the previous step must be done first: the merge writes the
+     * memtable and the per-object state that it reads and resets. */
+    {
+        const int rc = join_flush(c);
+        if (rc) return rc;
+    }
+    if (queue_run_copies(c, false)) return fail(c, STC_E_HIP, std::string("run copies: ") + g_last_fail);
     const int cu = c->cur, nx = cu ^ 1;
     const uint64_t kv_before = c->kv_count;
     if (ensure_records(c, kv_before + 2 * std::max<uint64_t>(c->last_expired + c->last_expired / 4, 1 << 16) + 2))
@@ -2739,7 +2833,6 @@ static int step_tail(StcCtx *c, uint32_t t, StepGeom &g, stc_step_out *out) {
     /* 10: flush */
     uint32_t flushed = 0;
     if (c->kv_count > c->cfg.kv_restriction) {
-        RunHost run;
         FlushArgs fa{};
         fa.n = n;
         fa.ctf_count = c->cfg.ctf_count;
@@ -2759,22 +2852,49 @@ static int step_tail(StcCtx *c, uint32_t t, StepGeom &g, stc_step_out *out) {
         fa.o_boxs = c->o_boxs;
         fa.sids = c->sids;
         fa.reset_memtable = 1;
-        FlushOut fo;
-        fo.acquire = pool_acquire;
-        fo.acquire_ctx = &c->pinned;
-        int rc = stc_flush_run(&c->flushw, &fa, &fo, s);
-        settle_deferred(c);
-        if (rc) {
-            release_block(c, fo.blk);
-            return fail(c, rc, std::string("flush: ") + g_last_fail);
+        if (c->cfg.async_flush) {
+            /* the flush on its own stream and thread, after this step's records are complete */
+            CK(hipEventRecord(c->fev[0], s));
+            c->frc = 0;
+            c->ferr.clear();
+            c->fout = FlushOut();
+            c->fout.acquire = pool_acquire;
+            c->fout.acquire_ctx = &c->pinned;
+            c->frun = true;
+            c->fthr = std::thread([c, fa]() {
+                if (hipSetDevice(c->device) != hipSuccess || hipStreamWaitEvent(c->fs, c->fev[0], 0) != hipSuccess ||
+                    hipEventRecord(c->fev[1], c->fs) != hipSuccess) {
+                    c->frc = STC_E_HIP;
+                    c->ferr = "flush stream set-up";
+                    return;
+                }
+                c->frc = stc_flush_run(&c->flushw, &fa, &c->fout, c->fs);
+                if (c->frc) {
+                    c->ferr = g_last_fail;
+                } else if (hipEventRecord(c->fev[2], c->fs) != hipSuccess) {
+                    c->frc = STC_E_HIP;
+                    c->ferr = "flush end event";
+                }
+            });
+        } else {
+            RunHost run;
+            FlushOut fo;
+            fo.acquire = pool_acquire;
+            fo.acquire_ctx = &c->pinned;
+            int rc = stc_flush_run(&c->flushw, &fa, &fo, s);
+            settle_deferred(c);
+            if (rc) {
+                release_block(c, fo.blk);
+                return fail(c, rc, std::string("flush: ") + g_last_fail);
+            }
+            run.info = fo.info;
+            run.ctfs.swap(fo.ctfs);
+            run.lay = fo.lay;
+            run.has_sorted = fo.has_sorted;
+            run.blk = fo.blk;
+            if (c->cfg.device_runs) CKRC(keep_run_on_device(c, run));
+            c->runs.push_back(std::move(run));
         }
-        run.info = fo.info;
-        run.ctfs.swap(fo.ctfs);
-        run.lay = fo.lay;
-        run.has_sorted = fo.has_sorted;
-        run.blk = fo.blk;
-        if (c->cfg.device_runs) CKRC(keep_run_on_device(c, run));
-        c->runs.push_back(std::move(run));
         /* the memtable's per-object state was reset inside the flush (reset_memtable) */
         c->kv_count = 0;
         c->oversize_oid_count = 0;
@@ -2796,7 +2916,7 @@ static int step_tail(StcCtx *c, uint32_t t, StepGeom &g, stc_step_out *out) {
 
     c->stepped = true;
     c->last_t = t;
-    publish(c);
+    if (!c->frun) publish(c); /* with a flush pending, the state is published when its run joins */
     if (out) {
         memset(out, 0, sizeof(*out));
         out->nodes = g.nodes;
@@ -2829,6 +2949,12 @@ static int step_tail(StcCtx *c, uint32_t t, StepGeom &g, stc_step_out *out) {
         out->new_keys = h.new_keys;
         if (g.units || g.nodes) hipEventElapsedTime(&out->ms_count_kernel, c->ev[10], c->ev[11]);
         if (schema_step) hipEventElapsedTime(&out->ms_schema, c->ev[12], c->ev[13]);
+        if (flushed && !c->cfg.async_flush) {
+            out->ms_flush_done = out->ms_flush;
+        } else if (c->fms_pending) {
+            out->ms_flush_done = c->flush_done_ms;
+            c->fms_pending = false;
+        }
     }
     return STC_OK;
 }
@@ -3021,6 +3147,7 @@ int stc_get_active(void *vc, uint64_t *keys, uint32_t *start, uint32_t *end, flo
 int stc_get_records(void *vc, uint64_t *keys, float *boxes, uint64_t cap, uint64_t *n) {
     StcCtx *c = (StcCtx *)vc;
     if (!c || !n) return STC_E_ARG;
+    JOIN_FLUSH(c);
     *n = c->kv_count;
     uint64_t m = std::min<uint64_t>(cap, c->kv_count);
     if (m && keys) CK(hipMemcpy(keys, c->rec_keys, m * 16, hipMemcpyDeviceToHost));
@@ -3110,13 +3237,16 @@ int stc_get_schema(void *vc, stc_qtnode *nodes, uint64_t cap, uint64_t *n) {
 int stc_num_runs(void *vc, uint32_t *n) {
     StcCtx *c = (StcCtx *)vc;
     if (!c || !n) return STC_E_ARG;
+    JOIN_FLUSH(c);
     *n = (uint32_t)c->runs.size();
     return STC_OK;
 }
 
 int stc_get_run_info(void *vc, uint32_t run, stc_run_info *info) {
     StcCtx *c = (StcCtx *)vc;
-    if (!c || !info || run >= c->runs.size()) return STC_E_ARG;
+    if (!c || !info) return STC_E_ARG;
+    JOIN_FLUSH(c);
+    if (run >= c->runs.size()) return STC_E_ARG;
     *info = c->runs[run].info;
     return STC_OK;
 }
@@ -3124,7 +3254,9 @@ int stc_get_run_info(void *vc, uint32_t run, stc_run_info *info) {
 int stc_get_run(void *vc, uint32_t run, stc_ctf_meta *ctfs, uint8_t *key_bytes, uint8_t *bitmaps, uint16_t *sids,
                 uint64_t *o_keys, float *o_boxes, uint8_t *o_bitmap, uint64_t *sorted_keys, float *sorted_boxes) {
     StcCtx *c = (StcCtx *)vc;
-    if (!c || run >= c->runs.size()) return STC_E_ARG;
+    if (!c) return STC_E_ARG;
+    JOIN_FLUSH(c);
+    if (run >= c->runs.size()) return STC_E_ARG;
     RunHost &R = c->runs[run];
     if (queue_run_copies(c)) return fail(c, STC_E_HIP, std::string("run copies: ") + g_last_fail);
     CK(hipEventSynchronize(R.blk.ev));
@@ -3151,6 +3283,7 @@ int stc_get_run(void *vc, uint32_t run, stc_ctf_meta *ctfs, uint8_t *key_bytes, 
 int stc_sync(void *vc) {
     StcCtx *c = (StcCtx *)vc;
     if (!c) return STC_E_ARG;
+    JOIN_FLUSH(c);
     if (queue_run_copies(c)) return fail(c, STC_E_HIP, std::string("run copies: ") + g_last_fail);
     CK(hipStreamSynchronize(c->s));
     if (c->flushw.cs) CK(hipStreamSynchronize(c->flushw.cs));
@@ -3312,7 +3445,9 @@ static bool write_file(const std::string &path, const void *a, size_t na, const 
 
 int stc_dump_run(void *vc, uint32_t run, const char *dir, const char *prefix, uint32_t ctb) {
     StcCtx *c = (StcCtx *)vc;
-    if (!c || !dir || !prefix || run >= c->runs.size()) return STC_E_ARG;
+    if (!c || !dir || !prefix) return STC_E_ARG;
+    JOIN_FLUSH(c);
+    if (run >= c->runs.size()) return STC_E_ARG;
     RunHost &R = c->runs[run];
     if (queue_run_copies(c)) return fail(c, STC_E_HIP, std::string("run copies: ") + g_last_fail);
     CK(hipEventSynchronize(R.blk.ev));
@@ -3386,17 +3521,11 @@ int stc_ctf_widths(stc_ctf_meta *ctf, const stc_box *map_mbr, uint32_t num_objec
 int stc_drop_runs(void *vc) {
     StcCtx *c = (StcCtx *)vc;
     if (!c) return STC_E_ARG;
-    wait_all_readers(c); /* the runs' device blocks are freed below */
-    for (auto &r : c->runs) {
-        release_block(c, r.blk, true);
-        if (r.dev) {
-            (void)hipStreamSynchronize(c->s);
-            (void)hipFree(r.dev);
-            r.dev = nullptr;
-        }
+    if (c->frun) { /* the runs go when the pending flush joins, its own with them */
+        c->drop_pending = true;
+        return STC_OK;
     }
-    c->runs.clear();
-    publish(c);
+    drop_all_runs(c);
     return STC_OK;
 }
 
@@ -3409,6 +3538,7 @@ int stc_drop_runs(void *vc) {
 int stc_compact_runs(void *vc, uint32_t first, uint32_t count) {
     StcCtx *c = (StcCtx *)vc;
     if (!c || count == 0) return STC_E_ARG;
+    JOIN_FLUSH(c);
     wait_all_readers(c); /* the merged runs' blocks are freed and the memtable buffers reused */
     if ((uint64_t)first + count > c->runs.size()) return fail(c, STC_E_ARG, "run range out of bounds");
     if (count == 1) return STC_OK;

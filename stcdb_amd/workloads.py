@@ -54,7 +54,8 @@ def map_box(gp):
 
 def pipeline_config(num_objects, reach_distance, min_meet_time, grid_capacity=200, kv_restriction=67108864,
                     ctf_count=100, compat=0, area_threshold=100.0, max_meet_time=4096, grid_amplify=2.0,
-                    start_time=0, num_meeting_buckets=0, keep_pairs=0, dynamic_schema=0, schema_update_delay=1):
+                    start_time=0, num_meeting_buckets=0, keep_pairs=0, dynamic_schema=0, schema_update_delay=1,
+                    async_flush=0):
     """The reference's configuration defaults (src/util/config.h:15-86):
     kv_restriction = G_bytes(2) * 33554432, CTF_count 100, grid_amplify 2,
     max_meet_time 4096; area_threshold 100 is the file_size = 10 entry of the
@@ -64,7 +65,7 @@ def pipeline_config(num_objects, reach_distance, min_meet_time, grid_capacity=20
                 kv_restriction=kv_restriction, ctf_count=ctf_count, compat=compat,
                 num_meeting_buckets=num_meeting_buckets, reach_distance=reach_distance,
                 area_threshold=area_threshold, keep_pairs=keep_pairs, dynamic_schema=dynamic_schema,
-                schema_update_delay=schema_update_delay)
+                schema_update_delay=schema_update_delay, async_flush=async_flush)
 
 
 WORKLOADS = {
