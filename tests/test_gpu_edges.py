@@ -23,7 +23,7 @@ def config(n, **over):
     return w["cfg"], w["box"]
 
 
-def run_frames(cfg, box, frames, tag):
+def run_frames(cfg, box, frames, tag, min_pairs=0):
     gpu = Pipeline(cfg, box)
     orc = O.Oracle(cfg, box)
     print(f"[edges {tag}] n={cfg['num_objects']} steps={len(frames)}", flush=True)
@@ -32,6 +32,7 @@ def run_frames(cfg, box, frames, tag):
         g = gpu.step(xy, t)
         o = orc.step(xy, t)
         band += g["ambiguous_pairs"]
+        assert g["pairs"] >= min_pairs, (tag, t, g["pairs"])
         for k, ko in [("leaves", "leaves"), ("nodes", "nodes"), ("overflow_points", "overflow"),
                       ("active_meetings", "active"), ("expired", "expired"), ("kv_count", "kv_count"),
                       ("flushed", "flushed")]:
@@ -199,3 +200,14 @@ def test_full_size_step_invariants():
     assert prev["flushed"] and prev["new_records"] > 0
     pipe.close()
     trace.close()
+
+
+def test_one_dense_cluster_every_segment_length():
+    """5,000 objects within a few metres: every pair is within reach, so the
+    owners' segments take every length from 0 to 4,999 slots (the place pass,
+    the wave sort of 513-1024, the block sort of 1025-4096 and the sort over
+    global scratch beyond), and the first step's pair sources overflow their
+    initial buffer (grown and collected again)"""
+    cfg, box = config(5000, grid_capacity=200)
+    frames = walk(box, 5000, 4, seed=9, step_m=0.2, spread=0.01)
+    run_frames(cfg, box, frames, "dense-cluster", min_pairs=10_000_000)  # ~95 % of the 12.5M possible pairs
