@@ -1973,6 +1973,7 @@ struct StcCtx {
     bool fms_pending = false;            /* a joined flush whose device time is not reported yet */
     float flush_done_ms = 0;
     bool drop_pending = false;           /* stc_drop_runs came while a flush thread ran */
+    bool lp_valid = false;               /* lp holds a position step's leaf order (a permutation of the points) */
     PinnedPool pinned;
     std::vector<PinnedBlock> deferred;   /* blocks of dropped runs whose host copy is not queued yet */
     QueryBufs *qb = nullptr;
@@ -2505,7 +2506,11 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
     hipLaunchKernelGGL(k_prep, dim3(blocks_for(n, 256)), dim3(256), 0, s, xy, n, c->map, c->codes.p, c->idx.p,
                        c->dlon.p, c->posf.p, cnt);
     CK(hipGetLastError());
-    CKRC(stc_radix_sort_u32(c->codes.p, c->idx.p, c->codes_alt.p, c->idx_alt.p, n, 28, c->temp.p, s));
+    /* the order the partition walks the points in: Morton order, or with a kept schema the previous
+     * step's leaf order (objects move a few metres a step, so it is as coherent, without the sort) */
+    const bool reuse_order = c->dyn && c->sch.present && c->lp_valid;
+    if (!reuse_order) CKRC(stc_radix_sort_u32(c->codes.p, c->idx.p, c->codes_alt.p, c->idx_alt.p, n, 28, c->temp.p, s));
+    const uint32_t *walk = reuse_order ? (const uint32_t *)c->lp.p : (const uint32_t *)c->idx.p;
 
     /* 3: tree: rebuilt from the step's points, or the persistent dynamic schema */
     uint32_t n_nodes, n_leaves;
@@ -2537,7 +2542,7 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
     if (!c->dyn)
         hipLaunchKernelGGL(k_leaf_nodes, dim3(blocks_for(n_nodes, 256)), dim3(256), 0, s, c->nodes.p, n_nodes,
                            c->leaf_node.p);
-    hipLaunchKernelGGL(k_partition, dim3(blocks_for(n, 256)), dim3(256), 0, s, xy, (const uint32_t *)c->idx.p, n,
+    hipLaunchKernelGGL(k_partition, dim3(blocks_for(n, 256)), dim3(256), 0, s, xy, walk, n,
                        (const Node *)tree_nodes(c), c->x_buffer, c->y_buffer, c->pt_leaf.p, c->pt_lv.p, c->leaf_count.p, cnt);
     CK(hipGetLastError());
     /* 5: leaves in id order */
@@ -2548,6 +2553,7 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
                             std::max(1, ilog2_ceil(n_leaves)), c->temp.p, s));
     hipLaunchKernelGGL(k_gather, dim3(blocks_for(n, 256)), dim3(256), 0, s, xy, c->dlon.p, c->lp.p, c->pt_leaf.p,
                        c->leaf_start.p, n, c->lxy.p, c->ld.p, c->pt_rank.p, c->lleaf.p);
+    c->lp_valid = true; /* lp: a permutation of the points, in this step's leaf order */
     hipLaunchKernelGGL(k_overflow, dim3(blocks_for(n_leaves, 256)), dim3(256), 0, s, c->leaf_count.p, n_leaves,
                        c->buf, cnt);
     CK(hipGetLastError());
