@@ -74,7 +74,7 @@ def cpu_sample_ids(gen):
 
 def cpu_threads():
     """the host threads a CPU baseline may use: the process's CPUs, capped by
-    OMP_NUM_THREADS (the GPU box's share of its host)"""
+    OMP_NUM_THREADS (the share of the host this process is given)"""
     n = len(os.sched_getaffinity(0))
     cap = os.environ.get("OMP_NUM_THREADS")
     return max(1, min(n, int(cap))) if cap and cap.isdigit() else n
