@@ -2336,9 +2336,14 @@ int stc_create(const stc_config *cfg, const stc_box *mbr, void **out) {
     };
     if (hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking) != hipSuccess) return bad("stream");
     if (hipStreamCreateWithFlags(&c->qs, hipStreamNonBlocking) != hipSuccess) return bad("query stream");
-    if (hipStreamCreateWithFlags(&c->fs, hipStreamNonBlocking) != hipSuccess) return bad("flush stream");
-    for (int i = 0; i < 3; i++)
-        if (hipEventCreate(&c->fev[i]) != hipSuccess) return bad("flush events");
+    /* the flush stream only when it is used: HIP spreads a process's streams over its hardware queues in
+     * creation order, and with four streams per context every context's main stream would share one
+     * queue (measured: 64 batched streams of 5M objects step at 163M instead of 310M point-updates/s) */
+    if (cfg->async_flush) {
+        if (hipStreamCreateWithFlags(&c->fs, hipStreamNonBlocking) != hipSuccess) return bad("flush stream");
+        for (int i = 0; i < 3; i++)
+            if (hipEventCreate(&c->fev[i]) != hipSuccess) return bad("flush events");
+    }
     for (int i = 0; i < 14; i++) hipEventCreate(&c->ev[i]);
     uint32_t n = c->n;
     size_t node_cap = 1 + 4ull * NLEVELS * (n / (c->cap + 1) + 1);
