@@ -326,6 +326,8 @@ def batched(args):
     import torch
     from stcdb_amd import DeviceTrace, Pipeline, batched_streams, step_batch
     from stcdb_amd.tuner import CANDIDATES, tune_leaf_capacity
+    if args.tune_steps < 2:
+        sys.exit("--tune-steps must be at least 2: the tuner times every frame after a candidate's first")
     streams = batched_streams()
     total = sum(w["cfg"]["num_objects"] for w in streams)
     W, K = args.warmup, args.steps
@@ -611,18 +613,18 @@ def main():
     slots = mean("table_buckets")  # pair slots of a step = buckets of its meeting array
     kernels = {
         # collect pass: the N points in (16 B position + 4 B id), a 4 B count per owner out, one
-        # 8 B key (partner, source) per slot out
-        "k_pair_collect": ("pair collect pass: each combination decided once, keys per owner", mean("ms_count_kernel"),
-                           24.0 * n + 8.0 * slots),
+        # 4 B source (the other point's index) per slot out
+        "k_pair_collect": ("pair collect pass: each combination decided once, sources per owner", mean("ms_count_kernel"),
+                           24.0 * n + 4.0 * slots),
         # place pass: each owner's 8 B run descriptor and 4 B sources in, one 16 B slot (key +
         # the partner's float position) per pair out
         "k_pair_place": ("pair place pass: owner segments sorted by partner", mean("ms_refine_kernel"),
                          8.0 * n + 20.0 * slots),
-        # merge: the step's slots (16 B) and the previous meetings (32 B) in, the owners' float
-        # positions (8 B each) once, the new meetings (32 B) and 2 records (32 B each) per
+        # merge: the step's slots (16 B) and the previous meetings (28 B) in, the owners' float
+        # positions (8 B each) once, the new meetings (28 B) and 2 records (32 B each) per
         # recorded meeting out
         "k_meet_merge": ("merge with the previous step's meetings, expiry, records", mean("ms_identify_kernel"),
-                         16.0 * slots + 32.0 * slots + 32.0 * slots + 8.0 * n + 32.0 * mean("new_records")),
+                         16.0 * slots + 28.0 * slots + 28.0 * slots + 8.0 * n + 32.0 * mean("new_records")),
     }
     kname = max(kernels, key=lambda k: kernels[k][1])
     desc, ms, alg = kernels[kname]

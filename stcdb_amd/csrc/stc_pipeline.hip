@@ -59,6 +59,9 @@
 #include <type_traits>
 
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4))); /* a native vector for 16-byte streaming accesses */
+/* 4-byte aligned vectors for the streaming stores of the 28-byte buckets (7 dwords) */
+typedef uint32_t u32x4u __attribute__((ext_vector_type(4), aligned(4)));
+typedef uint32_t u32x3u __attribute__((ext_vector_type(3), aligned(4)));
 
 #define ONE_DIM 16384u
 #define NLEVELS 15
@@ -1265,6 +1268,7 @@ struct MemtableDev {
 struct MergeArgs {
     uint32_t now, min_meet, max_meet, end_time_min;
     MapBox m;
+    uint32_t prev;         /* the step that built the previous array: its live meetings' end time */
 };
 
 /* a meeting that expires at `now` is recorded when it lasted long enough and
@@ -1275,16 +1279,17 @@ __device__ inline bool meeting_emits(const Bucket &b, const MergeArgs &a) {
            (double)lo1 >= a.m.lo1 && (double)hi1 <= a.m.hi1;
 }
 
-/* the two records of an expired meeting at k, k + 1 (geometry.cu:717-734) */
-__device__ inline void write_meeting_records(const Bucket &b, unsigned long long k, uint32_t end_time_min,
+/* the two records of an expired meeting at k, k + 1 (geometry.cu:717-734);
+ * end: the step that last saw it */
+__device__ inline void write_meeting_records(const Bucket &b, uint32_t end, unsigned long long k, uint32_t end_time_min,
                                              const MemtableDev &mt) {
     const float4 box = make_float4(o2f(b.mbr[0]), o2f(b.mbr[1]), o2f(b.mbr[2]), o2f(b.mbr[3]));
-    const uint32_t duration = b.end - b.start;
+    const uint32_t duration = end - b.start;
     const uint32_t pid = (uint32_t)(b.key >> 32), target = (uint32_t)b.key;
     for (int sdir = 0; sdir < 2; sdir++) {
         const uint32_t a = sdir ? target : pid, c = sdir ? pid : target;
         if (k + sdir < mt.cap) {
-            mt.keys[2 * (k + sdir)] = ((uint64_t)duration << 32) | (uint32_t)(b.end - end_time_min);
+            mt.keys[2 * (k + sdir)] = ((uint64_t)duration << 32) | (uint32_t)(end - end_time_min);
             mt.keys[2 * (k + sdir) + 1] = ((uint64_t)a << 26) | (uint64_t)c;
             mt.boxes[k + sdir] = box;
         }
@@ -1440,12 +1445,13 @@ __global__ __launch_bounds__(THREADS) void k_seg_sort(const uint32_t *__restrict
 #define MSTAGE 32u   /* expired meetings a merge wave stages before reserving their records */
 
 /* write the wave's staged meetings' records: one counter reservation per batch */
-__device__ inline void flush_stage(const Bucket *stage, uint32_t staged, uint32_t end_time_min, const MemtableDev &mt,
+__device__ inline void flush_stage(const Bucket *stage, uint32_t staged, const MergeArgs &ma, const MemtableDev &mt,
                                    Counters *cnt, int lane) {
     unsigned long long kb = 0;
     if (lane == 0) kb = atomicAdd(&cnt->kv_count, 2ull * staged);
     kb = __shfl(kb, 0, STC_WAVE);
-    for (uint32_t e = lane; e < staged; e += STC_WAVE) write_meeting_records(stage[e], kb + 2ull * e, end_time_min, mt);
+    for (uint32_t e = lane; e < staged; e += STC_WAVE)
+        write_meeting_records(stage[e], ma.prev, kb + 2ull * e, ma.end_time_min, mt);
     __builtin_amdgcn_wave_barrier();
 }
 
@@ -1611,7 +1617,6 @@ __global__ __launch_bounds__(256) void k_meet_merge(const S *__restrict__ pn, ui
                 Bucket nb;
                 nb.key = key;
                 nb.start = dead ? 0u : (cont ? ob.start : ma.now);
-                nb.end = dead ? 0xFFFFFFFFu : ma.now;
                 nb.mbr[0] = dead ? 0u : (cont ? min(ob.mbr[0], q0) : q0);
                 nb.mbr[1] = dead ? 0u : (cont ? min(ob.mbr[1], q1) : q1);
                 nb.mbr[2] = dead ? 0u : (cont ? max(ob.mbr[2], q2) : q2);
@@ -1623,11 +1628,11 @@ __global__ __launch_bounds__(256) void k_meet_merge(const S *__restrict__ pn, ui
                         else n_fresh++;
                     }
                     /* streamed out past the caches: the array is read again only by the next step */
-                    u32x4 *d = (u32x4 *)(mnew + i0 + i);
-                    const u32x4 lo = {(uint32_t)nb.key, (uint32_t)(nb.key >> 32), nb.start, nb.end};
-                    const u32x4 hi = {nb.mbr[0], nb.mbr[1], nb.mbr[2], nb.mbr[3]};
-                    __builtin_nontemporal_store(lo, d);
-                    __builtin_nontemporal_store(hi, d + 1);
+                    uint32_t *d = (uint32_t *)(mnew + i0 + i);
+                    const u32x4u lo = {(uint32_t)nb.key, (uint32_t)(nb.key >> 32), nb.start, nb.mbr[0]};
+                    const u32x3u hi = {nb.mbr[1], nb.mbr[2], nb.mbr[3]};
+                    __builtin_nontemporal_store(lo, (u32x4u *)d);
+                    __builtin_nontemporal_store(hi, (u32x3u *)(d + 4));
                 }
             }
             __builtin_amdgcn_wave_barrier();
@@ -1651,7 +1656,7 @@ __global__ __launch_bounds__(256) void k_meet_merge(const S *__restrict__ pn, ui
                     const uint32_t total = (uint32_t)__popcll(em);
                     if (staged + total > MSTAGE) {
                         __builtin_amdgcn_wave_barrier();
-                        flush_stage(stage, staged, ma.end_time_min, mt, cnt, lane);
+                        flush_stage(stage, staged, ma, mt, cnt, lane);
                         staged = 0;
                     }
                     const uint32_t put = staged + (uint32_t)__popcll(em & lt);
@@ -1661,7 +1666,7 @@ __global__ __launch_bounds__(256) void k_meet_merge(const S *__restrict__ pn, ui
                         unsigned long long kb = 0;
                         if (lane == 0) kb = atomicAdd(&cnt->kv_count, 2ull * (total - MSTAGE));
                         kb = __shfl(kb, 0, STC_WAVE);
-                        if (emit && put >= MSTAGE) write_meeting_records(ob, kb + 2ull * (put - MSTAGE), ma.end_time_min, mt);
+                        if (emit && put >= MSTAGE) write_meeting_records(ob, ma.prev, kb + 2ull * (put - MSTAGE), ma.end_time_min, mt);
                         staged = MSTAGE;
                     } else {
                         staged += total;
@@ -1675,7 +1680,7 @@ __global__ __launch_bounds__(256) void k_meet_merge(const S *__restrict__ pn, ui
     }
     if (staged) {
         __builtin_amdgcn_wave_barrier();
-        flush_stage(stage, staged, ma.end_time_min, mt, cnt, lane);
+        flush_stage(stage, staged, ma, mt, cnt, lane);
     }
     /* one update of each counter per workgroup: updates of one address serialise */
     __shared__ unsigned long long bsum[256 / STC_WAVE][4];
@@ -1763,7 +1768,7 @@ __global__ __launch_bounds__(1024) void k_na_prep(uint8_t *sc, uint32_t n) {
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) out[i] = k[i];
 }
 
-__device__ inline void na_emit(uint8_t *sc, uint32_t qi, uint32_t pid, uint32_t target, const Bucket &b) {
+__device__ inline void na_emit(uint8_t *sc, uint32_t qi, uint32_t pid, uint32_t target, const Bucket &b, uint32_t end) {
     const unsigned long long at = atomicAdd((unsigned long long *)sc, 1ull);
     if (at >= NA_MAX_HITS) return;
     stc_hit h;
@@ -1771,21 +1776,21 @@ __device__ inline void na_emit(uint8_t *sc, uint32_t qi, uint32_t pid, uint32_t 
     h.pid = pid;
     h.target = target;
     h.start = b.start;
-    h.end = b.end;
+    h.end = end;
     h.source = 0;
     for (int i = 0; i < 4; i++) h.mbr[i] = o2f(b.mbr[i]);
     ((stc_hit *)(sc + NA_HITS_AT))[at] = h;
 }
 
 /* every search id equal to `pid`: the hits seen from pid */
-__device__ inline void na_match(uint8_t *sc, uint32_t n, uint32_t pid, uint32_t target, const Bucket &b) {
+__device__ inline void na_match(uint8_t *sc, uint32_t n, uint32_t pid, uint32_t target, const Bucket &b, uint32_t end) {
     const unsigned long long *keys = (const unsigned long long *)(sc + NA_KEYS_AT);
     uint32_t lo = 0, hi = n;
     while (lo < hi) {
         const uint32_t mid = (lo + hi) >> 1;
         if ((uint32_t)(keys[mid] >> 32) < pid) lo = mid + 1; else hi = mid;
     }
-    for (uint32_t i = lo; i < n && (uint32_t)(keys[i] >> 32) == pid; i++) na_emit(sc, (uint32_t)keys[i], pid, target, b);
+    for (uint32_t i = lo; i < n && (uint32_t)(keys[i] >> 32) == pid; i++) na_emit(sc, (uint32_t)keys[i], pid, target, b, end);
 }
 
 /* live meetings of the last step established by t_est (start <= t_est) */
@@ -1794,10 +1799,10 @@ __global__ void k_na_live(const Bucket *__restrict__ t, uint64_t nb, uint32_t la
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nb; i += stride) {
         const Bucket b = t[i];
-        if (!bucket_live(b, last_t) || b.start > t_est) continue;
+        if (!bucket_live(b) || b.start > t_est) continue;
         const uint32_t a = (uint32_t)(b.key >> 32), c = (uint32_t)b.key;
-        na_match(sc, n, a, c, b);
-        na_match(sc, n, c, a, b);
+        na_match(sc, n, a, c, b, last_t);
+        na_match(sc, n, c, a, b, last_t);
     }
 }
 
@@ -2774,7 +2779,7 @@ static int step_tail(StcCtx *c, uint32_t t, StepGeom &g, stc_step_out *out) {
     const uint64_t kv_before = c->kv_count;
     if (ensure_records(c, kv_before + 2 * std::max<uint64_t>(c->last_expired + c->last_expired / 4, 1 << 16) + 2))
         return fail(c, STC_E_CAPACITY, "record buffer allocation failed");
-    const MergeArgs ma{t, c->cfg.min_meet_time, c->cfg.max_meet_time, c->end_time_min, c->map};
+    const MergeArgs ma{t, c->cfg.min_meet_time, c->cfg.max_meet_time, c->end_time_min, c->map, c->last_t};
     CKRC(sync_counters(c));
     Counters h = *c->hcnt;
     if (h.table_fail) return fail(c, STC_E_HIP, "a pair segment was longer than its sort's scratch");
@@ -3144,12 +3149,12 @@ int stc_get_active(void *vc, uint64_t *keys, uint32_t *start, uint32_t *end, flo
     if (c->mlen) CK(hipMemcpy(tb.data(), c->meet[c->cur].p, c->mlen * sizeof(Bucket), hipMemcpyDeviceToHost));
     std::vector<Bucket> live; /* the array is sorted by key; tombstones trail each segment */
     for (const Bucket &b : tb)
-        if (bucket_live(b, c->last_t)) live.push_back(b);
+        if (bucket_live(b)) live.push_back(b);
     *n = live.size();
     for (uint64_t i = 0; i < std::min<uint64_t>(cap, live.size()); i++) {
         if (keys) keys[i] = live[i].key;
         if (start) start[i] = live[i].start;
-        if (end) end[i] = live[i].end;
+        if (end) end[i] = c->last_t; /* every live meeting was seen by the last step */
         if (mbr) for (int q = 0; q < 4; q++) mbr[4 * i + q] = o2f(live[i].mbr[q]);
     }
     return STC_OK;

@@ -271,12 +271,13 @@ __global__ __launch_bounds__(256) void k_q_live(const Bucket *__restrict__ t, ui
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nb; i += stride) {
         Bucket b = t[i];
-        if (!bucket_live(b, last_t)) continue;
+        if (!bucket_live(b)) continue;
         const uint32_t a = (uint32_t)(b.key >> 32), c = (uint32_t)b.key;
         double m[4] = {o2f(b.mbr[0]), o2f(b.mbr[1]), o2f(b.mbr[2]), o2f(b.mbr[3])};
-        match_oid(Q, lc, a, c, b.start, b.end, 0, m);
-        match_oid(Q, lc, c, a, b.start, b.end, 0, m);
-        match_range(Q, lc, a, c, b.start, b.end, 0, m);
+        /* a live meeting ends at the step that built the array */
+        match_oid(Q, lc, a, c, b.start, last_t, 0, m);
+        match_oid(Q, lc, c, a, b.start, last_t, 0, m);
+        match_range(Q, lc, a, c, b.start, last_t, 0, m);
     }
     lc_flush(Q, lc);
 }

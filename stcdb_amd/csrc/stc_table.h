@@ -2,17 +2,19 @@
  * stc_table.h — layout of the live meetings, shared by the pipeline and the
  * query kernels.
  *
- * A bucket is the reference's meeting_unit (src/tracing/workbench.h:62-83):
- * 32 bytes of key, start, end and MBR. The MBR is stored as order-preserving
- * u32 images of the f32 bounds so that min/max updates are integer ops.
- * Keys are oid1 << 32 | oid2 with oid1 < oid2 < 2^26.
+ * A bucket is the reference's meeting_unit (src/tracing/workbench.h:62-83)
+ * without its end time: 28 bytes of key, start and MBR. The MBR is stored as
+ * order-preserving u32 images of the f32 bounds so that min/max updates are
+ * integer ops. Keys are oid1 << 32 | oid2 with oid1 < oid2 < 2^26.
  *
  * The live meetings after a step are one array of buckets sorted by key, one
  * segment per owner oid1 (a pair's smaller id). Bucket i of a step's array is
  * built from the step's pair slot i, so a slot that held a duplicate pair or
  * a rejected band pair leaves a dead bucket: key oid1 << 32 | DEAD_B (it sorts
- * after the owner's live pairs) and end DEAD_B. The array has no free buckets:
- * every bucket is live or dead, and a live bucket's end is the array's step.
+ * after the owner's live pairs). The array has no free buckets: every bucket
+ * is live or dead, and a live meeting was last seen at the step that built the
+ * array, which is therefore its end time; the readers take it from the array's
+ * step instead of from the bucket.
  */
 #ifndef STC_TABLE_H_
 #define STC_TABLE_H_
@@ -23,16 +25,16 @@
 
 #define DEAD_B 0xFFFFFFFFu /* partner word of a dead slot or bucket */
 
-struct __align__(32) Bucket {
+/* packed to 28 bytes (4-byte aligned): the merge streams two arrays of them per step */
+struct __attribute__((packed, aligned(4))) Bucket {
     unsigned long long key;
-    uint32_t start, end;
+    uint32_t start;
     uint32_t mbr[4];          /* ordered ints of low0, low1, high0, high1 */
 };
+static_assert(sizeof(Bucket) == 28, "a bucket is 28 bytes");
 
-/* a bucket of the array built at step last_t that holds a live meeting */
-__host__ __device__ inline bool bucket_live(const Bucket &b, uint32_t last_t) {
-    return (uint32_t)b.key != DEAD_B && b.end == last_t;
-}
+/* a bucket that holds a live meeting (not a dead slot) */
+__host__ __device__ inline bool bucket_live(const Bucket &b) { return (uint32_t)b.key != DEAD_B; }
 
 __host__ __device__ inline uint32_t f2o(float f) {
     uint32_t b;
