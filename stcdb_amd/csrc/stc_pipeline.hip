@@ -1030,16 +1030,12 @@ __device__ inline unsigned long long settle_key(uint32_t src, uint32_t a, const 
 /* Sort an owner's collected sources (srcs[0, slots), slots <= 64 * K) by
  * partner in registers and write its segment: live pairs in partner order,
  * then the dead slots (band combinations the host rejected, repeats of a
- * pair). The sources arrive as ascending runs (each source leaf's members and
- * the reverse list are in id order), so with a wave's LDS buffer (64 * K
- * keys) a segment that is one run is not sorted and one of two runs is merged
- * (the second run reversed behind the padding makes the keys bitonic, and the
- * last stage of the network merges them); more runs take the full sort. */
+ * pair). The sort key is (partner, source); each pair's position is read from
+ * its source after the sort. */
 template <int K>
 __device__ inline void write_sorted_segment(const uint32_t *srcs, uint32_t slots, uint32_t a, uint32_t w,
                                             const PairGeo &g, const unsigned long long *acc, uint32_t n_acc,
-                                            PairSlot *__restrict__ seg, int lane,
-                                            unsigned long long *runbuf = nullptr) {
+                                            PairSlot *__restrict__ seg, int lane) {
     const uint64_t lt = lanemask_lt(lane);
     unsigned long long x[K];
     uint32_t sv[K];
@@ -1050,43 +1046,7 @@ __device__ inline void write_sorted_segment(const uint32_t *srcs, uint32_t slots
     }
 #pragma unroll
     for (int r = 0; r < K; r++) x[r] = r * STC_WAVE + lane < slots ? settle_key(sv[r], a, g, acc, n_acc) : ~0ull;
-    if (runbuf) {
-        /* descents: places i (0 < i < slots) whose key is below its predecessor's */
-        uint32_t d = 0, p = 0;
-#pragma unroll
-        for (int r = 0; r < K; r++) {
-            const unsigned long long up = __shfl_up(x[r], 1, STC_WAVE);
-            const unsigned long long last = r > 0 ? __shfl(x[r > 0 ? r - 1 : 0], STC_WAVE - 1, STC_WAVE) : 0ull;
-            const unsigned long long prev = lane > 0 ? up : last;
-            const uint32_t i = r * STC_WAVE + lane;
-            const uint64_t m = __ballot(i < slots && x[r] < prev);
-            d += (uint32_t)__popcll(m);
-            if (m) p = r * STC_WAVE + 63 - (uint32_t)__clzll((long long)m);
-        }
-        if (d == 1) {
-            /* runs [0, p) and [p, slots): the second reversed at the end, the padding between */
-            constexpr uint32_t P = K * STC_WAVE;
-#pragma unroll
-            for (int r = 0; r < K; r++) {
-                const uint32_t i = r * STC_WAVE + lane;
-                const uint32_t t = i < p ? i : (i < slots ? P - 1 - (i - p) : p + (i - slots));
-                runbuf[t] = x[r];
-            }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-#pragma unroll
-            for (int r = 0; r < K; r++) x[r] = runbuf[r * STC_WAVE + lane];
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier(); /* the buffer is reused by the wave's next owner */
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            bitonic_stage_steps<K, P, P / 2>(x, lane);
-        } else if (d > 1) {
-            reg_bitonic<K>(x, lane);
-        }
-    } else {
-        reg_bitonic<K>(x, lane);
-    }
+    reg_bitonic<K>(x, lane);
     bool live[K];
     uint32_t nlive = 0;
 #pragma unroll
@@ -1133,6 +1093,123 @@ __device__ inline void write_sorted_segment(const uint32_t *srcs, uint32_t slots
             dr += (uint32_t)__popcll(md);
         }
     }
+}
+
+/* the wave's LDS accesses so far complete before its next ones */
+__device__ inline void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+/* The place pass's form of the above, with the wave's LDS buffer lds (64 * K
+ * 8-byte entries): each source's partner id and position are read together,
+ * straight after the source (one dependent read instead of two), the
+ * positions parked in lds by slot as the floats the slot stores, and the sort
+ * key is (partner, slot), so that the sorted slots find their positions in
+ * lds. The sources arrive as ascending runs (each source leaf's members and
+ * the reverse list are in id order), so a segment that is one run is not
+ * sorted and one of two runs is merged (the second run reversed behind the
+ * padding through lds makes the keys bitonic, and the last stage of the
+ * network merges them); more runs take the full sort. */
+template <int K>
+__device__ inline void place_segment(const uint32_t *srcs, uint32_t slots, uint32_t a, uint32_t w, const PairGeo &g,
+                                     const unsigned long long *acc, uint32_t n_acc, PairSlot *__restrict__ seg,
+                                     int lane, unsigned long long *lds) {
+    const uint64_t lt = lanemask_lt(lane);
+    constexpr uint32_t P = K * STC_WAVE;
+    unsigned long long x[K];
+    float2 pf[K];
+    uint32_t sv[K];
+#pragma unroll
+    for (int r = 0; r < K; r++) {
+        const uint32_t i = r * STC_WAVE + lane;
+        sv[r] = i < slots ? __builtin_nontemporal_load(srcs + i) : 0u;
+    }
+    /* the partner's id and position of every slot, all loads issued before the first wait (a lane
+     * past the end reads the owner's own entries) */
+    uint32_t q[K];
+    double2 pd[K];
+#pragma unroll
+    for (int r = 0; r < K; r++) {
+        const uint32_t i = r * STC_WAVE + lane;
+        const uint32_t src = sv[r], idx = src & ~(SRC_REV | SRC_AMB);
+        const bool rev = i < slots && (src & SRC_REV) != 0;
+        const uint32_t li = i < slots ? idx : w;
+        q[r] = rev ? g.rev[idx].pid : g.lp[li];
+        pd[r] = rev ? *(const double2 *)(g.rev + idx) : g.lxy[li];
+    }
+#pragma unroll
+    for (int r = 0; r < K; r++) {
+        const uint32_t i = r * STC_WAVE + lane;
+        uint32_t b = q[r];
+        if ((sv[r] & SRC_AMB) && !sorted_contains(acc, n_acc, ((unsigned long long)a << 32) | b)) b = DEAD_B;
+        x[r] = i < slots ? (((unsigned long long)b << 32) | i) : ~0ull;
+        pf[r] = make_float2((float)pd[r].x, (float)pd[r].y);
+    }
+    /* descents: places i (0 < i < slots) whose key is below its predecessor's */
+    uint32_t d = 0, p = 0;
+#pragma unroll
+    for (int r = 0; r < K; r++) {
+        const unsigned long long up = __shfl_up(x[r], 1, STC_WAVE);
+        const unsigned long long last = r > 0 ? __shfl(x[r > 0 ? r - 1 : 0], STC_WAVE - 1, STC_WAVE) : 0ull;
+        const unsigned long long prev = lane > 0 ? up : last;
+        const uint32_t i = r * STC_WAVE + lane;
+        const uint64_t m = __ballot(i < slots && x[r] < prev);
+        d += (uint32_t)__popcll(m);
+        if (m) p = r * STC_WAVE + 63 - (uint32_t)__clzll((long long)m);
+    }
+    if (d == 1) {
+        /* runs [0, p) and [p, slots): the second reversed at the end, the padding between */
+#pragma unroll
+        for (int r = 0; r < K; r++) {
+            const uint32_t i = r * STC_WAVE + lane;
+            const uint32_t t = i < p ? i : (i < slots ? P - 1 - (i - p) : p + (i - slots));
+            lds[t] = x[r];
+        }
+        wave_lds_sync();
+#pragma unroll
+        for (int r = 0; r < K; r++) x[r] = lds[r * STC_WAVE + lane];
+        wave_lds_sync(); /* the buffer takes the positions next */
+    }
+    float2 *lpos = (float2 *)lds;
+#pragma unroll
+    for (int r = 0; r < K; r++)
+        if (r * STC_WAVE + lane < slots) lpos[r * STC_WAVE + lane] = pf[r];
+    wave_lds_sync();
+    if (d == 1) bitonic_stage_steps<K, P, P / 2>(x, lane);
+    else if (d > 1) reg_bitonic<K>(x, lane);
+    bool live[K];
+    uint32_t nlive = 0;
+#pragma unroll
+    for (int r = 0; r < K; r++) {
+        const unsigned long long up = __shfl_up(x[r], 1, STC_WAVE);
+        const unsigned long long last = r > 0 ? __shfl(x[r > 0 ? r - 1 : 0], STC_WAVE - 1, STC_WAVE) : ~0ull;
+        const unsigned long long prev = lane > 0 ? up : last;
+        const uint32_t i = r * STC_WAVE + lane;
+        const uint32_t b = (uint32_t)(x[r] >> 32);
+        live[r] = i < slots && b != DEAD_B && (i == 0 || (uint32_t)(prev >> 32) != b);
+        nlive += (uint32_t)__popcll(__ballot(live[r]));
+    }
+    uint32_t lr = 0, dr = 0;
+#pragma unroll
+    for (int r = 0; r < K; r++) {
+        const uint32_t i = r * STC_WAVE + lane;
+        const bool valid = i < slots;
+        const uint64_t ml = __ballot(live[r]), md = __ballot(valid && !live[r]);
+        if (valid) {
+            /* the key and the partner's position, streamed out past the caches: the merge reads
+             * the slots once, a step's worth of traffic later */
+            const float2 pv = lpos[(uint32_t)x[r]];
+            const uint32_t partner = live[r] ? (uint32_t)(x[r] >> 32) : DEAD_B;
+            const u32x4 v = {partner, a, __float_as_uint(pv.x), __float_as_uint(pv.y)};
+            __builtin_nontemporal_store(v, (u32x4 *)(seg + (live[r] ? lr + (uint32_t)__popcll(ml & lt)
+                                                                    : nlive + dr + (uint32_t)__popcll(md & lt))));
+        }
+        lr += (uint32_t)__popcll(ml);
+        dr += (uint32_t)__popcll(md);
+    }
+    wave_lds_sync(); /* the buffer is reused by the wave's next owner */
 }
 
 
@@ -1205,10 +1282,10 @@ __global__ __launch_bounds__(256, 6) void k_pair_place(PairGeo g, const uint32_t
         const uint32_t a = (uint32_t)__builtin_amdgcn_readfirstlane((int)g.lp[w]);
         const uint32_t *keys = psrc + (uint32_t)__builtin_amdgcn_readfirstlane((int)run.x);
         PairSlot *seg = pn + poff[a];
-        if (slots <= STC_WAVE) write_sorted_segment<1>(keys, slots, a, (uint32_t)w, g, acc, n_acc, seg, lane, runbuf);
-        else if (slots <= 2 * STC_WAVE) write_sorted_segment<2>(keys, slots, a, (uint32_t)w, g, acc, n_acc, seg, lane, runbuf);
-        else if (slots <= 4 * STC_WAVE) write_sorted_segment<4>(keys, slots, a, (uint32_t)w, g, acc, n_acc, seg, lane, runbuf);
-        else write_sorted_segment<8>(keys, slots, a, (uint32_t)w, g, acc, n_acc, seg, lane, runbuf);
+        if (slots <= STC_WAVE) place_segment<1>(keys, slots, a, (uint32_t)w, g, acc, n_acc, seg, lane, runbuf);
+        else if (slots <= 2 * STC_WAVE) place_segment<2>(keys, slots, a, (uint32_t)w, g, acc, n_acc, seg, lane, runbuf);
+        else if (slots <= 4 * STC_WAVE) place_segment<4>(keys, slots, a, (uint32_t)w, g, acc, n_acc, seg, lane, runbuf);
+        else place_segment<8>(keys, slots, a, (uint32_t)w, g, acc, n_acc, seg, lane, runbuf);
     }
 }
 
