@@ -967,29 +967,34 @@ __device__ inline unsigned long long xor_lane(unsigned long long v, int lane) {
     return ((unsigned long long)hi << 32) | lo;
 }
 
+template <int J>
+__device__ inline uint32_t xor_lane(uint32_t v, int lane) {
+    return xor_lane32<J>(v, lane);
+}
+
 /* one compare-exchange step of a bitonic stage k at lane distance J: each
  * lane keeps the smaller key when its block of k ascends and it is the lower
- * lane of its pair (or neither), the larger otherwise */
-template <int K, int k, int J>
-__device__ inline void bitonic_lane_step(unsigned long long (&x)[K], int lane) {
+ * lane of its pair (or neither), the larger otherwise (keys of 32 or 64 bits) */
+template <int K, int k, int J, class T>
+__device__ inline void bitonic_lane_step(T (&x)[K], int lane) {
 #pragma unroll
     for (int r = 0; r < K; r++) {
-        const unsigned long long y = xor_lane<J>(x[r], lane);
+        const T y = xor_lane<J>(x[r], lane);
         const bool keep_min = (((r * STC_WAVE + lane) & k) == 0) == ((lane & J) == 0);
         const bool le = x[r] <= y;
         x[r] = le == keep_min ? x[r] : y;
     }
 }
 
-template <int K, int k, int J>
-__device__ inline void bitonic_stage_steps(unsigned long long (&x)[K], int lane) {
+template <int K, int k, int J, class T>
+__device__ inline void bitonic_stage_steps(T (&x)[K], int lane) {
     if constexpr (J >= STC_WAVE) { /* across registers of the lane */
 #pragma unroll
         for (int r = 0; r < K; r++) {
             const int r2 = r ^ (J / STC_WAVE);
             if (r2 > r) {
                 const bool asc = ((r * STC_WAVE) & k) == 0;
-                const unsigned long long u = x[r], v = x[r2];
+                const T u = x[r], v = x[r2];
                 const bool sw = asc ? (u > v) : (u < v);
                 x[r] = sw ? v : u;
                 x[r2] = sw ? u : v;
@@ -1001,8 +1006,8 @@ __device__ inline void bitonic_stage_steps(unsigned long long (&x)[K], int lane)
     if constexpr (J > 1) bitonic_stage_steps<K, k, J / 2>(x, lane);
 }
 
-template <int K, int k>
-__device__ inline void bitonic_stages(unsigned long long (&x)[K], int lane) {
+template <int K, int k, class T>
+__device__ inline void bitonic_stages(T (&x)[K], int lane) {
     bitonic_stage_steps<K, k, k / 2>(x, lane);
     if constexpr (k < STC_WAVE * K) bitonic_stages<K, 2 * k>(x, lane);
 }
@@ -1010,8 +1015,8 @@ __device__ inline void bitonic_stages(unsigned long long (&x)[K], int lane) {
 /* ascending bitonic sort of 64 * K keys held by a wave, element r * 64 + lane
  * in x[r] of the lane: stages across lanes exchange through DPP, swizzles and
  * permutes, stages across registers swap within the lane */
-template <int K>
-__device__ inline void reg_bitonic(unsigned long long (&x)[K], int lane) {
+template <int K, class T>
+__device__ inline void reg_bitonic(T (&x)[K], int lane) {
     bitonic_stages<K, 2>(x, lane);
 }
 
@@ -1107,18 +1112,24 @@ __device__ inline void wave_lds_sync() {
  * straight after the source (one dependent read instead of two), the
  * positions parked in lds by slot as the floats the slot stores, and the sort
  * key is (partner, slot), so that the sorted slots find their positions in
- * lds. The sources arrive as ascending runs (each source leaf's members and
- * the reverse list are in id order), so a segment that is one run is not
- * sorted and one of two runs is merged (the second run reversed behind the
- * padding through lds makes the keys bitonic, and the last stage of the
- * network merges them); more runs take the full sort. */
-template <int K>
+ * lds. With T 32 bits the key is partner << log2(64 K) | slot, the partner
+ * field pbits wide and all ones for a dead slot (ids are below 2^pbits - 1),
+ * which halves the sort's lane exchanges and selects; with T 64 bits it is
+ * partner << 32 | slot. The sources arrive as ascending runs (each source
+ * leaf's members and the reverse list are in id order), so a segment that is
+ * one run is not sorted and one of two runs is merged (the second run
+ * reversed behind the padding through lds makes the keys bitonic, and the last
+ * stage of the network merges them); more runs take the full sort. */
+template <int K, class T>
 __device__ inline void place_segment(const uint32_t *srcs, uint32_t slots, uint32_t a, uint32_t w, const PairGeo &g,
                                      const unsigned long long *acc, uint32_t n_acc, PairSlot *__restrict__ seg,
-                                     int lane, unsigned long long *lds) {
+                                     int lane, unsigned long long *lds, uint32_t pbits) {
     const uint64_t lt = lanemask_lt(lane);
     constexpr uint32_t P = K * STC_WAVE;
-    unsigned long long x[K];
+    constexpr int SB = sizeof(T) == 8 ? 32 : (K == 1 ? 6 : K == 2 ? 7 : K == 4 ? 8 : 9);
+    static_assert(K == 1 || K == 2 || K == 4 || K == 8, "segment width");
+    const uint32_t pdead = sizeof(T) == 8 ? DEAD_B : (1u << pbits) - 1u;
+    T x[K];
     float2 pf[K];
     uint32_t sv[K];
 #pragma unroll
@@ -1143,17 +1154,17 @@ __device__ inline void place_segment(const uint32_t *srcs, uint32_t slots, uint3
     for (int r = 0; r < K; r++) {
         const uint32_t i = r * STC_WAVE + lane;
         uint32_t b = q[r];
-        if ((sv[r] & SRC_AMB) && !sorted_contains(acc, n_acc, ((unsigned long long)a << 32) | b)) b = DEAD_B;
-        x[r] = i < slots ? (((unsigned long long)b << 32) | i) : ~0ull;
+        if ((sv[r] & SRC_AMB) && !sorted_contains(acc, n_acc, ((unsigned long long)a << 32) | b)) b = pdead;
+        x[r] = i < slots ? (((T)b << SB) | (T)i) : ~(T)0;
         pf[r] = make_float2((float)pd[r].x, (float)pd[r].y);
     }
     /* descents: places i (0 < i < slots) whose key is below its predecessor's */
     uint32_t d = 0, p = 0;
 #pragma unroll
     for (int r = 0; r < K; r++) {
-        const unsigned long long up = __shfl_up(x[r], 1, STC_WAVE);
-        const unsigned long long last = r > 0 ? __shfl(x[r > 0 ? r - 1 : 0], STC_WAVE - 1, STC_WAVE) : 0ull;
-        const unsigned long long prev = lane > 0 ? up : last;
+        const T up = __shfl_up(x[r], 1, STC_WAVE);
+        const T last = r > 0 ? __shfl(x[r > 0 ? r - 1 : 0], STC_WAVE - 1, STC_WAVE) : (T)0;
+        const T prev = lane > 0 ? up : last;
         const uint32_t i = r * STC_WAVE + lane;
         const uint64_t m = __ballot(i < slots && x[r] < prev);
         d += (uint32_t)__popcll(m);
@@ -1161,15 +1172,16 @@ __device__ inline void place_segment(const uint32_t *srcs, uint32_t slots, uint3
     }
     if (d == 1) {
         /* runs [0, p) and [p, slots): the second reversed at the end, the padding between */
+        T *lk = (T *)lds;
 #pragma unroll
         for (int r = 0; r < K; r++) {
             const uint32_t i = r * STC_WAVE + lane;
             const uint32_t t = i < p ? i : (i < slots ? P - 1 - (i - p) : p + (i - slots));
-            lds[t] = x[r];
+            lk[t] = x[r];
         }
         wave_lds_sync();
 #pragma unroll
-        for (int r = 0; r < K; r++) x[r] = lds[r * STC_WAVE + lane];
+        for (int r = 0; r < K; r++) x[r] = lk[r * STC_WAVE + lane];
         wave_lds_sync(); /* the buffer takes the positions next */
     }
     float2 *lpos = (float2 *)lds;
@@ -1183,12 +1195,12 @@ __device__ inline void place_segment(const uint32_t *srcs, uint32_t slots, uint3
     uint32_t nlive = 0;
 #pragma unroll
     for (int r = 0; r < K; r++) {
-        const unsigned long long up = __shfl_up(x[r], 1, STC_WAVE);
-        const unsigned long long last = r > 0 ? __shfl(x[r > 0 ? r - 1 : 0], STC_WAVE - 1, STC_WAVE) : ~0ull;
-        const unsigned long long prev = lane > 0 ? up : last;
+        const T up = __shfl_up(x[r], 1, STC_WAVE);
+        const T last = r > 0 ? __shfl(x[r > 0 ? r - 1 : 0], STC_WAVE - 1, STC_WAVE) : ~(T)0;
+        const T prev = lane > 0 ? up : last;
         const uint32_t i = r * STC_WAVE + lane;
-        const uint32_t b = (uint32_t)(x[r] >> 32);
-        live[r] = i < slots && b != DEAD_B && (i == 0 || (uint32_t)(prev >> 32) != b);
+        const uint32_t b = (uint32_t)(x[r] >> SB);
+        live[r] = i < slots && b != pdead && (i == 0 || (uint32_t)(prev >> SB) != b);
         nlive += (uint32_t)__popcll(__ballot(live[r]));
     }
     uint32_t lr = 0, dr = 0;
@@ -1200,8 +1212,8 @@ __device__ inline void place_segment(const uint32_t *srcs, uint32_t slots, uint3
         if (valid) {
             /* the key and the partner's position, streamed out past the caches: the merge reads
              * the slots once, a step's worth of traffic later */
-            const float2 pv = lpos[(uint32_t)x[r]];
-            const uint32_t partner = live[r] ? (uint32_t)(x[r] >> 32) : DEAD_B;
+            const float2 pv = lpos[(uint32_t)x[r] & (P - 1)];
+            const uint32_t partner = live[r] ? (uint32_t)(x[r] >> SB) : DEAD_B;
             const u32x4 v = {partner, a, __float_as_uint(pv.x), __float_as_uint(pv.y)};
             __builtin_nontemporal_store(v, (u32x4 *)(seg + (live[r] ? lr + (uint32_t)__popcll(ml & lt)
                                                                     : nlive + dr + (uint32_t)__popcll(md & lt))));
@@ -1263,13 +1275,13 @@ __global__ void k_huge_expand(PairGeo g, const uint32_t *__restrict__ owners, ui
 /* The place pass: per owner (one wave, owners in leaf order) with at most
  * WCAP slots, its collected keys sorted in registers and written to pn as the
  * owner's segment sorted by partner, its live pairs first and its dead slots
- * behind them, so that pn is sorted by key; each pair's MBR is computed from
- * its source's position when written. */
+ * behind them, so that pn is sorted by key. pbits: the bits of the object
+ * count; a segment sorts 32-bit keys when pbits and its slot index fit in 32. */
 __global__ __launch_bounds__(256, 6) void k_pair_place(PairGeo g, const uint32_t *__restrict__ poff,
                                                     const uint2 *__restrict__ pkrun,
                                                     const uint32_t *__restrict__ psrc,
                                                     const unsigned long long *__restrict__ acc, uint32_t n_acc,
-                                                    PairSlot *__restrict__ pn) {
+                                                    PairSlot *__restrict__ pn, uint32_t pbits) {
     __shared__ unsigned long long wrun[256 / STC_WAVE][WCAP];
     const int lane = threadIdx.x & (STC_WAVE - 1);
     unsigned long long *runbuf = wrun[threadIdx.x / STC_WAVE];
@@ -1282,10 +1294,16 @@ __global__ __launch_bounds__(256, 6) void k_pair_place(PairGeo g, const uint32_t
         const uint32_t a = (uint32_t)__builtin_amdgcn_readfirstlane((int)g.lp[w]);
         const uint32_t *keys = psrc + (uint32_t)__builtin_amdgcn_readfirstlane((int)run.x);
         PairSlot *seg = pn + poff[a];
-        if (slots <= STC_WAVE) place_segment<1>(keys, slots, a, (uint32_t)w, g, acc, n_acc, seg, lane, runbuf);
-        else if (slots <= 2 * STC_WAVE) place_segment<2>(keys, slots, a, (uint32_t)w, g, acc, n_acc, seg, lane, runbuf);
-        else if (slots <= 4 * STC_WAVE) place_segment<4>(keys, slots, a, (uint32_t)w, g, acc, n_acc, seg, lane, runbuf);
-        else place_segment<8>(keys, slots, a, (uint32_t)w, g, acc, n_acc, seg, lane, runbuf);
+#define PLACE(KK, BITS)                                                                                          \
+    do {                                                                                                         \
+        if (pbits + (BITS) <= 32u) place_segment<KK, uint32_t>(keys, slots, a, (uint32_t)w, g, acc, n_acc, seg, lane, runbuf, pbits); \
+        else place_segment<KK, unsigned long long>(keys, slots, a, (uint32_t)w, g, acc, n_acc, seg, lane, runbuf, pbits); \
+    } while (0)
+        if (slots <= STC_WAVE) PLACE(1, 6);
+        else if (slots <= 2 * STC_WAVE) PLACE(2, 7);
+        else if (slots <= 4 * STC_WAVE) PLACE(4, 8);
+        else PLACE(8, 9);
+#undef PLACE
     }
 }
 
@@ -2788,7 +2806,7 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
     const uint32_t wgrid = (uint32_t)std::min<uint64_t>((blocks_for((uint64_t)n * STC_WAVE, 256) + 7) & ~7ull, 65536u);
     hipLaunchKernelGGL(k_pair_place, dim3(wgrid), dim3(256), 0, s, pg, (const uint32_t *)c->moff[nx].p,
                        (const uint2 *)c->pkrun.p, (const uint32_t *)c->psrc.p, c->acc.p,
-                       (uint32_t)accepted.size(), c->pnew.p);
+                       (uint32_t)accepted.size(), c->pnew.p, 32u - (uint32_t)__builtin_clz(c->n));
     CK(hipGetLastError());
     CK(hipEventRecord(c->ev[7], s));
     if (h.big) { /* segments longer than the place pass sorts: by waves, by blocks, in global scratch */
