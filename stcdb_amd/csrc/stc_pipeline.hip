@@ -475,7 +475,7 @@ __global__ void k_lookup(const uint32_t *__restrict__ lp, const double2 *__restr
 /* meetings: pairs grouped by owner, merged with the live meetings      */
 /* ------------------------------------------------------------------ */
 /*
- * The live meetings after a step are an array of 32 B buckets sorted by key
+ * The live meetings after a step are an array of 28 B buckets sorted by key
  * (oid1 << 32 | oid2): one segment per owner oid1 (the pair's smaller id), in
  * owner order, each segment's live buckets followed by tombstones for the
  * step's duplicate or rejected pairs. A step finds its pairs owner by owner,
