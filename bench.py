@@ -616,15 +616,15 @@ def main():
         # 4 B source (the other point's index) per slot out
         "k_pair_collect": ("pair collect pass: each combination decided once, sources per owner", mean("ms_count_kernel"),
                            24.0 * n + 4.0 * slots),
-        # place pass: each owner's 8 B run descriptor and 4 B sources in, one 16 B slot (key +
-        # the partner's float position) per pair out
+        # place pass: each owner's 8 B run descriptor and 4 B sources in, one 12 B slot (the
+        # partner and its float position) per pair out
         "k_pair_place": ("pair place pass: owner segments sorted by partner", mean("ms_refine_kernel"),
-                         8.0 * n + 20.0 * slots),
-        # merge: the step's slots (16 B) and the previous meetings (28 B) in, the owners' float
-        # positions (8 B each) once, the new meetings (28 B) and 2 records (32 B each) per
-        # recorded meeting out
+                         8.0 * n + 16.0 * slots),
+        # merge: the step's slots (12 B) and the previous meetings (28 B) in, the owners' float
+        # positions (8 B each) and segment offsets (4 B each) once, the new meetings (28 B) and
+        # 2 records (32 B each) per recorded meeting out
         "k_meet_merge": ("merge with the previous step's meetings, expiry, records", mean("ms_identify_kernel"),
-                         16.0 * slots + 28.0 * slots + 28.0 * slots + 8.0 * n + 32.0 * mean("new_records")),
+                         12.0 * slots + 28.0 * slots + 28.0 * slots + 12.0 * n + 32.0 * mean("new_records")),
     }
     kname = max(kernels, key=lambda k: kernels[k][1])
     desc, ms, alg = kernels[kname]

@@ -46,6 +46,7 @@
 #include "../../include/stcdb.h"
 
 #include <algorithm>
+#include <type_traits>
 #include <cmath>
 #include <condition_variable>
 #include <mutex>
@@ -88,20 +89,28 @@ struct PairHit {              /* 24 B: key and the float MBR of the two position
     float mbr[4];
 };
 
-/* 16 B: key and the partner's position as floats. The owner's is posf[owner],
- * and the pair's MBR is the box of the two (the box of their float
- * conversions, as the reference's f_box of the two points). */
-struct __align__(16) PairSlot {
-    unsigned long long key;
+/* 12 B (packed, 4-byte aligned): the partner and its position as floats.
+ * The slots of a step are laid out owner by owner (slot i belongs to the
+ * owner a with poff[a] <= i < poff[a + 1]), so the owner is not stored; its
+ * position is posf[owner], and the pair's MBR is the box of the two (the box
+ * of their float conversions, as the reference's f_box of the two points). */
+struct __attribute__((packed, aligned(4))) PairSlot {
+    uint32_t partner;
     float px, py;
 };
+static_assert(sizeof(PairSlot) == 12, "a pair slot is 12 bytes");
 
-/* the ordered-int MBR of a pair slot (f2o of low0, low1, high0, high1) */
-__device__ inline uint4 slot_mbr(const PairHit &e, const float2 *) {
+/* a slot's key (owner << 32 | partner) and ordered-int MBR (f2o of low0, low1,
+ * high0, high1); a replayed PairHit carries both itself */
+__device__ inline unsigned long long slot_key(const PairHit &e, uint32_t) { return e.key; }
+__device__ inline unsigned long long slot_key(const PairSlot &e, uint32_t owner) {
+    return ((unsigned long long)owner << 32) | e.partner;
+}
+__device__ inline uint4 slot_mbr(const PairHit &e, const float2 *, uint32_t) {
     return make_uint4(f2o(e.mbr[0]), f2o(e.mbr[1]), f2o(e.mbr[2]), f2o(e.mbr[3]));
 }
-__device__ inline uint4 slot_mbr(const PairSlot &e, const float2 *posf) {
-    const float2 o = posf[(uint32_t)(e.key >> 32)];
+__device__ inline uint4 slot_mbr(const PairSlot &e, const float2 *posf, uint32_t owner) {
+    const float2 o = posf[owner];
     return make_uint4(f2o(fminf(o.x, e.px)), f2o(fminf(o.y, e.py)), f2o(fmaxf(o.x, e.px)), f2o(fmaxf(o.y, e.py)));
 }
 
@@ -1086,13 +1095,13 @@ __device__ inline void write_sorted_segment(const uint32_t *srcs, uint32_t slots
             const bool valid = i < slots;
             const uint64_t ml = __ballot(live[r]), md = __ballot(valid && !live[r]);
             if (valid) {
-                /* the key and the partner's position (the source's), streamed out past the caches:
-                 * the merge reads the slots once, a step's worth of traffic later */
+                /* the partner and its position (the source's), streamed out past the caches: the
+                 * merge reads the slots once, a step's worth of traffic later */
                 const uint32_t partner = live[r] ? (uint32_t)(x[r] >> 32) : DEAD_B;
                 const float px = (float)pos[b].x, py = (float)pos[b].y;
-                const u32x4 v = {partner, a, __float_as_uint(px), __float_as_uint(py)};
-                __builtin_nontemporal_store(v, (u32x4 *)(seg + (live[r] ? lr + (uint32_t)__popcll(ml & lt)
-                                                                        : nlive + dr + (uint32_t)__popcll(md & lt))));
+                const u32x3u v = {partner, __float_as_uint(px), __float_as_uint(py)};
+                __builtin_nontemporal_store(v, (u32x3u *)(seg + (live[r] ? lr + (uint32_t)__popcll(ml & lt)
+                                                                          : nlive + dr + (uint32_t)__popcll(md & lt))));
             }
             lr += (uint32_t)__popcll(ml);
             dr += (uint32_t)__popcll(md);
@@ -1210,13 +1219,13 @@ __device__ inline void place_segment(const uint32_t *srcs, uint32_t slots, uint3
         const bool valid = i < slots;
         const uint64_t ml = __ballot(live[r]), md = __ballot(valid && !live[r]);
         if (valid) {
-            /* the key and the partner's position, streamed out past the caches: the merge reads
-             * the slots once, a step's worth of traffic later */
+            /* the partner and its position, streamed out past the caches: the merge reads the
+             * slots once, a step's worth of traffic later */
             const float2 pv = lpos[(uint32_t)x[r] & (P - 1)];
             const uint32_t partner = live[r] ? (uint32_t)(x[r] >> SB) : DEAD_B;
-            const u32x4 v = {partner, a, __float_as_uint(pv.x), __float_as_uint(pv.y)};
-            __builtin_nontemporal_store(v, (u32x4 *)(seg + (live[r] ? lr + (uint32_t)__popcll(ml & lt)
-                                                                    : nlive + dr + (uint32_t)__popcll(md & lt))));
+            const u32x3u v = {partner, __float_as_uint(pv.x), __float_as_uint(pv.y)};
+            __builtin_nontemporal_store(v, (u32x3u *)(seg + (live[r] ? lr + (uint32_t)__popcll(ml & lt)
+                                                                      : nlive + dr + (uint32_t)__popcll(md & lt))));
         }
         lr += (uint32_t)__popcll(ml);
         dr += (uint32_t)__popcll(md);
@@ -1264,7 +1273,7 @@ __global__ void k_huge_expand(PairGeo g, const uint32_t *__restrict__ owners, ui
             const uint32_t src = (uint32_t)k;
             const double2 pos = (src & SRC_REV) ? *(const double2 *)(g.rev + (src & ~SRC_REV)) : g.lxy[src];
             PairSlot h;
-            h.key = ((unsigned long long)a << 32) | (uint32_t)(k >> 32);
+            h.partner = (uint32_t)(k >> 32);
             h.px = (float)pos.x;
             h.py = (float)pos.y;
             ptmp[base + i] = h;
@@ -1491,7 +1500,7 @@ __global__ __launch_bounds__(THREADS) void k_seg_sort(const uint32_t *__restrict
         uint32_t P = 1;
         while (P < slots) P <<= 1;
         for (uint32_t i = tid; i < P; i += THREADS)
-            keys[i] = i < slots ? (((unsigned long long)(uint32_t)ptmp[base + i].key << 32) | i) : ~0ull;
+            keys[i] = i < slots ? (((unsigned long long)ptmp[base + i].partner << 32) | i) : ~0ull;
         __syncthreads();
         block_bitonic<THREADS>(keys, P);
         uint32_t nlive = 0;
@@ -1524,7 +1533,7 @@ __global__ __launch_bounds__(THREADS) void k_seg_sort(const uint32_t *__restrict
                     dest = lr + rl;
                 } else {
                     dest = nlive + dr + rd;
-                    e.key = ((unsigned long long)a << 32) | DEAD_B;
+                    e.partner = DEAD_B;
                 }
                 pn[base + dest] = e;
             }
@@ -1550,6 +1559,16 @@ __device__ inline void flush_stage(const Bucket *stage, uint32_t staged, const M
     __builtin_amdgcn_wave_barrier();
 }
 
+/* the owner of each merge tile's first slot (slot i belongs to the owner a
+ * with poff[a] <= i < poff[a + 1]; each tile's first slot lies in exactly one
+ * non-empty segment) */
+__global__ void k_tile_owner(const uint32_t *__restrict__ poff, uint32_t n, uint32_t *__restrict__ towner) {
+    const uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= n) return;
+    const uint64_t p0 = poff[a], p1 = poff[a + 1];
+    for (uint64_t t = (p0 + MTILE - 1) / MTILE; t * MTILE < p1; t++) towner[t] = a;
+}
+
 /* Start of each merge tile's range in the previous meeting array: the number
  * of previous keys below the tile's first new key (found in that key's owner
  * segment). tb[n_tiles] = n_o. The search starts where the key's place among
@@ -1559,7 +1578,7 @@ __device__ inline void flush_stage(const Bucket *stage, uint32_t staged, const M
 template <class S>
 __global__ void k_tile_bounds(const S *__restrict__ pn, uint64_t n_p, const Bucket *__restrict__ mold,
                               const uint32_t *__restrict__ ooff, const uint32_t *__restrict__ poff, uint64_t n_o,
-                              uint32_t n_tiles, uint32_t *__restrict__ tb) {
+                              uint32_t n_tiles, const uint32_t *__restrict__ towner, uint32_t *__restrict__ tb) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t > n_tiles) return;
     if (t == 0 || t == n_tiles || (uint64_t)t * MTILE >= n_p) {
@@ -1567,7 +1586,9 @@ __global__ void k_tile_bounds(const S *__restrict__ pn, uint64_t n_p, const Buck
         return;
     }
     const uint64_t first = (uint64_t)t * MTILE;
-    const unsigned long long key = pn[first].key;
+    uint32_t owner = 0;
+    if constexpr (std::is_same<S, PairSlot>::value) owner = towner[t];
+    const unsigned long long key = slot_key(pn[first], owner);
     const uint32_t a = (uint32_t)(key >> 32);
     uint32_t lo = ooff[a], hi = ooff[a + 1];
     const uint32_t p0 = poff[a], np_a = poff[a + 1] - p0;
@@ -1646,7 +1667,10 @@ template <class S>
 __global__ __launch_bounds__(256) void k_meet_merge(const S *__restrict__ pn, uint64_t n_p,
                                                     const Bucket *__restrict__ mold, const uint32_t *__restrict__ tb,
                                                     uint32_t n_tiles, Bucket *__restrict__ mnew, MergeArgs ma,
-                                                    MemtableDev mt, const float2 *__restrict__ posf, Counters *cnt) {
+                                                    MemtableDev mt, const float2 *__restrict__ posf,
+                                                    const uint32_t *__restrict__ poff, const uint32_t *__restrict__ towner,
+                                                    uint32_t n_obj, Counters *cnt) {
+    constexpr bool SLOTS = std::is_same<S, PairSlot>::value; /* owners from poff, not from the slots */
     __shared__ unsigned long long wpk[256 / STC_WAVE][MTILE];
     __shared__ Bucket wob[256 / STC_WAVE][OCAP];
     __shared__ Bucket wstage[256 / STC_WAVE][MSTAGE];
@@ -1661,13 +1685,18 @@ __global__ __launch_bounds__(256) void k_meet_merge(const S *__restrict__ pn, ui
     const uint64_t plast = n_p ? n_p - 1 : 0;
     uint32_t staged = 0;
     unsigned long long n_live = 0, n_exp = 0, n_fresh = 0, n_cont = 0;
-    uint32_t o0 = 0, o1 = 0, on0 = 0, on1 = 0;
-    if (wid < n_tiles) { o0 = tb[wid]; o1 = tb[wid + 1]; }
-    for (uint32_t t = wid; t < n_tiles; t += nw, o0 = on0, o1 = on1) {
-        { /* the next tile's range, read ahead */
+    uint32_t o0 = 0, o1 = 0, on0 = 0, on1 = 0, a0 = 0, an0 = 0;
+    if (wid < n_tiles) {
+        o0 = tb[wid];
+        o1 = tb[wid + 1];
+        if constexpr (SLOTS) a0 = min(towner[wid], n_obj - 1); /* a step without pairs leaves tile 0 unset */
+    }
+    for (uint32_t t = wid; t < n_tiles; t += nw, o0 = on0, o1 = on1, a0 = an0) {
+        { /* the next tile's range (and first owner), read ahead */
             const uint32_t tn = t + nw < n_tiles ? t + nw : t;
             on0 = tb[tn];
             on1 = tb[tn + 1];
+            if constexpr (SLOTS) an0 = min(towner[tn], n_obj - 1);
         }
         const uint64_t i0 = (uint64_t)t * MTILE;
         const uint32_t np = i0 < n_p ? (uint32_t)min<uint64_t>(MTILE, n_p - i0) : 0u;
@@ -1680,6 +1709,12 @@ __global__ __launch_bounds__(256) void k_meet_merge(const S *__restrict__ pn, ui
         const uint32_t olast = o1 > o0 ? o1 - 1 : (n_o ? min(o0, n_o - 1) : 0u);
 #pragma unroll
         for (int k = 0; k < OPF; k++) po[k] = mold[min(o0 + lane + k * STC_WAVE, olast)];
+        /* the starts of the 64 owners after a0 (a0 owns the tile's first slot), read with the rest */
+        uint32_t bst = 0;
+        if constexpr (SLOTS) {
+            const uint64_t ai = (uint64_t)a0 + 1 + (uint32_t)lane;
+            bst = poff[ai < n_obj ? ai : n_obj];
+        }
         if (o1 < o0) { /* the pairs were not sorted by key: report it, use nothing */
             if (lane == 0) atomicOr(&cnt->table_fail, 2u);
             continue;
@@ -1691,6 +1726,33 @@ __global__ __launch_bounds__(256) void k_meet_merge(const S *__restrict__ pn, ui
 #pragma unroll
         for (int k = 0; k < OPF; k++) ob_l[lane + k * STC_WAVE] = po[k];
         __builtin_amdgcn_wave_barrier();
+        /* each slot's owner: a0 plus the number of later owners starting at or before the slot
+         * (the starts ascend; an empty segment's start equals the next one's, so both count and
+         * the later owner stands). Lanes past the tile's end take its last owner. */
+        uint32_t own[SPF];
+#pragma unroll
+        for (int k = 0; k < SPF; k++) own[k] = a0;
+        if constexpr (SLOTS) {
+            const uint64_t tend = i0 + np;
+            for (uint64_t c0 = 0;;) {
+                /* owners a0 + 1 + c0 + lane; past the last object the start reads as beyond the tile */
+                const bool in = (uint64_t)a0 + 1 + c0 + (uint32_t)lane <= n_obj && (uint64_t)bst < tend;
+                const uint64_t m = __ballot(in);
+                const uint32_t nb = (uint32_t)__popcll(m); /* a prefix of the lanes: the starts ascend */
+                for (uint32_t j = 0; j < nb; j++) {
+                    const uint32_t bj = (uint32_t)__builtin_amdgcn_readlane((int)bst, (int)j);
+#pragma unroll
+                    for (int k = 0; k < SPF; k++) {
+                        const uint64_t i = i0 + lane + k * STC_WAVE;
+                        own[k] += (i < tend ? i : tend - 1) >= bj;
+                    }
+                }
+                if (nb < STC_WAVE) break;
+                c0 += STC_WAVE;
+                const uint64_t ai = (uint64_t)a0 + 1 + c0 + (uint32_t)lane;
+                bst = poff[ai < n_obj ? ai : n_obj];
+            }
+        }
         /* the join, instantiated for previous meetings in LDS and, for a range longer than
          * OCAP, in global memory (so the LDS searches wait on nothing but LDS) */
         auto join = [&](const Bucket *obk) {
@@ -1701,11 +1763,11 @@ __global__ __launch_bounds__(256) void k_meet_merge(const S *__restrict__ pn, ui
                 const uint32_t i = lane + k * STC_WAVE;
                 const bool valid = i < np;
                 const S e = ps[k];
-                const unsigned long long key = e.key;
+                const unsigned long long key = slot_key(e, own[k]);
                 pk[i] = key;
                 const bool dead = (uint32_t)key == DEAD_B;
                 const uint32_t j = lower_bound_bucket(obk, no, key);
-                const uint4 q = slot_mbr(e, posf);
+                const uint4 q = slot_mbr(e, posf, own[k]);
                 const uint32_t q0 = q.x, q1 = q.y, q2 = q.z, q3 = q.w;
                 const Bucket ob = obk[j < no ? j : (no ? no - 1 : 0u)];
                 const bool cont = j < no && ob.key == key;
@@ -2032,6 +2094,7 @@ struct StcCtx {
     DBuf<Bucket> meet[2];
     DBuf<uint32_t> moff[2];
     DBuf<uint32_t> tbounds;              /* merge tiles' ranges in the previous meetings */
+    DBuf<uint32_t> towner;               /* merge tiles' first owners (the slots carry no owner) */
     int cur = 0;                         /* meet[cur] holds the meetings of the last step */
     uint64_t mlen = 0;                   /* buckets of meet[cur] (live and tombstones) */
     uint64_t plen = 0;                   /* entries of pnew (the last step's pair slots) */
@@ -2216,19 +2279,22 @@ static int step_pairs(StcCtx *c, std::vector<PairHit> &ph) {
     ph.resize(c->plen);
     if (c->pairs_boxed) {
         if (c->plen) CK(hipMemcpy(ph.data(), c->pbox.p, c->plen * sizeof(PairHit), hipMemcpyDeviceToHost));
-    } else if (c->plen) { /* slots: the MBR is the box of the owner's and the partner's float positions */
+    } else if (c->plen) { /* slots, owner by owner: the MBR is the box of the owner's and the partner's float positions */
         std::vector<PairSlot> sl(c->plen);
         std::vector<float2> pos(c->n);
+        std::vector<uint32_t> off(c->n + 1ull);
         CK(hipMemcpy(sl.data(), c->pnew.p, c->plen * sizeof(PairSlot), hipMemcpyDeviceToHost));
         CK(hipMemcpy(pos.data(), c->posf.p, (size_t)c->n * sizeof(float2), hipMemcpyDeviceToHost));
-        for (uint64_t i = 0; i < c->plen; i++) {
-            const float2 o = pos[(uint32_t)(sl[i].key >> 32)];
-            ph[i].key = sl[i].key;
-            ph[i].mbr[0] = std::fmin(o.x, sl[i].px);
-            ph[i].mbr[1] = std::fmin(o.y, sl[i].py);
-            ph[i].mbr[2] = std::fmax(o.x, sl[i].px);
-            ph[i].mbr[3] = std::fmax(o.y, sl[i].py);
-        }
+        CK(hipMemcpy(off.data(), c->moff[c->cur].p, (c->n + 1ull) * 4, hipMemcpyDeviceToHost));
+        for (uint32_t a = 0; a < c->n; a++)
+            for (uint64_t i = off[a]; i < off[a + 1] && i < c->plen; i++) {
+                const float2 o = pos[a];
+                ph[i].key = ((uint64_t)a << 32) | sl[i].partner;
+                ph[i].mbr[0] = std::fmin(o.x, sl[i].px);
+                ph[i].mbr[1] = std::fmin(o.y, sl[i].py);
+                ph[i].mbr[2] = std::fmax(o.x, sl[i].px);
+                ph[i].mbr[3] = std::fmax(o.y, sl[i].py);
+            }
     }
     ph.erase(std::remove_if(ph.begin(), ph.end(), [](const PairHit &e) { return (uint32_t)e.key == DEAD_B; }),
              ph.end());
@@ -2509,7 +2575,7 @@ int stc_destroy(void *vc) {
     c->rev.release(); c->rev_dl.release();
     c->pcount.release(); c->pnew.release(); c->acc.release();
     for (int i = 0; i < 2; i++) { c->meet[i].release(); c->moff[i].release(); }
-    c->ptmp.release(); c->tbounds.release(); c->psrc.release(); c->pkrun.release();
+    c->ptmp.release(); c->tbounds.release(); c->towner.release(); c->psrc.release(); c->pkrun.release();
     c->spill_big.release(); c->spill_huge.release(); c->spill_giant.release(); c->gsrt.release();
     c->units_in.release(); c->ukeys.release(); c->ukeys_alt.release();
     c->uidx.release(); c->uidx_alt.release(); c->utemp.release();
@@ -2881,17 +2947,20 @@ static int step_tail(StcCtx *c, uint32_t t, StepGeom &g, stc_step_out *out) {
     const Counters before = h;
     /* tiles of the step's pairs and their ranges in the previous meetings */
     const uint32_t n_tiles = (uint32_t)std::max<uint64_t>((c->plen + MTILE - 1) / MTILE, 1);
-    if (c->tbounds.ensure(n_tiles + 1ull)) return fail(c, STC_E_CAPACITY, "merge tile bounds");
-    if (c->pairs_boxed)
+    if (c->tbounds.ensure(n_tiles + 1ull) || c->towner.ensure(n_tiles)) return fail(c, STC_E_CAPACITY, "merge tile bounds");
+    if (c->pairs_boxed) {
         hipLaunchKernelGGL(k_tile_bounds<PairHit>, dim3(blocks_for(n_tiles + 1ull, 256)), dim3(256), 0, s,
                            (const PairHit *)c->pbox.p, c->plen, (const Bucket *)c->meet[cu].p,
                            (const uint32_t *)c->moff[cu].p, (const uint32_t *)c->moff[nx].p, c->mlen, n_tiles,
-                           c->tbounds.p);
-    else
+                           (const uint32_t *)nullptr, c->tbounds.p);
+    } else {
+        hipLaunchKernelGGL(k_tile_owner, dim3(blocks_for(c->n, 256)), dim3(256), 0, s, (const uint32_t *)c->moff[nx].p,
+                           c->n, c->towner.p);
         hipLaunchKernelGGL(k_tile_bounds<PairSlot>, dim3(blocks_for(n_tiles + 1ull, 256)), dim3(256), 0, s,
                            (const PairSlot *)c->pnew.p, c->plen, (const Bucket *)c->meet[cu].p,
                            (const uint32_t *)c->moff[cu].p, (const uint32_t *)c->moff[nx].p, c->mlen, n_tiles,
-                           c->tbounds.p);
+                           (const uint32_t *)c->towner.p, c->tbounds.p);
+    }
     CK(hipGetLastError());
     const uint32_t mgrid = std::max<uint32_t>(std::min<uint32_t>((n_tiles + 3) / 4, 65536), 1);
     /* the merge writes meet[nx] and, after a flush emptied the memtable, records from index 0 */
@@ -2908,11 +2977,13 @@ static int step_tail(StcCtx *c, uint32_t t, StepGeom &g, stc_step_out *out) {
         if (c->pairs_boxed)
             hipLaunchKernelGGL(k_meet_merge<PairHit>, dim3(mgrid), dim3(256), 0, s, (const PairHit *)c->pbox.p,
                                c->plen, (const Bucket *)c->meet[cu].p, (const uint32_t *)c->tbounds.p, n_tiles,
-                               c->meet[nx].p, ma, mt, (const float2 *)nullptr, cnt);
+                               c->meet[nx].p, ma, mt, (const float2 *)nullptr, (const uint32_t *)nullptr,
+                               (const uint32_t *)nullptr, c->n, cnt);
         else
             hipLaunchKernelGGL(k_meet_merge<PairSlot>, dim3(mgrid), dim3(256), 0, s, (const PairSlot *)c->pnew.p,
                                c->plen, (const Bucket *)c->meet[cu].p, (const uint32_t *)c->tbounds.p, n_tiles,
-                               c->meet[nx].p, ma, mt, (const float2 *)c->posf.p, cnt);
+                               c->meet[nx].p, ma, mt, (const float2 *)c->posf.p, (const uint32_t *)c->moff[nx].p,
+                               (const uint32_t *)c->towner.p, c->n, cnt);
         CK(hipGetLastError());
         CK(hipEventRecord(c->ev[9], s));
         CKRC(sync_counters(c));
