@@ -1685,18 +1685,27 @@ __global__ __launch_bounds__(256) void k_meet_merge(const S *__restrict__ pn, ui
     const uint64_t plast = n_p ? n_p - 1 : 0;
     uint32_t staged = 0;
     unsigned long long n_live = 0, n_exp = 0, n_fresh = 0, n_cont = 0;
-    uint32_t o0 = 0, o1 = 0, on0 = 0, on1 = 0, a0 = 0, an0 = 0;
+    /* a tile's first owner a0 and the next tile's a1 (n_obj - 1 after the last tile): the owners
+     * of the tile's slots lie in [a0, a1]; a step without pairs leaves tile 0 unset, hence the clamps */
+    auto tile_owner = [&](uint32_t t) { return t < n_tiles ? min(towner[t], n_obj - 1) : n_obj - 1; };
+    uint32_t o0 = 0, o1 = 0, on0 = 0, on1 = 0, a0 = 0, a1 = 0, an0 = 0, an1 = 0;
     if (wid < n_tiles) {
         o0 = tb[wid];
         o1 = tb[wid + 1];
-        if constexpr (SLOTS) a0 = min(towner[wid], n_obj - 1); /* a step without pairs leaves tile 0 unset */
+        if constexpr (SLOTS) {
+            a0 = tile_owner(wid);
+            a1 = max(a0, tile_owner(wid + 1));
+        }
     }
-    for (uint32_t t = wid; t < n_tiles; t += nw, o0 = on0, o1 = on1, a0 = an0) {
-        { /* the next tile's range (and first owner), read ahead */
+    for (uint32_t t = wid; t < n_tiles; t += nw, o0 = on0, o1 = on1, a0 = an0, a1 = an1) {
+        { /* the next tile's range (and owners), read ahead */
             const uint32_t tn = t + nw < n_tiles ? t + nw : t;
             on0 = tb[tn];
             on1 = tb[tn + 1];
-            if constexpr (SLOTS) an0 = min(towner[tn], n_obj - 1);
+            if constexpr (SLOTS) {
+                an0 = tile_owner(tn);
+                an1 = max(an0, tile_owner(tn + 1));
+            }
         }
         const uint64_t i0 = (uint64_t)t * MTILE;
         const uint32_t np = i0 < n_p ? (uint32_t)min<uint64_t>(MTILE, n_p - i0) : 0u;
@@ -1726,31 +1735,55 @@ __global__ __launch_bounds__(256) void k_meet_merge(const S *__restrict__ pn, ui
 #pragma unroll
         for (int k = 0; k < OPF; k++) ob_l[lane + k * STC_WAVE] = po[k];
         __builtin_amdgcn_wave_barrier();
-        /* each slot's owner: a0 plus the number of later owners starting at or before the slot
-         * (the starts ascend; an empty segment's start equals the next one's, so both count and
-         * the later owner stands). Lanes past the tile's end take its last owner. */
+        /* each slot's owner: a0 plus the number of the owners a0 + 1 .. a1 that start at or before
+         * the slot (their starts ascend; an empty segment's start equals the next one's, so both
+         * count and the later owner stands). The starts read with the tile's loads cover the next
+         * 64 owners; nb of them lie inside the tile (a prefix of the lanes). A dense tile has one to
+         * three, counted directly; up to 63 are searched by binary lifting over the lanes; a tile
+         * with more (a sparse stream's owners, most without pairs) lifts over poff itself. Lanes
+         * past the tile's end take its last owner. */
         uint32_t own[SPF];
 #pragma unroll
         for (int k = 0; k < SPF; k++) own[k] = a0;
         if constexpr (SLOTS) {
-            const uint64_t tend = i0 + np;
-            for (uint64_t c0 = 0;;) {
-                /* owners a0 + 1 + c0 + lane; past the last object the start reads as beyond the tile */
-                const bool in = (uint64_t)a0 + 1 + c0 + (uint32_t)lane <= n_obj && (uint64_t)bst < tend;
-                const uint64_t m = __ballot(in);
-                const uint32_t nb = (uint32_t)__popcll(m); /* a prefix of the lanes: the starts ascend */
-                for (uint32_t j = 0; j < nb; j++) {
-                    const uint32_t bj = (uint32_t)__builtin_amdgcn_readlane((int)bst, (int)j);
+            if (np) {
+                const uint32_t range = a1 - a0;
+                uint32_t slot[SPF], c[SPF];
 #pragma unroll
-                    for (int k = 0; k < SPF; k++) {
-                        const uint64_t i = i0 + lane + k * STC_WAVE;
-                        own[k] += (i < tend ? i : tend - 1) >= bj;
+                for (int k = 0; k < SPF; k++) {
+                    slot[k] = (uint32_t)min<uint64_t>(i0 + lane + k * STC_WAVE, i0 + np - 1);
+                    c[k] = 0;
+                }
+                const uint32_t nb =
+                    (uint32_t)__popcll(__ballot((uint32_t)lane < range && (uint64_t)bst < i0 + np));
+                if (nb <= 8) {
+                    for (uint32_t j = 0; j < nb; j++) {
+                        const uint32_t bj = (uint32_t)__builtin_amdgcn_readlane((int)bst, (int)j);
+#pragma unroll
+                        for (int k = 0; k < SPF; k++) c[k] += slot[k] >= bj;
+                    }
+                } else if (nb < STC_WAVE) {
+                    for (uint32_t st = STC_WAVE / 2; st; st >>= 1) {
+                        if (st > nb) continue; /* wave-uniform */
+#pragma unroll
+                        for (int k = 0; k < SPF; k++) {
+                            const uint32_t p = c[k] + st;
+                            const uint32_t v = (uint32_t)__shfl((int)bst, (int)min(p - 1, (uint32_t)STC_WAVE - 1), STC_WAVE);
+                            if (p <= nb && v <= slot[k]) c[k] = p;
+                        }
+                    }
+                } else {
+                    uint32_t st = 1u << (31 - __builtin_clz(range));
+                    for (; st; st >>= 1) {
+#pragma unroll
+                        for (int k = 0; k < SPF; k++) {
+                            const uint32_t p = c[k] + st;
+                            if (p <= range && poff[a0 + p] <= slot[k]) c[k] = p;
+                        }
                     }
                 }
-                if (nb < STC_WAVE) break;
-                c0 += STC_WAVE;
-                const uint64_t ai = (uint64_t)a0 + 1 + c0 + (uint32_t)lane;
-                bst = poff[ai < n_obj ? ai : n_obj];
+#pragma unroll
+                for (int k = 0; k < SPF; k++) own[k] = a0 + c[k];
             }
         }
         /* the join, instantiated for previous meetings in LDS and, for a range longer than
