@@ -59,8 +59,7 @@
 #include <vector>
 #include <type_traits>
 
-typedef uint32_t u32x4 __attribute__((ext_vector_type(4))); /* a native vector for 16-byte streaming accesses */
-/* 4-byte aligned vectors for the streaming stores of the 28-byte buckets (7 dwords) */
+/* 4-byte aligned vectors for the streaming stores of the 12-byte slots and 28-byte buckets */
 typedef uint32_t u32x4u __attribute__((ext_vector_type(4), aligned(4)));
 typedef uint32_t u32x3u __attribute__((ext_vector_type(3), aligned(4)));
 
