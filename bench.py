@@ -326,8 +326,6 @@ def batched(args):
     import torch
     from stcdb_amd import DeviceTrace, Pipeline, batched_streams, step_batch
     from stcdb_amd.tuner import CANDIDATES, tune_leaf_capacity
-    if args.tune_steps < 2:
-        sys.exit("--tune-steps must be at least 2: the tuner times every frame after a candidate's first")
     streams = batched_streams()
     total = sum(w["cfg"]["num_objects"] for w in streams)
     W, K = args.warmup, args.steps
@@ -508,6 +506,8 @@ def main():
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="copy a buffer of this size once before the steps (calibrates HBM byte counters)")
     args = ap.parse_args()
+    if args.tune_steps < 2:
+        ap.error("--tune-steps must be at least 2: the tuner times every frame after a candidate's first")
     if args.gpus != 1:
         print(f"bench.py measures one MI355X: --gpus {args.gpus} is not supported (the pipeline is single-GPU; "
               f"see the sharding design for several GPUs)", file=sys.stderr)

@@ -55,6 +55,13 @@ def test_batched_streams_are_seeded_and_within_bounds():
         assert 9.9 <= side_km <= 50.1
 
 
+def test_bench_rejects_a_single_tuner_frame():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--workload", "c4", "--tune-steps", "1"],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 2
+    assert "--tune-steps must be at least 2" in r.stderr
+
+
 def test_bench_refuses_more_than_one_gpu():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
                        text=True, timeout=300)
