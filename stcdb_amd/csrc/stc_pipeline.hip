@@ -1122,8 +1122,12 @@ __device__ inline void wave_lds_sync() {
  * key is (partner, slot), so that the sorted slots find their positions in
  * lds. With T 32 bits the key is partner << log2(64 K) | slot, the partner
  * field pbits wide and all ones for a dead slot (ids are below 2^pbits - 1),
- * which halves the sort's lane exchanges and selects; with T 64 bits it is
- * partner << 32 | slot. The sources arrive as ascending runs (each source
+ * which halves the sort's lane exchanges and selects; for K = 8 it is partner
+ * << 8 | slot / 2 (at most 510 slots, so that the padding stays above every
+ * key), the partners parked in lds beside the positions telling the two slots
+ * of a pair apart (two slots with one key hold one partner: either position
+ * serves); with T 64 bits it is partner << 32 | slot. The buffer lds holds
+ * 96 K 8-byte entries. The sources arrive as ascending runs (each source
  * leaf's members and the reverse list are in id order), so a segment that is
  * one run is not sorted and one of two runs is merged (the second run
  * reversed behind the padding through lds makes the keys bitonic, and the last
@@ -1134,7 +1138,8 @@ __device__ inline void place_segment(const uint32_t *srcs, uint32_t slots, uint3
                                      int lane, unsigned long long *lds, uint32_t pbits) {
     const uint64_t lt = lanemask_lt(lane);
     constexpr uint32_t P = K * STC_WAVE;
-    constexpr int SB = sizeof(T) == 8 ? 32 : (K == 1 ? 6 : K == 2 ? 7 : K == 4 ? 8 : 9);
+    constexpr bool PAIRED = sizeof(T) == 4 && K == 8; /* the key holds slot / 2 */
+    constexpr int SB = sizeof(T) == 8 ? 32 : (K == 1 ? 6 : K == 2 ? 7 : 8);
     static_assert(K == 1 || K == 2 || K == 4 || K == 8, "segment width");
     const uint32_t pdead = sizeof(T) == 8 ? DEAD_B : (1u << pbits) - 1u;
     T x[K];
@@ -1145,26 +1150,41 @@ __device__ inline void place_segment(const uint32_t *srcs, uint32_t slots, uint3
         const uint32_t i = r * STC_WAVE + lane;
         sv[r] = i < slots ? __builtin_nontemporal_load(srcs + i) : 0u;
     }
-    /* the partner's id and position of every slot, all loads issued before the first wait (a lane
-     * past the end reads the owner's own entries) */
-    uint32_t q[K];
-    double2 pd[K];
+    /* the partner's id and position of every slot, GB slots' loads issued before their first wait
+     * (a lane past the end reads the owner's own entries): four at a time for the widest segments,
+     * whose eight double positions in flight at once would spill */
+    constexpr int GB = K > 4 ? 4 : K;
 #pragma unroll
-    for (int r = 0; r < K; r++) {
-        const uint32_t i = r * STC_WAVE + lane;
-        const uint32_t src = sv[r], idx = src & ~(SRC_REV | SRC_AMB);
-        const bool rev = i < slots && (src & SRC_REV) != 0;
-        const uint32_t li = i < slots ? idx : w;
-        q[r] = rev ? g.rev[idx].pid : g.lp[li];
-        pd[r] = rev ? *(const double2 *)(g.rev + idx) : g.lxy[li];
+    for (int r0 = 0; r0 < K; r0 += GB) {
+        uint32_t q[GB];
+        double2 pd[GB];
+#pragma unroll
+        for (int u = 0; u < GB; u++) {
+            const int r = r0 + u;
+            const uint32_t i = r * STC_WAVE + lane;
+            const uint32_t src = sv[r], idx = src & ~(SRC_REV | SRC_AMB);
+            const bool rev = i < slots && (src & SRC_REV) != 0;
+            const uint32_t li = i < slots ? idx : w;
+            q[u] = rev ? g.rev[idx].pid : g.lp[li];
+            pd[u] = rev ? *(const double2 *)(g.rev + idx) : g.lxy[li];
+        }
+#pragma unroll
+        for (int u = 0; u < GB; u++) {
+            const int r = r0 + u;
+            const uint32_t i = r * STC_WAVE + lane;
+            uint32_t b = q[u];
+            if ((sv[r] & SRC_AMB) && !sorted_contains(acc, n_acc, ((unsigned long long)a << 32) | b)) b = pdead;
+            x[r] = i < slots ? (((T)b << SB) | (T)(PAIRED ? i >> 1 : i)) : ~(T)0;
+            pf[r] = make_float2((float)pd[u].x, (float)pd[u].y);
+        }
     }
+    /* PAIRED: each slot's key partner, parked past the first 64 K entries before the two-run
+     * reversal below permutes the keys (the reversal and the positions use the first 64 K) */
+    uint32_t *lpart = (uint32_t *)(lds + P);
+    if constexpr (PAIRED) {
 #pragma unroll
-    for (int r = 0; r < K; r++) {
-        const uint32_t i = r * STC_WAVE + lane;
-        uint32_t b = q[r];
-        if ((sv[r] & SRC_AMB) && !sorted_contains(acc, n_acc, ((unsigned long long)a << 32) | b)) b = pdead;
-        x[r] = i < slots ? (((T)b << SB) | (T)i) : ~(T)0;
-        pf[r] = make_float2((float)pd[r].x, (float)pd[r].y);
+        for (int r = 0; r < K; r++)
+            if (r * STC_WAVE + lane < slots) lpart[r * STC_WAVE + lane] = (uint32_t)(x[r] >> SB);
     }
     /* descents: places i (0 < i < slots) whose key is below its predecessor's */
     uint32_t d = 0, p = 0;
@@ -1220,7 +1240,12 @@ __device__ inline void place_segment(const uint32_t *srcs, uint32_t slots, uint3
         if (valid) {
             /* the partner and its position, streamed out past the caches: the merge reads the
              * slots once, a step's worth of traffic later */
-            const float2 pv = lpos[(uint32_t)x[r] & (P - 1)];
+            uint32_t si = (uint32_t)x[r] & (P - 1);
+            if constexpr (PAIRED) {
+                si = 2 * ((uint32_t)x[r] & (P / 2 - 1));
+                si += lpart[si] != (uint32_t)(x[r] >> SB);
+            }
+            const float2 pv = lpos[si];
             const uint32_t partner = live[r] ? (uint32_t)(x[r] >> SB) : DEAD_B;
             const u32x3u v = {partner, __float_as_uint(pv.x), __float_as_uint(pv.y)};
             __builtin_nontemporal_store(v, (u32x3u *)(seg + (live[r] ? lr + (uint32_t)__popcll(ml & lt)
@@ -1284,13 +1309,15 @@ __global__ void k_huge_expand(PairGeo g, const uint32_t *__restrict__ owners, ui
  * WCAP slots, its collected keys sorted in registers and written to pn as the
  * owner's segment sorted by partner, its live pairs first and its dead slots
  * behind them, so that pn is sorted by key. pbits: the bits of the object
- * count; a segment sorts 32-bit keys when pbits and its slot index fit in 32. */
-__global__ __launch_bounds__(256, 6) void k_pair_place(PairGeo g, const uint32_t *__restrict__ poff,
+ * count; a segment sorts 32-bit keys when pbits and its slot index (its slot
+ * pair index, for 257 to 510 slots) fit in 32. Five waves per SIMD: at six
+ * the 32-bit sort of the widest segments spills. */
+__global__ __launch_bounds__(256, 5) void k_pair_place(PairGeo g, const uint32_t *__restrict__ poff,
                                                     const uint2 *__restrict__ pkrun,
                                                     const uint32_t *__restrict__ psrc,
                                                     const unsigned long long *__restrict__ acc, uint32_t n_acc,
                                                     PairSlot *__restrict__ pn, uint32_t pbits) {
-    __shared__ unsigned long long wrun[256 / STC_WAVE][WCAP];
+    __shared__ unsigned long long wrun[256 / STC_WAVE][WCAP + WCAP / 2]; /* keys or positions, then partners */
     const int lane = threadIdx.x & (STC_WAVE - 1);
     unsigned long long *runbuf = wrun[threadIdx.x / STC_WAVE];
     const uint64_t wid = (uint32_t)__builtin_amdgcn_readfirstlane((int)((xcd_block_id() * blockDim.x + threadIdx.x) / STC_WAVE));
@@ -1310,7 +1337,8 @@ __global__ __launch_bounds__(256, 6) void k_pair_place(PairGeo g, const uint32_t
         if (slots <= STC_WAVE) PLACE(1, 6);
         else if (slots <= 2 * STC_WAVE) PLACE(2, 7);
         else if (slots <= 4 * STC_WAVE) PLACE(4, 8);
-        else PLACE(8, 9);
+        else if (slots <= 2 * 255) PLACE(8, 8); /* slot pairs below 255 keep the padding above every key */
+        else place_segment<8, unsigned long long>(keys, slots, a, (uint32_t)w, g, acc, n_acc, seg, lane, runbuf, pbits);
 #undef PLACE
     }
 }
