@@ -2854,8 +2854,11 @@ static int step_impl(StcCtx *c, const double2 *xy, uint32_t t, stc_step_out *out
                      c->ucount.p, c->rev_start.p, c->rev_pid.p,
                      c->rev.p, c->rev_dl.p, n, c->buf, r, c->band, r * r * (1.0 - 1e-12),
                      r * r * (1.0 + 1e-12)};
-    /* a multiple of 8 workgroups, for the XCD-aware owner order */
-    const uint32_t pgrid = (uint32_t)std::min<uint64_t>((blocks_for((uint64_t)n * STC_WAVE, 256) + 7) & ~7ull, 256u * 64u);
+    /* a multiple of 8 workgroups, for the XCD-aware owner order; at most 32k from 4M objects on (at 10M
+     * objects a step is 0.5 ms shorter than with 16k, measured three times each on one box), else 16k
+     * (with 32k, 64 batched streams of 48k-500k objects stepped 8 % slower) */
+    const uint32_t pcap = n >= (1u << 22) ? 256u * 128u : 256u * 64u;
+    const uint32_t pgrid = (uint32_t)std::min<uint64_t>((blocks_for((uint64_t)n * STC_WAVE, 256) + 7) & ~7ull, pcap);
     /* the last run's host copies overlap the collect pass, which computes more than it writes */
     if (queue_run_copies(c)) return fail(c, STC_E_HIP, std::string("run copies: ") + g_last_fail);
     if (c->pkrun.ensure(n)) return fail(c, STC_E_CAPACITY, "pair key runs");
